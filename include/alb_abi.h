@@ -1,0 +1,224 @@
+/*
+ * alb_abi.h — C ABI of the MI355X photon-tracing / photon-mapping library
+ * (libalbsim_hip.so).  Plain structs of scalars and pointers; no C++ or
+ * torch types cross this boundary.  Every entry point returns ALB_OK (0) or a
+ * negative alb_status and never throws; alb_last_error() gives the message.
+ *
+ * The reference (PyALBSim) has no native code and no FFI: its boundary for this
+ * path is the Python call run_linear(simulation_config, run_config, rng)
+ * (src/alb_sim/execution/linear.py:77-117).  The entry points below replace the
+ * NumPy/SciPy work that call performs:
+ *
+ *   alb_trace          <- Simulation.simulate_batch + simulate_photon_step +
+ *                         handle_water/enter/exit/seafloor + store_photons
+ *                         (src/alb_sim/core/simulation.py:71-130, 246-570)
+ *   alb_map_build      <- build_photon_map_data + PhotonMapIndex (scipy KDTree)
+ *                         (src/alb_sim/photon_mapping/build_photon_map_data.py:8-62,
+ *                          src/alb_sim/photon_mapping/photon_map_index.py:6-17)
+ *   alb_gather         <- Simulation.sample_photons (KDTree.query k-NN, energy
+ *                         multipliers, np.add.at into the waveform)
+ *                         (src/alb_sim/core/simulation.py:132-214)
+ *
+ * Conventions
+ *  - Every device buffer is owned by the caller.  Scratch memory is caller-
+ *    provided ("workspace"); its size comes from the *_workspace_size calls.
+ *  - Work is enqueued asynchronously on the given HIP stream (a hipStream_t
+ *    passed as void*; NULL = the default stream).
+ *  - Record and query sinks count every item they are offered, also past
+ *    capacity.  When a count exceeds capacity the caller reallocates and reruns;
+ *    Philox counters make the rerun produce the same items.
+ *  - Photon-type indices follow the reference enum order
+ *    (src/alb_sim/photon_mapping/photon_type.py:4-10).
+ */
+#ifndef ALB_ABI_H
+#define ALB_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALB_ABI_VERSION 1
+
+/* photon types, reference enum order (BOTTOM_REFLECTION = auto() = 1, ...) */
+enum alb_photon_type {
+  ALB_BOTTOM_REFLECTION = 0,
+  ALB_SCATTER = 1,
+  ALB_SURFACE_REFLECTION = 2,
+  ALB_SURFACE_TRANSMISSION_UP = 3,
+  ALB_N_TYPES = 4
+};
+
+enum alb_status {
+  ALB_OK = 0,
+  ALB_ERR_ARG = -1,
+  ALB_ERR_CAPACITY = -2,
+  ALB_ERR_HIP = -3,
+  ALB_ERR_UNSUPPORTED = -4,
+  ALB_ERR_WORKSPACE = -5
+};
+
+enum alb_rng_mode { ALB_RNG_PHILOX = 0, ALB_RNG_REPLAY = 1 };
+
+enum alb_pass { ALB_PASS_FORWARD = 0, ALB_PASS_BACKWARD = 1 };
+
+/* device event counters (uint64 each), written with atomics by alb_trace and
+ * alb_gather; used for the algorithmic-work (flop / byte) accounting */
+enum alb_counter {
+  ALB_CTR_PATHS = 0,         /* photons emitted                            */
+  ALB_CTR_AIR_LEGS = 1,      /* straight air legs solved in closed form     */
+  ALB_CTR_WATER_STEPS = 2,   /* in-water sub-steps (optical depth updates)  */
+  ALB_CTR_SCATTER = 3,       /* scatter events                              */
+  ALB_CTR_ENTER = 4,         /* air->water crossings                        */
+  ALB_CTR_EXIT = 5,          /* water->air crossings                        */
+  ALB_CTR_FLOOR = 6,         /* seafloor hits                               */
+  ALB_CTR_REPLAY_EXHAUSTED = 7, /* replay streams that ran out of uniforms  */
+  ALB_CTR_QUERIES = 8,       /* gather queries processed                    */
+  ALB_CTR_CANDIDATES = 9,    /* map records whose distance was evaluated    */
+  ALB_CTR_NODES = 10,        /* tree nodes visited                          */
+  ALB_CTR_OVERFLOW = 11,     /* gather queries that exceeded a kNN buffer   */
+  ALB_N_COUNTERS = 16
+};
+
+/* Scene constants, computed on the host in FP64 from SimulationConfig
+ * (src/alb_sim/physics/models/simulation.py:21-74, water.py:117-245,
+ *  sea_surface.py:30-33, laser.py:22-33).  Pointer members are device pointers
+ * for the HIP library and host pointers for the CPU oracle. */
+typedef struct alb_scene {
+  double y_ws;              /* water-surface plane y = -flying_height (m)          */
+  double y_sf;              /* seafloor plane y = -(flying_height + depth)         */
+  double laser_dir[3];      /* unit laser / sensor boresight                       */
+  float rot_laser[9];       /* float32 rotation +z -> laser_dir, row major         */
+  float pad0;
+  double divergence;        /* forward cone half-angle (rad)                       */
+  double rayleigh_sigma;    /* divergence / 3                                      */
+  double cos_fov;           /* cos(receiver half-angle)                            */
+  double aperture_radius;   /* receiver disk radius (m)                            */
+  double v_air;             /* float32(2.998e8)                                    */
+  double v_water;           /* float32(2.998e8 / n)                                */
+  double dt;                /* 1 / (sample_multiplier * sample_rate)               */
+  double n_water;           /* refractive index of layer 0                          */
+  double eta;               /* 1 / n_water (entry and exit, as the reference)      */
+  double F0;                /* ((1-n)/(1+n))^2                                     */
+  double alpha_ggx;         /* roughness^2                                         */
+  double surface_albedo;
+  double floor_albedo;
+  double absorption;        /* layer absorption coefficient a (1/m)               */
+  float alpha_l;            /* float32 lidar attenuation coefficient (1/m)         */
+  float omega0;             /* float32 single-scattering albedo                    */
+  int64_t n_substeps;       /* S = steps * sample_multiplier                       */
+  int32_t steps;            /* waveform has 2*steps bins                           */
+  int32_t mult;             /* sample_multiplier                                   */
+  int32_t k;                /* photon_mapping_k                                    */
+  int32_t phase_n;          /* entries of the phase tables (18000)                 */
+  const double* phase_theta;/* [phase_n] scattering-angle grid                     */
+  const double* phase_cdf;  /* [phase_n] normalised CDF                            */
+} alb_scene_t;
+
+typedef struct alb_rng {
+  int32_t mode;             /* ALB_RNG_PHILOX or ALB_RNG_REPLAY                     */
+  uint32_t key[2];          /* Philox4x32-10 key                                    */
+  uint32_t stream;          /* counter word 3 (pass / scene)                        */
+  const double* replay;     /* [n * replay_len] uniforms in [0,1), slot = local idx */
+  int64_t replay_len;       /* uniforms per photon slot                             */
+} alb_rng_t;
+
+/* Forward-pass record sink, one per photon type: float32 SoA like
+ * PhotonMapData (src/alb_sim/photon_mapping/photon_map_data.py:6-15) without
+ * the heat-map fields. */
+typedef struct alb_records {
+  float* x;
+  float* y;
+  float* z;
+  float* dx;
+  float* dy;
+  float* dz;
+  float* energy;
+  float* time;              /* sub-step index of the event (float like reference) */
+  int64_t* tag;             /* optional: (photon index << 16) | event ordinal     */
+  int64_t capacity;
+  unsigned long long* count;/* device counter                                     */
+} alb_records_t;
+
+/* Backward-pass query sink (sensor-path vertices), one per photon type. */
+typedef struct alb_queries {
+  double* x;
+  double* y;
+  double* z;
+  double* dx;
+  double* dy;
+  double* dz;
+  float* energy;
+  float* time;
+  int64_t* tag;             /* optional                                            */
+  int64_t capacity;
+  unsigned long long* count;
+} alb_queries_t;
+
+/* A built photon map: records sorted along a Morton curve plus a binary radix
+ * tree over them (internal node i covers sorted records [first, last]). */
+typedef struct alb_map {
+  int64_t n;                /* records                                             */
+  int32_t dims;             /* 2: key over (x,z) (planar maps), 3: over (x,y,z)    */
+  int32_t pad0;
+  float* x;                 /* sorted SoA [n]                                      */
+  float* y;
+  float* z;
+  float* dx;
+  float* dy;
+  float* dz;
+  float* energy;
+  float* time;
+  int32_t* perm;            /* [n] sorted position -> input record index          */
+  int32_t* node_child;      /* [2*(n-1)] left,right; a leaf is ~record             */
+  int32_t* node_range;      /* [2*(n-1)] first,last sorted record                  */
+  float* node_box;          /* [6*(n-1)] min x,y,z, max x,y,z                      */
+} alb_map_t;
+
+/* ---------------------------------------------------------------- entry points */
+
+int alb_abi_version(void);
+size_t alb_scene_size(void);           /* sizeof(alb_scene_t), layout check   */
+const char* alb_last_error(void);      /* thread-local message of last error  */
+
+/* Trace photons [photon_begin, photon_begin + n) of one pass.  Forward pass
+ * writes map records into out[4]; backward pass writes gather queries into
+ * q[4] (enter vertices go to both surface types).  counters: device
+ * uint64[ALB_N_COUNTERS] or NULL.
+ * Replaces core/simulation.py:71-107 (simulate_batch) with one lane per photon,
+ * closed-form air legs and exact sub-step quantisation in water. */
+int alb_trace(const alb_scene_t* scene, const alb_rng_t* rng, int32_t pass,
+              int64_t photon_begin, int64_t n, const alb_records_t* out,
+              const alb_queries_t* q, unsigned long long* counters,
+              void* stream);
+
+/* Workspace bytes needed to build a map of n records. */
+int alb_map_workspace_size(int64_t n, size_t* bytes);
+
+/* Sort n input records along a Morton curve (dims 2 or 3) into map's SoA and
+ * build the radix tree with bounding boxes.  Replaces
+ * build_photon_map_data.py:8-62 and photon_map_index.py:17 (KDTree build). */
+int alb_map_build(const alb_records_t* in, int64_t n, int32_t dims,
+                  alb_map_t* map, void* workspace, size_t workspace_bytes,
+                  void* stream);
+
+/* k-nearest-neighbour gather of queries [q_begin, q_begin + q_n) of one photon
+ * type against its map, accumulating FP64 energy into waveform[n_bins]
+ * (n_bins = 2*steps).  Bins [win0, win0 + win_len) are privatised in LDS.
+ * nbr_out (optional, [q_n * k] int32) receives each query's neighbour set as
+ * sorted-map indices in (distance, index) order; radius_out (optional,
+ * [q_n] double) the k-th neighbour distance.
+ * Replaces core/simulation.py:132-195 (sample_photons). */
+int alb_gather(const alb_scene_t* scene, const alb_map_t* map,
+               int32_t photon_type, const alb_queries_t* q, int64_t q_begin,
+               int64_t q_n, double* waveform, int64_t n_bins, int32_t win0,
+               int32_t win_len, int32_t* nbr_out, double* radius_out,
+               unsigned long long* counters, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ALB_ABI_H */

@@ -1,0 +1,264 @@
+"""CPU oracle for the photon-tracing / photon-mapping path — TEST INFRASTRUCTURE.
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py may import this package, and only as the checker (or as the timed CPU
+restatement).  The product (pyalbsim_amd) never imports it.
+
+The numerics live in alb_oracle.c (a literal, time-stepped restatement of
+src/alb_sim/core/simulation.py); this module builds it with gcc, loads it with
+ctypes and wraps it in NumPy-friendly calls.  Function-by-function citations
+and pinning are in alb_oracle.c and tests/test_oracle_*.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import threading
+
+import numpy as np
+
+from pyalbsim_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libalb_oracle.so")
+_lib = None
+_lock = threading.Lock()
+
+
+def build(force: bool = False) -> str:
+    """Compile alb_oracle.c with gcc (oracle/Makefile)."""
+    if force or not os.path.exists(LIB_PATH) or \
+            os.path.getmtime(LIB_PATH) < os.path.getmtime(os.path.join(HERE, "alb_oracle.c")):
+        subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    with _lock:
+        if _lib is None:
+            build()
+            L = C.CDLL(LIB_PATH)
+            P = C.c_void_p
+            L.orc_philox4x32_10.argtypes = [P, P, P]
+            L.orc_uniform_at.argtypes = [P, C.c_uint64, C.c_uint64]
+            L.orc_uniform_at.restype = C.c_double
+            L.orc_trace.argtypes = [P, P, C.c_int32, C.c_int64, C.c_int64, P, P, P]
+            L.orc_trace.restype = C.c_int64
+            L.orc_knn_one.argtypes = [P, P, P, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                      C.c_int32, P, P]
+            L.orc_knn_one.restype = C.c_int
+            L.orc_gather.argtypes = [P, C.c_int32, P, P, P, P, P, P, P, P, C.c_int64, P,
+                                     C.c_int64, P, C.c_int64, P, P]
+            L.orc_gather.restype = C.c_int
+            L.orc_rotation_from_z.argtypes = [P, P]
+            L.orc_interp.argtypes = [C.c_double, P, P, C.c_int]
+            L.orc_interp.restype = C.c_double
+            L.orc_refract.argtypes = [P, P, C.c_double, P]
+            L.orc_refract.restype = C.c_int
+            L.orc_brdf_reflect.argtypes = [P, P, C.c_double, C.c_double]
+            L.orc_brdf_reflect.restype = C.c_float
+            L.orc_btdf_transmit.argtypes = [P, P, C.c_double, C.c_double, C.c_double, C.c_double]
+            L.orc_btdf_transmit.restype = C.c_double
+            L.orc_scatter_mult.argtypes = [P, P, P, P, C.c_int]
+            L.orc_scatter_mult.restype = C.c_double
+            _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+# ----------------------------------------------------------------- small wrappers
+
+def philox4x32_10(ctr, key):
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox4x32_10(_ptr(c), _ptr(k), _ptr(out))
+    return out
+
+
+def uniforms(key, photon: int, n: int, stream: int = 0) -> np.ndarray:
+    r = abi.make_rng(key, stream)
+    return np.array([lib().orc_uniform_at(C.byref(r), photon, j) for j in range(n)])
+
+
+def rotation_from_z(v) -> np.ndarray:
+    vv = np.ascontiguousarray(v, dtype=np.float64)
+    R = np.zeros(9, dtype=np.float32)
+    lib().orc_rotation_from_z(_ptr(vv), _ptr(R))
+    return R.reshape(3, 3)
+
+
+def interp(x: float, xp: np.ndarray, fp: np.ndarray) -> float:
+    xp = np.ascontiguousarray(xp, dtype=np.float64)
+    fp = np.ascontiguousarray(fp, dtype=np.float64)
+    return lib().orc_interp(float(x), _ptr(xp), _ptr(fp), len(xp))
+
+
+def refract(d, n, eta):
+    dd = np.ascontiguousarray(d, dtype=np.float64)
+    nn = np.ascontiguousarray(n, dtype=np.float64)
+    out = np.zeros(3)
+    tir = lib().orc_refract(_ptr(dd), _ptr(nn), float(eta), _ptr(out))
+    return out, bool(tir)
+
+
+def brdf_reflect(dp, ds, alpha, F0) -> float:
+    a = np.ascontiguousarray(dp, dtype=np.float32)
+    b = np.ascontiguousarray(ds, dtype=np.float64)
+    return float(lib().orc_brdf_reflect(_ptr(a), _ptr(b), float(alpha), float(F0)))
+
+
+def btdf_transmit(dp, ds, alpha, n_w, F0, albedo) -> float:
+    a = np.ascontiguousarray(dp, dtype=np.float32)
+    b = np.ascontiguousarray(ds, dtype=np.float64)
+    return float(lib().orc_btdf_transmit(_ptr(a), _ptr(b), float(alpha), float(n_w), float(F0), float(albedo)))
+
+
+def scatter_mult(dp, ds, theta, cdf) -> float:
+    a = np.ascontiguousarray(dp, dtype=np.float32)
+    b = np.ascontiguousarray(ds, dtype=np.float64)
+    return float(lib().orc_scatter_mult(_ptr(a), _ptr(b), _ptr(theta), _ptr(cdf), len(theta)))
+
+
+# ----------------------------------------------------------------- scene packing
+
+class HostScene:
+    """alb_scene_t with host phase tables, kept alive with the struct."""
+
+    def __init__(self, model):
+        self.model = model
+        self.theta = np.ascontiguousarray(model.phase_theta, dtype=np.float64)
+        self.cdf = np.ascontiguousarray(model.phase_cdf, dtype=np.float64)
+        self.s = abi.make_scene(model, _ptr(self.theta), _ptr(self.cdf))
+
+
+_FIELDS_F32 = ("x", "y", "z", "dx", "dy", "dz", "energy", "time")
+
+
+def _alloc_sink(capacity: int, query: bool):
+    dt = np.float64 if query else np.float32
+    arrs = {}
+    for f in _FIELDS_F32:
+        if query and f in ("energy", "time"):
+            arrs[f] = np.zeros(capacity, dtype=np.float32)
+        else:
+            arrs[f] = np.zeros(capacity, dtype=dt if f not in ("energy", "time") else np.float32)
+    arrs["tag"] = np.zeros(capacity, dtype=np.int64)
+    arrs["count"] = np.zeros(1, dtype=np.uint64)
+    st = abi.Queries() if query else abi.Records()
+    for f in _FIELDS_F32:
+        setattr(st, f, _ptr(arrs[f]))
+    st.tag = _ptr(arrs["tag"])
+    st.capacity = capacity
+    st.count = _ptr(arrs["count"])
+    return st, arrs
+
+
+def trace(model, key, pass_: int, photon_begin: int, n: int, replay: np.ndarray | None = None,
+          stream: int | None = None, per_photon: int = 12):
+    """Trace n photons on the CPU; returns (sinks, counters, n_exhausted).
+
+    sinks[t] is a dict of NumPy arrays (records for the forward pass, queries
+    for the backward pass) truncated to the emitted count.
+    """
+    hs = HostScene(model)
+    rep = None
+    if replay is not None:
+        rep = np.ascontiguousarray(replay, dtype=np.float64)
+        assert rep.shape[0] == n
+    stream = pass_ if stream is None else stream
+    rng = abi.make_rng(key, stream, _ptr(rep) if rep is not None else 0,
+                       rep.shape[1] if rep is not None else 0)
+    cap = max(16, per_photon * n)
+    while True:
+        query = pass_ == abi.PASS_BACKWARD
+        sinks = [_alloc_sink(cap, query) for _ in range(abi.N_TYPES)]
+        rec_arr = (abi.Records * 4)()
+        q_arr = (abi.Queries * 4)()
+        for t in range(4):
+            if query:
+                q_arr[t] = sinks[t][0]
+                rec_arr[t] = abi.Records()
+            else:
+                rec_arr[t] = sinks[t][0]
+                q_arr[t] = abi.Queries()
+        ctr = np.zeros(abi.N_COUNTERS, dtype=np.uint64)
+        ex = lib().orc_trace(C.byref(hs.s), C.byref(rng), pass_, photon_begin, n,
+                             C.cast(rec_arr, C.c_void_p), C.cast(q_arr, C.c_void_p), _ptr(ctr))
+        counts = [int(s[1]["count"][0]) for s in sinks]
+        if max(counts) <= cap:
+            break
+        cap = max(counts) + 16
+    out = []
+    for t in range(4):
+        arrs = sinks[t][1]
+        c = counts[t]
+        out.append({f: arrs[f][:c].copy() for f in (*_FIELDS_F32, "tag")})
+    return out, ctr, int(ex)
+
+
+def knn(x, y, z, queries: np.ndarray, k: int):
+    """Brute-force k-NN ordered by (squared distance, index): (idx, d2)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    z = np.ascontiguousarray(z, dtype=np.float32)
+    q = np.ascontiguousarray(queries, dtype=np.float64)
+    nq = q.shape[0]
+    idx = np.zeros((nq, k), dtype=np.int32)
+    d2 = np.zeros((nq, k), dtype=np.float64)
+    L = lib()
+    for j in range(nq):
+        rc = L.orc_knn_one(_ptr(x), _ptr(y), _ptr(z), len(x), q[j, 0], q[j, 1], q[j, 2], k,
+                           idx[j].ctypes.data, d2[j].ctypes.data)
+        if rc != 0:
+            raise IndexError("k exceeds the photon map size")
+    return idx, d2
+
+
+def gather(model, ptype: int, mp: dict, qs: dict, n_bins: int | None = None,
+           want_nbrs: bool = False):
+    """FP64 waveform of one photon type from map arrays mp and queries qs."""
+    hs = HostScene(model)
+    n_bins = model.n_bins if n_bins is None else n_bins
+    m = {f: np.ascontiguousarray(mp[f], dtype=np.float32) for f in _FIELDS_F32}
+    nq = len(qs["x"])
+    qa = {f: np.ascontiguousarray(qs[f], dtype=np.float64 if f not in ("energy", "time") else np.float32)
+          for f in _FIELDS_F32}
+    qst = abi.Queries()
+    for f in _FIELDS_F32:
+        setattr(qst, f, _ptr(qa[f]))
+    qst.capacity = nq
+    wf = np.zeros(n_bins, dtype=np.float64)
+    nbr = np.zeros((nq, model.k), dtype=np.int32) if want_nbrs else None
+    rad = np.zeros(nq, dtype=np.float64) if want_nbrs else None
+    rc = lib().orc_gather(C.byref(hs.s), ptype, *[_ptr(m[f]) for f in _FIELDS_F32], len(m["x"]),
+                          C.byref(qst), nq, _ptr(wf), n_bins,
+                          _ptr(nbr) if want_nbrs else None, _ptr(rad) if want_nbrs else None)
+    if rc != 0:
+        if len(m["x"]) == 0:
+            raise KeyError(ptype)
+        raise IndexError("k exceeds the photon map size")
+    if want_nbrs:
+        return wf, nbr, rad
+    return wf
+
+
+def run(model, n_forward: int, n_backward: int, key, replay_fwd=None, replay_bwd=None):
+    """Whole path on the CPU: forward maps, backward queries, FP64 waveforms."""
+    maps, cf, exf = trace(model, key, abi.PASS_FORWARD, 0, n_forward, replay_fwd)
+    qs, cb, exb = trace(model, key, abi.PASS_BACKWARD, 0, n_backward, replay_bwd)
+    wfs = []
+    for t in range(4):
+        if len(qs[t]["x"]) == 0:
+            wfs.append(np.zeros(model.n_bins))
+            continue
+        if len(maps[t]["x"]) == 0:
+            raise KeyError(t)
+        wfs.append(gather(model, t, maps[t], qs[t]))
+    return {"waveforms": wfs, "maps": maps, "queries": qs, "counters_fwd": cf, "counters_bwd": cb,
+            "exhausted": exf + exb}

@@ -1,0 +1,688 @@
+/*
+ * alb_oracle.c — CPU oracle for the photon-tracing / photon-mapping path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (pyalbsim_amd, the HIP
+ * library) links, loads or calls this file; only tests/, the smoke check and
+ * the cpu_baseline leg of bench.py do, as the checker.
+ *
+ * It restates PyALBSim's algorithm literally: every photon is advanced by one
+ * fixed time step per iteration, exactly as Simulation.simulate_batch does for
+ * the whole batch (src/alb_sim/core/simulation.py:71-107), with the reference's
+ * FP64 positions / FP32 energies and optical depths, its state classification
+ * and its handler formulas.  The HIP trace solves straight legs in closed form
+ * instead; agreement of the two is what the parity tests check.
+ *
+ * Random numbers: a per-photon stream of FP64 uniforms, either replayed from a
+ * host buffer or drawn from Philox4x32-10 (Salmon et al., SC'11), consumed in
+ * per-photon event order:
+ *   forward emission  : theta (Rayleigh, rejected above the divergence), phi
+ *   backward emission : disk radius, disk angle, cone z, cone phi
+ *   enter: tau_target; scatter: theta, phi, tau_target; exit: roulette;
+ *   floor: two uniforms of the cosine-hemisphere map.
+ *
+ * Each function cites the reference lines it restates.  Pinning status of each
+ * function (what independent check covers it) is listed with the tests in
+ * tests/test_oracle_*.py; every function here is pinned by at least one of:
+ * Philox known-answer vectors, closed forms, distribution tests, a ballistic
+ * known-answer scene, brute-force / scipy cross-checks, an FP64 re-sum.
+ *
+ * Build: gcc -O2 -fPIC -shared -ffp-contract=off (see oracle/Makefile); no
+ * fused multiply-add, so every product and sum rounds as NumPy's do.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/alb_abi.h"
+
+#define ORC_PI 3.141592653589793
+#define ORC_FLT_EPS 1.1920928955078125e-07 /* np.finfo(np.float32).eps */
+
+/* ------------------------------------------------------------------ Philox */
+
+static inline void orc_mulhilo(uint32_t a, uint32_t b, uint32_t* hi, uint32_t* lo) {
+  uint64_t p = (uint64_t)a * (uint64_t)b;
+  *hi = (uint32_t)(p >> 32);
+  *lo = (uint32_t)p;
+}
+
+/* Philox4x32 with 10 rounds, Salmon, Moraes, Dror & Shaw, "Parallel random
+ * numbers: as easy as 1, 2, 3", SC'11: multipliers 0xD2511F53 / 0xCD9E8D57,
+ * Weyl key increments 0x9E3779B9 / 0xBB67AE85. */
+void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+  uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+  uint32_t k0 = key_in[0], k1 = key_in[1];
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    uint32_t hi0, lo0, hi1, lo1;
+    orc_mulhilo(0xD2511F53u, c0, &hi0, &lo0);
+    orc_mulhilo(0xCD9E8D57u, c2, &hi1, &lo1);
+    uint32_t n0 = hi1 ^ c1 ^ k0;
+    uint32_t n1 = lo1;
+    uint32_t n2 = hi0 ^ c3 ^ k1;
+    uint32_t n3 = lo0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+typedef struct {
+  const alb_rng_t* rng;
+  uint64_t photon;   /* global photon index (Philox counter words 1,2) */
+  int64_t slot;      /* local photon index (replay slot)               */
+  uint64_t draw;     /* next draw index                                */
+  int exhausted;
+} orc_stream;
+
+/* Uniform in [0,1) with 53 random bits: draw j uses Philox block j/2, words
+ * (0,1) for even j and (2,3) for odd j: u = ((a << 21) | (b >> 11)) * 2^-53. */
+static double orc_uniform(orc_stream* s) {
+  if (s->rng->mode == ALB_RNG_REPLAY) {
+    if ((int64_t)s->draw >= s->rng->replay_len) {
+      s->exhausted = 1;
+      return 0.5;
+    }
+    return s->rng->replay[s->slot * s->rng->replay_len + (int64_t)(s->draw++)];
+  }
+  uint64_t j = s->draw++;
+  uint32_t ctr[4] = {(uint32_t)(j >> 1), (uint32_t)s->photon, (uint32_t)(s->photon >> 32), s->rng->stream};
+  uint32_t out[4];
+  orc_philox4x32_10(ctr, s->rng->key, out);
+  uint32_t a = (j & 1) ? out[2] : out[0];
+  uint32_t b = (j & 1) ? out[3] : out[1];
+  uint64_t m = ((uint64_t)a << 21) | (uint64_t)(b >> 11);
+  return (double)m * 0x1.0p-53;
+}
+
+double orc_uniform_at(const alb_rng_t* rng, uint64_t photon, uint64_t draw) {
+  orc_stream s = {rng, photon, 0, draw, 0};
+  return orc_uniform(&s);
+}
+
+/* -------------------------------------------------------------- vector math */
+
+/* np.linalg.norm(v, axis=1) on float64: sqrt((x*x + y*y) + z*z)
+ * (src/alb_sim/math/vector_math.py:22-24) */
+static inline double orc_norm3(const double v[3]) {
+  double s = v[0] * v[0] + v[1] * v[1];
+  s = s + v[2] * v[2];
+  return sqrt(s);
+}
+
+/* normalize_batch (src/alb_sim/math/vector_math.py:14-19): zero norm -> 1 */
+static inline void orc_normalize3(double v[3]) {
+  double n = orc_norm3(v);
+  if (n == 0.0) n = 1.0;
+  v[0] = v[0] / n; v[1] = v[1] / n; v[2] = v[2] / n;
+}
+
+/* normalize_batch on a float32 array: the whole computation stays float32 */
+static inline void orc_normalize3f(float v[3]) {
+  float s = v[0] * v[0] + v[1] * v[1];
+  s = s + v[2] * v[2];
+  float n = sqrtf(s);
+  if (n == 0.0f) n = 1.0f;
+  v[0] = v[0] / n; v[1] = v[1] / n; v[2] = v[2] / n;
+}
+
+/* local @ R.T with R float32, result float64: d_i = sum_j local_j * R[i][j]
+ * (src/alb_sim/math/direction_cone.py:46-50, 112-117) */
+static inline void orc_rotate_f32(const float R[9], const double l[3], double out[3]) {
+  for (int i = 0; i < 3; ++i) {
+    double s = l[0] * (double)R[3 * i + 0];
+    s = s + l[1] * (double)R[3 * i + 1];
+    s = s + l[2] * (double)R[3 * i + 2];
+    out[i] = s;
+  }
+}
+
+/* |a - b| <= atol + rtol*|b| with numpy's defaults (np.isclose) */
+static inline int orc_isclose(double a, double b) {
+  return fabs(a - b) <= (1e-08 + 1e-05 * fabs(b));
+}
+
+/* rotation_from_z_batch for one vector, cast to float32
+ * (src/alb_sim/math/rotation_matrix.py:39-102) */
+void orc_rotation_from_z(const double vin[3], float R[9]) {
+  double v[3] = {vin[0], vin[1], vin[2]};
+  orc_normalize3(v);
+  double c = v[2];
+  double vx = -v[1], vy = v[0], vz = 0.0;
+  double s2 = vx * vx + vy * vy;
+  double M[9];
+  if (orc_isclose(c, 1.0)) {
+    double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    memcpy(M, I, sizeof M);
+  } else if (orc_isclose(c, -1.0)) {
+    double F[9] = {1, 0, 0, 0, -1, 0, 0, 0, -1};
+    memcpy(M, F, sizeof M);
+  } else {
+    double K[9] = {0.0, -vz, vy, vz, 0.0, -vx, -vy, vx, 0.0};
+    double f = (1.0 - c) / s2;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double kk = K[3 * i + 0] * K[0 * 3 + j];
+        kk = kk + K[3 * i + 1] * K[1 * 3 + j];
+        kk = kk + K[3 * i + 2] * K[2 * 3 + j];
+        double id = (i == j) ? 1.0 : 0.0;
+        M[3 * i + j] = (id + K[3 * i + j]) + kk * f;
+      }
+  }
+  for (int i = 0; i < 9; ++i) R[i] = (float)M[i];
+}
+
+/* np.interp(x, xp, fp) with numpy's clamping and exact-node rules */
+double orc_interp(double x, const double* xp, const double* fp, int n) {
+  if (x < xp[0]) return fp[0];
+  if (x > xp[n - 1]) return fp[n - 1];
+  if (x == xp[n - 1]) return fp[n - 1];
+  int lo = 0, hi = n - 1; /* xp[lo] <= x < xp[hi] */
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (xp[mid] <= x) lo = mid; else hi = mid;
+  }
+  if (xp[lo] == x) return fp[lo];
+  double slope = (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]);
+  double r = slope * (x - xp[lo]) + fp[lo];
+  if (isnan(r)) {
+    r = slope * (x - xp[lo + 1]) + fp[lo + 1];
+    if (isnan(r) && fp[lo] == fp[lo + 1]) r = fp[lo];
+  }
+  return r;
+}
+
+/* Schlick Fresnel (src/alb_sim/physics/reflection/fresnel.py:14-29) */
+static inline double orc_schlick(double cos_t, double F0) {
+  return F0 + (1.0 - F0) * pow(1.0 - cos_t, 5.0);
+}
+
+/* calculate_refraction_direction (src/alb_sim/physics/reflection/
+ * snell_refraction.py:7-45): returns the TIR flag */
+int orc_refract(const double d[3], const double n[3], double eta, double out[3]) {
+  double ci = (-d[0]) * n[0];
+  ci = ci + (-d[1]) * n[1];
+  ci = ci + (-d[2]) * n[2];
+  if (ci < -1.0) ci = -1.0;
+  if (ci > 1.0) ci = 1.0;
+  double sin2t = (eta * eta) * (1.0 - ci * ci);
+  int tir = sin2t > 1.0;
+  double ct = sqrt(fmax(0.0, 1.0 - sin2t));
+  double g = eta * ci - ct;
+  for (int i = 0; i < 3; ++i) out[i] = eta * d[i] + g * n[i];
+  orc_normalize3(out);
+  return tir;
+}
+
+/* ---------------------------------------------------------- record sinks */
+
+static void orc_emit_record(const alb_records_t* r, const double p[3], const double d[3],
+                            float E, float t, int64_t tag) {
+  unsigned long long i = (*r->count)++;
+  if ((int64_t)i >= r->capacity) return;
+  r->x[i] = (float)p[0]; r->y[i] = (float)p[1]; r->z[i] = (float)p[2];
+  r->dx[i] = (float)d[0]; r->dy[i] = (float)d[1]; r->dz[i] = (float)d[2];
+  r->energy[i] = E;
+  r->time[i] = t;
+  if (r->tag) r->tag[i] = tag;
+}
+
+static void orc_emit_query(const alb_queries_t* q, const double p[3], const double d[3],
+                           float E, float t, int64_t tag) {
+  unsigned long long i = (*q->count)++;
+  if ((int64_t)i >= q->capacity) return;
+  q->x[i] = p[0]; q->y[i] = p[1]; q->z[i] = p[2];
+  q->dx[i] = d[0]; q->dy[i] = d[1]; q->dz[i] = d[2];
+  q->energy[i] = E;
+  q->time[i] = t;
+  if (q->tag) q->tag[i] = tag;
+}
+
+/* ------------------------------------------------------------ the stepper */
+
+enum { ST_AIR = 0, ST_ENTER = 1, ST_WATER = 2, ST_FLOOR = 3, ST_EXIT = 4 };
+
+/* -log of the uniform rounded to float32, kept in float32
+ * (src/alb_sim/core/simulation.py:370-371, 428-429) */
+static inline float orc_tau_target(double u) {
+  float uf = (float)u;
+  return (float)(-log((double)uf));
+}
+
+/* Trace one photon through all S sub-steps.  Returns 0, or 1 if its replay
+ * stream ran out (the photon is then abandoned). */
+static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pass,
+                            uint64_t photon, int64_t slot, const alb_records_t* out,
+                            const alb_queries_t* q, unsigned long long* ctr) {
+  orc_stream st = {rng, photon, slot, 0, 0};
+  double p[3] = {0.0, 0.0, 0.0};
+  double d[3];
+  const int fwd = (pass == ALB_PASS_FORWARD);
+  const double two_pi = 2.0 * ORC_PI;
+
+  /* emission: physics/models/simulation.py:130-177 */
+  if (fwd) {
+    /* sample_directions_in_cone_gaussian (math/direction_cone.py:53-117) */
+    double theta;
+    do {
+      double u = orc_uniform(&st);
+      theta = sc->rayleigh_sigma * sqrt(-2.0 * log(1.0 - u));
+    } while (theta > sc->divergence && !st.exhausted);
+    double phi = two_pi * orc_uniform(&st);
+    double s = sin(theta), c = cos(theta);
+    double l[3] = {s * cos(phi), s * sin(phi), c};
+    double w[3];
+    orc_rotate_f32(sc->rot_laser, l, w);
+    for (int i = 0; i < 3; ++i) d[i] = (double)(float)w[i];
+  } else {
+    /* sample_disk_points (math/disk_sampling.py:7-53) + cone-uniform
+     * (math/direction_cone.py:7-50); disk point ADDED to the direction (Q1) */
+    double xi1 = orc_uniform(&st);
+    double xi2 = orc_uniform(&st);
+    double r = sc->aperture_radius * sqrt(xi1);
+    double th = two_pi * xi2;
+    double ld[3] = {r * cos(th), r * sin(th), 0.0};
+    double disk[3];
+    orc_rotate_f32(sc->rot_laser, ld, disk);
+    double z = sc->cos_fov + (1.0 - sc->cos_fov) * orc_uniform(&st);
+    double phi = two_pi * orc_uniform(&st);
+    double sn = sqrt(1.0 - z * z);
+    double l[3] = {sn * cos(phi), sn * sin(phi), z};
+    double w[3];
+    orc_rotate_f32(sc->rot_laser, l, w);
+    for (int i = 0; i < 3; ++i) d[i] = disk[i] + (double)(float)w[i];
+  }
+  if (ctr) ctr[ALB_CTR_PATHS]++;
+  if (st.exhausted) return 1;
+
+  float E = 1.0f, tau = 0.0f, tau_t = INFINITY;
+  int64_t event_no = 0;
+  const double y_ws = sc->y_ws, y_sf = sc->y_sf;
+  const double up[3] = {0.0, 1.0, 0.0}, down[3] = {0.0, -1.0, 0.0};
+  const double alpha = (double)sc->alpha_l;
+
+  for (int64_t s = 0; s < sc->n_substeps; ++s) {
+    double y = p[1];
+    /* above the surface and not descending: no event can ever happen again */
+    if (y > y_ws && d[1] >= 0.0) break;
+    /* velocities_by_position (physics/models/simulation.py:214-242) */
+    double v = (y > y_ws) ? sc->v_air : sc->v_water;
+    double nx[3];
+    for (int i = 0; i < 3; ++i) nx[i] = p[i] + (d[i] * v) * sc->dt;
+    double yn = nx[1];
+    /* evaluate_state (core/simulation.py:220-244), later masks win */
+    int state = ST_AIR;
+    if (y > y_ws && yn <= y_ws) state = ST_ENTER;
+    if (y <= y_ws && yn > y_sf) state = ST_WATER;
+    if (y > y_sf && yn <= y_sf) state = ST_FLOOR;
+    if (y <= y_ws && yn > y_ws) state = ST_EXIT;
+    const float t_evt = (float)s;
+    const int64_t tag = (int64_t)(photon << 16) | (event_no & 0xFFFF);
+
+    if (state == ST_AIR) {
+      p[0] = nx[0]; p[1] = nx[1]; p[2] = nx[2];
+    } else if (state == ST_WATER) {
+      /* handle_water (core/simulation.py:328-373) */
+      double sv[3] = {nx[0] - p[0], nx[1] - p[1], nx[2] - p[2]};
+      double len = orc_norm3(sv);
+      tau = (float)((double)tau + alpha * len);
+      int scatter = tau >= tau_t;
+      p[0] = nx[0]; p[1] = nx[1]; p[2] = nx[2];
+      if (ctr) ctr[ALB_CTR_WATER_STEPS]++;
+      if (scatter) {
+        if (fwd) orc_emit_record(&out[ALB_SCATTER], p, d, E, t_evt, tag);
+        else orc_emit_query(&q[ALB_SCATTER], p, d, E, t_evt, tag);
+        event_no++;
+        if (ctr) ctr[ALB_CTR_SCATTER]++;
+        /* sample_scattering_directions_batch (physics/scatter/direction.py:7-51) */
+        double u = orc_uniform(&st);
+        double theta = orc_interp(u, sc->phase_cdf, sc->phase_theta, sc->phase_n);
+        double phi = two_pi * orc_uniform(&st);
+        double l[3] = {sin(theta) * cos(phi), sin(theta) * sin(phi), cos(theta)};
+        float R[9];
+        orc_rotation_from_z(d, R);
+        double w[3];
+        for (int i = 0; i < 3; ++i) {
+          double acc = (double)R[3 * i + 0] * l[0];
+          acc = acc + (double)R[3 * i + 1] * l[1];
+          acc = acc + (double)R[3 * i + 2] * l[2];
+          w[i] = acc;
+        }
+        for (int i = 0; i < 3; ++i) d[i] = (double)(float)w[i]; /* stored float32 */
+        E = E * sc->omega0;
+        tau = tau - tau_t;
+        tau_t = orc_tau_target(orc_uniform(&st));
+      }
+    } else if (state == ST_ENTER) {
+      /* handle_enter (core/simulation.py:375-431) */
+      double sv[3] = {nx[0] - p[0], nx[1] - p[1], nx[2] - p[2]};
+      double f = (y_ws - p[1]) / sv[1];
+      if (f < 0.0) f = 0.0;
+      if (f > 1.0) f = 1.0;
+      double P[3];
+      for (int i = 0; i < 3; ++i) P[i] = p[i] + f * sv[i];
+      double ci = -d[1];
+      double Rf = orc_schlick(ci, sc->F0);
+      double T = 1.0 - Rf;
+      if (fwd) {
+        orc_emit_record(&out[ALB_SURFACE_REFLECTION], P, d, E, t_evt, tag);
+      } else {
+        orc_emit_query(&q[ALB_SURFACE_REFLECTION], P, d, E, t_evt, tag);
+        orc_emit_query(&q[ALB_SURFACE_TRANSMISSION_UP], P, d, E, t_evt, tag);
+      }
+      event_no++;
+      if (ctr) ctr[ALB_CTR_ENTER]++;
+      double nd[3];
+      orc_refract(d, up, sc->eta, nd);
+      d[0] = nd[0]; d[1] = nd[1]; d[2] = nd[2];
+      E = (float)((double)E * T);
+      double len = orc_norm3(sv);
+      double rem = (1.0 - f) * len;
+      for (int i = 0; i < 3; ++i) p[i] = P[i] + d[i] * rem;
+      tau_t = orc_tau_target(orc_uniform(&st));
+    } else if (state == ST_EXIT) {
+      /* handle_exit (core/simulation.py:433-502) */
+      double sv[3] = {nx[0] - p[0], nx[1] - p[1], nx[2] - p[2]};
+      double f = (y_ws - p[1]) / sv[1];
+      if (f < 0.0) f = 0.0;
+      if (f > 1.0) f = 1.0;
+      double P[3];
+      for (int i = 0; i < 3; ++i) P[i] = p[i] + f * sv[i];
+      double ci = d[1];
+      double Rf = orc_schlick(ci, sc->F0);
+      int reflect = orc_uniform(&st) < Rf;
+      if (!reflect && fwd) orc_emit_record(&out[ALB_SURFACE_TRANSMISSION_UP], P, d, E, t_evt, tag);
+      event_no++;
+      if (ctr) ctr[ALB_CTR_EXIT]++;
+      double rd[3];
+      int tir = orc_refract(d, down, sc->eta, rd);
+      reflect = reflect || tir;
+      if (reflect) {
+        d[1] = -d[1];
+      } else {
+        d[0] = rd[0]; d[1] = rd[1]; d[2] = rd[2];
+      }
+      double len = orc_norm3(sv);
+      double rem = (1.0 - f) * len;
+      for (int i = 0; i < 3; ++i) p[i] = P[i] + d[i] * rem;
+      if (reflect) tau = (float)((double)tau + alpha * len);
+    } else { /* ST_FLOOR */
+      /* handle_seafloor (core/simulation.py:504-570) */
+      double sv[3] = {nx[0] - p[0], nx[1] - p[1], nx[2] - p[2]};
+      double f = (y_sf - p[1]) / sv[1];
+      if (f < 0.0) f = 0.0;
+      if (f > 1.0) f = 1.0;
+      double P[3];
+      for (int i = 0; i < 3; ++i) P[i] = p[i] + f * sv[i];
+      if (fwd) orc_emit_record(&out[ALB_BOTTOM_REFLECTION], P, d, E, t_evt, tag);
+      else orc_emit_query(&q[ALB_BOTTOM_REFLECTION], P, d, E, t_evt, tag);
+      event_no++;
+      if (ctr) ctr[ALB_CTR_FLOOR]++;
+      /* cosine-weighted hemisphere about +y (physics/reflection/lambertian.py:7-27) */
+      double u1 = orc_uniform(&st);
+      double u2 = orc_uniform(&st);
+      double sr = sqrt(u1), ph = two_pi * u2;
+      d[0] = sr * cos(ph);
+      d[1] = sqrt(1.0 - u1);
+      d[2] = sr * sin(ph);
+      double len = orc_norm3(sv);
+      double rem = (1.0 - f) * len;
+      for (int i = 0; i < 3; ++i) p[i] = P[i] + d[i] * rem;
+      tau = (float)((double)tau + alpha * len);
+      E = E * (float)sc->floor_albedo;
+    }
+    if (st.exhausted) return 1;
+  }
+  return 0;
+}
+
+/* Trace photons [photon_begin, photon_begin+n) of one pass.  Same sink
+ * semantics as alb_trace (host pointers).  Returns the number of photons whose
+ * replay stream ran out. */
+int64_t orc_trace(const alb_scene_t* sc, const alb_rng_t* rng, int32_t pass,
+                  int64_t photon_begin, int64_t n, const alb_records_t* out,
+                  const alb_queries_t* q, unsigned long long* counters) {
+  int64_t exhausted = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    int ex = orc_trace_photon(sc, rng, pass, (uint64_t)(photon_begin + i), i, out, q, counters);
+    if (ex) {
+      exhausted++;
+      if (counters) counters[ALB_CTR_REPLAY_EXHAUSTED]++;
+    }
+  }
+  return exhausted;
+}
+
+/* --------------------------------------------------------------- k-NN */
+
+/* squared distance in FP64, terms summed x, y, z in order */
+static inline double orc_d2(double qx, double qy, double qz, float x, float y, float z) {
+  double dx = (double)x - qx, dy = (double)y - qy, dz = (double)z - qz;
+  double s = dx * dx + dy * dy;
+  return s + dz * dz;
+}
+
+/* (d2, idx) lexicographic "a before b" */
+static inline int orc_before(double da, int32_t ia, double db, int32_t ib) {
+  return da < db || (da == db && ia < ib);
+}
+
+/* Exact k nearest neighbours of one query by brute force over all n points,
+ * ordered by (squared distance, index).  Replaces KDTree.query(pos, k)
+ * (core/simulation.py:139-142).  Writes idx[k], d2[k]; returns 0, or -1 when
+ * k > n (the reference raises IndexError there, core/simulation.py:143-153). */
+int orc_knn_one(const float* x, const float* y, const float* z, int64_t n, double qx,
+                double qy, double qz, int32_t k, int32_t* idx, double* d2) {
+  if (k > n || k <= 0) return -1;
+  int32_t m = 0; /* max-heap of the best k on (d2, idx) */
+  for (int64_t i = 0; i < n; ++i) {
+    double di = orc_d2(qx, qy, qz, x[i], y[i], z[i]);
+    if (m < k) {
+      int32_t c = m++;
+      while (c > 0) {
+        int32_t par = (c - 1) >> 1;
+        if (orc_before(d2[par], idx[par], di, (int32_t)i)) {
+          d2[c] = d2[par]; idx[c] = idx[par]; c = par;
+        } else break;
+      }
+      d2[c] = di; idx[c] = (int32_t)i;
+    } else if (orc_before(di, (int32_t)i, d2[0], idx[0])) {
+      int32_t c = 0;
+      for (;;) {
+        int32_t l = 2 * c + 1, r = l + 1, big = c;
+        double bd = di; int32_t bi = (int32_t)i;
+        if (l < k && orc_before(bd, bi, d2[l], idx[l])) { big = l; bd = d2[l]; bi = idx[l]; }
+        if (r < k && orc_before(bd, bi, d2[r], idx[r])) { big = r; }
+        if (big == c) break;
+        d2[c] = d2[big]; idx[c] = idx[big]; c = big;
+      }
+      d2[c] = di; idx[c] = (int32_t)i;
+    }
+  }
+  /* heap sort ascending */
+  for (int32_t end = k - 1; end > 0; --end) {
+    double td = d2[0]; int32_t ti = idx[0];
+    d2[0] = d2[end]; idx[0] = idx[end];
+    d2[end] = td; idx[end] = ti;
+    int32_t c = 0;
+    for (;;) {
+      int32_t l = 2 * c + 1, r = l + 1, big = c;
+      if (l < end && orc_before(d2[big], idx[big], d2[l], idx[l])) big = l;
+      if (r < end && orc_before(d2[big], idx[big], d2[r], idx[r])) big = r;
+      if (big == c) break;
+      double sd = d2[c]; int32_t si = idx[c];
+      d2[c] = d2[big]; idx[c] = idx[big];
+      d2[big] = sd; idx[big] = si;
+      c = big;
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------ multipliers */
+
+static inline double orc_ggx_d(double ndh, double alpha) {
+  double a2 = alpha * alpha;
+  double den = (a2 - 1.0) * ndh * ndh + 1.0;
+  den = ORC_PI * den * den;
+  return a2 / (den + ORC_FLT_EPS);
+}
+
+static inline double orc_g1(double x, double alpha) {
+  double a2 = alpha * alpha;
+  double den = x + sqrt(a2 + (1.0 - a2) * (x * x));
+  return 2.0 * x / den;
+}
+
+static inline double orc_clip(double x, double lo, double hi) {
+  return x < lo ? lo : (x > hi ? hi : x);
+}
+
+/* microfacet_reflected_energy(incident=-d_p, outgoing=-d_s, n=+y), float32 out
+ * (physics/reflection/microfacet_brdf.py:96-147; physics/models/sea_surface.py:35-62) */
+float orc_brdf_reflect(const float dp[3], const double ds[3], double alpha, double F0) {
+  float wi[3] = {-dp[0], -dp[1], -dp[2]};
+  orc_normalize3f(wi);
+  double wo[3] = {-ds[0], -ds[1], -ds[2]};
+  {
+    double s = wo[0] * wo[0] + wo[1] * wo[1];
+    s = s + wo[2] * wo[2];
+    double nn = sqrt(s);
+    if (nn != 0.0) { wo[0] /= nn; wo[1] /= nn; wo[2] /= nn; }
+  }
+  double h[3] = {(double)wi[0] + wo[0], (double)wi[1] + wo[1], (double)wi[2] + wo[2]};
+  orc_normalize3(h);
+  double ndwi = orc_clip((double)wi[1], 0.0, 1.0);
+  double ndwo = orc_clip(wo[1], 0.0, 1.0);
+  double widh = (double)wi[0] * h[0];
+  widh = widh + (double)wi[1] * h[1];
+  widh = widh + (double)wi[2] * h[2];
+  widh = orc_clip(widh, 0.0, 1.0);
+  double ndh = orc_clip(h[1], 0.0, 1.0);
+  if (!(ndwi > 0.0 && ndwo > 0.0)) return 0.0f;
+  double D = orc_ggx_d(ndh, alpha);
+  double G = orc_g1(ndwi, alpha) * orc_g1(ndwo, alpha);
+  double F = orc_schlick(widh, F0);
+  double den = 4.0 * ndwi * ndwo + ORC_FLT_EPS;
+  return (float)((D * G * F) / den);
+}
+
+/* microfacet_transmitted_energy(-d_p, -d_s, +y, r, n, 1, F0) + albedo/pi
+ * (physics/reflection/microfacet_brdf.py:150-226; physics/models/sea_surface.py:64-100) */
+double orc_btdf_transmit(const float dp[3], const double ds[3], double alpha, double n_w,
+                         double F0, double surface_albedo) {
+  float wi[3] = {dp[0], dp[1], dp[2]};
+  orc_normalize3f(wi);
+  double wo[3] = {-ds[0], -ds[1], -ds[2]};
+  {
+    double s = wo[0] * wo[0] + wo[1] * wo[1];
+    s = s + wo[2] * wo[2];
+    double nn = sqrt(s);
+    if (nn != 0.0) { wo[0] /= nn; wo[1] /= nn; wo[2] /= nn; }
+  }
+  double eta_i = n_w, eta_o = 1.0, eta = eta_i / eta_o;
+  double h[3] = {(double)wi[0] + eta * wo[0], (double)wi[1] + eta * wo[1], (double)wi[2] + eta * wo[2]};
+  orc_normalize3(h);
+  double ndwi = orc_clip((double)wi[1], -1.0, 1.0);
+  double ndwo = orc_clip(wo[1], -1.0, 1.0);
+  double ndh = orc_clip(h[1], 0.0, 1.0);
+  double widh = (double)wi[0] * h[0];
+  widh = widh + (double)wi[1] * h[1];
+  widh = widh + (double)wi[2] * h[2];
+  widh = orc_clip(widh, 0.0, 1.0);
+  double wodh = h[0] * wo[0];
+  wodh = wodh + h[1] * wo[1];
+  wodh = wodh + h[2] * wo[2];
+  wodh = orc_clip(wodh, 0.0, 1.0);
+  double btdf = 0.0;
+  if (ndwi * ndwo < 0.0) {
+    double D = orc_ggx_d(ndh, alpha);
+    double G = orc_g1(fabs(ndwi), alpha) * orc_g1(fabs(ndwo), alpha);
+    double F = orc_schlick(widh, F0);
+    double den = eta_i * widh + eta_o * wodh;
+    double factor = (eta_o * eta_o * wodh) / (den * den + ORC_FLT_EPS);
+    btdf = (1.0 - F) * D * G * factor / (fabs(ndwi) * fabs(ndwo) + ORC_FLT_EPS);
+  }
+  return btdf + surface_albedo / ORC_PI;
+}
+
+/* calculate_energy_batch: phase-function mass of the 1/18000-pi angle bin
+ * (physics/scatter/energy.py:8-52).  delta is float32 arithmetic because
+ * np.pi - np.float32 eps stays float32 under NumPy 2 promotion. */
+double orc_scatter_mult(const float dp[3], const double ds[3], const double* theta,
+                        const double* cdf, int n) {
+  float pd[3] = {dp[0], dp[1], dp[2]};
+  orc_normalize3f(pd);
+  double sd[3] = {-ds[0], -ds[1], -ds[2]};
+  {
+    double s = sd[0] * sd[0] + sd[1] * sd[1];
+    s = s + sd[2] * sd[2];
+    double nn = sqrt(s);
+    if (nn != 0.0) { sd[0] /= nn; sd[1] /= nn; sd[2] /= nn; }
+  }
+  double c = (double)pd[0] * sd[0];
+  c = c + (double)pd[1] * sd[1];
+  c = c + (double)pd[2] * sd[2];
+  c = orc_clip(c, -1.0, 1.0);
+  double ang = acos(c);
+  float delta_f = ((float)ORC_PI - (float)ORC_FLT_EPS) / (float)n;
+  double delta = (double)delta_f;
+  double q = (ang - ORC_FLT_EPS) / delta;
+  int64_t i = (int64_t)q; /* astype(int) truncates toward zero */
+  if (i < 0) i = 0;
+  if (i > n - 1) i = n - 1;
+  double lo = (double)i * delta;
+  double hi = (double)(i + 1) * delta;
+  return orc_interp(hi, theta, cdf, n) - orc_interp(lo, theta, cdf, n);
+}
+
+/* ---------------------------------------------------------------- gather */
+
+/* sample_photons for queries [0, nq) of one type against one map
+ * (core/simulation.py:132-195), brute-force k-NN, FP64 accumulation in query
+ * order then neighbour order.  nbr_out (optional) [nq*k]; radius_out
+ * (optional) [nq].  Returns 0, or -1 (k > n or empty map). */
+int orc_gather(const alb_scene_t* sc, int32_t type, const float* x, const float* y,
+               const float* z, const float* dx, const float* dy, const float* dz,
+               const float* energy, const float* time, int64_t n, const alb_queries_t* q,
+               int64_t nq, double* waveform, int64_t n_bins, int32_t* nbr_out,
+               double* radius_out) {
+  const int32_t k = sc->k;
+  if (nq == 0) return 0;
+  if (n <= 0 || k > n) return -1;
+  int32_t* idx = (int32_t*)malloc(sizeof(int32_t) * (size_t)k);
+  double* d2 = (double*)malloc(sizeof(double) * (size_t)k);
+  for (int64_t j = 0; j < nq; ++j) {
+    double qx = q->x[j], qy = q->y[j], qz = q->z[j];
+    double ds[3] = {q->dx[j], q->dy[j], q->dz[j]};
+    orc_knn_one(x, y, z, n, qx, qy, qz, k, idx, d2);
+    double r = sqrt(d2[k - 1]);
+    double ksize;
+    if (type == ALB_SCATTER) ksize = (4.0 / 3.0) * ORC_PI * pow(r, 3.0);
+    else ksize = ORC_PI * (r * r);
+    double knorm = 1.0 / ksize;
+    double m_bottom = (sc->floor_albedo / ORC_PI) * fmax(0.0, -ds[1]);
+    if (nbr_out) memcpy(nbr_out + j * k, idx, sizeof(int32_t) * (size_t)k);
+    if (radius_out) radius_out[j] = r;
+    for (int32_t t = 0; t < k; ++t) {
+      int32_t i = idx[t];
+      float dp[3] = {dx[i], dy[i], dz[i]};
+      double m;
+      if (type == ALB_BOTTOM_REFLECTION) m = m_bottom;
+      else if (type == ALB_SCATTER) m = orc_scatter_mult(dp, ds, sc->phase_theta, sc->phase_cdf, sc->phase_n);
+      else if (type == ALB_SURFACE_REFLECTION) m = (double)orc_brdf_reflect(dp, ds, sc->alpha_ggx, sc->F0);
+      else m = orc_btdf_transmit(dp, ds, sc->alpha_ggx, sc->n_water, sc->F0, sc->surface_albedo);
+      float ee = q->energy[j] * energy[i];
+      double e = (double)ee * knorm * m;
+      float tt = q->time[j] + time[i];
+      int64_t bin = (int64_t)(tt / (float)sc->mult);
+      if (bin >= 0 && bin < n_bins) waveform[bin] += e;
+    }
+  }
+  free(idx);
+  free(d2);
+  return 0;
+}

@@ -1,0 +1,121 @@
+"""ctypes mirror of include/alb_abi.h (layout checked against alb_scene_size())."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+ALB_ABI_VERSION = 1
+N_TYPES = 4
+N_COUNTERS = 16
+
+ALB_OK = 0
+ALB_ERR_ARG = -1
+ALB_ERR_CAPACITY = -2
+ALB_ERR_HIP = -3
+ALB_ERR_UNSUPPORTED = -4
+ALB_ERR_WORKSPACE = -5
+
+RNG_PHILOX = 0
+RNG_REPLAY = 1
+PASS_FORWARD = 0
+PASS_BACKWARD = 1
+
+CTR_PATHS, CTR_AIR_LEGS, CTR_WATER_STEPS, CTR_SCATTER, CTR_ENTER, CTR_EXIT, CTR_FLOOR, \
+    CTR_REPLAY_EXHAUSTED, CTR_QUERIES, CTR_CANDIDATES, CTR_NODES, CTR_OVERFLOW = range(12)
+COUNTER_NAMES = ["paths", "air_legs", "water_steps", "scatter", "enter", "exit", "floor",
+                 "replay_exhausted", "queries", "candidates", "nodes", "overflow"]
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("y_ws", C.c_double), ("y_sf", C.c_double), ("laser_dir", C.c_double * 3),
+        ("rot_laser", C.c_float * 9), ("pad0", C.c_float),
+        ("divergence", C.c_double), ("rayleigh_sigma", C.c_double), ("cos_fov", C.c_double),
+        ("aperture_radius", C.c_double), ("v_air", C.c_double), ("v_water", C.c_double),
+        ("dt", C.c_double), ("n_water", C.c_double), ("eta", C.c_double), ("F0", C.c_double),
+        ("alpha_ggx", C.c_double), ("surface_albedo", C.c_double), ("floor_albedo", C.c_double),
+        ("absorption", C.c_double), ("alpha_l", C.c_float), ("omega0", C.c_float),
+        ("n_substeps", C.c_int64), ("steps", C.c_int32), ("mult", C.c_int32), ("k", C.c_int32),
+        ("phase_n", C.c_int32), ("phase_theta", C.c_void_p), ("phase_cdf", C.c_void_p),
+    ]
+
+
+class Rng(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("key", C.c_uint32 * 2), ("stream", C.c_uint32),
+                ("replay", C.c_void_p), ("replay_len", C.c_int64)]
+
+
+class Records(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("dx", C.c_void_p),
+                ("dy", C.c_void_p), ("dz", C.c_void_p), ("energy", C.c_void_p), ("time", C.c_void_p),
+                ("tag", C.c_void_p), ("capacity", C.c_int64), ("count", C.c_void_p)]
+
+
+class Queries(C.Structure):
+    _fields_ = list(Records._fields_)
+
+
+class Map(C.Structure):
+    _fields_ = [("n", C.c_int64), ("dims", C.c_int32), ("pad0", C.c_int32),
+                ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("dx", C.c_void_p),
+                ("dy", C.c_void_p), ("dz", C.c_void_p), ("energy", C.c_void_p), ("time", C.c_void_p),
+                ("perm", C.c_void_p), ("node_child", C.c_void_p), ("node_range", C.c_void_p),
+                ("node_box", C.c_void_p)]
+
+
+def make_scene(model, theta_ptr: int, cdf_ptr: int) -> Scene:
+    """Pack a SceneModel; the phase-table pointers are device or host addresses."""
+    s = Scene()
+    s.y_ws = model.y_ws
+    s.y_sf = model.y_sf
+    for i in range(3):
+        s.laser_dir[i] = float(model.laser_dir[i])
+    rot = np.asarray(model.rot_laser, dtype=np.float32).reshape(9)
+    for i in range(9):
+        s.rot_laser[i] = float(rot[i])
+    s.divergence = model.divergence
+    s.rayleigh_sigma = model.divergence / 3.0
+    s.cos_fov = float(np.cos(model.fov))
+    s.aperture_radius = model.aperture_radius
+    s.v_air = model.v_air
+    s.v_water = model.v_water
+    s.dt = model.dt
+    s.n_water = model.n_water
+    s.eta = 1.0 / model.n_water
+    s.F0 = model.F0
+    s.alpha_ggx = model.roughness * model.roughness
+    s.surface_albedo = model.surface_albedo
+    s.floor_albedo = model.floor_albedo
+    s.absorption = model.absorption
+    s.alpha_l = model.alpha_l
+    s.omega0 = model.omega0
+    s.n_substeps = model.n_substeps
+    s.steps = model.steps
+    s.mult = model.mult
+    s.k = model.k
+    s.phase_n = len(model.phase_theta)
+    s.phase_theta = theta_ptr
+    s.phase_cdf = cdf_ptr
+    return s
+
+
+def make_rng(key, stream: int = 0, replay_ptr: int = 0, replay_len: int = 0) -> Rng:
+    r = Rng()
+    if replay_ptr:
+        r.mode = RNG_REPLAY
+        r.replay = replay_ptr
+        r.replay_len = int(replay_len)
+    else:
+        r.mode = RNG_PHILOX
+    r.key[0] = int(key[0]) & 0xFFFFFFFF
+    r.key[1] = int(key[1]) & 0xFFFFFFFF
+    r.stream = int(stream) & 0xFFFFFFFF
+    return r
+
+
+def key_from_generator(rng: np.random.Generator):
+    """Philox key derived from a numpy Generator (consumes two draws)."""
+    a = int(rng.integers(0, 2 ** 32))
+    b = int(rng.integers(0, 2 ** 32))
+    return (a, b)
