@@ -172,6 +172,8 @@ typedef struct alb_map {
   float* energy;
   float* time;
   int32_t* perm;            /* [n] sorted position -> input record index          */
+  uint64_t* keys;           /* [n] sorted Morton keys                              */
+  double* meta;             /* [8] grid origin x,y,z, inverse cell width x,y,z     */
   int32_t* node_child;      /* [2*(n-1)] left,right; a leaf is ~record             */
   int32_t* node_range;      /* [2*(n-1)] first,last sorted record                  */
   float* node_box;          /* [6*(n-1)] min x,y,z, max x,y,z                      */

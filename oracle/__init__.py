@@ -63,6 +63,12 @@ def lib():
             L.orc_btdf_transmit.restype = C.c_double
             L.orc_scatter_mult.argtypes = [P, P, P, P, C.c_int]
             L.orc_scatter_mult.restype = C.c_double
+            L.orc_emission_dir.argtypes = [P, P, C.c_int32, C.c_uint64, P]
+            L.orc_lambert.argtypes = [C.c_double, C.c_double, P]
+            L.orc_scatter_direction.argtypes = [P, P, C.c_double, C.c_double, P]
+            for name in ("orc_ggx_d_x", "orc_g1_x", "orc_schlick_x"):
+                getattr(L, name).argtypes = [C.c_double, C.c_double]
+                getattr(L, name).restype = C.c_double
             _lib = L
     return _lib
 
@@ -123,6 +129,43 @@ def scatter_mult(dp, ds, theta, cdf) -> float:
     a = np.ascontiguousarray(dp, dtype=np.float32)
     b = np.ascontiguousarray(ds, dtype=np.float64)
     return float(lib().orc_scatter_mult(_ptr(a), _ptr(b), _ptr(theta), _ptr(cdf), len(theta)))
+
+
+def ggx_d(ndh, alpha):
+    return lib().orc_ggx_d_x(float(ndh), float(alpha))
+
+
+def smith_g1(x, alpha):
+    return lib().orc_g1_x(float(x), float(alpha))
+
+
+def schlick(c, F0):
+    return lib().orc_schlick_x(float(c), float(F0))
+
+
+def lambert(u1, u2):
+    out = np.zeros(3)
+    lib().orc_lambert(float(u1), float(u2), _ptr(out))
+    return out
+
+
+def emission_dirs(model, key, pass_: int, n: int) -> np.ndarray:
+    """Starting directions of photons 0..n-1 of one pass (Philox streams)."""
+    hs = HostScene(model)
+    r = abi.make_rng(key, pass_)
+    out = np.zeros((n, 3))
+    L = lib()
+    for i in range(n):
+        L.orc_emission_dir(C.byref(hs.s), C.byref(r), pass_, i, out[i].ctypes.data)
+    return out
+
+
+def scatter_direction(model, din, u1, u2):
+    hs = HostScene(model)
+    d = np.ascontiguousarray(din, dtype=np.float64)
+    out = np.zeros(3)
+    lib().orc_scatter_direction(C.byref(hs.s), _ptr(d), float(u1), float(u2), _ptr(out))
+    return out
 
 
 # ----------------------------------------------------------------- scene packing

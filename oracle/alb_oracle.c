@@ -252,6 +252,80 @@ static inline float orc_tau_target(double u) {
   return (float)(-log((double)uf));
 }
 
+/* Starting direction of one photon (the position is the origin for both
+ * passes, core/simulation.py:75): forward = Gaussian-cone laser sample,
+ * backward = aperture disk point added to a cone-uniform direction (Q1). */
+static void orc_emission(const alb_scene_t* sc, orc_stream* stp, int fwd, double d[3]) {
+  orc_stream st_local = *stp;
+  orc_stream* st = &st_local;
+  const double two_pi = 2.0 * ORC_PI;
+  if (fwd) {
+    /* sample_directions_in_cone_gaussian (math/direction_cone.py:53-117) */
+    double theta;
+    do {
+      double u = orc_uniform(st);
+      theta = sc->rayleigh_sigma * sqrt(-2.0 * log(1.0 - u));
+    } while (theta > sc->divergence && !st->exhausted);
+    double phi = two_pi * orc_uniform(st);
+    double s = sin(theta), c = cos(theta);
+    double l[3] = {s * cos(phi), s * sin(phi), c};
+    double w[3];
+    orc_rotate_f32(sc->rot_laser, l, w);
+    for (int i = 0; i < 3; ++i) d[i] = (double)(float)w[i];
+  } else {
+    /* sample_disk_points (math/disk_sampling.py:7-53) + cone-uniform
+     * (math/direction_cone.py:7-50); disk point ADDED to the direction (Q1) */
+    double xi1 = orc_uniform(st);
+    double xi2 = orc_uniform(st);
+    double r = sc->aperture_radius * sqrt(xi1);
+    double th = two_pi * xi2;
+    double ld[3] = {r * cos(th), r * sin(th), 0.0};
+    double disk[3];
+    orc_rotate_f32(sc->rot_laser, ld, disk);
+    double z = sc->cos_fov + (1.0 - sc->cos_fov) * orc_uniform(st);
+    double phi = two_pi * orc_uniform(st);
+    double sn = sqrt(1.0 - z * z);
+    double l[3] = {sn * cos(phi), sn * sin(phi), z};
+    double w[3];
+    orc_rotate_f32(sc->rot_laser, l, w);
+    for (int i = 0; i < 3; ++i) d[i] = disk[i] + (double)(float)w[i];
+  }
+  *stp = st_local;
+}
+
+/* exported for the sampler distribution tests */
+void orc_emission_dir(const alb_scene_t* sc, const alb_rng_t* rng, int32_t pass, uint64_t photon, double out[3]) {
+  orc_stream st = {rng, photon, 0, 0, 0};
+  orc_emission(sc, &st, pass == ALB_PASS_FORWARD, out);
+}
+
+/* cosine-weighted hemisphere about +y from two uniforms (the floor reflection;
+ * same distribution as normalize(y + isotropic unit vector) of
+ * physics/reflection/lambertian.py:7-27) */
+void orc_lambert(double u1, double u2, double out[3]) {
+  const double sr = sqrt(u1), ph = 2.0 * ORC_PI * u2;
+  out[0] = sr * cos(ph);
+  out[1] = sqrt(1.0 - u1);
+  out[2] = sr * sin(ph);
+}
+
+/* new direction after a scatter: theta by inverse CDF of the phase table,
+ * phi uniform, rotated from +z onto the incoming direction with a float32
+ * matrix, stored float32 (physics/scatter/direction.py:7-51) */
+void orc_scatter_direction(const alb_scene_t* sc, const double din[3], double u1, double u2, double out[3]) {
+  double theta = orc_interp(u1, sc->phase_cdf, sc->phase_theta, sc->phase_n);
+  double phi = 2.0 * ORC_PI * u2;
+  double l[3] = {sin(theta) * cos(phi), sin(theta) * sin(phi), cos(theta)};
+  float R[9];
+  orc_rotation_from_z(din, R);
+  for (int i = 0; i < 3; ++i) {
+    double acc = (double)R[3 * i + 0] * l[0];
+    acc = acc + (double)R[3 * i + 1] * l[1];
+    acc = acc + (double)R[3 * i + 2] * l[2];
+    out[i] = (double)(float)acc;
+  }
+}
+
 /* Trace one photon through all S sub-steps.  Returns 0, or 1 if its replay
  * stream ran out (the photon is then abandoned). */
 static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pass,
@@ -261,40 +335,9 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
   double p[3] = {0.0, 0.0, 0.0};
   double d[3];
   const int fwd = (pass == ALB_PASS_FORWARD);
-  const double two_pi = 2.0 * ORC_PI;
 
   /* emission: physics/models/simulation.py:130-177 */
-  if (fwd) {
-    /* sample_directions_in_cone_gaussian (math/direction_cone.py:53-117) */
-    double theta;
-    do {
-      double u = orc_uniform(&st);
-      theta = sc->rayleigh_sigma * sqrt(-2.0 * log(1.0 - u));
-    } while (theta > sc->divergence && !st.exhausted);
-    double phi = two_pi * orc_uniform(&st);
-    double s = sin(theta), c = cos(theta);
-    double l[3] = {s * cos(phi), s * sin(phi), c};
-    double w[3];
-    orc_rotate_f32(sc->rot_laser, l, w);
-    for (int i = 0; i < 3; ++i) d[i] = (double)(float)w[i];
-  } else {
-    /* sample_disk_points (math/disk_sampling.py:7-53) + cone-uniform
-     * (math/direction_cone.py:7-50); disk point ADDED to the direction (Q1) */
-    double xi1 = orc_uniform(&st);
-    double xi2 = orc_uniform(&st);
-    double r = sc->aperture_radius * sqrt(xi1);
-    double th = two_pi * xi2;
-    double ld[3] = {r * cos(th), r * sin(th), 0.0};
-    double disk[3];
-    orc_rotate_f32(sc->rot_laser, ld, disk);
-    double z = sc->cos_fov + (1.0 - sc->cos_fov) * orc_uniform(&st);
-    double phi = two_pi * orc_uniform(&st);
-    double sn = sqrt(1.0 - z * z);
-    double l[3] = {sn * cos(phi), sn * sin(phi), z};
-    double w[3];
-    orc_rotate_f32(sc->rot_laser, l, w);
-    for (int i = 0; i < 3; ++i) d[i] = disk[i] + (double)(float)w[i];
-  }
+  orc_emission(sc, &st, fwd, d);
   if (ctr) ctr[ALB_CTR_PATHS]++;
   if (st.exhausted) return 1;
 
@@ -338,20 +381,11 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
         event_no++;
         if (ctr) ctr[ALB_CTR_SCATTER]++;
         /* sample_scattering_directions_batch (physics/scatter/direction.py:7-51) */
-        double u = orc_uniform(&st);
-        double theta = orc_interp(u, sc->phase_cdf, sc->phase_theta, sc->phase_n);
-        double phi = two_pi * orc_uniform(&st);
-        double l[3] = {sin(theta) * cos(phi), sin(theta) * sin(phi), cos(theta)};
-        float R[9];
-        orc_rotation_from_z(d, R);
-        double w[3];
-        for (int i = 0; i < 3; ++i) {
-          double acc = (double)R[3 * i + 0] * l[0];
-          acc = acc + (double)R[3 * i + 1] * l[1];
-          acc = acc + (double)R[3 * i + 2] * l[2];
-          w[i] = acc;
-        }
-        for (int i = 0; i < 3; ++i) d[i] = (double)(float)w[i]; /* stored float32 */
+        double u1 = orc_uniform(&st);
+        double u2 = orc_uniform(&st);
+        double nd[3];
+        orc_scatter_direction(sc, d, u1, u2, nd);
+        d[0] = nd[0]; d[1] = nd[1]; d[2] = nd[2];
         E = E * sc->omega0;
         tau = tau - tau_t;
         tau_t = orc_tau_target(orc_uniform(&st));
@@ -424,10 +458,7 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
       /* cosine-weighted hemisphere about +y (physics/reflection/lambertian.py:7-27) */
       double u1 = orc_uniform(&st);
       double u2 = orc_uniform(&st);
-      double sr = sqrt(u1), ph = two_pi * u2;
-      d[0] = sr * cos(ph);
-      d[1] = sqrt(1.0 - u1);
-      d[2] = sr * sin(ph);
+      orc_lambert(u1, u2, d);
       double len = orc_norm3(sv);
       double rem = (1.0 - f) * len;
       for (int i = 0; i < 3; ++i) p[i] = P[i] + d[i] * rem;
@@ -638,6 +669,10 @@ double orc_scatter_mult(const float dp[3], const double ds[3], const double* the
   double hi = (double)(i + 1) * delta;
   return orc_interp(hi, theta, cdf, n) - orc_interp(lo, theta, cdf, n);
 }
+
+double orc_ggx_d_x(double ndh, double alpha) { return orc_ggx_d(ndh, alpha); }
+double orc_g1_x(double x, double alpha) { return orc_g1(x, alpha); }
+double orc_schlick_x(double c, double F0) { return orc_schlick(c, F0); }
 
 /* ---------------------------------------------------------------- gather */
 

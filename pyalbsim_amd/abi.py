@@ -16,6 +16,8 @@ ALB_ERR_HIP = -3
 ALB_ERR_UNSUPPORTED = -4
 ALB_ERR_WORKSPACE = -5
 
+BOTTOM_REFLECTION, SCATTER, SURFACE_REFLECTION, SURFACE_TRANSMISSION_UP = range(4)
+
 RNG_PHILOX = 0
 RNG_REPLAY = 1
 PASS_FORWARD = 0
@@ -60,7 +62,8 @@ class Map(C.Structure):
     _fields_ = [("n", C.c_int64), ("dims", C.c_int32), ("pad0", C.c_int32),
                 ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("dx", C.c_void_p),
                 ("dy", C.c_void_p), ("dz", C.c_void_p), ("energy", C.c_void_p), ("time", C.c_void_p),
-                ("perm", C.c_void_p), ("node_child", C.c_void_p), ("node_range", C.c_void_p),
+                ("perm", C.c_void_p), ("keys", C.c_void_p), ("meta", C.c_void_p),
+                ("node_child", C.c_void_p), ("node_range", C.c_void_p),
                 ("node_box", C.c_void_p)]
 
 

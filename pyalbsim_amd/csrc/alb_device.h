@@ -1,0 +1,329 @@
+// alb_device.h — device-side building blocks shared by the trace and gather
+// kernels: Philox4x32-10 counter RNG, the per-photon uniform stream, wave
+// helpers and the surface / phase-function formulas of the reference
+// (src/alb_sim/physics/reflection/*.py, physics/scatter/energy.py).
+//
+// Arithmetic is compiled with -ffp-contract=off so every product and sum is
+// rounded separately, in the order NumPy evaluates the reference expressions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/alb_abi.h"
+
+#define ALB_PI 3.141592653589793
+#define ALB_FLT_EPS 1.1920928955078125e-07  // np.finfo(np.float32).eps
+
+namespace alb {
+
+// ------------------------------------------------------------------ Philox
+// Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11).
+struct u32x4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint32_t lo0 = 0xD2511F53u * c.x;
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c.x);
+    const uint32_t lo1 = 0xCD9E8D57u * c.z;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z);
+    c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+  }
+  return c;
+}
+
+__device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
+  const uint64_t m = ((uint64_t)a << 21) | (uint64_t)(b >> 11);
+  return (double)m * 0x1.0p-53;
+}
+
+// Per-photon stream of FP64 uniforms in [0,1): draw j comes from Philox block
+// j/2 (counter = {j/2, photon lo, photon hi, stream}), words (x,y) for even j
+// and (z,w) for odd j; or from the caller's replay buffer.
+struct Stream {
+  uint64_t photon;
+  int64_t slot;
+  uint32_t draw;
+  bool exhausted;
+  double cached;
+};
+
+__device__ __forceinline__ double uniform(const alb_rng_t& rng, Stream& s) {
+  if (rng.mode == ALB_RNG_REPLAY) {
+    if ((int64_t)s.draw >= rng.replay_len) {
+      s.exhausted = true;
+      return 0.5;
+    }
+    return rng.replay[s.slot * rng.replay_len + (int64_t)(s.draw++)];
+  }
+  const uint32_t j = s.draw++;
+  if (j & 1u) return s.cached;
+  const u32x4 r = philox4x32_10(
+      u32x4{j >> 1, (uint32_t)s.photon, (uint32_t)(s.photon >> 32), rng.stream}, rng.key[0], rng.key[1]);
+  s.cached = u53(r.z, r.w);
+  return u53(r.x, r.y);
+}
+
+// ------------------------------------------------------------- wave helpers
+__device__ __forceinline__ uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// number of set bits of mask below this lane
+__device__ __forceinline__ uint32_t mask_rank(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ uint64_t bcast_u64(uint64_t v, int src) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, src);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ double bcast_f64(double v, int src) {
+  return __longlong_as_double((long long)bcast_u64((uint64_t)__double_as_longlong(v), src));
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ------------------------------------------------------------- vector math
+__device__ __forceinline__ double norm3(double x, double y, double z) {
+  double s = x * x + y * y;
+  s = s + z * z;
+  return sqrt(s);
+}
+
+// np.interp(x, xp, fp) for increasing xp: clamps, exact nodes, nan fallback
+__device__ __forceinline__ double interp(double x, const double* __restrict__ xp,
+                                         const double* __restrict__ fp, int n) {
+  if (x < xp[0]) return fp[0];
+  if (x >= xp[n - 1]) return fp[n - 1];
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (xp[mid] <= x) lo = mid; else hi = mid;
+  }
+  if (xp[lo] == x) return fp[lo];
+  const double slope = (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]);
+  double r = slope * (x - xp[lo]) + fp[lo];
+  if (isnan(r)) {
+    r = slope * (x - xp[lo + 1]) + fp[lo + 1];
+    if (isnan(r) && fp[lo] == fp[lo + 1]) r = fp[lo];
+  }
+  return r;
+}
+
+// np.interp on the theta grid itself (uniform spacing up to rounding): locate
+// the node by division and correct by one in either direction.
+__device__ __forceinline__ double interp_uniform_grid(double x, const double* __restrict__ xp,
+                                                      const double* __restrict__ fp, int n) {
+  if (x < xp[0]) return fp[0];
+  if (x >= xp[n - 1]) return fp[n - 1];
+  const double step = (xp[n - 1] - xp[0]) / (double)(n - 1);
+  int lo = (int)((x - xp[0]) / step);
+  lo = lo < 0 ? 0 : (lo > n - 2 ? n - 2 : lo);
+  while (lo > 0 && xp[lo] > x) --lo;
+  while (lo < n - 2 && xp[lo + 1] <= x) ++lo;
+  if (xp[lo] == x) return fp[lo];
+  const double slope = (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]);
+  double r = slope * (x - xp[lo]) + fp[lo];
+  if (isnan(r)) {
+    r = slope * (x - xp[lo + 1]) + fp[lo + 1];
+    if (isnan(r) && fp[lo] == fp[lo + 1]) r = fp[lo];
+  }
+  return r;
+}
+
+__device__ __forceinline__ double clipd(double x, double lo, double hi) {
+  return x < lo ? lo : (x > hi ? hi : x);
+}
+
+// Schlick: F0 + (1 - F0)(1 - cos)^5 (physics/reflection/fresnel.py:14-29)
+__device__ __forceinline__ double schlick(double c, double F0) {
+  return F0 + (1.0 - F0) * pow(1.0 - c, 5.0);
+}
+
+// GGX D with alpha2 = alpha^2 (microfacet_brdf.py:18-39)
+__device__ __forceinline__ double ggx_d(double ndh, double alpha) {
+  const double a2 = alpha * alpha;
+  double den = (a2 - 1.0) * ndh * ndh + 1.0;
+  den = ALB_PI * den * den;
+  return a2 / (den + ALB_FLT_EPS);
+}
+
+// Smith G1 (microfacet_brdf.py:42-67)
+__device__ __forceinline__ double smith_g1(double x, double alpha) {
+  const double a2 = alpha * alpha;
+  const double den = x + sqrt(a2 + (1.0 - a2) * (x * x));
+  return 2.0 * x / den;
+}
+
+// float32 normalisation of a stored map direction (normalize_batch on float32)
+__device__ __forceinline__ void normalize_f32(float& x, float& y, float& z) {
+  float s = x * x + y * y;
+  s = s + z * z;
+  float n = sqrtf(s);
+  if (n == 0.0f) n = 1.0f;
+  x = x / n;
+  y = y / n;
+  z = z / n;
+}
+
+__device__ __forceinline__ void normalize_f64_keep_zero(double& x, double& y, double& z) {
+  const double n = norm3(x, y, z);
+  if (n != 0.0) {
+    x = x / n;
+    y = y / n;
+    z = z / n;
+  }
+}
+
+// Cook-Torrance reflection towards the sensor, float32 result
+// (microfacet_brdf.py:96-147 called as sea_surface.py:35-62: wi = -d_p, wo = -d_s)
+__device__ __forceinline__ float brdf_reflect(float px, float py, float pz, double sx, double sy,
+                                              double sz, double alpha, double F0) {
+  float wx = -px, wy = -py, wz = -pz;
+  normalize_f32(wx, wy, wz);
+  double ox = -sx, oy = -sy, oz = -sz;
+  normalize_f64_keep_zero(ox, oy, oz);
+  double hx = (double)wx + ox, hy = (double)wy + oy, hz = (double)wz + oz;
+  {
+    double n = norm3(hx, hy, hz);
+    if (n == 0.0) n = 1.0;
+    hx = hx / n;
+    hy = hy / n;
+    hz = hz / n;
+  }
+  const double ndwi = clipd((double)wy, 0.0, 1.0);
+  const double ndwo = clipd(oy, 0.0, 1.0);
+  double widh = (double)wx * hx;
+  widh = widh + (double)wy * hy;
+  widh = widh + (double)wz * hz;
+  widh = clipd(widh, 0.0, 1.0);
+  const double ndh = clipd(hy, 0.0, 1.0);
+  if (!(ndwi > 0.0 && ndwo > 0.0)) return 0.0f;
+  const double D = ggx_d(ndh, alpha);
+  const double G = smith_g1(ndwi, alpha) * smith_g1(ndwo, alpha);
+  const double F = schlick(widh, F0);
+  const double den = 4.0 * ndwi * ndwo + ALB_FLT_EPS;
+  return (float)((D * G * F) / den);
+}
+
+// Walter BTDF + surface albedo / pi (microfacet_brdf.py:150-226 called as
+// sea_surface.py:64-100 with eta_i = n_water, eta_o = 1)
+__device__ __forceinline__ double btdf_transmit(float px, float py, float pz, double sx, double sy,
+                                                double sz, double alpha, double n_w, double F0,
+                                                double albedo) {
+  float wx = px, wy = py, wz = pz;
+  normalize_f32(wx, wy, wz);
+  double ox = -sx, oy = -sy, oz = -sz;
+  normalize_f64_keep_zero(ox, oy, oz);
+  const double eta_i = n_w, eta_o = 1.0, eta = eta_i / eta_o;
+  double hx = (double)wx + eta * ox, hy = (double)wy + eta * oy, hz = (double)wz + eta * oz;
+  {
+    double n = norm3(hx, hy, hz);
+    if (n == 0.0) n = 1.0;
+    hx = hx / n;
+    hy = hy / n;
+    hz = hz / n;
+  }
+  const double ndwi = clipd((double)wy, -1.0, 1.0);
+  const double ndwo = clipd(oy, -1.0, 1.0);
+  double btdf = 0.0;
+  if (ndwi * ndwo < 0.0) {
+    const double ndh = clipd(hy, 0.0, 1.0);
+    double widh = (double)wx * hx;
+    widh = widh + (double)wy * hy;
+    widh = widh + (double)wz * hz;
+    widh = clipd(widh, 0.0, 1.0);
+    double wodh = hx * ox;
+    wodh = wodh + hy * oy;
+    wodh = wodh + hz * oz;
+    wodh = clipd(wodh, 0.0, 1.0);
+    const double D = ggx_d(ndh, alpha);
+    const double G = smith_g1(fabs(ndwi), alpha) * smith_g1(fabs(ndwo), alpha);
+    const double F = schlick(widh, F0);
+    const double den = eta_i * widh + eta_o * wodh;
+    const double factor = (eta_o * eta_o * wodh) / (den * den + ALB_FLT_EPS);
+    btdf = (1.0 - F) * D * G * factor / (fabs(ndwi) * fabs(ndwo) + ALB_FLT_EPS);
+  }
+  return btdf + albedo / ALB_PI;
+}
+
+// Phase-function mass of the scattering-angle bin towards the sensor
+// (physics/scatter/energy.py:8-52).  delta is float32: pi - float32 eps
+// stays float32 under NumPy 2 promotion.
+__device__ __forceinline__ double scatter_mult(float px, float py, float pz, double sx, double sy,
+                                               double sz, const double* __restrict__ theta,
+                                               const double* __restrict__ cdf, int n) {
+  normalize_f32(px, py, pz);
+  double ox = -sx, oy = -sy, oz = -sz;
+  normalize_f64_keep_zero(ox, oy, oz);
+  double c = (double)px * ox;
+  c = c + (double)py * oy;
+  c = c + (double)pz * oz;
+  c = clipd(c, -1.0, 1.0);
+  const double ang = acos(c);
+  const float delta_f = (3.14159265358979323846f - 1.1920928955078125e-07f) / (float)n;
+  const double delta = (double)delta_f;
+  long long i = (long long)((ang - ALB_FLT_EPS) / delta);
+  if (i < 0) i = 0;
+  if (i > n - 1) i = n - 1;
+  const double lo = (double)i * delta;
+  const double hi = (double)(i + 1) * delta;
+  return interp_uniform_grid(hi, theta, cdf, n) - interp_uniform_grid(lo, theta, cdf, n);
+}
+
+// ----------------------------------------------------------- Morton keys
+__device__ __forceinline__ uint64_t spread2(uint64_t v) {  // 32 bits -> even bit positions
+  v &= 0xFFFFFFFFull;
+  v = (v | (v << 16)) & 0x0000FFFF0000FFFFull;
+  v = (v | (v << 8)) & 0x00FF00FF00FF00FFull;
+  v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0Full;
+  v = (v | (v << 2)) & 0x3333333333333333ull;
+  v = (v | (v << 1)) & 0x5555555555555555ull;
+  return v;
+}
+
+__device__ __forceinline__ uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
+  v &= 0x1FFFFFull;
+  v = (v | (v << 32)) & 0x1F00000000FFFFull;
+  v = (v | (v << 16)) & 0x1F0000FF0000FFull;
+  v = (v | (v << 8)) & 0x100F00F00F00F00Full;
+  v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+
+__device__ __forceinline__ uint64_t quant(double p, double lo, double inv, double maxc) {
+  double c = floor((p - lo) * inv);
+  c = c < 0.0 ? 0.0 : (c > maxc ? maxc : c);
+  return (uint64_t)c;
+}
+
+// Morton key of a point on the map grid described by meta (origin, inverse
+// cell width, dims, bits per axis); used by the map build and the gather
+__device__ __forceinline__ uint64_t morton_key(double px, double py, double pz, const double* meta) {
+  const int dims = (int)meta[6];
+  const double maxc = ldexp(1.0, (int)meta[7]) - 1.0;
+  const uint64_t qx = quant(px, meta[0], meta[3], maxc);
+  const uint64_t qz = quant(pz, meta[2], meta[5], maxc);
+  if (dims == 2) return (spread2(qx) << 1) | spread2(qz);
+  const uint64_t qy = quant(py, meta[1], meta[4], maxc);
+  return (spread3(qx) << 2) | (spread3(qy) << 1) | spread3(qz);
+}
+
+}  // namespace alb

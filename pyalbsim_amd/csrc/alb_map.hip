@@ -1,0 +1,312 @@
+// alb_map.hip — photon-map build: Morton-ordered records + binary radix tree.
+//
+// Replaces build_photon_map_data (src/alb_sim/photon_mapping/
+// build_photon_map_data.py:8-62: concatenation into float32 SoA) and the
+// scipy KDTree build of PhotonMapIndex (photon_map_index.py:17).
+//
+//  1. bounding box of the records (block reduction + ordered-int atomics)
+//  2. quantise each record to a uniform grid (2^32 cells per axis over (x,z)
+//     for the planar surface / floor maps, 2^21 over (x,y,z) for scatter
+//     events) and interleave the cell coordinates into a 64-bit Morton key
+//  3. sort (key, record index) pairs (rocPRIM radix sort via hipCUB)
+//  4. gather the float32 SoA into Morton order
+//  5. build the binary radix tree over the sorted keys (Karras, "Maximizing
+//     parallelism in the construction of BVHs, octrees, and k-d trees", HPG
+//     2012): internal node i covers a contiguous record range; equal keys are
+//     split by index
+//  6. exact float32 bounding boxes per node, bottom-up: a child's prefix
+//     length is always longer than its parent's, so one launch per prefix
+//     length (longest first) finds every child box already written.
+#include <hipcub/hipcub.hpp>
+
+#include "alb_device.h"
+#include "alb_internal.h"
+
+namespace alb {
+
+constexpr int kLevels = 128;  // prefix lengths 0..63 (+64..95 for equal keys)
+
+__device__ __forceinline__ uint32_t f2ord(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+__global__ void bbox_init_kernel(uint32_t* bb, uint32_t* hist) {
+  if (threadIdx.x < 3) bb[threadIdx.x] = 0xFFFFFFFFu;
+  else if (threadIdx.x < 6) bb[threadIdx.x] = 0u;
+  for (int i = threadIdx.x; i < kLevels; i += blockDim.x) hist[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void bbox_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                   const float* __restrict__ z, int64_t n, uint32_t* bb) {
+  uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t a = f2ord(x[i]), b = f2ord(y[i]), c = f2ord(z[i]);
+    mn[0] = min(mn[0], a); mx[0] = max(mx[0], a);
+    mn[1] = min(mn[1], b); mx[1] = max(mx[1], b);
+    mn[2] = min(mn[2], c); mx[2] = max(mx[2], c);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      mn[k] = min(mn[k], (uint32_t)__shfl_xor((int)mn[k], off));
+      mx[k] = max(mx[k], (uint32_t)__shfl_xor((int)mx[k], off));
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      atomicMin(&bb[k], mn[k]);
+      atomicMax(&bb[3 + k], mx[k]);
+    }
+  }
+}
+
+// meta = {origin x,y,z, inverse cell width x,y,z, dims, bits per axis}
+__global__ void meta_kernel(const uint32_t* bb, double* meta, int dims) {
+  const int bits = dims == 2 ? 32 : 21;
+  const double cells = ldexp(1.0, bits);
+  for (int k = 0; k < 3; ++k) {
+    const double lo = (double)ord2f(bb[k]);
+    const double hi = (double)ord2f(bb[3 + k]);
+    double ext = hi - lo;
+    if (!(ext > 0.0)) ext = 1.0;
+    meta[k] = lo;
+    meta[3 + k] = cells / ext;
+  }
+  meta[6] = (double)dims;
+  meta[7] = (double)bits;
+}
+
+__global__ __launch_bounds__(256) void key_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                  const float* __restrict__ z, int64_t n,
+                                                  const double* __restrict__ meta, uint64_t* __restrict__ keys,
+                                                  int32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = morton_key((double)x[i], (double)y[i], (double)z[i], meta);
+  vals[i] = (int32_t)i;
+}
+
+__global__ __launch_bounds__(256) void permute_kernel(alb_records_t in, alb_map_t m) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.n) return;
+  const int32_t j = m.perm[i];
+  m.x[i] = in.x[j]; m.y[i] = in.y[j]; m.z[i] = in.z[j];
+  m.dx[i] = in.dx[j]; m.dy[i] = in.dy[j]; m.dz[i] = in.dz[j];
+  m.energy[i] = in.energy[j];
+  m.time[i] = in.time[j];
+}
+
+// common-prefix length of keys i and j; equal keys continue with the index bits
+__device__ __forceinline__ int delta(const uint64_t* __restrict__ k, int64_t n, int64_t i, int64_t j) {
+  if (j < 0 || j >= n) return -1;
+  const uint64_t a = k[i], b = k[j];
+  if (a == b) return 64 + __clz((uint32_t)i ^ (uint32_t)j);
+  return __clzll(a ^ b);
+}
+
+__device__ __forceinline__ void build_node(const uint64_t* __restrict__ k, int64_t n, int64_t i,
+                                           int32_t* __restrict__ child, int32_t* __restrict__ range,
+                                           uint8_t* __restrict__ level, uint32_t* lh);
+
+__global__ __launch_bounds__(256) void radix_tree_kernel(const uint64_t* __restrict__ k, int64_t n,
+                                                         int32_t* __restrict__ child, int32_t* __restrict__ range,
+                                                         uint8_t* __restrict__ level, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t lh[kLevels];
+  for (int t = threadIdx.x; t < kLevels; t += blockDim.x) lh[t] = 0u;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n - 1) build_node(k, n, i, child, range, level, lh);
+  __syncthreads();
+  for (int t = threadIdx.x; t < kLevels; t += blockDim.x)
+    if (lh[t]) atomicAdd(&hist[t], lh[t]);
+}
+
+__device__ __forceinline__ void build_node(const uint64_t* __restrict__ k, int64_t n, int64_t i,
+                                           int32_t* __restrict__ child, int32_t* __restrict__ range,
+                                           uint8_t* __restrict__ level, uint32_t* lh) {
+  const int d = (delta(k, n, i, i + 1) - delta(k, n, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta(k, n, i, i - d);
+  int64_t lmax = 2;
+  while (delta(k, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  int64_t l = 0;
+  for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
+    if (delta(k, n, i, i + (l + t) * d) > dmin) l += t;
+  const int64_t j = i + l * d;
+  const int dnode = delta(k, n, i, j);
+  int64_t s = 0;
+  int64_t div = 2;
+  int64_t t = (l + div - 1) / div;
+  while (true) {
+    if (delta(k, n, i, i + (s + t) * d) > dnode) s += t;
+    if (t <= 1) break;
+    div <<= 1;
+    t = (l + div - 1) / div;
+  }
+  const int64_t g = i + s * d + (d < 0 ? -1 : 0);
+  const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+  child[2 * i + 0] = (lo == g) ? ~(int32_t)g : (int32_t)g;
+  child[2 * i + 1] = (hi == g + 1) ? ~(int32_t)(g + 1) : (int32_t)(g + 1);
+  range[2 * i + 0] = (int32_t)lo;
+  range[2 * i + 1] = (int32_t)hi;
+  const int lv = dnode < 0 ? 0 : (dnode >= kLevels ? kLevels - 1 : dnode);
+  level[i] = (uint8_t)lv;
+  atomicAdd(&lh[lv], 1u);
+}
+
+__global__ void level_scan_kernel(const uint32_t* hist, uint32_t* offset, uint32_t* cursor) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t acc = 0;
+  for (int i = 0; i < kLevels; ++i) {
+    offset[i] = acc;
+    cursor[i] = acc;
+    acc += hist[i];
+  }
+  offset[kLevels] = acc;
+}
+
+__global__ __launch_bounds__(256) void level_scatter_kernel(const uint8_t* __restrict__ level, int64_t m,
+                                                            uint32_t* cursor, int32_t* __restrict__ order) {
+  __shared__ uint32_t cnt[kLevels], base[kLevels];
+  for (int t = threadIdx.x; t < kLevels; t += blockDim.x) cnt[t] = 0u;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lv = i < m ? (int)level[i] : 0;
+  const uint32_t local = i < m ? atomicAdd(&cnt[lv], 1u) : 0u;
+  __syncthreads();
+  for (int t = threadIdx.x; t < kLevels; t += blockDim.x)
+    if (cnt[t]) base[t] = atomicAdd(&cursor[t], cnt[t]);
+  __syncthreads();
+  if (i < m) order[base[lv] + local] = (int32_t)i;
+}
+
+__device__ __forceinline__ void child_box(int32_t c, const alb_map_t& m, float b[6]) {
+  if (c < 0) {
+    const int32_t r = ~c;
+    b[0] = b[3] = m.x[r];
+    b[1] = b[4] = m.y[r];
+    b[2] = b[5] = m.z[r];
+  } else {
+    const float* nb = m.node_box + 6 * (int64_t)c;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) b[k] = nb[k];
+  }
+}
+
+__global__ __launch_bounds__(256) void box_level_kernel(alb_map_t m, const int32_t* __restrict__ order,
+                                                        const uint32_t* __restrict__ offset, int lv) {
+  const uint32_t beg = offset[lv], end = offset[lv + 1];
+  for (uint32_t t = beg + blockIdx.x * blockDim.x + threadIdx.x; t < end; t += gridDim.x * blockDim.x) {
+    const int32_t node = order[t];
+    float a[6], b[6];
+    child_box(m.node_child[2 * node + 0], m, a);
+    child_box(m.node_child[2 * node + 1], m, b);
+    float* o = m.node_box + 6 * (int64_t)node;
+    o[0] = fminf(a[0], b[0]);
+    o[1] = fminf(a[1], b[1]);
+    o[2] = fminf(a[2], b[2]);
+    o[3] = fmaxf(a[3], b[3]);
+    o[4] = fmaxf(a[4], b[4]);
+    o[5] = fmaxf(a[5], b[5]);
+  }
+}
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct WsLayout {
+  size_t keys_in, vals_in, level, order, small, temp, total, temp_bytes;
+};
+
+static int layout(int64_t n, WsLayout* L) {
+  size_t temp_bytes = 0;
+  const int nn = (int)(n > 1 ? n : 1);
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, (const uint64_t*)nullptr,
+                                                    (uint64_t*)nullptr, (const int32_t*)nullptr,
+                                                    (int32_t*)nullptr, nn, 0, 64, (hipStream_t)0);
+  if (e != hipSuccess) return set_error(ALB_ERR_HIP, "alb_map: sort size query failed: %s", hipGetErrorString(e));
+  size_t off = 0;
+  L->keys_in = off; off = align256(off + 8 * (size_t)nn);
+  L->vals_in = off; off = align256(off + 4 * (size_t)nn);
+  L->level = off; off = align256(off + (size_t)nn);
+  L->order = off; off = align256(off + 4 * (size_t)nn);
+  L->small = off; off = align256(off + 4 * (3 * kLevels + 8) + 64);
+  L->temp = off; off = align256(off + temp_bytes);
+  L->total = off;
+  L->temp_bytes = temp_bytes;
+  return ALB_OK;
+}
+
+}  // namespace alb
+
+extern "C" int alb_map_workspace_size(int64_t n, size_t* bytes) {
+  using namespace alb;
+  if (!bytes || n < 0) return set_error(ALB_ERR_ARG, "alb_map_workspace_size: bad arguments");
+  if (n > 0x7FFFFFF0) return set_error(ALB_ERR_ARG, "alb_map_workspace_size: more than 2^31 records");
+  WsLayout L;
+  const int rc = layout(n, &L);
+  if (rc != ALB_OK) return rc;
+  *bytes = L.total;
+  return ALB_OK;
+}
+
+extern "C" int alb_map_build(const alb_records_t* in, int64_t n, int32_t dims, alb_map_t* map,
+                             void* workspace, size_t workspace_bytes, void* stream) {
+  using namespace alb;
+  if (!in || !map) return set_error(ALB_ERR_ARG, "alb_map_build: null argument");
+  if (dims != 2 && dims != 3) return set_error(ALB_ERR_ARG, "alb_map_build: dims must be 2 or 3");
+  if (n < 0 || n > 0x7FFFFFF0) return set_error(ALB_ERR_ARG, "alb_map_build: record count out of range");
+  map->n = n;
+  map->dims = dims;
+  if (n == 0) return ALB_OK;
+  if (!map->x || !map->y || !map->z || !map->dx || !map->dy || !map->dz || !map->energy || !map->time ||
+      !map->perm || !map->keys || !map->meta)
+    return set_error(ALB_ERR_ARG, "alb_map_build: map arrays missing");
+  if (n > 1 && (!map->node_child || !map->node_range || !map->node_box))
+    return set_error(ALB_ERR_ARG, "alb_map_build: tree arrays missing");
+  WsLayout L;
+  int rc = layout(n, &L);
+  if (rc != ALB_OK) return rc;
+  if (!workspace || workspace_bytes < L.total)
+    return set_error(ALB_ERR_WORKSPACE, "alb_map_build: workspace too small (%zu < %zu)", workspace_bytes, L.total);
+  char* ws = (char*)workspace;
+  uint64_t* keys_in = (uint64_t*)(ws + L.keys_in);
+  int32_t* vals_in = (int32_t*)(ws + L.vals_in);
+  uint8_t* level = (uint8_t*)(ws + L.level);
+  int32_t* order = (int32_t*)(ws + L.order);
+  uint32_t* small = (uint32_t*)(ws + L.small);
+  uint32_t* bb = small;                // 6
+  uint32_t* hist = small + 8;          // kLevels
+  uint32_t* offset = hist + kLevels;   // kLevels + 1
+  uint32_t* cursor = offset + kLevels + 1;
+  hipStream_t s = (hipStream_t)stream;
+  const int B = 256;
+  const unsigned grid = (unsigned)((n + B - 1) / B);
+
+  hipLaunchKernelGGL(bbox_init_kernel, dim3(1), dim3(128), 0, s, bb, hist);
+  hipLaunchKernelGGL(bbox_kernel, dim3(grid < 2048 ? grid : 2048), dim3(B), 0, s, in->x, in->y, in->z, n, bb);
+  hipLaunchKernelGGL(meta_kernel, dim3(1), dim3(1), 0, s, bb, map->meta, dims);
+  hipLaunchKernelGGL(key_kernel, dim3(grid), dim3(B), 0, s, in->x, in->y, in->z, n, map->meta, keys_in, vals_in);
+  if ((rc = check_launch("alb_map_build keys")) != ALB_OK) return rc;
+  size_t tb = L.temp_bytes;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs((void*)(ws + L.temp), tb, keys_in, map->keys, vals_in,
+                                                    map->perm, (int)n, 0, dims == 2 ? 64 : 63, s);
+  if (e != hipSuccess) return set_error(ALB_ERR_HIP, "alb_map_build: sort failed: %s", hipGetErrorString(e));
+  alb_map_t m = *map;
+  hipLaunchKernelGGL(permute_kernel, dim3(grid), dim3(B), 0, s, *in, m);
+  if (n > 1) {
+    const unsigned g2 = (unsigned)((n - 1 + B - 1) / B);
+    hipLaunchKernelGGL(radix_tree_kernel, dim3(g2), dim3(B), 0, s, map->keys, n, map->node_child,
+                       map->node_range, level, hist);
+    hipLaunchKernelGGL(level_scan_kernel, dim3(1), dim3(1), 0, s, hist, offset, cursor);
+    hipLaunchKernelGGL(level_scatter_kernel, dim3(g2), dim3(B), 0, s, level, n - 1, cursor, order);
+    const unsigned g3 = g2 < 1024 ? g2 : 1024;
+    for (int lv = kLevels - 1; lv >= 0; --lv)
+      hipLaunchKernelGGL(box_level_kernel, dim3(g3), dim3(B), 0, s, m, order, offset, lv);
+  }
+  return check_launch("alb_map_build");
+}

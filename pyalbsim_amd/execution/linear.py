@@ -1,0 +1,134 @@
+"""run_linear — the drop-in entry point (src/alb_sim/execution/linear.py:77-117).
+
+Same signature and return contract as the reference: a dict keyed by the four
+PhotonType members (enum order) holding waveforms of 2*steps bins of width
+1/sample_rate.  Underneath, the forward pass, the photon-map build and the
+backward gather run as HIP kernels on the current GPU.
+
+Differences from the reference, by design:
+  * photons are independent Philox4x32-10 streams keyed from ``rng`` (two
+    draws of ``rng.integers``), so the result does not depend on
+    photons_per_batch_* — only on the photon totals and the key;
+  * waveform bins are accumulated in FP64 (returned as float64 unless
+    ``as_float32``);
+  * heat maps are not computed (the reference computes and discards them).
+"""
+from __future__ import annotations
+
+import secrets
+import time
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from .. import abi, runtime
+from ..config.run import RunConfig
+from ..config.simulation import SimulationConfig
+from ..model import SceneModel, build_scene
+from ..photon_mapping.photon_type import PhotonType
+
+
+@dataclass
+class SimulationResult:
+    """Device-side outcome of one scene: waveforms plus the intermediate maps."""
+
+    model: SceneModel
+    waveforms: object  # torch float64 [4, n_bins] on the device
+    map_counts: list
+    query_counts: list
+    counters_fwd: np.ndarray
+    counters_bwd: np.ndarray
+    maps: list = field(default_factory=list)
+    queries: list = field(default_factory=list)
+    timings: dict = field(default_factory=dict)
+
+
+def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=None, keep=False,
+             tags: bool = False, dscene: Optional[runtime.DeviceScene] = None, sync_timing: bool = False,
+             strict: bool = True) -> SimulationResult:
+    """Forward trace -> map build -> backward trace -> gather, all on the GPU.
+
+    strict: raise KeyError when a photon type has gather queries but an empty
+    map and IndexError when k exceeds a map's size, as the reference does
+    (core/simulation.py:134-153); strict=False gives such types zero signal.
+    """
+    import torch
+
+    ds = dscene or runtime.DeviceScene(model)
+    win = window or model.gate()
+    t = {}
+
+    def mark(name):
+        if sync_timing:
+            torch.cuda.synchronize()
+        t[name] = time.perf_counter()
+
+    mark("start")
+    cf = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device="cuda")
+    cb = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device="cuda")
+    recs, rcounts = runtime.trace(ds, key, abi.PASS_FORWARD, n_forward, tags=tags, counters=cf)
+    mark("forward")
+    maps = [runtime.build_map(recs[ti], rcounts[ti], runtime.MAP_DIMS[ti]) for ti in range(abi.N_TYPES)]
+    if not keep:
+        del recs
+    mark("build")
+    qs, qcounts = runtime.trace(ds, key, abi.PASS_BACKWARD, n_backward, tags=tags, counters=cb)
+    mark("backward")
+    wf = torch.zeros((abi.N_TYPES, model.n_bins), dtype=torch.float64, device="cuda")
+    for ti in range(abi.N_TYPES):
+        if qcounts[ti] == 0:
+            continue
+        if rcounts[ti] == 0 or model.k > rcounts[ti]:
+            if not strict:
+                continue
+            if rcounts[ti] == 0:
+                raise KeyError(PhotonType(ti + 1))
+            raise IndexError(f"photon_mapping_k = {model.k} exceeds the {PhotonType(ti + 1).name} map size "
+                             f"{rcounts[ti]}")
+        runtime.gather(ds, maps[ti], ti, qs[ti], qcounts[ti], wf[ti], win, counters=cb)
+    mark("gather")
+    res = SimulationResult(model, wf, rcounts, qcounts, None, None, timings=t)
+    ctr_f, ctr_b = cf.cpu().numpy().view(np.uint64), cb.cpu().numpy().view(np.uint64)
+    if ctr_b[abi.CTR_OVERFLOW]:
+        raise runtime.AlbError("gather node stack overflow")
+    res.counters_fwd, res.counters_bwd = ctr_f, ctr_b
+    if keep:
+        res.maps = maps
+        res.queries = qs
+    return res
+
+
+def photon_map_stats_table(counts) -> str:
+    from tabulate import tabulate
+
+    rows = [[pt.name, counts[pt.abi_index], f"{counts[pt.abi_index] * 32 / 2 ** 20:.1f}"] for pt in PhotonType]
+    return tabulate(rows, headers=["Photon type", "Records", "Device MiB (32 B/record)"], tablefmt="github")
+
+
+def run_linear(simulation_config: SimulationConfig, run_config: RunConfig,
+               rng: Optional[np.random.Generator] = None, *, as_float32: bool = False,
+               window=None, verbose: bool = True) -> dict:
+    """Full forward + backward simulation of one scene on the GPU.
+
+    Same contract as the reference run_linear (execution/linear.py:77-117):
+    returns {PhotonType: waveform}, consumes ``rng`` (a fresh secrets-seeded
+    generator when None).
+    """
+    if rng is None:
+        rng = np.random.default_rng(secrets.randbits(128))
+    key = abi.key_from_generator(rng)
+    model = build_scene(simulation_config)
+    n_f = run_config.photons_per_batch_forward * run_config.batches_forward
+    n_b = run_config.photons_per_batch_backward * run_config.batches_backward
+    if verbose:
+        print("Starting forwards pass")
+    t0 = time.perf_counter()
+    res = simulate(model, n_f, n_b, key, window=window)
+    wf = res.waveforms.cpu().numpy()
+    if verbose:
+        print(f"Finished forwards and backwards pass: {n_f} + {n_b} photon paths in "
+              f"{time.perf_counter() - t0:.2f} s")
+        print("\n" + photon_map_stats_table(res.map_counts) + "\n")
+    dt = np.float32 if as_float32 else np.float64
+    return {pt: wf[pt.abi_index].astype(dt) for pt in PhotonType}

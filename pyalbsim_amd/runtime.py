@@ -1,0 +1,234 @@
+"""Host runtime over the HIP library: device buffers (torch tensors used only
+as allocations), ctypes calls into libalbsim_hip.so, capacity handling.
+
+There is no CPU fallback: every call requires the built HIP library and a
+GPU, and raises otherwise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+from .model import SceneModel
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libalbsim_hip.so")
+_lib = None
+
+# header entry points, checked by the ABI test
+EXPORTS = ["alb_abi_version", "alb_scene_size", "alb_last_error", "alb_trace", "alb_map_workspace_size",
+           "alb_map_build", "alb_gather"]
+
+
+class AlbError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libalbsim_hip.so and declare its C signatures (no GPU needed)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise AlbError(f"HIP library not built: {path} is missing (run python -m pyalbsim_amd.build)")
+    L = C.CDLL(path)
+    P = C.c_void_p
+    L.alb_abi_version.restype = C.c_int
+    L.alb_scene_size.restype = C.c_size_t
+    L.alb_last_error.restype = C.c_char_p
+    L.alb_trace.argtypes = [P, P, C.c_int32, C.c_int64, C.c_int64, P, P, P, P]
+    L.alb_trace.restype = C.c_int
+    L.alb_map_workspace_size.argtypes = [C.c_int64, C.POINTER(C.c_size_t)]
+    L.alb_map_workspace_size.restype = C.c_int
+    L.alb_map_build.argtypes = [P, C.c_int64, C.c_int32, P, P, C.c_size_t, P]
+    L.alb_map_build.restype = C.c_int
+    L.alb_gather.argtypes = [P, P, C.c_int32, P, C.c_int64, C.c_int64, P, C.c_int64, C.c_int32, C.c_int32,
+                             P, P, P, P]
+    L.alb_gather.restype = C.c_int
+    if L.alb_abi_version() != abi.ALB_ABI_VERSION:
+        raise AlbError("libalbsim_hip.so ABI version mismatch")
+    if L.alb_scene_size() != C.sizeof(abi.Scene):
+        raise AlbError("alb_scene_t layout mismatch between include/alb_abi.h and pyalbsim_amd/abi.py")
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str):
+    if rc != abi.ALB_OK:
+        msg = load_library().alb_last_error().decode(errors="replace")
+        if rc == abi.ALB_ERR_CAPACITY:
+            raise MemoryError(f"{what}: {msg}")
+        raise AlbError(f"{what} failed ({rc}): {msg}")
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise AlbError("no GPU available: the photon-tracing path runs only on the HIP device")
+    return torch
+
+
+def _stream_ptr():
+    torch = _torch()
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _p(t) -> int:
+    return int(t.data_ptr()) if t is not None else 0
+
+
+class DeviceScene:
+    """Scene constants with the phase tables resident on the device."""
+
+    def __init__(self, model: SceneModel, device="cuda"):
+        torch = _torch()
+        self.model = model
+        self.theta = torch.as_tensor(np.ascontiguousarray(model.phase_theta), dtype=torch.float64, device=device)
+        self.cdf = torch.as_tensor(np.ascontiguousarray(model.phase_cdf), dtype=torch.float64, device=device)
+        self.s = abi.make_scene(model, _p(self.theta), _p(self.cdf))
+        self.device = device
+
+
+REC_F32 = ("x", "y", "z", "dx", "dy", "dz", "energy", "time")
+
+
+@dataclass
+class Sink:
+    """Device SoA sink for records (float32 geometry) or queries (float64 geometry)."""
+
+    tensors: dict
+    capacity: int
+    query: bool
+    struct: object = None
+
+    @staticmethod
+    def alloc(capacity: int, query: bool, tags: bool, device="cuda") -> "Sink":
+        torch = _torch()
+        capacity = max(int(capacity), 1)
+        gdt = torch.float64 if query else torch.float32
+        t = {}
+        for f in REC_F32:
+            t[f] = torch.empty(capacity, dtype=torch.float32 if f in ("energy", "time") else gdt, device=device)
+        t["tag"] = torch.empty(capacity, dtype=torch.int64, device=device) if tags else None
+        t["count"] = torch.zeros(1, dtype=torch.int64, device=device)
+        st = abi.Queries() if query else abi.Records()
+        for f in REC_F32:
+            setattr(st, f, _p(t[f]))
+        st.tag = _p(t["tag"])
+        st.capacity = capacity
+        st.count = _p(t["count"])
+        return Sink(t, capacity, query, st)
+
+    def view(self, n: int) -> dict:
+        return {f: (self.tensors[f][:n] if self.tensors[f] is not None else None) for f in (*REC_F32, "tag")}
+
+
+# expected records per photon and type (BOTTOM, SCATTER, SURF_REFL, SURF_TRANS) before a pilot
+_DEFAULT_RATES = (1.5, 8.0, 1.0, 1.5)
+
+
+def trace(dscene: DeviceScene, key, pass_: int, n: int, begin: int = 0, replay=None, tags: bool = False,
+          rates=None, counters=None, stream=None, pilot: int = 65536):
+    """Trace photons [begin, begin + n) of one pass on the GPU.
+
+    Returns (sinks, counts): sinks[t] is a Sink holding counts[t] items.
+    Capacities come from a pilot run over the first photons (same counters, so
+    it does not change the result); an overflow reruns the whole range with
+    exact capacities.  replay: device float64 tensor [n, L] of uniforms.
+    """
+    torch = _torch()
+    L = load_library()
+    query = pass_ == abi.PASS_BACKWARD
+    rp = _p(replay) if replay is not None else 0
+    rlen = int(replay.shape[1]) if replay is not None else 0
+    rng = abi.make_rng(key, pass_, rp, rlen)
+    s = stream if stream is not None else _stream_ptr()
+    if rates is None and n > 4 * pilot and replay is None:
+        sinks, counts = trace(dscene, key, pass_, pilot, begin, None, False, _DEFAULT_RATES, None, stream, pilot)
+        rates = [max(c / pilot, 0.05) for c in counts]
+        del sinks
+    rates = rates or _DEFAULT_RATES
+    caps = [int(r * n * 1.05) + 1024 for r in rates]
+    for attempt in range(2):
+        sinks = [Sink.alloc(caps[t], query, tags) for t in range(abi.N_TYPES)]
+        arr_r = (abi.Records * 4)()
+        arr_q = (abi.Queries * 4)()
+        for t in range(4):
+            if query:
+                arr_q[t] = sinks[t].struct
+            else:
+                arr_r[t] = sinks[t].struct
+        ctr_t = counters if counters is not None else None
+        if ctr_t is not None and attempt == 1:
+            ctr_t.zero_()
+        rc = L.alb_trace(C.byref(dscene.s), C.byref(rng), pass_, begin, n,
+                         C.cast(arr_r, C.c_void_p) if not query else None,
+                         C.cast(arr_q, C.c_void_p) if query else None, _p(ctr_t), C.c_void_p(s))
+        _check(rc, "alb_trace")
+        counts = [int(x) for x in torch.cat([sk.tensors["count"] for sk in sinks]).cpu().tolist()]
+        if all(counts[t] <= caps[t] for t in range(4)):
+            return sinks, counts
+        caps = [max(counts[t], 1) for t in range(4)]
+    raise AlbError("alb_trace: capacity overflow persisted after resizing")
+
+
+@dataclass
+class DeviceMap:
+    n: int
+    dims: int
+    tensors: dict = field(default_factory=dict)
+    struct: object = None
+
+
+def build_map(records: Sink, n: int, dims: int, stream=None) -> DeviceMap:
+    """Morton-sort n records of a forward sink into a map with a radix tree."""
+    torch = _torch()
+    L = load_library()
+    dev = records.tensors["x"].device
+    m = DeviceMap(n=n, dims=dims)
+    t = m.tensors
+    for f in REC_F32:
+        t[f] = torch.empty(max(n, 1), dtype=torch.float32, device=dev)
+    t["perm"] = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    t["keys"] = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+    t["meta"] = torch.zeros(8, dtype=torch.float64, device=dev)
+    ni = max(n - 1, 1)
+    t["node_child"] = torch.empty(2 * ni, dtype=torch.int32, device=dev)
+    t["node_range"] = torch.empty(2 * ni, dtype=torch.int32, device=dev)
+    t["node_box"] = torch.empty(6 * ni, dtype=torch.float32, device=dev)
+    st = abi.Map()
+    st.n = n
+    st.dims = dims
+    for f in ("x", "y", "z", "dx", "dy", "dz", "energy", "time", "perm", "keys", "meta", "node_child",
+              "node_range", "node_box"):
+        setattr(st, f, _p(t[f]))
+    m.struct = st
+    if n == 0:
+        return m
+    ws_bytes = C.c_size_t(0)
+    _check(L.alb_map_workspace_size(n, C.byref(ws_bytes)), "alb_map_workspace_size")
+    ws = torch.empty(ws_bytes.value, dtype=torch.uint8, device=dev)
+    s = stream if stream is not None else _stream_ptr()
+    _check(L.alb_map_build(C.byref(records.struct), n, dims, C.byref(st), _p(ws), ws_bytes.value, C.c_void_p(s)),
+           "alb_map_build")
+    m.tensors["_ws"] = ws  # keep alive until the stream has consumed it
+    return m
+
+
+def gather(dscene: DeviceScene, dmap: DeviceMap, ptype: int, queries: Sink, q_n: int, waveform, window,
+           q_begin: int = 0, nbr_out=None, radius_out=None, counters=None, stream=None):
+    """Accumulate the k-NN gather of q_n queries of one type into waveform (float64 device tensor)."""
+    L = load_library()
+    win0, win_len = window
+    s = stream if stream is not None else _stream_ptr()
+    rc = L.alb_gather(C.byref(dscene.s), C.byref(dmap.struct), ptype, C.byref(queries.struct), q_begin, q_n,
+                      _p(waveform), int(waveform.numel()), int(win0), int(win_len), _p(nbr_out), _p(radius_out),
+                      _p(counters), C.c_void_p(s))
+    _check(rc, "alb_gather")
+
+
+MAP_DIMS = (2, 3, 2, 2)  # BOTTOM and the two surface maps are planar; SCATTER is volumetric

@@ -1,0 +1,56 @@
+"""The C-ABI library loads on a host without a GPU and exports every entry
+point declared in include/alb_abi.h; struct layouts agree with the ctypes mirror."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from pyalbsim_amd import abi, runtime
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "alb_abi.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(alb_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_expected_entry_points():
+    assert _declared() == sorted(runtime.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = runtime.load_library()
+    for name in _declared():
+        assert hasattr(lib, name), name
+
+
+def test_layout_and_version():
+    lib = runtime.load_library()
+    assert lib.alb_abi_version() == abi.ALB_ABI_VERSION
+    assert lib.alb_scene_size() == C.sizeof(abi.Scene)
+    assert C.sizeof(abi.Rng) == 32
+    assert C.sizeof(abi.Records) == 88 and C.sizeof(abi.Queries) == 88
+
+
+def test_last_error_is_set_by_argument_checks():
+    lib = runtime.load_library()
+    rc = lib.alb_trace(None, None, 0, 0, 1, None, None, None, None)
+    assert rc == abi.ALB_ERR_ARG
+    assert b"null" in lib.alb_last_error()
+    rc = lib.alb_map_build(None, 0, 2, None, None, 0, None)
+    assert rc == abi.ALB_ERR_ARG
+
+
+def test_product_path_fails_loudly_without_gpu():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("host has a GPU")
+    from pyalbsim_amd.model import build_scene
+    from tests.scenes import single_shot
+
+    with pytest.raises(runtime.AlbError):
+        runtime.DeviceScene(build_scene(single_shot()))

@@ -1,0 +1,64 @@
+"""Configuration dataclasses: defaults, validation errors and the drop-in
+module paths of the reference (src/alb_sim/config/*.py)."""
+import dataclasses
+
+import pytest
+
+from pyalbsim_amd.config import (LaserConfig, RunConfig, SceneConfig, SeaFloorConfig, SeaSurfaceConfig,
+                                 SensorConfig, SimulationConfig, TurbidityLayerConfig)
+from pyalbsim_amd.utils.parameter_profile import ExponentialParameter, LinearParameter
+
+
+def test_defaults():
+    c = SimulationConfig()
+    assert c.sample_multiplier == 10 and c.photon_mapping_k == 100
+    assert c.scene.flying_height == 135 and c.laser.nadir_angle == 15.0
+    assert c.sensor.sample_rate == 2_000_000_000 and c.sensor.field_of_view == 5e-3
+    layer = c.water.layers[0]
+    assert layer.height == 3 and layer.scattering_coefficient == 2.5 and layer.absorption_coefficient == 0.169
+    assert isinstance(layer.salinity, float) and layer.num_sublayers == 60
+    r = RunConfig()
+    assert (r.photons_per_batch_forward, r.photons_per_batch_backward, r.batches_forward, r.batches_backward) == \
+        (10_000, 5_000, 10, 5)
+    f = {x.name: x for x in dataclasses.fields(SensorConfig)}
+    assert f["sample_rate"].metadata["unit"] == "Hz"
+
+
+@pytest.mark.parametrize("make", [
+    lambda: LaserConfig(nadir_angle=90), lambda: SceneConfig(flying_height=0),
+    lambda: SeaFloorConfig(albedo=1.5), lambda: SeaSurfaceConfig(albedo=-0.1),
+    lambda: SensorConfig(aperture_radius=-1), lambda: TurbidityLayerConfig(height=0),
+    lambda: TurbidityLayerConfig(sublayer_dz=None), lambda: ExponentialParameter(0.0, 1.0),
+])
+def test_validation_errors(make):
+    with pytest.raises(ValueError):
+        make()
+
+
+def test_bool_parameter_rejected():
+    with pytest.raises(TypeError):
+        TurbidityLayerConfig(scattering_coefficient=True)
+
+
+def test_profiles():
+    assert LinearParameter(1.0, 3.0).at(0.5) == pytest.approx(2.0)
+    assert ExponentialParameter(1.0, 4.0).at(0.5) == pytest.approx(2.0)
+
+
+def test_frozen():
+    with pytest.raises(dataclasses.FrozenInstanceError):
+        SimulationConfig().sample_multiplier = 3
+
+
+def test_dropin_module_paths():
+    from alb_sim.config.run import RunConfig as R
+    from alb_sim.config.sea_floor import SeaFloorConfig as SF
+    from alb_sim.config.simulation import SimulationConfig as S
+    from alb_sim.config.water import TurbidityLayerConfig as T, WaterConfig as W
+    from alb_sim.execution.linear import run_linear
+    from alb_sim.photon_mapping.photon_type import PhotonType
+
+    assert R is RunConfig and S is SimulationConfig and SF is SeaFloorConfig and T is TurbidityLayerConfig
+    assert W is not None and callable(run_linear)
+    assert [p.name for p in PhotonType] == ["BOTTOM_REFLECTION", "SCATTER", "SURFACE_REFLECTION",
+                                            "SURFACE_TRANSMISSION_UP"]

@@ -1,0 +1,116 @@
+"""HIP photon-map build + k-NN gather vs brute force / the CPU oracle.
+
+* neighbour sets: for each query the GPU's k nearest records (indices into the
+  Morton-sorted map, (distance, index) order) must equal the brute-force set
+  over the same map arrays, bit for bit, together with the k-th distance;
+* waveforms: the GPU FP64 waveform of each photon type must equal the oracle's
+  sequential FP64 re-sum over the same map and queries to 1e-9 relative (the
+  only difference is the order of FP64 additions);
+* map build: the sorted SoA is a permutation of the input records and the
+  tree's node boxes contain their records.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from pyalbsim_amd import abi, runtime
+from pyalbsim_amd.model import build_scene
+from tests.scenes import run_linear_script_scene, single_shot
+
+pytestmark = pytest.mark.gpu
+
+F = ("x", "y", "z", "dx", "dy", "dz", "energy", "time")
+
+
+def _setup(model, n_f, n_b, key):
+    ds = runtime.DeviceScene(model)
+    recs, rc = runtime.trace(ds, key, abi.PASS_FORWARD, n_f)
+    maps = [runtime.build_map(recs[t], rc[t], runtime.MAP_DIMS[t]) for t in range(4)]
+    qs, qc = runtime.trace(ds, key, abi.PASS_BACKWARD, n_b)
+    return ds, recs, rc, maps, qs, qc
+
+
+def _np(d, n):
+    return {f: d[f][:n].cpu().numpy() for f in d if d[f] is not None and f in F + ("perm",)}
+
+
+@pytest.mark.parametrize("scene", ["single_shot", "turbid"])
+def test_map_build_is_a_permutation_with_containing_boxes(gpu, scene):
+    model = build_scene(single_shot() if scene == "single_shot" else run_linear_script_scene())
+    ds, recs, rc, maps, qs, qc = _setup(model, 5000, 100, (3, 4))
+    for t in range(4):
+        n = rc[t]
+        if n < 2:
+            continue
+        src = {f: recs[t].tensors[f][:n].cpu().numpy() for f in F}
+        m = maps[t].tensors
+        perm = m["perm"][:n].cpu().numpy()
+        assert np.array_equal(np.sort(perm), np.arange(n))
+        for f in F:
+            assert np.array_equal(m[f][:n].cpu().numpy(), src[f][perm])
+        keys = m["keys"][:n].cpu().numpy().view(np.uint64)
+        assert np.all(keys[1:] >= keys[:-1])
+        rng_ = m["node_range"][: 2 * (n - 1)].cpu().numpy().reshape(-1, 2)
+        box = m["node_box"][: 6 * (n - 1)].cpu().numpy().reshape(-1, 6)
+        xyz = np.stack([m["x"][:n].cpu().numpy(), m["y"][:n].cpu().numpy(), m["z"][:n].cpu().numpy()], 1)
+        assert rng_[0, 0] == 0 and rng_[0, 1] == n - 1
+        sample = np.random.default_rng(0).choice(n - 1, size=min(n - 1, 400), replace=False)
+        for i in sample:
+            lo, hi = rng_[i]
+            pts = xyz[lo:hi + 1]
+            assert np.array_equal(box[i, :3], pts.min(0)) and np.array_equal(box[i, 3:], pts.max(0))
+
+
+@pytest.mark.parametrize("scene", ["single_shot", "turbid"])
+def test_knn_sets_bit_exact_and_waveform_matches(gpu, scene):
+    torch = gpu
+    model = build_scene(single_shot() if scene == "single_shot" else run_linear_script_scene())
+    ds, recs, rc, maps, qs, qc = _setup(model, 20000, 1500, (11, 22))
+    k = model.k
+    win = model.gate()
+    for t in range(4):
+        if qc[t] == 0 or rc[t] < k:
+            continue
+        nq = min(qc[t], 600)
+        wf = torch.zeros(model.n_bins, dtype=torch.float64, device="cuda")
+        nbr = torch.zeros((nq, k), dtype=torch.int32, device="cuda")
+        rad = torch.zeros(nq, dtype=torch.float64, device="cuda")
+        runtime.gather(ds, maps[t], t, qs[t], nq, wf, win, nbr_out=nbr, radius_out=rad)
+        torch.cuda.synchronize()
+        mp = {f: maps[t].tensors[f][: rc[t]].cpu().numpy() for f in F}
+        qv = {f: qs[t].tensors[f][:nq].cpu().numpy() for f in F}
+        qpos = np.stack([qv["x"], qv["y"], qv["z"]], 1)
+        idx, d2 = oracle.knn(mp["x"], mp["y"], mp["z"], qpos, k)
+        g_idx = nbr.cpu().numpy()
+        assert np.array_equal(g_idx, idx), f"type {t}: {np.sum(np.any(g_idx != idx, 1))} of {nq} queries differ"
+        assert np.array_equal(rad.cpu().numpy(), np.sqrt(d2[:, -1]))
+        ref = oracle.gather(model, t, mp, qv)
+        got = wf.cpu().numpy()
+        scale = np.max(np.abs(ref))
+        assert scale > 0
+        assert np.max(np.abs(got - ref)) <= 1e-9 * scale, f"type {t}"
+
+
+def test_gather_window_placement_does_not_change_waveform(gpu):
+    torch = gpu
+    model = build_scene(single_shot())
+    ds, recs, rc, maps, qs, qc = _setup(model, 20000, 2000, (1, 2))
+    t = abi.SCATTER
+    outs = []
+    for win in [model.gate(), (0, 0), (model.gate()[0] + 300, 2048)]:
+        wf = torch.zeros(model.n_bins, dtype=torch.float64, device="cuda")
+        runtime.gather(ds, maps[t], t, qs[t], qc[t], wf, win)
+        outs.append(wf.cpu().numpy())
+    s = np.max(np.abs(outs[0]))
+    for o in outs[1:]:
+        assert np.max(np.abs(o - outs[0])) <= 1e-12 * s
+
+
+def test_gather_errors_like_reference(gpu):
+    """k larger than the map -> IndexError; empty map with queries -> KeyError
+    (core/simulation.py:134-153)."""
+    from pyalbsim_amd.execution.linear import simulate
+
+    model = build_scene(single_shot(k=100))
+    with pytest.raises((IndexError, KeyError)):
+        simulate(model, 20, 200, (1, 1))
