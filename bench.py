@@ -1,0 +1,181 @@
+"""Benchmark: photon paths per second on one MI355X, single ALB shot.
+
+One step = one full pass of the hot path over one shot: forward trace of
+N_f photons (map records into HBM), four photon-map builds, backward trace of
+N_b sensor paths, k-NN gather into FP64 waveforms.  Default workload: the
+400 m / 5 m scene at N_f = 1e8, N_b = 2.5e7 (k = 100).  value = (N_f + N_b) /
+(wall seconds per step), wall time bracketed by device synchronisation.
+
+Also reported: per-phase times (HIP events on the launch stream), the
+roofline position of the dominant kernel (algorithmic flops or bytes from the
+kernels' own event counters / its time) and a CPU baseline: the time-stepped
+C oracle on a bounded sample of the same scene, on this host's cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--forward N]
+       [--backward N] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "photon paths/s on 1 GPU, plain vs batched; adjoint/forward time ratio"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+# algorithmic work per event (flops: add/sub/mul/div/cmp = 1, FMA = 2,
+# sqrt/log/exp/sin/cos/acos/pow = 1)
+F_FWD_EMIT, F_BWD_EMIT, F_AIR, F_WATER_STEP = 30, 50, 12, 24
+F_SCATTER, F_ENTER, F_EXIT, F_FLOOR = 95, 70, 80, 70
+M_TYPE = (0, 30, 85, 0)  # per-neighbour multiplier flops by photon type
+
+
+def trace_flops(ctr, forward: bool) -> float:
+    from pyalbsim_amd import abi
+
+    return float((F_FWD_EMIT if forward else F_BWD_EMIT) * ctr[abi.CTR_PATHS] + F_AIR * ctr[abi.CTR_AIR_LEGS]
+                 + F_WATER_STEP * ctr[abi.CTR_WATER_STEPS] + F_SCATTER * ctr[abi.CTR_SCATTER]
+                 + F_ENTER * ctr[abi.CTR_ENTER] + F_EXIT * ctr[abi.CTR_EXIT] + F_FLOOR * ctr[abi.CTR_FLOOR])
+
+
+def cpu_baseline(model, key, n_f: int, n_b: int, threads: int):
+    """Time the C oracle (time-stepped, brute-force k-NN) on n_f + n_b paths."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    import oracle
+    from pyalbsim_amd import abi
+
+    oracle.lib()
+    t0 = time.perf_counter()
+
+    def chunks(n):
+        step = max(1, (n + threads - 1) // threads)
+        return [(b, min(step, n - b)) for b in range(0, n, step)]
+
+    with ThreadPoolExecutor(threads) as ex:
+        fw = list(ex.map(lambda bc: oracle.trace(model, key, abi.PASS_FORWARD, bc[0], bc[1])[0], chunks(n_f)))
+        bw = list(ex.map(lambda bc: oracle.trace(model, key, abi.PASS_BACKWARD, bc[0], bc[1])[0], chunks(n_b)))
+        maps = [{f: np.concatenate([c[t][f] for c in fw]) for f in fw[0][t]} for t in range(4)]
+        qs = [{f: np.concatenate([c[t][f] for c in bw]) for f in bw[0][t]} for t in range(4)]
+        jobs = []
+        for t in range(4):
+            nq = len(qs[t]["x"])
+            if nq == 0 or len(maps[t]["x"]) < model.k:
+                continue
+            for b, c in chunks(nq):
+                jobs.append((t, {f: v[b:b + c] for f, v in qs[t].items()}))
+        list(ex.map(lambda j: oracle.gather(model, j[0], maps[j[0]], j[1]), jobs))
+    wall = time.perf_counter() - t0
+    return (n_f + n_b) / wall, wall
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--forward", type=float, default=1e8)
+    ap.add_argument("--backward", type=float, default=2.5e7)
+    ap.add_argument("--cpu-forward", type=int, default=16000)
+    ap.add_argument("--cpu-backward", type=int, default=4000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from pyalbsim_amd import abi, runtime
+    from pyalbsim_amd.execution.linear import simulate
+    from pyalbsim_amd.model import build_scene
+    from tests.scenes import single_shot
+
+    runtime.load_library()
+    model = build_scene(single_shot())
+    n_f, n_b = int(args.forward), int(args.backward)
+    ds = runtime.DeviceScene(model)
+    key = (0x5EED, 0)
+
+    def step(events=None):
+        return simulate(model, n_f, n_b, key, dscene=ds, events=events)
+
+    for i in range(args.warmup):
+        step()
+        print(f"warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    events = {}
+    t0 = time.perf_counter()
+    res = None
+    for i in range(args.steps):
+        res = step(events)
+        print(f"step {i + 1}/{args.steps} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = wall / args.steps * 1e3
+    value = (n_f + n_b) / (wall / args.steps)
+
+    phases = {}
+    names = ["start", "forward", "build", "backward", "gather"]
+    for a, b in zip(names[:-1], names[1:]):
+        phases[b] = sum(s.elapsed_time(e) for s, e in zip(events[a], events[b])) / args.steps
+    cf, cb = res.counters_fwd, res.counters_bwd
+    flops_fwd = trace_flops(cf, True)
+    flops_bwd = trace_flops(cb, False)
+    nq = float(cb[abi.CTR_QUERIES])
+    cand = float(cb[abi.CTR_CANDIDATES])
+    nodes = float(cb[abi.CTR_NODES])
+    k = model.k
+    qc = res.query_counts
+    g_flops = sum(qc[t] * (4 + (7 + M_TYPE[t]) * k) for t in range(4)) + 15 * cand
+    g_bytes = 12 * cand + 20 * k * nq + 40 * nodes
+    t_trace = (phases["forward"] + phases["backward"]) / 1e3
+    t_gather = phases["gather"] / 1e3
+    if t_gather >= t_trace:
+        roof = {"kernel": "gather_kernel", "bound": "hbm", "achieved": g_bytes / t_gather / 1e9,
+                "peak": PEAK_HBM_GBS, "unit": "GB/s"}
+    else:
+        roof = {"kernel": "trace_kernel", "bound": "valu", "achieved": (flops_fwd + flops_bwd) / t_trace / 1e12,
+                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "photon paths/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": f"single ALB shot (400 m, nadir, 1 mrad, FOV 20 mrad, 5 m water, a=0.1, b=0.2, "
+                               f"albedo 0.15, 0.1 ns bins), plain form, {n_f:.3g} forward + {n_b:.3g} "
+                               f"backward photons, k=100",
+                   "forward_photons": n_f, "backward_photons": n_b, "k": k,
+                   "map_records": res.map_counts, "gather_queries": qc},
+        "phases_ms": phases,
+        "roofline": roof,
+        "trace_tflops": (flops_fwd + flops_bwd) / t_trace / 1e12,
+        "gather_gbs": g_bytes / t_gather / 1e9 if t_gather > 0 else None,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        rate, cwall = cpu_baseline(model, key, args.cpu_forward, args.cpu_backward, threads)
+        out["cpu_baseline"] = {"value": rate, "unit": "photon paths/s", "cores": threads, "kind": "port",
+                               "sample": f"time-stepped C oracle, same scene, {args.cpu_forward} forward + "
+                                         f"{args.cpu_backward} backward photons, brute-force k-NN, "
+                                         f"{cwall:.1f} s wall"}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

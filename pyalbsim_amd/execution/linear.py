@@ -46,9 +46,11 @@ class SimulationResult:
 
 def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=None, keep=False,
              tags: bool = False, dscene: Optional[runtime.DeviceScene] = None, sync_timing: bool = False,
-             strict: bool = True) -> SimulationResult:
+             strict: bool = True, events: Optional[dict] = None) -> SimulationResult:
     """Forward trace -> map build -> backward trace -> gather, all on the GPU.
 
+    events: optional dict; HIP events are recorded on the launch stream at the
+    phase boundaries start / forward / build / backward / gather.
     strict: raise KeyError when a photon type has gather queries but an empty
     map and IndexError when k exceeds a map's size, as the reference does
     (core/simulation.py:134-153); strict=False gives such types zero signal.
@@ -60,6 +62,10 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
     t = {}
 
     def mark(name):
+        if events is not None:
+            ev = torch.cuda.Event(enable_timing=True)
+            ev.record(torch.cuda.current_stream())
+            events.setdefault(name, []).append(ev)
         if sync_timing:
             torch.cuda.synchronize()
         t[name] = time.perf_counter()
