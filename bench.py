@@ -164,6 +164,9 @@ def main(argv=None):
         "roofline": roof,
         "trace_tflops": (flops_fwd + flops_bwd) / t_trace / 1e12,
         "gather_gbs": g_bytes / t_gather / 1e9 if t_gather > 0 else None,
+        "gather_candidates_per_query": cand / max(nq, 1.0),
+        "gather_nodes_per_query": nodes / max(nq, 1.0),
+        "water_steps_per_path": float(cf[abi.CTR_WATER_STEPS]) / max(float(cf[abi.CTR_PATHS]), 1.0),
         "cpu_baseline": None,
     }
     if not args.no_cpu:

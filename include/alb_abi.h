@@ -158,7 +158,16 @@ typedef struct alb_queries {
 } alb_queries_t;
 
 /* A built photon map: records sorted along a Morton curve plus a binary radix
- * tree over them (internal node i covers sorted records [first, last]). */
+ * tree over them.  Internal node i (0 <= i < n-1; node 0 is the root) is 64
+ * bytes:
+ *   int32 first, last    sorted records it covers
+ *   int32 split          left child covers [first, split], right [split+1, last];
+ *                        an internal child's node index is split (left) or
+ *                        split+1 (right)
+ *   int32 leaf_mask      bit 0: left child is the single record split,
+ *                        bit 1: right child is the single record split+1
+ *   float left_box[6]    min x,y,z, max x,y,z of the left child's records
+ *   float right_box[6]   same for the right child */
 typedef struct alb_map {
   int64_t n;                /* records                                             */
   int32_t dims;             /* 2: key over (x,z) (planar maps), 3: over (x,y,z)    */
@@ -174,9 +183,7 @@ typedef struct alb_map {
   int32_t* perm;            /* [n] sorted position -> input record index          */
   uint64_t* keys;           /* [n] sorted Morton keys                              */
   double* meta;             /* [8] grid origin x,y,z, inverse cell width x,y,z     */
-  int32_t* node_child;      /* [2*(n-1)] left,right; a leaf is ~record             */
-  int32_t* node_range;      /* [2*(n-1)] first,last sorted record                  */
-  float* node_box;          /* [6*(n-1)] min x,y,z, max x,y,z                      */
+  void* nodes;              /* [n-1] 64-byte internal nodes (layout above)         */
 } alb_map_t;
 
 /* ---------------------------------------------------------------- entry points */
@@ -206,17 +213,22 @@ int alb_map_build(const alb_records_t* in, int64_t n, int32_t dims,
                   alb_map_t* map, void* workspace, size_t workspace_bytes,
                   void* stream);
 
+/* Workspace bytes alb_gather needs for q_n queries (it processes at most
+ * 2^22 queries per round and reuses the space). */
+int alb_gather_workspace_size(int64_t q_n, int32_t k, size_t* bytes);
+
 /* k-nearest-neighbour gather of queries [q_begin, q_begin + q_n) of one photon
  * type against its map, accumulating FP64 energy into waveform[n_bins]
  * (n_bins = 2*steps).  Bins [win0, win0 + win_len) are privatised in LDS.
  * nbr_out (optional, [q_n * k] int32) receives each query's neighbour set as
- * sorted-map indices in (distance, index) order; radius_out (optional,
- * [q_n] double) the k-th neighbour distance.
- * Replaces core/simulation.py:132-195 (sample_photons). */
+ * sorted-map indices in (distance, index) order and radius_out ([q_n]
+ * double) the k-th neighbour distance; when they are given, no workspace is
+ * needed.  Replaces core/simulation.py:132-195 (sample_photons). */
 int alb_gather(const alb_scene_t* scene, const alb_map_t* map,
                int32_t photon_type, const alb_queries_t* q, int64_t q_begin,
                int64_t q_n, double* waveform, int64_t n_bins, int32_t win0,
                int32_t win_len, int32_t* nbr_out, double* radius_out,
+               void* workspace, size_t workspace_bytes,
                unsigned long long* counters, void* stream);
 
 #ifdef __cplusplus

@@ -63,8 +63,7 @@ class Map(C.Structure):
                 ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("dx", C.c_void_p),
                 ("dy", C.c_void_p), ("dz", C.c_void_p), ("energy", C.c_void_p), ("time", C.c_void_p),
                 ("perm", C.c_void_p), ("keys", C.c_void_p), ("meta", C.c_void_p),
-                ("node_child", C.c_void_p), ("node_range", C.c_void_p),
-                ("node_box", C.c_void_p)]
+                ("nodes", C.c_void_p)]
 
 
 def make_scene(model, theta_ptr: int, cdf_ptr: int) -> Scene:

@@ -287,6 +287,15 @@ __device__ __forceinline__ double scatter_mult(float px, float py, float pz, dou
   return interp_uniform_grid(hi, theta, cdf, n) - interp_uniform_grid(lo, theta, cdf, n);
 }
 
+// ------------------------------------------------------------- tree nodes
+// 64-byte internal node of the photon-map radix tree (layout in alb_abi.h)
+struct __attribute__((aligned(16))) Node {
+  int32_t first, last, split, leaf_mask;
+  float lb[6];  // left child box: min x,y,z, max x,y,z
+  float rb[6];  // right child box
+};
+static_assert(sizeof(Node) == 64, "tree node must be one 64-byte line");
+
 // ----------------------------------------------------------- Morton keys
 __device__ __forceinline__ uint64_t spread2(uint64_t v) {  // 32 bits -> even bit positions
   v &= 0xFFFFFFFFull;

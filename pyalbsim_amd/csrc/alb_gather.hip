@@ -5,22 +5,26 @@
 // type-specific energy multiplier, the area / volume kernel 1/(pi r_k^2) or
 // 1/(4/3 pi r_k^3) from the k-th distance, and np.add.at into the waveform.
 //
-// One wavefront handles one query:
-//  1. seed: the query's Morton key is located in the sorted keys with a
-//     64-ary wave search; the W records around it are the first candidates
-//     and their k-th smallest squared distance is the pruning bound theta;
-//  2. traversal of the radix tree, up to 64 nodes per step (one per lane),
-//     pruning nodes whose exact float32 box is farther than theta; nodes of at
-//     most 64 records are scanned one record per lane, and records with
-//     d2 <= theta are appended to an LDS candidate buffer;
-//  3. when the buffer fills it is bitonic-sorted by (d2, index) and cut back
-//     to k entries, which tightens theta.  The final sort gives the neighbour
-//     set in (distance, index) order — bit-identical to brute force, because
-//     squared distances are evaluated in FP64 in one fixed order and box
-//     distances are, by monotone rounding, never larger than those of the
-//     records inside;
-//  4. each neighbour's FP64 energy goes into an LDS-privatised window of the
-//     waveform (ds_add_f64); bins outside the window use global FP64 atomics.
+// Two kernels per chunk of queries:
+//  knn_kernel — one wavefront per query:
+//   1. seed: descend the radix tree into the child whose box is nearer while
+//      it still holds >= k records; the (at most kSeed) records reached give a
+//      first bound theta = k-th smallest squared distance;
+//   2. traversal, up to 64 nodes per step (one per lane): each node carries its
+//      children's exact float32 boxes, so a child is dropped, pushed (with its
+//      box distance, re-checked when popped) or, at <= 64 records, scanned;
+//      scanned records with d2 <= theta go to an LDS candidate buffer, four
+//      record ranges at a time so that their loads are in flight together;
+//   3. a full buffer is bitonic-sorted by (d2, index) and cut back to k, which
+//      tightens theta; the final sort gives the neighbour set in (distance,
+//      index) order — bit-identical to brute force, because squared distances
+//      are evaluated in FP64 in one fixed order and box distances are, by
+//      monotone rounding, never larger than those of the records inside.
+//  contrib_kernel — one thread per (query, neighbour): energy multiplier,
+//   kernel normalisation and time bin; FP64 sums go to an LDS-privatised
+//   window of the waveform (ds_add_f64), bins outside it to global atomics.
+//   Keeping the FP64 surface / phase-function code out of knn_kernel keeps the
+//   traversal at low register pressure (higher occupancy).
 #include <limits.h>
 
 #include "alb_device.h"
@@ -28,25 +32,36 @@
 
 namespace alb {
 
-constexpr int kWaves = 4;         // wavefronts per block
-constexpr int kStack = 512;       // per-wave node stack (LDS)
-constexpr int kBuf = 512;         // per-wave candidate buffer (LDS)
+constexpr int kWaves = 4;         // wavefronts per k-NN block
+constexpr int kStack = 256;       // per-wave node stack (LDS)
+constexpr int kSeed = 192;        // records of the descent seed
+constexpr int kBuf = 256;         // per-wave candidate buffer (LDS), power of two
 constexpr int kBucket = 64;       // nodes with <= kBucket records are scanned
 constexpr int kMaxWindow = 2048;  // LDS-privatised waveform bins
-constexpr int kMaxK = kBuf - 128;
+constexpr int kMaxK = kSeed;      // the seed must hold k records
+constexpr int64_t kChunk = 1 << 22;  // queries per k-NN / contribution round
 
-struct GatherArgs {
+struct KnnArgs {
+  alb_map_t map;
+  alb_queries_t q;
+  int64_t q_begin, q_n;
+  int32_t k;
+  int32_t* nbr;      // [q_n * k]
+  double* radius;    // [q_n]
+  unsigned long long* ctr;
+};
+
+struct ContribArgs {
   alb_scene_t sc;
   alb_map_t map;
   alb_queries_t q;
   int64_t q_begin, q_n;
   int32_t type;
+  const int32_t* nbr;
+  const double* radius;
   double* wf;
   int64_t n_bins;
   int32_t win0, win_len;
-  int32_t* nbr_out;
-  double* radius_out;
-  unsigned long long* ctr;
 };
 
 __device__ __forceinline__ double dist2(double qx, double qy, double qz, float x, float y, float z) {
@@ -123,28 +138,26 @@ __device__ int64_t wave_lower_bound(const uint64_t* __restrict__ keys, int64_t n
   return lo + __popcll(ballot(less));
 }
 
-__global__ __launch_bounds__(256) void gather_kernel(GatherArgs a) {
-  __shared__ double hist[kMaxWindow];
-  __shared__ int32_t stack_s[kWaves][kStack];
+__global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
+  __shared__ int32_t sn_s[kWaves][kStack];
+  __shared__ double sd_s[kWaves][kStack];
   __shared__ double bd_s[kWaves][kBuf];
   __shared__ int32_t bi_s[kWaves][kBuf];
-  __shared__ int32_t rlo_s[kWaves][64];
-  __shared__ int32_t rhi_s[kWaves][64];
-
-  for (int t = threadIdx.x; t < a.win_len; t += blockDim.x) hist[t] = 0.0;
-  __syncthreads();
+  __shared__ int32_t rlo_s[kWaves][128];
+  __shared__ int32_t rhi_s[kWaves][128];
 
   const uint32_t lane = lane_id();
   const int w = threadIdx.x >> 6;
-  int32_t* stack = stack_s[w];
+  int32_t* sn = sn_s[w];
+  double* sd = sd_s[w];
   double* bd = bd_s[w];
   int32_t* bi = bi_s[w];
   int32_t* rlo = rlo_s[w];
   int32_t* rhi = rhi_s[w];
   const alb_map_t& m = a.map;
+  const Node* __restrict__ nodes = (const Node*)m.nodes;
   const int64_t n = m.n;
-  const int k = a.sc.k;
-  const int W = (int)(n < (int64_t)(((k + 63) / 64) * 64 + 64) ? n : (int64_t)(((k + 63) / 64) * 64 + 64));
+  const int k = a.k;
   unsigned long long c_q = 0, c_cand = 0, c_nodes = 0, c_ovf = 0;
 
   for (int64_t qi = (int64_t)blockIdx.x * kWaves + w; qi < a.q_n; qi += (int64_t)gridDim.x * kWaves) {
@@ -152,160 +165,177 @@ __global__ __launch_bounds__(256) void gather_kernel(GatherArgs a) {
     const double qx = a.q.x[qg], qy = a.q.y[qg], qz = a.q.z[qg];
     c_q++;
 
-    // ---- 1. Morton-window seed
-    const uint64_t key = morton_key(qx, qy, qz, m.meta);
-    const int64_t pos = wave_lower_bound(m.keys, n, key, lane);
-    int64_t w0 = pos - W / 2;
-    if (w0 > n - W) w0 = n - W;
-    if (w0 < 0) w0 = 0;
-    const int64_t w1 = w0 + W - 1;
-    int buf_n = 0;
-    for (int t = 0; t < W; t += 64) {
-      const int64_t r = w0 + t + (int64_t)lane;
-      const bool ok = t + (int)lane < W;
-      if (ok) {
-        bd[t + lane] = dist2(qx, qy, qz, m.x[r], m.y[r], m.z[r]);
-        bi[t + lane] = (int32_t)r;
+    // ---- 1. seed
+    int32_t s0 = 0, s1 = (int32_t)n - 1;
+    if (n > kSeed) {
+      int32_t node = 0;
+      for (;;) {
+        const Node nd = nodes[node];
+        c_nodes++;
+        const double dl = box_dist2(qx, qy, qz, nd.lb);
+        const double dr = box_dist2(qx, qy, qz, nd.rb);
+        const bool go_left = dl <= dr;
+        const int32_t cnt_near = go_left ? nd.split - nd.first + 1 : nd.last - nd.split;
+        const bool near_leaf = go_left ? (nd.leaf_mask & 1) : (nd.leaf_mask & 2);
+        if (cnt_near >= k && !near_leaf && cnt_near > kSeed) {
+          node = go_left ? nd.split : nd.split + 1;
+          continue;
+        }
+        if (cnt_near >= k && !near_leaf) {  // the near child fits the seed buffer
+          s0 = go_left ? nd.first : nd.split + 1;
+          s1 = go_left ? nd.split : nd.last;
+        } else if (nd.last - nd.first + 1 <= kSeed) {
+          s0 = nd.first;
+          s1 = nd.last;
+        } else if (go_left) {
+          s0 = nd.first;
+          s1 = nd.first + kSeed - 1;
+        } else {
+          s0 = nd.last - kSeed + 1;
+          s1 = nd.last;
+        }
+        break;
       }
     }
-    buf_n = W;
-    c_cand += W;
-    wave_sort(bd, bi, buf_n, lane);
-    buf_n = k;
+    const int W = s1 - s0 + 1;
+    for (int t = (int)lane; t < W; t += 64) {
+      const int32_t r = s0 + t;
+      bd[t] = dist2(qx, qy, qz, m.x[r], m.y[r], m.z[r]);
+      bi[t] = r;
+    }
+    c_cand += (unsigned long long)W;
+    wave_sort(bd, bi, W, lane);
+    int buf_n = k;
     double theta = bd[k - 1];
 
-    // ---- 2. pruned traversal of the rest of the map
+    // ---- 2. pruned traversal
     if (W < n) {
-      int sp = 0;
-      if (lane == 0) stack[0] = 0;
-      sp = 1;
+      if (lane == 0) {
+        sn[0] = 0;
+        sd[0] = 0.0;
+      }
+      int sp = 1;
       wave_sync();
       while (sp > 0) {
         const int p = (sp >= kStack - 128) ? 1 : (sp < 64 ? sp : 64);
-        const bool have = (int)lane < p;
-        const int32_t node = have ? stack[sp - 1 - (int)lane] : 0;
+        bool have = (int)lane < p;
+        int32_t node = 0;
+        if (have) {
+          node = sn[sp - 1 - (int)lane];
+          have = sd[sp - 1 - (int)lane] <= theta;
+        }
         sp -= p;
         wave_sync();
-        int kind = 0;  // 1: expand children, 2: scan record range
-        int32_t lo = 0, hi = -1, c0 = 0, c1 = 0;
+        // per child (0 = farther, 1 = nearer): 1 = push, 2 = scan
+        int kind0 = 0, kind1 = 0;
+        int32_t lo0 = 0, hi0 = -1, lo1 = 0, hi1 = -1, id0 = 0, id1 = 0;
+        double md0 = 0.0, md1 = 0.0;
         if (have) {
-          if (node < 0) {
-            const int32_t r = ~node;
-            if (!(r >= w0 && r <= w1)) {
-              kind = 2;
-              lo = hi = r;
-            }
-          } else {
-            lo = m.node_range[2 * node];
-            hi = m.node_range[2 * node + 1];
-            if (!(lo >= w0 && hi <= w1)) {
-              const double md2 = box_dist2(qx, qy, qz, m.node_box + 6 * (int64_t)node);
-              if (md2 <= theta) {
-                if (hi - lo + 1 <= kBucket) {
-                  kind = 2;
-                } else {
-                  kind = 1;
-                  c0 = m.node_child[2 * node];
-                  c1 = m.node_child[2 * node + 1];
-                }
-              }
-            }
+          const Node nd = nodes[node];
+          const double dl = box_dist2(qx, qy, qz, nd.lb);
+          const double dr = box_dist2(qx, qy, qz, nd.rb);
+          const bool left_near = dl <= dr;
+          const int32_t fl = left_near ? nd.split + 1 : nd.first, fh = left_near ? nd.last : nd.split;
+          const int32_t nl = left_near ? nd.first : nd.split + 1, nh = left_near ? nd.split : nd.last;
+          const double fd = left_near ? dr : dl, ndd = left_near ? dl : dr;
+          const bool fleaf = left_near ? (nd.leaf_mask & 2) : (nd.leaf_mask & 1);
+          const bool nleaf = left_near ? (nd.leaf_mask & 1) : (nd.leaf_mask & 2);
+          const int32_t fid = left_near ? nd.split + 1 : nd.split, nid = left_near ? nd.split : nd.split + 1;
+          if (fd <= theta && !(fl >= s0 && fh <= s1)) {
+            lo0 = fl; hi0 = fh; id0 = fid; md0 = fd;
+            kind0 = (fleaf || fh - fl + 1 <= kBucket) ? 2 : 1;
+          }
+          if (ndd <= theta && !(nl >= s0 && nh <= s1)) {
+            lo1 = nl; hi1 = nh; id1 = nid; md1 = ndd;
+            kind1 = (nleaf || nh - nl + 1 <= kBucket) ? 2 : 1;
           }
         }
-        c_nodes += (unsigned long long)p;
-        const uint64_t m1 = ballot(kind == 1);
-        const int np = 2 * __popcll(m1);
-        if (sp + np > kStack) {  // cannot happen with the pop policy above; refuse rather than corrupt
+        c_nodes += (unsigned long long)__popcll(ballot(have));
+        const uint64_t pa = ballot(kind0 == 1), pb = ballot(kind1 == 1);
+        const int npush = __popcll(pa) + __popcll(pb);
+        if (sp + npush > kStack) {  // excluded by the pop policy; refuse rather than corrupt
           c_ovf++;
           break;
         }
-        if (kind == 1) {
-          const int off = sp + 2 * (int)mask_rank(m1);
-          stack[off] = c1;
-          stack[off + 1] = c0;
+        {
+          int off = sp + (int)mask_rank(pa) + (int)mask_rank(pb);
+          if (kind0 == 1) {
+            sn[off] = id0;
+            sd[off] = md0;
+            ++off;
+          }
+          if (kind1 == 1) {
+            sn[off] = id1;
+            sd[off] = md1;
+          }
         }
-        sp += np;
-        const uint64_t m2 = ballot(kind == 2);
-        const int nr = __popcll(m2);
-        if (kind == 2) {
-          const int r = (int)mask_rank(m2);
-          rlo[r] = lo;
-          rhi[r] = hi;
+        sp += npush;
+        const uint64_t ra = ballot(kind0 == 2), rb = ballot(kind1 == 2);
+        const int nr = __popcll(ra) + __popcll(rb);
+        {
+          int off = (int)mask_rank(ra) + (int)mask_rank(rb);
+          if (kind0 == 2) {
+            rlo[off] = lo0;
+            rhi[off] = hi0;
+            ++off;
+          }
+          if (kind1 == 2) {
+            rlo[off] = lo1;
+            rhi[off] = hi1;
+          }
         }
         wave_sync();
-        for (int r = 0; r < nr; ++r) {
-          const int32_t rl = rlo[r], rh = rhi[r];
-          const int64_t idx = (int64_t)rl + lane;
-          bool take = false;
-          double d2 = 0.0;
-          if (idx <= rh && !(idx >= w0 && idx <= w1)) {
-            d2 = dist2(qx, qy, qz, m.x[idx], m.y[idx], m.z[idx]);
-            take = d2 <= theta;
+        // scan the ranges four at a time: all twelve coordinate loads issue together
+        for (int r0 = 0; r0 < nr; r0 += 4) {
+          double d2v[4];
+          bool tk[4];
+          int32_t ix[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            tk[u] = false;
+            d2v[u] = 0.0;
+            ix[u] = 0;
+            if (r0 + u < nr) {
+              const int32_t rl = rlo[r0 + u], rh = rhi[r0 + u];
+              const int32_t idx = rl + (int32_t)lane;
+              ix[u] = idx;
+              c_cand += (unsigned long long)(rh - rl + 1);
+              if (idx <= rh && !(idx >= s0 && idx <= s1)) {
+                d2v[u] = dist2(qx, qy, qz, m.x[idx], m.y[idx], m.z[idx]);
+                tk[u] = d2v[u] <= theta;
+              }
+            }
           }
-          c_cand += (unsigned long long)(rh - rl + 1);
-          const uint64_t mt = ballot(take);
-          if (mt == 0) continue;
-          if (buf_n + __popcll(mt) > kBuf) {
-            wave_sort(bd, bi, buf_n, lane);
-            buf_n = k;
-            theta = bd[k - 1];
-            take = take && d2 <= theta;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint64_t mt = ballot(tk[u]);
+            if (mt == 0) continue;
+            bool take = tk[u];
+            if (buf_n + __popcll(mt) > kBuf) {
+              wave_sort(bd, bi, buf_n, lane);
+              buf_n = k;
+              theta = bd[k - 1];
+              take = take && d2v[u] <= theta;
+            }
+            const uint64_t mt2 = ballot(take);
+            if (take) {
+              const int slot = buf_n + (int)mask_rank(mt2);
+              bd[slot] = d2v[u];
+              bi[slot] = ix[u];
+            }
+            buf_n += __popcll(mt2);
+            wave_sync();
           }
-          const uint64_t mt2 = ballot(take);
-          if (take) {
-            const int slot = buf_n + (int)mask_rank(mt2);
-            bd[slot] = d2;
-            bi[slot] = (int32_t)idx;
-          }
-          buf_n += __popcll(mt2);
-          wave_sync();
         }
       }
     }
 
     // ---- 3. final neighbour set in (distance, index) order
     wave_sort(bd, bi, buf_n, lane);
-    const double r2 = bd[k - 1];
-    const double rk = sqrt(r2);
-
-    // ---- 4. energies into the waveform
-    const double sdx = a.q.dx[qg], sdy = a.q.dy[qg], sdz = a.q.dz[qg];
-    const float qE = a.q.energy[qg], qt = a.q.time[qg];
-    double ksize;
-    if (a.type == ALB_SCATTER) ksize = (4.0 / 3.0) * ALB_PI * pow(rk, 3.0);
-    else ksize = ALB_PI * (rk * rk);
-    const double knorm = 1.0 / ksize;
-    const double m_bottom = (a.sc.floor_albedo / ALB_PI) * fmax(0.0, -sdy);
-    const float fmult = (float)a.sc.mult;
-    for (int t = (int)lane; t < k; t += 64) {
-      const int32_t i = bi[t];
-      double mul;
-      if (a.type == ALB_BOTTOM_REFLECTION) {
-        mul = m_bottom;
-      } else if (a.type == ALB_SCATTER) {
-        mul = scatter_mult(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, a.sc.phase_theta, a.sc.phase_cdf,
-                           a.sc.phase_n);
-      } else if (a.type == ALB_SURFACE_REFLECTION) {
-        mul = (double)brdf_reflect(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, a.sc.alpha_ggx, a.sc.F0);
-      } else {
-        mul = btdf_transmit(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, a.sc.alpha_ggx, a.sc.n_water, a.sc.F0,
-                            a.sc.surface_albedo);
-      }
-      const float ee = qE * m.energy[i];
-      const double e = (double)ee * knorm * mul;
-      const float tt = qt + m.time[i];
-      const int64_t bin = (int64_t)(tt / fmult);
-      if (bin >= a.win0 && bin < (int64_t)a.win0 + a.win_len) atomicAdd(&hist[bin - a.win0], e);
-      else if (bin >= 0 && bin < a.n_bins) unsafeAtomicAdd(&a.wf[bin], e);
-      if (a.nbr_out) a.nbr_out[qi * k + t] = i;
-    }
-    if (a.radius_out && lane == 0) a.radius_out[qi] = rk;
+    for (int t = (int)lane; t < k; t += 64) a.nbr[qi * k + t] = bi[t];
+    if (lane == 0) a.radius[qi] = sqrt(bd[k - 1]);
     wave_sync();
   }
-
-  __syncthreads();
-  for (int t = threadIdx.x; t < a.win_len; t += blockDim.x)
-    if (hist[t] != 0.0) unsafeAtomicAdd(&a.wf[a.win0 + t], hist[t]);
   if (a.ctr && lane == 0) {
     if (c_q) atomicAdd(&a.ctr[ALB_CTR_QUERIES], c_q);
     if (c_cand) atomicAdd(&a.ctr[ALB_CTR_CANDIDATES], c_cand);
@@ -314,47 +344,110 @@ __global__ __launch_bounds__(256) void gather_kernel(GatherArgs a) {
   }
 }
 
+__global__ __launch_bounds__(256) void contrib_kernel(ContribArgs a) {
+  extern __shared__ double hist[];  // [win_len]
+  for (int t = threadIdx.x; t < a.win_len; t += blockDim.x) hist[t] = 0.0;
+  __syncthreads();
+  const int k = a.sc.k;
+  const int64_t total = a.q_n * (int64_t)k;
+  const float fmult = (float)a.sc.mult;
+  const alb_map_t& m = a.map;
+  for (int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; it < total;
+       it += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t qi = it / k;
+    const int64_t qg = a.q_begin + qi;
+    const int32_t i = a.nbr[it];
+    const double rk = a.radius[qi];
+    const double sdx = a.q.dx[qg], sdy = a.q.dy[qg], sdz = a.q.dz[qg];
+    double ksize;
+    if (a.type == ALB_SCATTER) ksize = (4.0 / 3.0) * ALB_PI * pow(rk, 3.0);
+    else ksize = ALB_PI * (rk * rk);
+    const double knorm = 1.0 / ksize;
+    double mul;
+    if (a.type == ALB_BOTTOM_REFLECTION) {
+      mul = (a.sc.floor_albedo / ALB_PI) * fmax(0.0, -sdy);
+    } else if (a.type == ALB_SCATTER) {
+      mul = scatter_mult(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, a.sc.phase_theta, a.sc.phase_cdf,
+                         a.sc.phase_n);
+    } else if (a.type == ALB_SURFACE_REFLECTION) {
+      mul = (double)brdf_reflect(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, a.sc.alpha_ggx, a.sc.F0);
+    } else {
+      mul = btdf_transmit(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, a.sc.alpha_ggx, a.sc.n_water, a.sc.F0,
+                          a.sc.surface_albedo);
+    }
+    const float ee = a.q.energy[qg] * m.energy[i];
+    const double e = (double)ee * knorm * mul;
+    const float tt = a.q.time[qg] + m.time[i];
+    const int64_t bin = (int64_t)(tt / fmult);
+    if (bin >= a.win0 && bin < (int64_t)a.win0 + a.win_len) atomicAdd(&hist[bin - a.win0], e);
+    else if (bin >= 0 && bin < a.n_bins) unsafeAtomicAdd(&a.wf[bin], e);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < a.win_len; t += blockDim.x)
+    if (hist[t] != 0.0) unsafeAtomicAdd(&a.wf[a.win0 + t], hist[t]);
+}
+
+static int64_t chunk_for(int64_t q_n, size_t ws_bytes, int k) {
+  const size_t per = (size_t)k * sizeof(int32_t) + sizeof(double);
+  int64_t c = (int64_t)(ws_bytes / per);
+  if (c > kChunk) c = kChunk;
+  if (c > q_n) c = q_n;
+  return c;
+}
+
 }  // namespace alb
+
+extern "C" int alb_gather_workspace_size(int64_t q_n, int32_t k, size_t* bytes) {
+  using namespace alb;
+  if (!bytes || q_n < 0 || k <= 0) return set_error(ALB_ERR_ARG, "alb_gather_workspace_size: bad arguments");
+  const int64_t c = q_n < kChunk ? q_n : kChunk;
+  *bytes = (size_t)(c > 0 ? c : 1) * ((size_t)k * sizeof(int32_t) + sizeof(double)) + 256;
+  return ALB_OK;
+}
 
 extern "C" int alb_gather(const alb_scene_t* scene, const alb_map_t* map, int32_t photon_type,
                           const alb_queries_t* q, int64_t q_begin, int64_t q_n, double* waveform,
                           int64_t n_bins, int32_t win0, int32_t win_len, int32_t* nbr_out, double* radius_out,
-                          unsigned long long* counters, void* stream) {
+                          void* workspace, size_t workspace_bytes, unsigned long long* counters, void* stream) {
   using namespace alb;
   if (!scene || !map || !q || !waveform) return set_error(ALB_ERR_ARG, "alb_gather: null argument");
   if (photon_type < 0 || photon_type >= ALB_N_TYPES) return set_error(ALB_ERR_ARG, "alb_gather: bad photon type");
   if (q_n < 0 || q_begin < 0) return set_error(ALB_ERR_ARG, "alb_gather: negative query range");
   if (q_n == 0) return ALB_OK;
   if (map->n <= 0) return set_error(ALB_ERR_ARG, "alb_gather: empty photon map for type %d", photon_type);
-  if (scene->k <= 0 || scene->k > map->n)
-    return set_error(ALB_ERR_ARG, "alb_gather: k = %d exceeds the map size %lld", scene->k, (long long)map->n);
-  if (scene->k > kMaxK) return set_error(ALB_ERR_UNSUPPORTED, "alb_gather: k = %d above %d", scene->k, kMaxK);
+  const int k = scene->k;
+  if (k <= 0 || k > map->n)
+    return set_error(ALB_ERR_ARG, "alb_gather: k = %d exceeds the map size %lld", k, (long long)map->n);
+  if (k > kMaxK) return set_error(ALB_ERR_UNSUPPORTED, "alb_gather: k = %d above %d", k, kMaxK);
   if (win_len < 0 || win_len > kMaxWindow || win0 < 0 || (int64_t)win0 + win_len > n_bins)
     return set_error(ALB_ERR_ARG, "alb_gather: window [%d, %d) invalid (max length %d, %lld bins)", win0,
                      win0 + win_len, kMaxWindow, (long long)n_bins);
-  if (!map->keys || !map->meta || !map->x || (map->n > 1 && (!map->node_box || !map->node_child)))
+  if (!map->keys || !map->meta || !map->x || (map->n > 1 && !map->nodes))
     return set_error(ALB_ERR_ARG, "alb_gather: map not built");
-  GatherArgs a;
-  a.sc = *scene;
-  a.map = *map;
-  a.q = *q;
-  a.q_begin = q_begin;
-  a.q_n = q_n;
-  a.type = photon_type;
-  a.wf = waveform;
-  a.n_bins = n_bins;
-  a.win0 = win0;
-  a.win_len = win_len;
-  a.nbr_out = nbr_out;
-  a.radius_out = radius_out;
-  a.ctr = counters;
+  if ((nbr_out == nullptr) != (radius_out == nullptr))
+    return set_error(ALB_ERR_ARG, "alb_gather: nbr_out and radius_out go together");
+  const int64_t chunk = nbr_out ? q_n : chunk_for(q_n, workspace ? workspace_bytes : 0, k);
+  if (chunk <= 0) return set_error(ALB_ERR_WORKSPACE, "alb_gather: workspace too small");
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return set_error(ALB_ERR_HIP, "alb_gather: no HIP device");
-  int64_t blocks = (q_n + kWaves - 1) / kWaves;
-  const int64_t cap = (int64_t)cus * 8;
-  if (blocks > cap) blocks = cap;
-  hipLaunchKernelGGL(gather_kernel, dim3((unsigned)blocks), dim3(64 * kWaves), 0, (hipStream_t)stream, a);
-  return check_launch("alb_gather");
+  hipStream_t s = (hipStream_t)stream;
+  for (int64_t c0 = 0; c0 < q_n; c0 += chunk) {
+    const int64_t cn = (q_n - c0) < chunk ? (q_n - c0) : chunk;
+    int32_t* nbr = nbr_out ? nbr_out + c0 * k : (int32_t*)workspace;
+    double* rad = radius_out ? radius_out + c0
+                             : (double*)((char*)workspace + (((size_t)chunk * k * sizeof(int32_t) + 7) & ~(size_t)7));
+    KnnArgs ka{*map, *q, q_begin + c0, cn, k, nbr, rad, counters};
+    int64_t kb = (cn + kWaves - 1) / kWaves;
+    if (kb > (int64_t)cus * 16) kb = (int64_t)cus * 16;
+    hipLaunchKernelGGL(knn_kernel, dim3((unsigned)kb), dim3(64 * kWaves), 0, s, ka);
+    ContribArgs ca{*scene, *map, *q, q_begin + c0, cn, photon_type, nbr, rad, waveform, n_bins, win0, win_len};
+    int64_t cb = (cn * k + 255) / 256;
+    if (cb > (int64_t)cus * 8) cb = (int64_t)cus * 8;
+    hipLaunchKernelGGL(contrib_kernel, dim3((unsigned)cb), dim3(256), sizeof(double) * (size_t)win_len, s, ca);
+    const int rc = check_launch("alb_gather");
+    if (rc != ALB_OK) return rc;
+  }
+  return ALB_OK;
 }

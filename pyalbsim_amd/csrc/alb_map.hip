@@ -111,25 +111,23 @@ __device__ __forceinline__ int delta(const uint64_t* __restrict__ k, int64_t n, 
 }
 
 __device__ __forceinline__ void build_node(const uint64_t* __restrict__ k, int64_t n, int64_t i,
-                                           int32_t* __restrict__ child, int32_t* __restrict__ range,
-                                           uint8_t* __restrict__ level, uint32_t* lh);
+                                           Node* __restrict__ nodes, uint8_t* __restrict__ level, uint32_t* lh);
 
 __global__ __launch_bounds__(256) void radix_tree_kernel(const uint64_t* __restrict__ k, int64_t n,
-                                                         int32_t* __restrict__ child, int32_t* __restrict__ range,
-                                                         uint8_t* __restrict__ level, uint32_t* __restrict__ hist) {
+                                                         Node* __restrict__ nodes, uint8_t* __restrict__ level,
+                                                         uint32_t* __restrict__ hist) {
   __shared__ uint32_t lh[kLevels];
   for (int t = threadIdx.x; t < kLevels; t += blockDim.x) lh[t] = 0u;
   __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n - 1) build_node(k, n, i, child, range, level, lh);
+  if (i < n - 1) build_node(k, n, i, nodes, level, lh);
   __syncthreads();
   for (int t = threadIdx.x; t < kLevels; t += blockDim.x)
     if (lh[t]) atomicAdd(&hist[t], lh[t]);
 }
 
 __device__ __forceinline__ void build_node(const uint64_t* __restrict__ k, int64_t n, int64_t i,
-                                           int32_t* __restrict__ child, int32_t* __restrict__ range,
-                                           uint8_t* __restrict__ level, uint32_t* lh) {
+                                           Node* __restrict__ nodes, uint8_t* __restrict__ level, uint32_t* lh) {
   const int d = (delta(k, n, i, i + 1) - delta(k, n, i, i - 1)) >= 0 ? 1 : -1;
   const int dmin = delta(k, n, i, i - d);
   int64_t lmax = 2;
@@ -150,10 +148,11 @@ __device__ __forceinline__ void build_node(const uint64_t* __restrict__ k, int64
   }
   const int64_t g = i + s * d + (d < 0 ? -1 : 0);
   const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
-  child[2 * i + 0] = (lo == g) ? ~(int32_t)g : (int32_t)g;
-  child[2 * i + 1] = (hi == g + 1) ? ~(int32_t)(g + 1) : (int32_t)(g + 1);
-  range[2 * i + 0] = (int32_t)lo;
-  range[2 * i + 1] = (int32_t)hi;
+  Node* nd = nodes + i;
+  nd->first = (int32_t)lo;
+  nd->last = (int32_t)hi;
+  nd->split = (int32_t)g;
+  nd->leaf_mask = (lo == g ? 1 : 0) | (hi == g + 1 ? 2 : 0);
   const int lv = dnode < 0 ? 0 : (dnode >= kLevels ? kLevels - 1 : dnode);
   level[i] = (uint8_t)lv;
   atomicAdd(&lh[lv], 1u);
@@ -185,34 +184,39 @@ __global__ __launch_bounds__(256) void level_scatter_kernel(const uint8_t* __res
   if (i < m) order[base[lv] + local] = (int32_t)i;
 }
 
-__device__ __forceinline__ void child_box(int32_t c, const alb_map_t& m, float b[6]) {
-  if (c < 0) {
-    const int32_t r = ~c;
-    b[0] = b[3] = m.x[r];
-    b[1] = b[4] = m.y[r];
-    b[2] = b[5] = m.z[r];
+// box of one child: the record itself for a leaf, else the union of the child
+// node's two child boxes (written by an earlier, longer-prefix level)
+__device__ __forceinline__ void child_box(const alb_map_t& m, const Node* __restrict__ nodes, int32_t c, bool leaf,
+                                          float b[6]) {
+  if (leaf) {
+    b[0] = b[3] = m.x[c];
+    b[1] = b[4] = m.y[c];
+    b[2] = b[5] = m.z[c];
   } else {
-    const float* nb = m.node_box + 6 * (int64_t)c;
+    const Node& cn = nodes[c];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) b[k] = nb[k];
+    for (int a = 0; a < 3; ++a) {
+      b[a] = fminf(cn.lb[a], cn.rb[a]);
+      b[3 + a] = fmaxf(cn.lb[3 + a], cn.rb[3 + a]);
+    }
   }
 }
 
 __global__ __launch_bounds__(256) void box_level_kernel(alb_map_t m, const int32_t* __restrict__ order,
                                                         const uint32_t* __restrict__ offset, int lv) {
+  Node* nodes = (Node*)m.nodes;
   const uint32_t beg = offset[lv], end = offset[lv + 1];
   for (uint32_t t = beg + blockIdx.x * blockDim.x + threadIdx.x; t < end; t += gridDim.x * blockDim.x) {
     const int32_t node = order[t];
+    Node& nd = nodes[node];
     float a[6], b[6];
-    child_box(m.node_child[2 * node + 0], m, a);
-    child_box(m.node_child[2 * node + 1], m, b);
-    float* o = m.node_box + 6 * (int64_t)node;
-    o[0] = fminf(a[0], b[0]);
-    o[1] = fminf(a[1], b[1]);
-    o[2] = fminf(a[2], b[2]);
-    o[3] = fmaxf(a[3], b[3]);
-    o[4] = fmaxf(a[4], b[4]);
-    o[5] = fmaxf(a[5], b[5]);
+    child_box(m, nodes, nd.split, nd.leaf_mask & 1, a);
+    child_box(m, nodes, nd.split + 1, nd.leaf_mask & 2, b);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      nd.lb[c] = a[c];
+      nd.rb[c] = b[c];
+    }
   }
 }
 
@@ -266,7 +270,7 @@ extern "C" int alb_map_build(const alb_records_t* in, int64_t n, int32_t dims, a
   if (!map->x || !map->y || !map->z || !map->dx || !map->dy || !map->dz || !map->energy || !map->time ||
       !map->perm || !map->keys || !map->meta)
     return set_error(ALB_ERR_ARG, "alb_map_build: map arrays missing");
-  if (n > 1 && (!map->node_child || !map->node_range || !map->node_box))
+  if (n > 1 && !map->nodes)
     return set_error(ALB_ERR_ARG, "alb_map_build: tree arrays missing");
   WsLayout L;
   int rc = layout(n, &L);
@@ -300,8 +304,7 @@ extern "C" int alb_map_build(const alb_records_t* in, int64_t n, int32_t dims, a
   hipLaunchKernelGGL(permute_kernel, dim3(grid), dim3(B), 0, s, *in, m);
   if (n > 1) {
     const unsigned g2 = (unsigned)((n - 1 + B - 1) / B);
-    hipLaunchKernelGGL(radix_tree_kernel, dim3(g2), dim3(B), 0, s, map->keys, n, map->node_child,
-                       map->node_range, level, hist);
+    hipLaunchKernelGGL(radix_tree_kernel, dim3(g2), dim3(B), 0, s, map->keys, n, (Node*)map->nodes, level, hist);
     hipLaunchKernelGGL(level_scan_kernel, dim3(1), dim3(1), 0, s, hist, offset, cursor);
     hipLaunchKernelGGL(level_scatter_kernel, dim3(g2), dim3(B), 0, s, level, n - 1, cursor, order);
     const unsigned g3 = g2 < 1024 ? g2 : 1024;

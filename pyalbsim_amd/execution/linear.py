@@ -82,6 +82,7 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
     qs, qcounts = runtime.trace(ds, key, abi.PASS_BACKWARD, n_backward, tags=tags, counters=cb)
     mark("backward")
     wf = torch.zeros((abi.N_TYPES, model.n_bins), dtype=torch.float64, device="cuda")
+    keep_alive = []
     for ti in range(abi.N_TYPES):
         if qcounts[ti] == 0:
             continue
@@ -92,13 +93,14 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
                 raise KeyError(PhotonType(ti + 1))
             raise IndexError(f"photon_mapping_k = {model.k} exceeds the {PhotonType(ti + 1).name} map size "
                              f"{rcounts[ti]}")
-        runtime.gather(ds, maps[ti], ti, qs[ti], qcounts[ti], wf[ti], win, counters=cb)
+        keep_alive.append(runtime.gather(ds, maps[ti], ti, qs[ti], qcounts[ti], wf[ti], win, counters=cb))
     mark("gather")
     res = SimulationResult(model, wf, rcounts, qcounts, None, None, timings=t)
     ctr_f, ctr_b = cf.cpu().numpy().view(np.uint64), cb.cpu().numpy().view(np.uint64)
     if ctr_b[abi.CTR_OVERFLOW]:
         raise runtime.AlbError("gather node stack overflow")
     res.counters_fwd, res.counters_bwd = ctr_f, ctr_b
+    del keep_alive  # the counter read-back above synchronised the stream
     if keep:
         res.maps = maps
         res.queries = qs

@@ -20,7 +20,7 @@ _lib = None
 
 # header entry points, checked by the ABI test
 EXPORTS = ["alb_abi_version", "alb_scene_size", "alb_last_error", "alb_trace", "alb_map_workspace_size",
-           "alb_map_build", "alb_gather"]
+           "alb_map_build", "alb_gather_workspace_size", "alb_gather"]
 
 
 class AlbError(RuntimeError):
@@ -45,8 +45,10 @@ def load_library(path: str = LIB_PATH):
     L.alb_map_workspace_size.restype = C.c_int
     L.alb_map_build.argtypes = [P, C.c_int64, C.c_int32, P, P, C.c_size_t, P]
     L.alb_map_build.restype = C.c_int
+    L.alb_gather_workspace_size.argtypes = [C.c_int64, C.c_int32, C.POINTER(C.c_size_t)]
+    L.alb_gather_workspace_size.restype = C.c_int
     L.alb_gather.argtypes = [P, P, C.c_int32, P, C.c_int64, C.c_int64, P, C.c_int64, C.c_int32, C.c_int32,
-                             P, P, P, P]
+                             P, P, P, C.c_size_t, P, P]
     L.alb_gather.restype = C.c_int
     if L.alb_abi_version() != abi.ALB_ABI_VERSION:
         raise AlbError("libalbsim_hip.so ABI version mismatch")
@@ -197,14 +199,11 @@ def build_map(records: Sink, n: int, dims: int, stream=None) -> DeviceMap:
     t["keys"] = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
     t["meta"] = torch.zeros(8, dtype=torch.float64, device=dev)
     ni = max(n - 1, 1)
-    t["node_child"] = torch.empty(2 * ni, dtype=torch.int32, device=dev)
-    t["node_range"] = torch.empty(2 * ni, dtype=torch.int32, device=dev)
-    t["node_box"] = torch.empty(6 * ni, dtype=torch.float32, device=dev)
+    t["nodes"] = torch.empty((ni, 16), dtype=torch.int32, device=dev)  # 64-byte nodes
     st = abi.Map()
     st.n = n
     st.dims = dims
-    for f in ("x", "y", "z", "dx", "dy", "dz", "energy", "time", "perm", "keys", "meta", "node_child",
-              "node_range", "node_box"):
+    for f in ("x", "y", "z", "dx", "dy", "dz", "energy", "time", "perm", "keys", "meta", "nodes"):
         setattr(st, f, _p(t[f]))
     m.struct = st
     if n == 0:
@@ -222,13 +221,22 @@ def build_map(records: Sink, n: int, dims: int, stream=None) -> DeviceMap:
 def gather(dscene: DeviceScene, dmap: DeviceMap, ptype: int, queries: Sink, q_n: int, waveform, window,
            q_begin: int = 0, nbr_out=None, radius_out=None, counters=None, stream=None):
     """Accumulate the k-NN gather of q_n queries of one type into waveform (float64 device tensor)."""
+    torch = _torch()
     L = load_library()
     win0, win_len = window
     s = stream if stream is not None else _stream_ptr()
+    ws = None
+    ws_bytes = 0
+    if nbr_out is None:
+        b = C.c_size_t(0)
+        _check(L.alb_gather_workspace_size(q_n, dscene.model.k, C.byref(b)), "alb_gather_workspace_size")
+        ws_bytes = b.value
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=waveform.device)
     rc = L.alb_gather(C.byref(dscene.s), C.byref(dmap.struct), ptype, C.byref(queries.struct), q_begin, q_n,
                       _p(waveform), int(waveform.numel()), int(win0), int(win_len), _p(nbr_out), _p(radius_out),
-                      _p(counters), C.c_void_p(s))
+                      _p(ws), ws_bytes, _p(counters), C.c_void_p(s))
     _check(rc, "alb_gather")
+    return ws  # the caller keeps it alive until the stream has run
 
 
 MAP_DIMS = (2, 3, 2, 2)  # BOTTOM and the two surface maps are planar; SCATTER is volumetric

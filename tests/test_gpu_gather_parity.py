@@ -50,15 +50,27 @@ def test_map_build_is_a_permutation_with_containing_boxes(gpu, scene):
             assert np.array_equal(m[f][:n].cpu().numpy(), src[f][perm])
         keys = m["keys"][:n].cpu().numpy().view(np.uint64)
         assert np.all(keys[1:] >= keys[:-1])
-        rng_ = m["node_range"][: 2 * (n - 1)].cpu().numpy().reshape(-1, 2)
-        box = m["node_box"][: 6 * (n - 1)].cpu().numpy().reshape(-1, 6)
+        nodes = m["nodes"][: n - 1].cpu().numpy()
+        first, last, split, mask = nodes[:, 0], nodes[:, 1], nodes[:, 2], nodes[:, 3]
+        boxes = nodes[:, 4:].view(np.float32)
         xyz = np.stack([m["x"][:n].cpu().numpy(), m["y"][:n].cpu().numpy(), m["z"][:n].cpu().numpy()], 1)
-        assert rng_[0, 0] == 0 and rng_[0, 1] == n - 1
+        assert first[0] == 0 and last[0] == n - 1
+        assert np.all((first <= split) & (split < last))
+        # every record is a leaf of exactly one node
+        leaves = np.concatenate([split[(mask & 1) > 0], split[(mask & 2) > 0] + 1])
+        assert np.array_equal(np.sort(leaves), np.arange(n))
         sample = np.random.default_rng(0).choice(n - 1, size=min(n - 1, 400), replace=False)
         for i in sample:
-            lo, hi = rng_[i]
-            pts = xyz[lo:hi + 1]
-            assert np.array_equal(box[i, :3], pts.min(0)) and np.array_equal(box[i, 3:], pts.max(0))
+            lp = xyz[first[i]:split[i] + 1]
+            rp = xyz[split[i] + 1:last[i] + 1]
+            assert np.array_equal(boxes[i, :3], lp.min(0)) and np.array_equal(boxes[i, 3:6], lp.max(0))
+            assert np.array_equal(boxes[i, 6:9], rp.min(0)) and np.array_equal(boxes[i, 9:], rp.max(0))
+            if not mask[i] & 1:
+                c = split[i]
+                assert first[c] == first[i] and last[c] == split[i]
+            if not mask[i] & 2:
+                c = split[i] + 1
+                assert first[c] == split[i] + 1 and last[c] == last[i]
 
 
 @pytest.mark.parametrize("scene", ["single_shot", "turbid"])
