@@ -33,6 +33,7 @@ PEAK_HBM_GBS = 8000.0
 # algorithmic work per event (flops: add/sub/mul/div/cmp = 1, FMA = 2,
 # sqrt/log/exp/sin/cos/acos/pow = 1)
 F_FWD_EMIT, F_BWD_EMIT, F_AIR, F_WATER_STEP = 30, 50, 12, 24
+F_DEPTH_SOLVE = 12  # one closed-form optical-depth jump (or single step)
 F_SCATTER, F_ENTER, F_EXIT, F_FLOOR = 95, 70, 80, 70
 M_TYPE = (0, 30, 85, 0)  # per-neighbour multiplier flops by photon type
 
@@ -40,8 +41,9 @@ M_TYPE = (0, 30, 85, 0)  # per-neighbour multiplier flops by photon type
 def trace_flops(ctr, forward: bool) -> float:
     from pyalbsim_amd import abi
 
+    # the water sub-steps are reproduced in closed form: count the jumps, not the sub-steps
     return float((F_FWD_EMIT if forward else F_BWD_EMIT) * ctr[abi.CTR_PATHS] + F_AIR * ctr[abi.CTR_AIR_LEGS]
-                 + F_WATER_STEP * ctr[abi.CTR_WATER_STEPS] + F_SCATTER * ctr[abi.CTR_SCATTER]
+                 + F_DEPTH_SOLVE * ctr[abi.CTR_DEPTH_SOLVES] + F_SCATTER * ctr[abi.CTR_SCATTER]
                  + F_ENTER * ctr[abi.CTR_ENTER] + F_EXIT * ctr[abi.CTR_EXIT] + F_FLOOR * ctr[abi.CTR_FLOOR])
 
 
@@ -167,6 +169,7 @@ def main(argv=None):
         "gather_candidates_per_query": cand / max(nq, 1.0),
         "gather_nodes_per_query": nodes / max(nq, 1.0),
         "water_steps_per_path": float(cf[abi.CTR_WATER_STEPS]) / max(float(cf[abi.CTR_PATHS]), 1.0),
+        "depth_solves_per_path": float(cf[abi.CTR_DEPTH_SOLVES]) / max(float(cf[abi.CTR_PATHS]), 1.0),
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -69,7 +69,8 @@ enum alb_pass { ALB_PASS_FORWARD = 0, ALB_PASS_BACKWARD = 1 };
 enum alb_counter {
   ALB_CTR_PATHS = 0,         /* photons emitted                            */
   ALB_CTR_AIR_LEGS = 1,      /* straight air legs solved in closed form     */
-  ALB_CTR_WATER_STEPS = 2,   /* in-water sub-steps (optical depth updates)  */
+  ALB_CTR_WATER_STEPS = 2,   /* in-water sub-steps covered (float32 optical
+                                depth updates reproduced)                   */
   ALB_CTR_SCATTER = 3,       /* scatter events                              */
   ALB_CTR_ENTER = 4,         /* air->water crossings                        */
   ALB_CTR_EXIT = 5,          /* water->air crossings                        */
@@ -79,6 +80,7 @@ enum alb_counter {
   ALB_CTR_CANDIDATES = 9,    /* map records whose distance was evaluated    */
   ALB_CTR_NODES = 10,        /* tree nodes visited                          */
   ALB_CTR_OVERFLOW = 11,     /* gather queries that exceeded a kNN buffer   */
+  ALB_CTR_DEPTH_SOLVES = 12, /* optical-depth binade jumps and single steps */
   ALB_N_COUNTERS = 16
 };
 
@@ -213,8 +215,8 @@ int alb_map_build(const alb_records_t* in, int64_t n, int32_t dims,
                   alb_map_t* map, void* workspace, size_t workspace_bytes,
                   void* stream);
 
-/* Workspace bytes alb_gather needs for q_n queries (it processes at most
- * 2^22 queries per round and reuses the space). */
+/* Workspace bytes alb_gather needs for q_n queries (query sort buffers plus
+ * the neighbour rows of at most 2^22 queries, reused round by round). */
 int alb_gather_workspace_size(int64_t q_n, int32_t k, size_t* bytes);
 
 /* k-nearest-neighbour gather of queries [q_begin, q_begin + q_n) of one photon
@@ -222,8 +224,8 @@ int alb_gather_workspace_size(int64_t q_n, int32_t k, size_t* bytes);
  * (n_bins = 2*steps).  Bins [win0, win0 + win_len) are privatised in LDS.
  * nbr_out (optional, [q_n * k] int32) receives each query's neighbour set as
  * sorted-map indices in (distance, index) order and radius_out ([q_n]
- * double) the k-th neighbour distance; when they are given, no workspace is
- * needed.  Replaces core/simulation.py:132-195 (sample_photons). */
+ * double) the k-th neighbour distance.  Queries are processed in Morton order
+ * of their positions.  Replaces core/simulation.py:132-195 (sample_photons). */
 int alb_gather(const alb_scene_t* scene, const alb_map_t* map,
                int32_t photon_type, const alb_queries_t* q, int64_t q_begin,
                int64_t q_n, double* waveform, int64_t n_bins, int32_t win0,

@@ -24,9 +24,9 @@ PASS_FORWARD = 0
 PASS_BACKWARD = 1
 
 CTR_PATHS, CTR_AIR_LEGS, CTR_WATER_STEPS, CTR_SCATTER, CTR_ENTER, CTR_EXIT, CTR_FLOOR, \
-    CTR_REPLAY_EXHAUSTED, CTR_QUERIES, CTR_CANDIDATES, CTR_NODES, CTR_OVERFLOW = range(12)
+    CTR_REPLAY_EXHAUSTED, CTR_QUERIES, CTR_CANDIDATES, CTR_NODES, CTR_OVERFLOW, CTR_DEPTH_SOLVES = range(13)
 COUNTER_NAMES = ["paths", "air_legs", "water_steps", "scatter", "enter", "exit", "floor",
-                 "replay_exhausted", "queries", "candidates", "nodes", "overflow"]
+                 "replay_exhausted", "queries", "candidates", "nodes", "overflow", "depth_solves"]
 
 
 class Scene(C.Structure):

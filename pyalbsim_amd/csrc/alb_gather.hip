@@ -20,12 +20,16 @@
 //      index) order — bit-identical to brute force, because squared distances
 //      are evaluated in FP64 in one fixed order and box distances are, by
 //      monotone rounding, never larger than those of the records inside.
+//  Queries are first sorted by the Morton key of their position on the map's
+//  grid, so neighbouring wavefronts walk the same nodes and buckets (L2 hits).
 //  contrib_kernel — one thread per (query, neighbour): energy multiplier,
 //   kernel normalisation and time bin; FP64 sums go to an LDS-privatised
 //   window of the waveform (ds_add_f64), bins outside it to global atomics.
 //   Keeping the FP64 surface / phase-function code out of knn_kernel keeps the
 //   traversal at low register pressure (higher occupancy).
 #include <limits.h>
+
+#include <hipcub/hipcub.hpp>
 
 #include "alb_device.h"
 #include "alb_internal.h"
@@ -41,13 +45,18 @@ constexpr int kMaxWindow = 2048;  // LDS-privatised waveform bins
 constexpr int kMaxK = kSeed;      // the seed must hold k records
 constexpr int64_t kChunk = 1 << 22;  // queries per k-NN / contribution round
 
+// Query slots are processed in sorted order: slot si (c0 <= si < c0 + q_n)
+// is query q_begin + order[si].  Output rows are slot - c0, or the query's own
+// index when direct (caller-provided neighbour arrays).
 struct KnnArgs {
   alb_map_t map;
   alb_queries_t q;
-  int64_t q_begin, q_n;
+  int64_t q_begin, c0, q_n;
+  const int32_t* order;
+  int32_t direct;
   int32_t k;
-  int32_t* nbr;      // [q_n * k]
-  double* radius;    // [q_n]
+  int32_t* nbr;      // [rows * k]
+  double* radius;    // [rows]
   unsigned long long* ctr;
 };
 
@@ -55,7 +64,9 @@ struct ContribArgs {
   alb_scene_t sc;
   alb_map_t map;
   alb_queries_t q;
-  int64_t q_begin, q_n;
+  int64_t q_begin, c0, q_n;
+  const int32_t* order;
+  int32_t direct;
   int32_t type;
   const int32_t* nbr;
   const double* radius;
@@ -97,10 +108,12 @@ __device__ void wave_sort(double* d, int32_t* idx, int n, uint32_t lane) {
     idx[t] = INT_MAX;
   }
   wave_sync();
-  for (int size = 2; size <= N; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+  for (int lsize = 1; (1 << lsize) <= N; ++lsize) {
+    const int size = 1 << lsize;
+    for (int lstr = lsize - 1; lstr >= 0; --lstr) {
+      const int stride = 1 << lstr;
       for (int t = (int)lane; t < (N >> 1); t += 64) {
-        const int a = (t / stride) * 2 * stride + (t & (stride - 1));
+        const int a = ((t >> lstr) << (lstr + 1)) | (t & (stride - 1));
         const int b = a + stride;
         const bool up = (a & size) == 0;
         const double da = d[a], db = d[b];
@@ -160,8 +173,10 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   const int k = a.k;
   unsigned long long c_q = 0, c_cand = 0, c_nodes = 0, c_ovf = 0;
 
-  for (int64_t qi = (int64_t)blockIdx.x * kWaves + w; qi < a.q_n; qi += (int64_t)gridDim.x * kWaves) {
-    const int64_t qg = a.q_begin + qi;
+  for (int64_t si = (int64_t)blockIdx.x * kWaves + w; si < a.q_n; si += (int64_t)gridDim.x * kWaves) {
+    const int64_t qrel = a.order[a.c0 + si];
+    const int64_t qg = a.q_begin + qrel;
+    const int64_t qi = a.direct ? qrel : si;  // output row
     const double qx = a.q.x[qg], qy = a.q.y[qg], qz = a.q.z[qg];
     c_q++;
 
@@ -354,8 +369,8 @@ __global__ __launch_bounds__(256) void contrib_kernel(ContribArgs a) {
   const alb_map_t& m = a.map;
   for (int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; it < total;
        it += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t qi = it / k;
-    const int64_t qg = a.q_begin + qi;
+    const int64_t qi = it / k;  // row
+    const int64_t qg = a.q_begin + (a.direct ? qi : (int64_t)a.order[a.c0 + qi]);
     const int32_t i = a.nbr[it];
     const double rk = a.radius[qi];
     const double sdx = a.q.dx[qg], sdy = a.q.dy[qg], sdz = a.q.dz[qg];
@@ -387,12 +402,42 @@ __global__ __launch_bounds__(256) void contrib_kernel(ContribArgs a) {
     if (hist[t] != 0.0) unsafeAtomicAdd(&a.wf[a.win0 + t], hist[t]);
 }
 
-static int64_t chunk_for(int64_t q_n, size_t ws_bytes, int k) {
-  const size_t per = (size_t)k * sizeof(int32_t) + sizeof(double);
-  int64_t c = (int64_t)(ws_bytes / per);
-  if (c > kChunk) c = kChunk;
-  if (c > q_n) c = q_n;
-  return c;
+__global__ __launch_bounds__(256) void query_key_kernel(alb_queries_t q, int64_t q_begin, int64_t q_n,
+                                                        const double* __restrict__ meta, uint64_t* __restrict__ keys,
+                                                        int32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= q_n) return;
+  keys[i] = morton_key(q.x[q_begin + i], q.y[q_begin + i], q.z[q_begin + i], meta);
+  vals[i] = (int32_t)i;
+}
+
+static size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct GatherWs {
+  size_t keys_a, keys_b, vals_a, order, temp, temp_bytes, nbr, rad, total;
+  int64_t chunk;
+};
+
+// workspace: query sort buffers, then one chunk of neighbour rows
+static int gather_layout(int64_t q_n, int k, int64_t chunk, GatherWs* L) {
+  size_t tb = 0;
+  const int nn = (int)(q_n > 1 ? q_n : 1);
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                                    (const int32_t*)nullptr, (int32_t*)nullptr, nn, 0, 64,
+                                                    (hipStream_t)0);
+  if (e != hipSuccess) return set_error(ALB_ERR_HIP, "alb_gather: sort size query failed: %s", hipGetErrorString(e));
+  size_t off = 0;
+  L->keys_a = off; off = al256(off + 8 * (size_t)nn);
+  L->keys_b = off; off = al256(off + 8 * (size_t)nn);
+  L->vals_a = off; off = al256(off + 4 * (size_t)nn);
+  L->order = off; off = al256(off + 4 * (size_t)nn);
+  L->temp = off; off = al256(off + tb);
+  L->temp_bytes = tb;
+  L->chunk = chunk;
+  L->nbr = off; off = al256(off + (size_t)(chunk > 0 ? chunk : 1) * (size_t)k * sizeof(int32_t));
+  L->rad = off; off = al256(off + (size_t)(chunk > 0 ? chunk : 1) * sizeof(double));
+  L->total = off;
+  return ALB_OK;
 }
 
 }  // namespace alb
@@ -400,8 +445,11 @@ static int64_t chunk_for(int64_t q_n, size_t ws_bytes, int k) {
 extern "C" int alb_gather_workspace_size(int64_t q_n, int32_t k, size_t* bytes) {
   using namespace alb;
   if (!bytes || q_n < 0 || k <= 0) return set_error(ALB_ERR_ARG, "alb_gather_workspace_size: bad arguments");
-  const int64_t c = q_n < kChunk ? q_n : kChunk;
-  *bytes = (size_t)(c > 0 ? c : 1) * ((size_t)k * sizeof(int32_t) + sizeof(double)) + 256;
+  if (q_n > 0x7FFFFFF0) return set_error(ALB_ERR_ARG, "alb_gather_workspace_size: more than 2^31 queries");
+  GatherWs L;
+  const int rc = gather_layout(q_n, k, q_n < kChunk ? q_n : kChunk, &L);
+  if (rc != ALB_OK) return rc;
+  *bytes = L.total;
   return ALB_OK;
 }
 
@@ -426,27 +474,45 @@ extern "C" int alb_gather(const alb_scene_t* scene, const alb_map_t* map, int32_
     return set_error(ALB_ERR_ARG, "alb_gather: map not built");
   if ((nbr_out == nullptr) != (radius_out == nullptr))
     return set_error(ALB_ERR_ARG, "alb_gather: nbr_out and radius_out go together");
-  const int64_t chunk = nbr_out ? q_n : chunk_for(q_n, workspace ? workspace_bytes : 0, k);
-  if (chunk <= 0) return set_error(ALB_ERR_WORKSPACE, "alb_gather: workspace too small");
+  if (q_n > 0x7FFFFFF0) return set_error(ALB_ERR_ARG, "alb_gather: more than 2^31 queries");
+  const bool direct = nbr_out != nullptr;
+  GatherWs L;
+  int rc = gather_layout(q_n, k, direct ? 0 : (q_n < kChunk ? q_n : kChunk), &L);
+  if (rc != ALB_OK) return rc;
+  if (!workspace || workspace_bytes < L.total)
+    return set_error(ALB_ERR_WORKSPACE, "alb_gather: workspace too small (%zu < %zu)", workspace_bytes, L.total);
+  char* ws = (char*)workspace;
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return set_error(ALB_ERR_HIP, "alb_gather: no HIP device");
   hipStream_t s = (hipStream_t)stream;
+  // Morton order of the queries on the map's grid
+  uint64_t* keys_a = (uint64_t*)(ws + L.keys_a);
+  uint64_t* keys_b = (uint64_t*)(ws + L.keys_b);
+  int32_t* vals_a = (int32_t*)(ws + L.vals_a);
+  int32_t* order = (int32_t*)(ws + L.order);
+  hipLaunchKernelGGL(query_key_kernel, dim3((unsigned)((q_n + 255) / 256)), dim3(256), 0, s, *q, q_begin, q_n,
+                     map->meta, keys_a, vals_a);
+  size_t tb = L.temp_bytes;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs((void*)(ws + L.temp), tb, keys_a, keys_b, vals_a, order,
+                                                    (int)q_n, 0, map->dims == 2 ? 64 : 63, s);
+  if (e != hipSuccess) return set_error(ALB_ERR_HIP, "alb_gather: query sort failed: %s", hipGetErrorString(e));
+  const int64_t chunk = direct ? q_n : L.chunk;
   for (int64_t c0 = 0; c0 < q_n; c0 += chunk) {
     const int64_t cn = (q_n - c0) < chunk ? (q_n - c0) : chunk;
-    int32_t* nbr = nbr_out ? nbr_out + c0 * k : (int32_t*)workspace;
-    double* rad = radius_out ? radius_out + c0
-                             : (double*)((char*)workspace + (((size_t)chunk * k * sizeof(int32_t) + 7) & ~(size_t)7));
-    KnnArgs ka{*map, *q, q_begin + c0, cn, k, nbr, rad, counters};
+    int32_t* nbr = direct ? nbr_out : (int32_t*)(ws + L.nbr);
+    double* rad = direct ? radius_out : (double*)(ws + L.rad);
+    KnnArgs ka{*map, *q, q_begin, c0, cn, order, direct ? 1 : 0, k, nbr, rad, counters};
     int64_t kb = (cn + kWaves - 1) / kWaves;
     if (kb > (int64_t)cus * 16) kb = (int64_t)cus * 16;
     hipLaunchKernelGGL(knn_kernel, dim3((unsigned)kb), dim3(64 * kWaves), 0, s, ka);
-    ContribArgs ca{*scene, *map, *q, q_begin + c0, cn, photon_type, nbr, rad, waveform, n_bins, win0, win_len};
+    ContribArgs ca{*scene, *map, *q, q_begin, c0, cn, order, direct ? 1 : 0, photon_type, nbr, rad,
+                   waveform, n_bins, win0, win_len};
     int64_t cb = (cn * k + 255) / 256;
     if (cb > (int64_t)cus * 8) cb = (int64_t)cus * 8;
     hipLaunchKernelGGL(contrib_kernel, dim3((unsigned)cb), dim3(256), sizeof(double) * (size_t)win_len, s, ca);
-    const int rc = check_launch("alb_gather");
+    rc = check_launch("alb_gather");
     if (rc != ALB_OK) return rc;
   }
   return ALB_OK;

@@ -7,9 +7,10 @@
 //  * air legs are straight and draw no random numbers, so the sub-step at
 //    which the photon crosses the surface is solved in closed form (FP64);
 //  * in water the sub-step structure matters because the optical depth is a
-//    float32 running sum (core/simulation.py:339-344): the lane replays exactly
-//    that float32 accumulation, one add per sub-step, until the target is
-//    reached or the closed-form boundary sub-step (floor / surface) comes;
+//    float32 running sum (core/simulation.py:339-344): the lane reproduces that
+//    float32 accumulation exactly, jumping a whole float32 binade at a time
+//    (advance_optical_depth), until the target is reached or the closed-form
+//    boundary sub-step (floor / surface) comes;
 //  * events happen at the sub-step index the reference would give them and
 //    use the reference's handler formulas (intersection fraction, Schlick,
 //    Snell with eta = 1/n both ways, FF inverse-CDF sampling with a float32
@@ -105,6 +106,90 @@ __device__ __forceinline__ void rotate(const float R[9], double l0, double l1, d
   s = l0 * (double)R[6]; s = s + l1 * (double)R[7]; s = s + l2 * (double)R[8]; o2 = s;
 }
 
+
+// Exact emulation of up to jmax water sub-steps of the float32 optical-depth
+// update tau = float32(float64(tau) + inc) (core/simulation.py:339-344, inc =
+// alpha * step length) with a stop at the first sub-step where tau >= tau_t.
+// Within one float32 binade [top/2, top) with ulp u, float64(tau) + inc rounds
+// to tau + I (I = the same float64 value for every tau of the binade, because
+// tau is a multiple of u and u = 2^29 float64 ulps), and rounding that to
+// float32 adds round(I/u) * u, again the same for every step unless I/u ends in
+// exactly .5 (a tie, whose rounding depends on the parity of tau).  So a
+// binade is crossed in one jump; the sub-step that leaves the binade and any
+// tie are taken one at a time with the literal formula.  Returns the number
+// of sub-steps taken; *hit tells whether the last one reached tau_t.
+__device__ __forceinline__ int64_t advance_optical_depth(float& tau, float tau_t, double inc, int64_t jmax,
+                                                        bool* hit, unsigned long long& solves) {
+  int64_t j = 0;
+  *hit = false;
+  const double tt = (double)tau_t;
+  while (j < jmax) {
+    ++solves;
+    const double t0 = (double)tau;
+    bool single = !(tau >= 1.17549435e-38f);  // zero (or subnormal): literal step
+    double top = 0.0, I = 0.0, u = 0.0, nu = 0.0;
+    if (!single) {
+      int e;
+      frexpf(tau, &e);  // tau in [2^(e-1), 2^e)
+      top = ldexp(1.0, e);
+      u = ldexp(1.0, e - 24);
+      const double sum = t0 + inc;
+      if (!(sum < top)) {
+        single = true;  // this sub-step leaves the binade
+      } else {
+        I = sum - t0;  // exact
+        const double q = I / u;
+        const double fl = floor(q);
+        const double fr = q - fl;
+        if (fr == 0.5) single = true;
+        else nu = fr > 0.5 ? fl + 1.0 : fl;
+      }
+    }
+    if (single) {
+      tau = (float)(t0 + inc);
+      ++j;
+      if (tau >= tau_t) {
+        *hit = true;
+        return j;
+      }
+      continue;
+    }
+    const int64_t left = jmax - j;
+    if (nu == 0.0) {  // the add no longer changes tau
+      if (t0 >= tt) {
+        ++j;
+        *hit = true;
+        return j;
+      }
+      return jmax;
+    }
+    const double d = nu * u;
+    // sub-steps whose float64 sum stays inside the binade: t0 + c*d + I < top
+    int64_t c = (int64_t)ceil((top - I - t0) / d);
+    if (c < 1) c = 1;
+    while (c > 1 && !(t0 + (double)(c - 1) * d + I < top)) --c;
+    while (t0 + (double)c * d + I < top) ++c;
+    // first sub-step reaching the target: t0 + nt*d >= tau_t, nt >= 1
+    int64_t nt = 1;
+    if (t0 + d < tt) {
+      nt = (int64_t)ceil((tt - t0) / d);
+      if (nt < 1) nt = 1;
+      while (nt > 1 && t0 + (double)(nt - 1) * d >= tt) --nt;
+      while (t0 + (double)nt * d < tt) ++nt;
+    }
+    if (nt <= c && nt <= left) {
+      tau = (float)(t0 + (double)nt * d);
+      j += nt;
+      *hit = true;
+      return j;
+    }
+    const int64_t take = c < left ? c : left;
+    tau = (float)(t0 + (double)take * d);
+    j += take;
+  }
+  return j;
+}
+
 __device__ __forceinline__ float tau_target(double u) {
   const float uf = (float)u;
   return (float)(-log((double)uf));
@@ -125,7 +210,7 @@ __global__ __launch_bounds__(256) void trace_kernel(alb_scene_t sc, alb_rng_t rn
   const double two_pi = 2.0 * ALB_PI;
   double px = 0.0, py = 0.0, pz = 0.0, dx = 0.0, dy = -1.0, dz = 0.0;
   unsigned long long c_paths = alive ? 1 : 0, c_air = 0, c_ws = 0, c_sc = 0, c_en = 0, c_ex = 0,
-                     c_fl = 0, c_exh = 0;
+                     c_fl = 0, c_exh = 0, c_solve = 0;
 
   if (alive) {
     if (FWD) {
@@ -241,16 +326,10 @@ __global__ __launch_bounds__(256) void trace_kernel(alb_scene_t sc, alb_rng_t rn
         const double len = norm3(Dx, Dy, Dz);
         const double inc = alpha * len;
         if (!__builtin_isinf(tau_t)) {
-          // float32 optical-depth accumulation, one rounding per sub-step
-          float t_ = tau;
-          for (; j < jmax; ++j) {
-            t_ = (float)((double)t_ + inc);
-            if (t_ >= tau_t) {
-              scattered = true;
-              break;
-            }
-          }
-          tau = t_;
+          bool hit = false;
+          const int64_t taken = advance_optical_depth(tau, tau_t, inc, jmax, &hit, c_solve);
+          scattered = hit;
+          j = hit ? taken - 1 : taken;  // index of the scattering sub-step
         } else {
           j = jmax;
         }
@@ -399,6 +478,8 @@ __global__ __launch_bounds__(256) void trace_kernel(alb_scene_t sc, alb_rng_t rn
       const unsigned long long tot = wave_sum(v[c]);
       if (lane == 0 && tot) atomicAdd(&ctr[c], tot);
     }
+    const unsigned long long ts = wave_sum(c_solve);
+    if (lane == 0 && ts) atomicAdd(&ctr[ALB_CTR_DEPTH_SOLVES], ts);
   }
 }
 

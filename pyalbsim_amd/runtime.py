@@ -225,13 +225,10 @@ def gather(dscene: DeviceScene, dmap: DeviceMap, ptype: int, queries: Sink, q_n:
     L = load_library()
     win0, win_len = window
     s = stream if stream is not None else _stream_ptr()
-    ws = None
-    ws_bytes = 0
-    if nbr_out is None:
-        b = C.c_size_t(0)
-        _check(L.alb_gather_workspace_size(q_n, dscene.model.k, C.byref(b)), "alb_gather_workspace_size")
-        ws_bytes = b.value
-        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=waveform.device)
+    b = C.c_size_t(0)
+    _check(L.alb_gather_workspace_size(q_n, dscene.model.k, C.byref(b)), "alb_gather_workspace_size")
+    ws_bytes = b.value
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=waveform.device)
     rc = L.alb_gather(C.byref(dscene.s), C.byref(dmap.struct), ptype, C.byref(queries.struct), q_begin, q_n,
                       _p(waveform), int(waveform.numel()), int(win0), int(win_len), _p(nbr_out), _p(radius_out),
                       _p(ws), ws_bytes, _p(counters), C.c_void_p(s))
