@@ -168,6 +168,9 @@ def main(argv=None):
         "gather_gbs": g_bytes / t_gather / 1e9 if t_gather > 0 else None,
         "gather_candidates_per_query": cand / max(nq, 1.0),
         "gather_nodes_per_query": nodes / max(nq, 1.0),
+        "knn_steps_per_query": float(cb[abi.CTR_KNN_STEPS]) / max(nq, 1.0),
+        "knn_sorts_per_query": float(cb[abi.CTR_KNN_SORTS]) / max(nq, 1.0),
+        "knn_descent_per_query": float(cb[abi.CTR_KNN_DESCENT]) / max(nq, 1.0),
         "water_steps_per_path": float(cf[abi.CTR_WATER_STEPS]) / max(float(cf[abi.CTR_PATHS]), 1.0),
         "depth_solves_per_path": float(cf[abi.CTR_DEPTH_SOLVES]) / max(float(cf[abi.CTR_PATHS]), 1.0),
         "cpu_baseline": None,

@@ -81,6 +81,9 @@ enum alb_counter {
   ALB_CTR_NODES = 10,        /* tree nodes visited                          */
   ALB_CTR_OVERFLOW = 11,     /* gather queries that exceeded a kNN buffer   */
   ALB_CTR_DEPTH_SOLVES = 12, /* optical-depth binade jumps and single steps */
+  ALB_CTR_KNN_STEPS = 13,    /* k-NN traversal steps (node batches)          */
+  ALB_CTR_KNN_SORTS = 14,    /* candidate-buffer sorts                       */
+  ALB_CTR_KNN_DESCENT = 15,  /* seed-descent levels                          */
   ALB_N_COUNTERS = 16
 };
 

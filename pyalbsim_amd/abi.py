@@ -24,9 +24,11 @@ PASS_FORWARD = 0
 PASS_BACKWARD = 1
 
 CTR_PATHS, CTR_AIR_LEGS, CTR_WATER_STEPS, CTR_SCATTER, CTR_ENTER, CTR_EXIT, CTR_FLOOR, \
-    CTR_REPLAY_EXHAUSTED, CTR_QUERIES, CTR_CANDIDATES, CTR_NODES, CTR_OVERFLOW, CTR_DEPTH_SOLVES = range(13)
+    CTR_REPLAY_EXHAUSTED, CTR_QUERIES, CTR_CANDIDATES, CTR_NODES, CTR_OVERFLOW, CTR_DEPTH_SOLVES, \
+    CTR_KNN_STEPS, CTR_KNN_SORTS, CTR_KNN_DESCENT = range(16)
 COUNTER_NAMES = ["paths", "air_legs", "water_steps", "scatter", "enter", "exit", "floor",
-                 "replay_exhausted", "queries", "candidates", "nodes", "overflow", "depth_solves"]
+                 "replay_exhausted", "queries", "candidates", "nodes", "overflow", "depth_solves",
+                 "knn_steps", "knn_sorts", "knn_descent"]
 
 
 class Scene(C.Structure):

@@ -8,18 +8,21 @@
 // Two kernels per chunk of queries:
 //  knn_kernel — one wavefront per query:
 //   1. seed: descend the radix tree into the child whose box is nearer while
-//      it still holds >= k records; the (at most kSeed) records reached give a
-//      first bound theta = k-th smallest squared distance;
-//   2. traversal, up to 64 nodes per step (one per lane): each node carries its
-//      children's exact float32 boxes, so a child is dropped, pushed (with its
-//      box distance, re-checked when popped) or, at <= 64 records, scanned;
-//      scanned records with d2 <= theta go to an LDS candidate buffer, four
-//      record ranges at a time so that their loads are in flight together;
-//   3. a full buffer is bitonic-sorted by (d2, index) and cut back to k, which
-//      tightens theta; the final sort gives the neighbour set in (distance,
-//      index) order — bit-identical to brute force, because squared distances
-//      are evaluated in FP64 in one fixed order and box distances are, by
-//      monotone rounding, never larger than those of the records inside.
+//      it still holds >= k records, stacking every child not taken; the (at
+//      most kSeed) records reached give a first bound theta = k-th smallest
+//      squared distance;
+//   2. traversal of the stacked subtrees, up to 64 nodes per step (one per
+//      lane): each node carries its children's exact float32 boxes, so a child
+//      is dropped, pushed (with its box distance, re-checked when popped) or,
+//      at <= 64 records, scanned; scanned records with d2 <= theta go to an
+//      LDS candidate buffer, four record ranges at a time so that their loads
+//      are in flight together;
+//   3. a full buffer is cut back to its k smallest (d2, index) entries by a
+//      wave-wide radix select (8-bit digits of the FP64 bit patterns, ties by
+//      index), which tightens theta.  The result is the brute-force neighbour
+//      set bit for bit, because squared distances are evaluated in FP64 in one
+//      fixed order and box distances are, by monotone rounding, never larger
+//      than those of the records inside.
 //  Queries are first sorted by the Morton key of their position on the map's
 //  grid, so neighbouring wavefronts walk the same nodes and buckets (L2 hits).
 //  contrib_kernel — one thread per (query, neighbour): energy multiplier,
@@ -38,7 +41,7 @@ namespace alb {
 
 constexpr int kWaves = 4;         // wavefronts per k-NN block
 constexpr int kStack = 256;       // per-wave node stack (LDS)
-constexpr int kSeed = 192;        // records of the descent seed
+constexpr int kSeed = 192;        // records of the descent seed (<= kBuf)
 constexpr int kBuf = 256;         // per-wave candidate buffer (LDS), power of two
 constexpr int kBucket = 64;       // nodes with <= kBucket records are scanned
 constexpr int kMaxWindow = 2048;  // LDS-privatised waveform bins
@@ -129,6 +132,109 @@ __device__ void wave_sort(double* d, int32_t* idx, int n, uint32_t lane) {
   }
 }
 
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t o = __shfl_up(v, off);
+    if ((int)lane >= off) v += o;
+  }
+  return v;
+}
+
+// Radix selection over at most kBuf buffer entries: the key of entry t is
+// the bit pattern of d[t] (non-negative doubles order like their bits) when
+// by_index is false, else idx[t] restricted to entries with d[t] == eq.
+// Returns the key of rank kk (1-based) among the eligible entries.
+__device__ uint64_t wave_select(const double* d, const int32_t* idx, int n, int kk, bool by_index, double eq,
+                                uint32_t* hist, uint32_t lane) {
+  uint64_t prefix = 0, pmask = 0;
+  const int top = by_index ? 24 : 56;
+  for (int shift = top; shift >= 0; shift -= 8) {
+    for (int t = (int)lane; t < 256; t += 64) hist[t] = 0u;
+    wave_sync();
+    for (int t = (int)lane; t < n; t += 64) {
+      bool ok = true;
+      uint64_t key;
+      if (by_index) {
+        ok = d[t] == eq;
+        key = (uint64_t)(uint32_t)idx[t];
+      } else {
+        key = (uint64_t)__double_as_longlong(d[t]);
+      }
+      if (ok && (key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+    }
+    wave_sync();
+    const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2], h3 = hist[4 * lane + 3];
+    const uint32_t loc = h0 + h1 + h2 + h3;
+    const uint32_t incl = wave_incl_scan(loc, lane);
+    const uint32_t excl = incl - loc;
+    const uint64_t bal = ballot(excl < (uint32_t)kk && (uint32_t)kk <= incl);
+    const int L = __ffsll((long long)bal) - 1;
+    const uint32_t e = __shfl(excl, L), g0 = __shfl(h0, L), g1 = __shfl(h1, L), g2 = __shfl(h2, L);
+    uint32_t digit = 4u * (uint32_t)L, before_cnt = e;
+    if ((uint32_t)kk > e + g0) {
+      digit += 1; before_cnt += g0;
+      if ((uint32_t)kk > e + g0 + g1) {
+        digit += 1; before_cnt += g1;
+        if ((uint32_t)kk > e + g0 + g1 + g2) {
+          digit += 1; before_cnt += g2;
+        }
+      }
+    }
+    kk -= (int)before_cnt;
+    prefix |= (uint64_t)digit << shift;
+    pmask |= (uint64_t)0xFF << shift;
+    wave_sync();
+  }
+  return prefix;
+}
+
+// Keep the k smallest entries by (d, idx) at the front of the buffer (any
+// order) and return the k-th smallest d.
+__device__ double wave_keep_k(double* d, int32_t* idx, int n, int k, uint32_t* hist, uint32_t lane) {
+  const double th = __longlong_as_double((long long)wave_select(d, idx, n, k, false, 0.0, hist, lane));
+  int lt = 0, eq = 0;
+  for (int t = (int)lane; t < n; t += 64) {
+    lt += d[t] < th;
+    eq += d[t] == th;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    lt += __shfl_xor(lt, off);
+    eq += __shfl_xor(eq, off);
+  }
+  int32_t icut = INT_MAX;  // entries equal to th are kept up to this index
+  if (lt + eq > k) icut = (int32_t)wave_select(d, idx, n, k - lt, true, th, hist, lane);
+  double vd[kBuf / 64];
+  int32_t vi[kBuf / 64];
+  bool keep[kBuf / 64];
+#pragma unroll
+  for (int u = 0; u < kBuf / 64; ++u) {
+    const int t = (int)lane + 64 * u;
+    keep[u] = false;
+    if (t < n) {
+      vd[u] = d[t];
+      vi[u] = idx[t];
+      keep[u] = vd[u] < th || (vd[u] == th && vi[u] <= icut);
+    }
+  }
+  wave_sync();
+  int base = 0;
+#pragma unroll
+  for (int u = 0; u < kBuf / 64; ++u) {
+    const uint64_t m = ballot(keep[u]);
+    if (keep[u]) {
+      const int slot = base + (int)mask_rank(m);
+      d[slot] = vd[u];
+      idx[slot] = vi[u];
+    }
+    base += __popcll(m);
+  }
+  wave_sync();
+  return th;
+}
+
 // first index in [0, n) whose key is >= key (n if none); wave-cooperative
 __device__ int64_t wave_lower_bound(const uint64_t* __restrict__ keys, int64_t n, uint64_t key, uint32_t lane) {
   int64_t lo = 0, hi = n;
@@ -158,6 +264,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   __shared__ int32_t bi_s[kWaves][kBuf];
   __shared__ int32_t rlo_s[kWaves][128];
   __shared__ int32_t rhi_s[kWaves][128];
+  __shared__ uint32_t hist_s[kWaves][256];
 
   const uint32_t lane = lane_id();
   const int w = threadIdx.x >> 6;
@@ -167,11 +274,12 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   int32_t* bi = bi_s[w];
   int32_t* rlo = rlo_s[w];
   int32_t* rhi = rhi_s[w];
+  uint32_t* hist = hist_s[w];
   const alb_map_t& m = a.map;
   const Node* __restrict__ nodes = (const Node*)m.nodes;
   const int64_t n = m.n;
   const int k = a.k;
-  unsigned long long c_q = 0, c_cand = 0, c_nodes = 0, c_ovf = 0;
+  unsigned long long c_q = 0, c_cand = 0, c_nodes = 0, c_ovf = 0, c_steps = 0, c_sorts = 0, c_desc = 0;
 
   for (int64_t si = (int64_t)blockIdx.x * kWaves + w; si < a.q_n; si += (int64_t)gridDim.x * kWaves) {
     const int64_t qrel = a.order[a.c0 + si];
@@ -180,36 +288,50 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     const double qx = a.q.x[qg], qy = a.q.y[qg], qz = a.q.z[qg];
     c_q++;
 
-    // ---- 1. seed
+    // ---- 1. seed descent; every child not taken goes on the stack (with its
+    // box distance), so that stack + seed cover the whole map.  Stack entries
+    // are node indices, or ~record for a single record.
+    int sp = 0;
     int32_t s0 = 0, s1 = (int32_t)n - 1;
     if (n > kSeed) {
       int32_t node = 0;
       for (;;) {
         const Node nd = nodes[node];
         c_nodes++;
+        c_desc++;
         const double dl = box_dist2(qx, qy, qz, nd.lb);
         const double dr = box_dist2(qx, qy, qz, nd.rb);
         const bool go_left = dl <= dr;
         const int32_t cnt_near = go_left ? nd.split - nd.first + 1 : nd.last - nd.split;
         const bool near_leaf = go_left ? (nd.leaf_mask & 1) : (nd.leaf_mask & 2);
+        const bool far_leaf = go_left ? (nd.leaf_mask & 2) : (nd.leaf_mask & 1);
+        const int32_t far_id = go_left ? nd.split + 1 : nd.split;
+        bool stop = true;
         if (cnt_near >= k && !near_leaf && cnt_near > kSeed) {
-          node = go_left ? nd.split : nd.split + 1;
-          continue;
-        }
-        if (cnt_near >= k && !near_leaf) {  // the near child fits the seed buffer
+          stop = false;
+        } else if (cnt_near >= k && !near_leaf) {  // the near child fits the seed buffer
           s0 = go_left ? nd.first : nd.split + 1;
           s1 = go_left ? nd.split : nd.last;
         } else if (nd.last - nd.first + 1 <= kSeed) {
           s0 = nd.first;
           s1 = nd.last;
-        } else if (go_left) {
+        } else if (go_left) {  // near child (< k records) plus the first records of the far one
           s0 = nd.first;
           s1 = nd.first + kSeed - 1;
         } else {
           s0 = nd.last - kSeed + 1;
           s1 = nd.last;
         }
-        break;
+        const bool far_in_seed = stop && (nd.last - nd.first + 1 <= kSeed);
+        if (!far_in_seed) {
+          if (lane == 0) {
+            sn[sp] = far_leaf ? ~far_id : far_id;
+            sd[sp] = go_left ? dr : dl;
+          }
+          ++sp;
+        }
+        if (stop) break;
+        node = go_left ? nd.split : nd.split + 1;
       }
     }
     const int W = s1 - s0 + 1;
@@ -219,34 +341,36 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
       bi[t] = r;
     }
     c_cand += (unsigned long long)W;
-    wave_sort(bd, bi, W, lane);
+    wave_sync();
+    double theta = wave_keep_k(bd, bi, W, k, hist, lane);
     int buf_n = k;
-    double theta = bd[k - 1];
 
-    // ---- 2. pruned traversal
-    if (W < n) {
-      if (lane == 0) {
-        sn[0] = 0;
-        sd[0] = 0.0;
+    // ---- 2. pruned traversal of the stacked subtrees
+    while (sp > 0) {
+      c_steps++;
+      const int p = (sp >= kStack - 128) ? 1 : (sp < 64 ? sp : 64);
+      bool have = (int)lane < p;
+      int32_t node = 0;
+      if (have) {
+        node = sn[sp - 1 - (int)lane];
+        have = sd[sp - 1 - (int)lane] <= theta;
       }
-      int sp = 1;
+      sp -= p;
       wave_sync();
-      while (sp > 0) {
-        const int p = (sp >= kStack - 128) ? 1 : (sp < 64 ? sp : 64);
-        bool have = (int)lane < p;
-        int32_t node = 0;
-        if (have) {
-          node = sn[sp - 1 - (int)lane];
-          have = sd[sp - 1 - (int)lane] <= theta;
-        }
-        sp -= p;
-        wave_sync();
-        // per child (0 = farther, 1 = nearer): 1 = push, 2 = scan
-        int kind0 = 0, kind1 = 0;
-        int32_t lo0 = 0, hi0 = -1, lo1 = 0, hi1 = -1, id0 = 0, id1 = 0;
-        double md0 = 0.0, md1 = 0.0;
-        if (have) {
-          const Node nd = nodes[node];
+      // per child (0 = farther, 1 = nearer): 1 = push, 2 = scan
+      int kind0 = 0, kind1 = 0;
+      int32_t lo0 = 0, hi0 = -1, lo1 = 0, hi1 = -1, id0 = 0, id1 = 0;
+      double md0 = 0.0, md1 = 0.0;
+      if (have && node < 0) {  // a single record
+        kind0 = 2;
+        lo0 = hi0 = ~node;
+      } else if (have) {
+        const Node nd = nodes[node];
+        if (nd.last - nd.first + 1 <= kBucket) {  // small subtree from the descent: scan it
+          kind0 = 2;
+          lo0 = nd.first;
+          hi0 = nd.last;
+        } else {
           const double dl = box_dist2(qx, qy, qz, nd.lb);
           const double dr = box_dist2(qx, qy, qz, nd.rb);
           const bool left_near = dl <= dr;
@@ -265,90 +389,92 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
             kind1 = (nleaf || nh - nl + 1 <= kBucket) ? 2 : 1;
           }
         }
-        c_nodes += (unsigned long long)__popcll(ballot(have));
-        const uint64_t pa = ballot(kind0 == 1), pb = ballot(kind1 == 1);
-        const int npush = __popcll(pa) + __popcll(pb);
-        if (sp + npush > kStack) {  // excluded by the pop policy; refuse rather than corrupt
-          c_ovf++;
-          break;
+      }
+      c_nodes += (unsigned long long)__popcll(ballot(have));
+      const uint64_t pa = ballot(kind0 == 1), pb = ballot(kind1 == 1);
+      const int npush = __popcll(pa) + __popcll(pb);
+      if (sp + npush > kStack) {  // excluded by the pop policy; refuse rather than corrupt
+        c_ovf++;
+        break;
+      }
+      {
+        int off = sp + (int)mask_rank(pa) + (int)mask_rank(pb);
+        if (kind0 == 1) {
+          sn[off] = id0;
+          sd[off] = md0;
+          ++off;
         }
-        {
-          int off = sp + (int)mask_rank(pa) + (int)mask_rank(pb);
-          if (kind0 == 1) {
-            sn[off] = id0;
-            sd[off] = md0;
-            ++off;
-          }
-          if (kind1 == 1) {
-            sn[off] = id1;
-            sd[off] = md1;
-          }
+        if (kind1 == 1) {
+          sn[off] = id1;
+          sd[off] = md1;
         }
-        sp += npush;
-        const uint64_t ra = ballot(kind0 == 2), rb = ballot(kind1 == 2);
-        const int nr = __popcll(ra) + __popcll(rb);
-        {
-          int off = (int)mask_rank(ra) + (int)mask_rank(rb);
-          if (kind0 == 2) {
-            rlo[off] = lo0;
-            rhi[off] = hi0;
-            ++off;
-          }
-          if (kind1 == 2) {
-            rlo[off] = lo1;
-            rhi[off] = hi1;
-          }
+      }
+      sp += npush;
+      const uint64_t ra = ballot(kind0 == 2), rb = ballot(kind1 == 2);
+      const int nr = __popcll(ra) + __popcll(rb);
+      {
+        int off = (int)mask_rank(ra) + (int)mask_rank(rb);
+        if (kind0 == 2) {
+          rlo[off] = lo0;
+          rhi[off] = hi0;
+          ++off;
         }
-        wave_sync();
-        // scan the ranges four at a time: all twelve coordinate loads issue together
-        for (int r0 = 0; r0 < nr; r0 += 4) {
-          double d2v[4];
-          bool tk[4];
-          int32_t ix[4];
+        if (kind1 == 2) {
+          rlo[off] = lo1;
+          rhi[off] = hi1;
+        }
+      }
+      wave_sync();
+      // scan the ranges four at a time: all twelve coordinate loads issue together
+      for (int r0 = 0; r0 < nr; r0 += 4) {
+        double d2v[4];
+        bool tk[4];
+        int32_t ix[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            tk[u] = false;
-            d2v[u] = 0.0;
-            ix[u] = 0;
-            if (r0 + u < nr) {
-              const int32_t rl = rlo[r0 + u], rh = rhi[r0 + u];
-              const int32_t idx = rl + (int32_t)lane;
-              ix[u] = idx;
-              c_cand += (unsigned long long)(rh - rl + 1);
-              if (idx <= rh && !(idx >= s0 && idx <= s1)) {
-                d2v[u] = dist2(qx, qy, qz, m.x[idx], m.y[idx], m.z[idx]);
-                tk[u] = d2v[u] <= theta;
-              }
+        for (int u = 0; u < 4; ++u) {
+          tk[u] = false;
+          d2v[u] = 0.0;
+          ix[u] = 0;
+          if (r0 + u < nr) {
+            const int32_t rl = rlo[r0 + u], rh = rhi[r0 + u];
+            const int32_t idx = rl + (int32_t)lane;
+            ix[u] = idx;
+            c_cand += (unsigned long long)(rh - rl + 1);
+            if (idx <= rh && !(idx >= s0 && idx <= s1)) {
+              d2v[u] = dist2(qx, qy, qz, m.x[idx], m.y[idx], m.z[idx]);
+              tk[u] = d2v[u] <= theta;
             }
           }
+        }
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const uint64_t mt = ballot(tk[u]);
-            if (mt == 0) continue;
-            bool take = tk[u];
-            if (buf_n + __popcll(mt) > kBuf) {
-              wave_sort(bd, bi, buf_n, lane);
-              buf_n = k;
-              theta = bd[k - 1];
-              take = take && d2v[u] <= theta;
-            }
-            const uint64_t mt2 = ballot(take);
-            if (take) {
-              const int slot = buf_n + (int)mask_rank(mt2);
-              bd[slot] = d2v[u];
-              bi[slot] = ix[u];
-            }
-            buf_n += __popcll(mt2);
-            wave_sync();
+        for (int u = 0; u < 4; ++u) {
+          const uint64_t mt = ballot(tk[u]);
+          if (mt == 0) continue;
+          bool take = tk[u];
+          if (buf_n + __popcll(mt) > kBuf) {
+            c_sorts++;
+            theta = wave_keep_k(bd, bi, buf_n, k, hist, lane);
+            buf_n = k;
+            take = take && d2v[u] <= theta;
           }
+          const uint64_t mt2 = ballot(take);
+          if (take) {
+            const int slot = buf_n + (int)mask_rank(mt2);
+            bd[slot] = d2v[u];
+            bi[slot] = ix[u];
+          }
+          buf_n += __popcll(mt2);
+          wave_sync();
         }
       }
     }
 
-    // ---- 3. final neighbour set in (distance, index) order
-    wave_sort(bd, bi, buf_n, lane);
+    // ---- 3. the k nearest: cut the buffer back to k; in (distance, index)
+    // order when the caller asked for the neighbour sets
+    const double r2 = buf_n > k ? wave_keep_k(bd, bi, buf_n, k, hist, lane) : theta;
+    if (a.direct) wave_sort(bd, bi, k, lane);
     for (int t = (int)lane; t < k; t += 64) a.nbr[qi * k + t] = bi[t];
-    if (lane == 0) a.radius[qi] = sqrt(bd[k - 1]);
+    if (lane == 0) a.radius[qi] = sqrt(r2);
     wave_sync();
   }
   if (a.ctr && lane == 0) {
@@ -356,6 +482,9 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     if (c_cand) atomicAdd(&a.ctr[ALB_CTR_CANDIDATES], c_cand);
     if (c_nodes) atomicAdd(&a.ctr[ALB_CTR_NODES], c_nodes);
     if (c_ovf) atomicAdd(&a.ctr[ALB_CTR_OVERFLOW], c_ovf);
+    if (c_steps) atomicAdd(&a.ctr[ALB_CTR_KNN_STEPS], c_steps);
+    if (c_sorts) atomicAdd(&a.ctr[ALB_CTR_KNN_SORTS], c_sorts);
+    if (c_desc) atomicAdd(&a.ctr[ALB_CTR_KNN_DESCENT], c_desc);
   }
 }
 
