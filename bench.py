@@ -110,18 +110,19 @@ def main(argv=None):
     ds = runtime.DeviceScene(model)
     key = (0x5EED, 0)
 
-    def step(events=None):
-        return simulate(model, n_f, n_b, key, dscene=ds, events=events)
+    def step(events=None, kernel_events=None):
+        return simulate(model, n_f, n_b, key, dscene=ds, events=events, kernel_events=kernel_events)
 
     for i in range(args.warmup):
         step()
         print(f"warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     events = {}
+    kev = {}
     t0 = time.perf_counter()
     res = None
     for i in range(args.steps):
-        res = step(events)
+        res = step(events, kev)
         print(f"step {i + 1}/{args.steps} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -141,17 +142,27 @@ def main(argv=None):
     k = model.k
     qc = res.query_counts
     g_flops = sum(qc[t] * (4 + (7 + M_TYPE[t]) * k) for t in range(4)) + 15 * cand
-    g_bytes = 12 * cand + 20 * k * nq + 40 * nodes
+    # k-NN kernel: candidate coordinates (12 B), neighbour indices written (4 B x k), node lines (64 B)
+    knn_bytes = 12 * cand + 4 * k * nq + 64 * nodes
+    knn_ms = [s_.elapsed_time(e_) for s_, e_ in kev["knn"]]
+    contrib_ms = [s_.elapsed_time(e_) for s_, e_ in kev["contrib"]]
+    knn_launches_per_step = len(knn_ms) / args.steps
+    knn_avg_ms = sum(knn_ms) / len(knn_ms)
     t_trace = (phases["forward"] + phases["backward"]) / 1e3
-    t_gather = phases["gather"] / 1e3
-    if t_gather >= t_trace:
-        roof = {"kernel": "gather_kernel", "bound": "hbm", "achieved": g_bytes / t_gather / 1e9,
-                "peak": PEAK_HBM_GBS, "unit": "GB/s"}
+    t_knn = sum(knn_ms) / args.steps / 1e3
+    if t_knn >= t_trace:
+        # bytes per launch: the step's algorithmic bytes spread over its launches
+        roof = {"kernel": "knn_kernel", "bound": "hbm",
+                "achieved": knn_bytes / knn_launches_per_step / (knn_avg_ms / 1e3) / 1e9,
+                "peak": PEAK_HBM_GBS, "unit": "GB/s", "avg_launch_ms": knn_avg_ms,
+                "launches_per_step": knn_launches_per_step}
     else:
         roof = {"kernel": "trace_kernel", "bound": "valu", "achieved": (flops_fwd + flops_bwd) / t_trace / 1e12,
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = None
+    g_bytes = knn_bytes + (20 + 48) * k * nq
+    t_gather = phases["gather"] / 1e3
 
     out = {
         "metric": METRIC, "value": value, "unit": "photon paths/s", "n_gpus": 1, "steps": args.steps,
@@ -163,6 +174,7 @@ def main(argv=None):
                    "forward_photons": n_f, "backward_photons": n_b, "k": k,
                    "map_records": res.map_counts, "gather_queries": qc},
         "phases_ms": phases,
+        "kernel_ms_per_step": {"knn": sum(knn_ms) / args.steps, "contrib": sum(contrib_ms) / args.steps},
         "roofline": roof,
         "trace_tflops": (flops_fwd + flops_bwd) / t_trace / 1e12,
         "gather_gbs": g_bytes / t_gather / 1e9 if t_gather > 0 else None,

@@ -120,6 +120,9 @@ typedef struct alb_scene {
   int32_t phase_n;          /* entries of the phase tables (18000)                 */
   const double* phase_theta;/* [phase_n] scattering-angle grid                     */
   const double* phase_cdf;  /* [phase_n] normalised CDF                            */
+  const double* phase_mass; /* [phase_n] CDF mass of each scattering-angle bin used
+                               by the gather (bin i spans [i, i+1) * float32(pi -
+                               eps)/phase_n), evaluated with np.interp semantics */
 } alb_scene_t;
 
 typedef struct alb_rng {
@@ -228,13 +231,35 @@ int alb_gather_workspace_size(int64_t q_n, int32_t k, size_t* bytes);
  * nbr_out (optional, [q_n * k] int32) receives each query's neighbour set as
  * sorted-map indices in (distance, index) order and radius_out ([q_n]
  * double) the k-th neighbour distance.  Queries are processed in Morton order
- * of their positions.  Replaces core/simulation.py:132-195 (sample_photons). */
+ * of their positions.  Same result as alb_gather_order + alb_knn +
+ * alb_contrib.  Replaces core/simulation.py:132-195 (sample_photons). */
 int alb_gather(const alb_scene_t* scene, const alb_map_t* map,
                int32_t photon_type, const alb_queries_t* q, int64_t q_begin,
                int64_t q_n, double* waveform, int64_t n_bins, int32_t win0,
                int32_t win_len, int32_t* nbr_out, double* radius_out,
                void* workspace, size_t workspace_bytes,
                unsigned long long* counters, void* stream);
+
+/* The three stages of alb_gather, for callers that time or overlap them.
+ * alb_gather_order: order[q_n] (int32) = query offsets sorted by the Morton
+ *   key of their positions on the map's grid; workspace from
+ *   alb_gather_workspace_size.
+ * alb_knn: the k nearest records of the queries in sort slots [c0, c0 + cn):
+ *   row r (= slot - c0, or the query offset when rows_by_query != 0) of
+ *   nbr[rows * k] gets the neighbour indices, radius[r] the k-th distance and
+ *   knorm[r] the kernel normalisation 1/(pi r^2), or 1/(4/3 pi r^3) for
+ *   photon_type ALB_SCATTER.  Neighbours are in (distance, index) order when
+ *   rows_by_query != 0, else in no particular order.
+ * alb_contrib: adds the energies of those rows to the waveform. */
+int alb_gather_order(const alb_map_t* map, const alb_queries_t* q, int64_t q_begin, int64_t q_n,
+                     int32_t* order, void* workspace, size_t workspace_bytes, void* stream);
+int alb_knn(const alb_map_t* map, int32_t photon_type, int32_t k, const alb_queries_t* q,
+            int64_t q_begin, const int32_t* order, int64_t c0, int64_t cn, int32_t rows_by_query,
+            int32_t* nbr, double* radius, double* knorm, unsigned long long* counters, void* stream);
+int alb_contrib(const alb_scene_t* scene, const alb_map_t* map, int32_t photon_type,
+                const alb_queries_t* q, int64_t q_begin, const int32_t* order, int64_t c0, int64_t cn,
+                int32_t rows_by_query, const int32_t* nbr, const double* knorm, double* waveform,
+                int64_t n_bins, int32_t win0, int32_t win_len, void* stream);
 
 #ifdef __cplusplus
 }

@@ -42,6 +42,7 @@ class Scene(C.Structure):
         ("absorption", C.c_double), ("alpha_l", C.c_float), ("omega0", C.c_float),
         ("n_substeps", C.c_int64), ("steps", C.c_int32), ("mult", C.c_int32), ("k", C.c_int32),
         ("phase_n", C.c_int32), ("phase_theta", C.c_void_p), ("phase_cdf", C.c_void_p),
+        ("phase_mass", C.c_void_p),
     ]
 
 
@@ -68,7 +69,7 @@ class Map(C.Structure):
                 ("nodes", C.c_void_p)]
 
 
-def make_scene(model, theta_ptr: int, cdf_ptr: int) -> Scene:
+def make_scene(model, theta_ptr: int, cdf_ptr: int, mass_ptr: int = 0) -> Scene:
     """Pack a SceneModel; the phase-table pointers are device or host addresses."""
     s = Scene()
     s.y_ws = model.y_ws
@@ -101,6 +102,7 @@ def make_scene(model, theta_ptr: int, cdf_ptr: int) -> Scene:
     s.phase_n = len(model.phase_theta)
     s.phase_theta = theta_ptr
     s.phase_cdf = cdf_ptr
+    s.phase_mass = mass_ptr
     return s
 
 

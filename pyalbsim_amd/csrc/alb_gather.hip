@@ -58,8 +58,10 @@ struct KnnArgs {
   const int32_t* order;
   int32_t direct;
   int32_t k;
+  int32_t volume;    // kernel normalisation by volume (scatter map) instead of area
   int32_t* nbr;      // [rows * k]
   double* radius;    // [rows]
+  double* knorm;     // [rows]
   unsigned long long* ctr;
 };
 
@@ -72,7 +74,7 @@ struct ContribArgs {
   int32_t direct;
   int32_t type;
   const int32_t* nbr;
-  const double* radius;
+  const double* knorm;
   double* wf;
   int64_t n_bins;
   int32_t win0, win_len;
@@ -488,6 +490,28 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   }
 }
 
+// kernel normalisation per row: 1 / (pi r_k^2), or 1 / ((4/3) pi r_k^3) for
+// the scatter map (core/simulation.py:163-187); a separate pass keeps the FP64
+// pow out of the traversal kernel's register budget
+__global__ __launch_bounds__(256) void knorm_kernel(const double* __restrict__ radius, double* __restrict__ knorm,
+                                                    int64_t rows, int32_t volume) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const double rk = radius[r];
+  const double ksize = volume ? (4.0 / 3.0) * ALB_PI * pow(rk, 3.0) : ALB_PI * (rk * rk);
+  knorm[r] = 1.0 / ksize;
+}
+
+__global__ __launch_bounds__(256) void knorm_perm_kernel(const double* __restrict__ radius, double* __restrict__ knorm,
+                                                         const int32_t* __restrict__ rows, int64_t n, int32_t volume) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t r = rows[t];
+  const double rk = radius[r];
+  const double ksize = volume ? (4.0 / 3.0) * ALB_PI * pow(rk, 3.0) : ALB_PI * (rk * rk);
+  knorm[r] = 1.0 / ksize;
+}
+
 __global__ __launch_bounds__(256) void contrib_kernel(ContribArgs a) {
   extern __shared__ double hist[];  // [win_len]
   for (int t = threadIdx.x; t < a.win_len; t += blockDim.x) hist[t] = 0.0;
@@ -501,18 +525,13 @@ __global__ __launch_bounds__(256) void contrib_kernel(ContribArgs a) {
     const int64_t qi = it / k;  // row
     const int64_t qg = a.q_begin + (a.direct ? qi : (int64_t)a.order[a.c0 + qi]);
     const int32_t i = a.nbr[it];
-    const double rk = a.radius[qi];
+    const double knorm = a.knorm[qi];
     const double sdx = a.q.dx[qg], sdy = a.q.dy[qg], sdz = a.q.dz[qg];
-    double ksize;
-    if (a.type == ALB_SCATTER) ksize = (4.0 / 3.0) * ALB_PI * pow(rk, 3.0);
-    else ksize = ALB_PI * (rk * rk);
-    const double knorm = 1.0 / ksize;
     double mul;
     if (a.type == ALB_BOTTOM_REFLECTION) {
       mul = (a.sc.floor_albedo / ALB_PI) * fmax(0.0, -sdy);
     } else if (a.type == ALB_SCATTER) {
-      mul = scatter_mult(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, a.sc.phase_theta, a.sc.phase_cdf,
-                         a.sc.phase_n);
+      mul = scatter_mass(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, a.sc.phase_mass, a.sc.phase_n);
     } else if (a.type == ALB_SURFACE_REFLECTION) {
       mul = (double)brdf_reflect(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, a.sc.alpha_ggx, a.sc.F0);
     } else {
@@ -543,8 +562,8 @@ __global__ __launch_bounds__(256) void query_key_kernel(alb_queries_t q, int64_t
 static size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct GatherWs {
-  size_t keys_a, keys_b, vals_a, order, temp, temp_bytes, nbr, rad, total;
-  int64_t chunk;
+  size_t keys_a, keys_b, vals_a, order, temp, temp_bytes, nbr, rad, knorm, total;
+  int64_t chunk, knorm_rows;
 };
 
 // workspace: query sort buffers, then one chunk of neighbour rows
@@ -565,6 +584,9 @@ static int gather_layout(int64_t q_n, int k, int64_t chunk, GatherWs* L) {
   L->chunk = chunk;
   L->nbr = off; off = al256(off + (size_t)(chunk > 0 ? chunk : 1) * (size_t)k * sizeof(int32_t));
   L->rad = off; off = al256(off + (size_t)(chunk > 0 ? chunk : 1) * sizeof(double));
+  // per-query kernel normalisation; whole query range so that rows may follow query order
+  L->knorm_rows = q_n > chunk ? q_n : chunk;
+  L->knorm = off; off = al256(off + (size_t)(L->knorm_rows > 0 ? L->knorm_rows : 1) * sizeof(double));
   L->total = off;
   return ALB_OK;
 }
@@ -582,66 +604,142 @@ extern "C" int alb_gather_workspace_size(int64_t q_n, int32_t k, size_t* bytes) 
   return ALB_OK;
 }
 
+static int device_cus(int* cus) {
+  int dev = 0;
+  *cus = 256;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return alb::set_error(ALB_ERR_HIP, "alb_gather: no HIP device");
+  return ALB_OK;
+}
+
+static int check_map(const alb_map_t* map, int k, const char* who) {
+  using namespace alb;
+  if (!map) return set_error(ALB_ERR_ARG, "%s: null map", who);
+  if (map->n <= 0) return set_error(ALB_ERR_ARG, "%s: empty photon map", who);
+  if (k <= 0 || k > map->n)
+    return set_error(ALB_ERR_ARG, "%s: k = %d exceeds the map size %lld", who, k, (long long)map->n);
+  if (k > kMaxK) return set_error(ALB_ERR_UNSUPPORTED, "%s: k = %d above %d", who, k, kMaxK);
+  if (!map->keys || !map->meta || !map->x || (map->n > 1 && !map->nodes))
+    return set_error(ALB_ERR_ARG, "%s: map not built", who);
+  return ALB_OK;
+}
+
+extern "C" int alb_gather_order(const alb_map_t* map, const alb_queries_t* q, int64_t q_begin, int64_t q_n,
+                                int32_t* order, void* workspace, size_t workspace_bytes, void* stream) {
+  using namespace alb;
+  if (!map || !q || !order) return set_error(ALB_ERR_ARG, "alb_gather_order: null argument");
+  if (q_n < 0 || q_begin < 0 || q_n > 0x7FFFFFF0) return set_error(ALB_ERR_ARG, "alb_gather_order: bad query range");
+  if (q_n == 0) return ALB_OK;
+  if (!map->meta) return set_error(ALB_ERR_ARG, "alb_gather_order: map not built");
+  GatherWs L;
+  int rc = gather_layout(q_n, 1, 0, &L);
+  if (rc != ALB_OK) return rc;
+  if (!workspace || workspace_bytes < L.total)
+    return set_error(ALB_ERR_WORKSPACE, "alb_gather_order: workspace too small (%zu < %zu)", workspace_bytes, L.total);
+  char* ws = (char*)workspace;
+  hipStream_t s = (hipStream_t)stream;
+  uint64_t* keys_a = (uint64_t*)(ws + L.keys_a);
+  uint64_t* keys_b = (uint64_t*)(ws + L.keys_b);
+  int32_t* vals_a = (int32_t*)(ws + L.vals_a);
+  hipLaunchKernelGGL(query_key_kernel, dim3((unsigned)((q_n + 255) / 256)), dim3(256), 0, s, *q, q_begin, q_n,
+                     map->meta, keys_a, vals_a);
+  size_t tb = L.temp_bytes;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs((void*)(ws + L.temp), tb, keys_a, keys_b, vals_a, order,
+                                                    (int)q_n, 0, map->dims == 2 ? 64 : 63, s);
+  if (e != hipSuccess) return set_error(ALB_ERR_HIP, "alb_gather_order: query sort failed: %s", hipGetErrorString(e));
+  return check_launch("alb_gather_order");
+}
+
+extern "C" int alb_knn(const alb_map_t* map, int32_t photon_type, int32_t k, const alb_queries_t* q,
+                       int64_t q_begin, const int32_t* order, int64_t c0, int64_t cn, int32_t rows_by_query,
+                       int32_t* nbr, double* radius, double* knorm, unsigned long long* counters, void* stream) {
+  using namespace alb;
+  int rc = check_map(map, k, "alb_knn");
+  if (rc != ALB_OK) return rc;
+  if (!q || !order || !nbr || !radius || !knorm) return set_error(ALB_ERR_ARG, "alb_knn: null argument");
+  if (photon_type < 0 || photon_type >= ALB_N_TYPES) return set_error(ALB_ERR_ARG, "alb_knn: bad photon type");
+  if (cn < 0 || c0 < 0) return set_error(ALB_ERR_ARG, "alb_knn: bad slot range");
+  if (cn == 0) return ALB_OK;
+  int cus;
+  if ((rc = device_cus(&cus)) != ALB_OK) return rc;
+  KnnArgs ka{*map, *q, q_begin, c0, cn, order, rows_by_query ? 1 : 0, k, photon_type == ALB_SCATTER ? 1 : 0,
+             nbr, radius, knorm, counters};
+  int64_t kb = (cn + kWaves - 1) / kWaves;
+  if (kb > (int64_t)cus * 16) kb = (int64_t)cus * 16;
+  hipLaunchKernelGGL(knn_kernel, dim3((unsigned)kb), dim3(64 * kWaves), 0, (hipStream_t)stream, ka);
+  // rows written: [0, cn) in slot order, or the query offsets of the slots when rows follow queries
+  if (rows_by_query) {
+    // rows are query offsets: normalise every row of this chunk's queries (all rows when one chunk)
+    hipLaunchKernelGGL(knorm_perm_kernel, dim3((unsigned)((cn + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       radius, knorm, order + c0, cn, photon_type == ALB_SCATTER ? 1 : 0);
+  } else {
+    hipLaunchKernelGGL(knorm_kernel, dim3((unsigned)((cn + 255) / 256)), dim3(256), 0, (hipStream_t)stream, radius,
+                       knorm, cn, photon_type == ALB_SCATTER ? 1 : 0);
+  }
+  return check_launch("alb_knn");
+}
+
+extern "C" int alb_contrib(const alb_scene_t* scene, const alb_map_t* map, int32_t photon_type,
+                           const alb_queries_t* q, int64_t q_begin, const int32_t* order, int64_t c0, int64_t cn,
+                           int32_t rows_by_query, const int32_t* nbr, const double* knorm, double* waveform,
+                           int64_t n_bins, int32_t win0, int32_t win_len, void* stream) {
+  using namespace alb;
+  if (!scene || !q || !nbr || !knorm || !waveform || !order) return set_error(ALB_ERR_ARG, "alb_contrib: null argument");
+  int rc = check_map(map, scene->k, "alb_contrib");
+  if (rc != ALB_OK) return rc;
+  if (photon_type < 0 || photon_type >= ALB_N_TYPES) return set_error(ALB_ERR_ARG, "alb_contrib: bad photon type");
+  if (photon_type == ALB_SCATTER && !scene->phase_mass)
+    return set_error(ALB_ERR_ARG, "alb_contrib: scene without phase_mass table");
+  if (win_len < 0 || win_len > kMaxWindow || win0 < 0 || (int64_t)win0 + win_len > n_bins)
+    return set_error(ALB_ERR_ARG, "alb_contrib: window [%d, %d) invalid (max length %d, %lld bins)", win0,
+                     win0 + win_len, kMaxWindow, (long long)n_bins);
+  if (cn <= 0) return ALB_OK;
+  int cus;
+  if ((rc = device_cus(&cus)) != ALB_OK) return rc;
+  ContribArgs ca{*scene, *map, *q, q_begin, c0, cn, order, rows_by_query ? 1 : 0, photon_type, nbr, knorm,
+                 waveform, n_bins, win0, win_len};
+  int64_t cb = (cn * scene->k + 255) / 256;
+  if (cb > (int64_t)cus * 8) cb = (int64_t)cus * 8;
+  hipLaunchKernelGGL(contrib_kernel, dim3((unsigned)cb), dim3(256), sizeof(double) * (size_t)win_len,
+                     (hipStream_t)stream, ca);
+  return check_launch("alb_contrib");
+}
+
 extern "C" int alb_gather(const alb_scene_t* scene, const alb_map_t* map, int32_t photon_type,
                           const alb_queries_t* q, int64_t q_begin, int64_t q_n, double* waveform,
                           int64_t n_bins, int32_t win0, int32_t win_len, int32_t* nbr_out, double* radius_out,
                           void* workspace, size_t workspace_bytes, unsigned long long* counters, void* stream) {
   using namespace alb;
   if (!scene || !map || !q || !waveform) return set_error(ALB_ERR_ARG, "alb_gather: null argument");
-  if (photon_type < 0 || photon_type >= ALB_N_TYPES) return set_error(ALB_ERR_ARG, "alb_gather: bad photon type");
-  if (q_n < 0 || q_begin < 0) return set_error(ALB_ERR_ARG, "alb_gather: negative query range");
+  if (q_n < 0 || q_begin < 0 || q_n > 0x7FFFFFF0) return set_error(ALB_ERR_ARG, "alb_gather: bad query range");
   if (q_n == 0) return ALB_OK;
-  if (map->n <= 0) return set_error(ALB_ERR_ARG, "alb_gather: empty photon map for type %d", photon_type);
   const int k = scene->k;
-  if (k <= 0 || k > map->n)
-    return set_error(ALB_ERR_ARG, "alb_gather: k = %d exceeds the map size %lld", k, (long long)map->n);
-  if (k > kMaxK) return set_error(ALB_ERR_UNSUPPORTED, "alb_gather: k = %d above %d", k, kMaxK);
-  if (win_len < 0 || win_len > kMaxWindow || win0 < 0 || (int64_t)win0 + win_len > n_bins)
-    return set_error(ALB_ERR_ARG, "alb_gather: window [%d, %d) invalid (max length %d, %lld bins)", win0,
-                     win0 + win_len, kMaxWindow, (long long)n_bins);
-  if (!map->keys || !map->meta || !map->x || (map->n > 1 && !map->nodes))
-    return set_error(ALB_ERR_ARG, "alb_gather: map not built");
+  int rc = check_map(map, k, "alb_gather");
+  if (rc != ALB_OK) return rc;
   if ((nbr_out == nullptr) != (radius_out == nullptr))
     return set_error(ALB_ERR_ARG, "alb_gather: nbr_out and radius_out go together");
-  if (q_n > 0x7FFFFFF0) return set_error(ALB_ERR_ARG, "alb_gather: more than 2^31 queries");
   const bool direct = nbr_out != nullptr;
   GatherWs L;
-  int rc = gather_layout(q_n, k, direct ? 0 : (q_n < kChunk ? q_n : kChunk), &L);
+  rc = gather_layout(q_n, k, q_n < kChunk ? q_n : kChunk, &L);
   if (rc != ALB_OK) return rc;
   if (!workspace || workspace_bytes < L.total)
     return set_error(ALB_ERR_WORKSPACE, "alb_gather: workspace too small (%zu < %zu)", workspace_bytes, L.total);
   char* ws = (char*)workspace;
-  int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-    return set_error(ALB_ERR_HIP, "alb_gather: no HIP device");
-  hipStream_t s = (hipStream_t)stream;
-  // Morton order of the queries on the map's grid
-  uint64_t* keys_a = (uint64_t*)(ws + L.keys_a);
-  uint64_t* keys_b = (uint64_t*)(ws + L.keys_b);
-  int32_t* vals_a = (int32_t*)(ws + L.vals_a);
   int32_t* order = (int32_t*)(ws + L.order);
-  hipLaunchKernelGGL(query_key_kernel, dim3((unsigned)((q_n + 255) / 256)), dim3(256), 0, s, *q, q_begin, q_n,
-                     map->meta, keys_a, vals_a);
-  size_t tb = L.temp_bytes;
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs((void*)(ws + L.temp), tb, keys_a, keys_b, vals_a, order,
-                                                    (int)q_n, 0, map->dims == 2 ? 64 : 63, s);
-  if (e != hipSuccess) return set_error(ALB_ERR_HIP, "alb_gather: query sort failed: %s", hipGetErrorString(e));
+  rc = alb_gather_order(map, q, q_begin, q_n, order, workspace, workspace_bytes, stream);
+  if (rc != ALB_OK) return rc;
   const int64_t chunk = direct ? q_n : L.chunk;
   for (int64_t c0 = 0; c0 < q_n; c0 += chunk) {
     const int64_t cn = (q_n - c0) < chunk ? (q_n - c0) : chunk;
     int32_t* nbr = direct ? nbr_out : (int32_t*)(ws + L.nbr);
     double* rad = direct ? radius_out : (double*)(ws + L.rad);
-    KnnArgs ka{*map, *q, q_begin, c0, cn, order, direct ? 1 : 0, k, nbr, rad, counters};
-    int64_t kb = (cn + kWaves - 1) / kWaves;
-    if (kb > (int64_t)cus * 16) kb = (int64_t)cus * 16;
-    hipLaunchKernelGGL(knn_kernel, dim3((unsigned)kb), dim3(64 * kWaves), 0, s, ka);
-    ContribArgs ca{*scene, *map, *q, q_begin, c0, cn, order, direct ? 1 : 0, photon_type, nbr, rad,
-                   waveform, n_bins, win0, win_len};
-    int64_t cb = (cn * k + 255) / 256;
-    if (cb > (int64_t)cus * 8) cb = (int64_t)cus * 8;
-    hipLaunchKernelGGL(contrib_kernel, dim3((unsigned)cb), dim3(256), sizeof(double) * (size_t)win_len, s, ca);
-    rc = check_launch("alb_gather");
+    double* kn = (double*)(ws + L.knorm) + (direct ? 0 : 0);
+    if (direct && L.knorm_rows < q_n) return set_error(ALB_ERR_WORKSPACE, "alb_gather: workspace too small");
+    rc = alb_knn(map, photon_type, k, q, q_begin, order, c0, cn, direct ? 1 : 0, nbr, rad, kn, counters, stream);
+    if (rc != ALB_OK) return rc;
+    rc = alb_contrib(scene, map, photon_type, q, q_begin, order, c0, cn, direct ? 1 : 0, nbr, kn, waveform, n_bins,
+                     win0, win_len, stream);
     if (rc != ALB_OK) return rc;
   }
   return ALB_OK;

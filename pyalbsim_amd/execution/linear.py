@@ -46,11 +46,14 @@ class SimulationResult:
 
 def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=None, keep=False,
              tags: bool = False, dscene: Optional[runtime.DeviceScene] = None, sync_timing: bool = False,
-             strict: bool = True, events: Optional[dict] = None) -> SimulationResult:
+             strict: bool = True, events: Optional[dict] = None,
+             kernel_events: Optional[dict] = None) -> SimulationResult:
     """Forward trace -> map build -> backward trace -> gather, all on the GPU.
 
     events: optional dict; HIP events are recorded on the launch stream at the
     phase boundaries start / forward / build / backward / gather.
+    kernel_events: optional dict of (start, stop) event pairs around every
+    k-NN ("knn") and contribution ("contrib") launch.
     strict: raise KeyError when a photon type has gather queries but an empty
     map and IndexError when k exceeds a map's size, as the reference does
     (core/simulation.py:134-153); strict=False gives such types zero signal.
@@ -93,7 +96,8 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
                 raise KeyError(PhotonType(ti + 1))
             raise IndexError(f"photon_mapping_k = {model.k} exceeds the {PhotonType(ti + 1).name} map size "
                              f"{rcounts[ti]}")
-        keep_alive.append(runtime.gather(ds, maps[ti], ti, qs[ti], qcounts[ti], wf[ti], win, counters=cb))
+        keep_alive.append(runtime.gather(ds, maps[ti], ti, qs[ti], qcounts[ti], wf[ti], win, counters=cb,
+                                         events=kernel_events))
     mark("gather")
     res = SimulationResult(model, wf, rcounts, qcounts, None, None, timings=t)
     ctr_f, ctr_b = cf.cpu().numpy().view(np.uint64), cb.cpu().numpy().view(np.uint64)

@@ -80,6 +80,22 @@ def fournier_forand_table(junge_slope: float, m: float):
     return theta, pf, cdf
 
 
+def phase_bin_mass(theta: np.ndarray, cdf: np.ndarray) -> np.ndarray:
+    """CDF mass of each of the len(theta) scattering-angle bins the gather uses.
+
+    Bin i spans [i delta, (i+1) delta) with delta = float32(pi - float32 eps) /
+    len(theta) in float32 arithmetic (NumPy 2 keeps pi - np.float32 in
+    float32); the mass is np.interp at the upper minus np.interp at the lower
+    edge (physics/scatter/energy.py:42-52), so a lookup equals that formula.
+    """
+    n = len(theta)
+    delta = np.float32(np.float32(np.pi) - np.finfo(np.float32).eps) / np.float32(n)
+    i = np.arange(n, dtype=np.int64)
+    lo = i * np.float64(delta)
+    hi = (i + 1) * np.float64(delta)
+    return np.interp(hi, theta, cdf) - np.interp(lo, theta, cdf)
+
+
 def backscatter_fraction(junge_slope: float, m: float) -> float:
     """Closed-form FF backscatter fraction B_p."""
     v = (3 - junge_slope) / 2

@@ -20,7 +20,8 @@ _lib = None
 
 # header entry points, checked by the ABI test
 EXPORTS = ["alb_abi_version", "alb_scene_size", "alb_last_error", "alb_trace", "alb_map_workspace_size",
-           "alb_map_build", "alb_gather_workspace_size", "alb_gather"]
+           "alb_map_build", "alb_gather_workspace_size", "alb_gather", "alb_gather_order", "alb_knn",
+           "alb_contrib"]
 
 
 class AlbError(RuntimeError):
@@ -50,6 +51,13 @@ def load_library(path: str = LIB_PATH):
     L.alb_gather.argtypes = [P, P, C.c_int32, P, C.c_int64, C.c_int64, P, C.c_int64, C.c_int32, C.c_int32,
                              P, P, P, C.c_size_t, P, P]
     L.alb_gather.restype = C.c_int
+    L.alb_gather_order.argtypes = [P, P, C.c_int64, C.c_int64, P, P, C.c_size_t, P]
+    L.alb_gather_order.restype = C.c_int
+    L.alb_knn.argtypes = [P, C.c_int32, C.c_int32, P, C.c_int64, P, C.c_int64, C.c_int64, C.c_int32, P, P, P, P, P]
+    L.alb_knn.restype = C.c_int
+    L.alb_contrib.argtypes = [P, P, C.c_int32, P, C.c_int64, P, C.c_int64, C.c_int64, C.c_int32, P, P, P, C.c_int64,
+                              C.c_int32, C.c_int32, P]
+    L.alb_contrib.restype = C.c_int
     if L.alb_abi_version() != abi.ALB_ABI_VERSION:
         raise AlbError("libalbsim_hip.so ABI version mismatch")
     if L.alb_scene_size() != C.sizeof(abi.Scene):
@@ -91,7 +99,11 @@ class DeviceScene:
         self.model = model
         self.theta = torch.as_tensor(np.ascontiguousarray(model.phase_theta), dtype=torch.float64, device=device)
         self.cdf = torch.as_tensor(np.ascontiguousarray(model.phase_cdf), dtype=torch.float64, device=device)
-        self.s = abi.make_scene(model, _p(self.theta), _p(self.cdf))
+        from .model import phase_bin_mass
+
+        self.mass = torch.as_tensor(phase_bin_mass(model.phase_theta, model.phase_cdf), dtype=torch.float64,
+                                    device=device)
+        self.s = abi.make_scene(model, _p(self.theta), _p(self.cdf), _p(self.mass))
         self.device = device
 
 
@@ -218,22 +230,60 @@ def build_map(records: Sink, n: int, dims: int, stream=None) -> DeviceMap:
     return m
 
 
+CHUNK = 1 << 22  # queries per k-NN / contribution round
+
+
 def gather(dscene: DeviceScene, dmap: DeviceMap, ptype: int, queries: Sink, q_n: int, waveform, window,
-           q_begin: int = 0, nbr_out=None, radius_out=None, counters=None, stream=None):
-    """Accumulate the k-NN gather of q_n queries of one type into waveform (float64 device tensor)."""
+           q_begin: int = 0, nbr_out=None, radius_out=None, counters=None, stream=None, events=None):
+    """Accumulate the k-NN gather of q_n queries of one type into waveform (float64 device tensor).
+
+    Runs alb_gather_order, then alb_knn + alb_contrib per chunk of CHUNK
+    queries.  nbr_out / radius_out (optional, given together) receive the
+    neighbour sets in query order, (distance, index) sorted.  events: optional
+    dict collecting (start, stop) HIP event pairs of the knn and contrib
+    launches under keys "knn" and "contrib".
+    """
     torch = _torch()
     L = load_library()
+    if q_n == 0:
+        return
+    k = dscene.model.k
     win0, win_len = window
     s = stream if stream is not None else _stream_ptr()
+    dev = waveform.device
+    direct = nbr_out is not None
+    if direct != (radius_out is not None):
+        raise ValueError("nbr_out and radius_out go together")
     b = C.c_size_t(0)
-    _check(L.alb_gather_workspace_size(q_n, dscene.model.k, C.byref(b)), "alb_gather_workspace_size")
-    ws_bytes = b.value
-    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=waveform.device)
-    rc = L.alb_gather(C.byref(dscene.s), C.byref(dmap.struct), ptype, C.byref(queries.struct), q_begin, q_n,
-                      _p(waveform), int(waveform.numel()), int(win0), int(win_len), _p(nbr_out), _p(radius_out),
-                      _p(ws), ws_bytes, _p(counters), C.c_void_p(s))
-    _check(rc, "alb_gather")
-    return ws  # the caller keeps it alive until the stream has run
+    _check(L.alb_gather_workspace_size(q_n, 1, C.byref(b)), "alb_gather_workspace_size")
+    ws = torch.empty(b.value, dtype=torch.uint8, device=dev)
+    order = torch.empty(q_n, dtype=torch.int32, device=dev)
+    _check(L.alb_gather_order(C.byref(dmap.struct), C.byref(queries.struct), q_begin, q_n, _p(order), _p(ws),
+                              b.value, C.c_void_p(s)), "alb_gather_order")
+    chunk = q_n if direct else min(q_n, CHUNK)
+    knorm = torch.empty(q_n if direct else chunk, dtype=torch.float64, device=dev)
+    nbr = nbr_out if direct else torch.empty((chunk, k), dtype=torch.int32, device=dev)
+    rad = radius_out if direct else torch.empty(chunk, dtype=torch.float64, device=dev)
+
+    def ev():
+        e = torch.cuda.Event(enable_timing=True)
+        e.record(torch.cuda.current_stream())
+        return e
+
+    for c0 in range(0, q_n, chunk):
+        cn = min(chunk, q_n - c0)
+        e0 = ev() if events is not None else None
+        _check(L.alb_knn(C.byref(dmap.struct), ptype, k, C.byref(queries.struct), q_begin, _p(order), c0, cn,
+                         1 if direct else 0, _p(nbr), _p(rad), _p(knorm), _p(counters), C.c_void_p(s)), "alb_knn")
+        e1 = ev() if events is not None else None
+        _check(L.alb_contrib(C.byref(dscene.s), C.byref(dmap.struct), ptype, C.byref(queries.struct), q_begin,
+                             _p(order), c0, cn, 1 if direct else 0, _p(nbr), _p(knorm), _p(waveform),
+                             int(waveform.numel()), int(win0), int(win_len), C.c_void_p(s)), "alb_contrib")
+        if events is not None:
+            e2 = ev()
+            events.setdefault("knn", []).append((e0, e1))
+            events.setdefault("contrib", []).append((e1, e2))
+    return (ws, order, knorm, nbr, rad)  # the caller keeps these alive until the stream has run
 
 
 MAP_DIMS = (2, 3, 2, 2)  # BOTTOM and the two surface maps are planar; SCATTER is volumetric
