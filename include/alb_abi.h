@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define ALB_ABI_VERSION 1
+#define ALB_ABI_VERSION 2
 
 /* photon types, reference enum order (BOTTOM_REFLECTION = auto() = 1, ...) */
 enum alb_photon_type {
@@ -84,7 +84,9 @@ enum alb_counter {
   ALB_CTR_KNN_STEPS = 13,    /* k-NN traversal steps (node batches)          */
   ALB_CTR_KNN_SORTS = 14,    /* candidate-buffer sorts                       */
   ALB_CTR_KNN_DESCENT = 15,  /* seed-descent levels                          */
-  ALB_N_COUNTERS = 16
+  ALB_CTR_SMALL_SCENE = 16,  /* batched queries whose scene map holds fewer
+                                than k records (rows left at -1 / NaN)      */
+  ALB_N_COUNTERS = 17
 };
 
 /* Scene constants, computed on the host in FP64 from SimulationConfig
@@ -148,6 +150,7 @@ typedef struct alb_records {
   int64_t* tag;             /* optional: (photon index << 16) | event ordinal     */
   int64_t capacity;
   unsigned long long* count;/* device counter                                     */
+  int32_t* scene;           /* optional: scene of each record (batched traces)     */
 } alb_records_t;
 
 /* Backward-pass query sink (sensor-path vertices), one per photon type. */
@@ -163,6 +166,7 @@ typedef struct alb_queries {
   int64_t* tag;             /* optional                                            */
   int64_t capacity;
   unsigned long long* count;
+  int32_t* scene;           /* optional: scene of each query (batched traces)      */
 } alb_queries_t;
 
 /* A built photon map: records sorted along a Morton curve plus a binary radix
@@ -194,6 +198,19 @@ typedef struct alb_map {
   void* nodes;              /* [n-1] 64-byte internal nodes (layout above)         */
 } alb_map_t;
 
+/* Scene index of a map (or query set) holding records of many scenes: after
+ * alb_map_build_batched, scene s owns sorted records [first[s], last[s]]
+ * (first = 0, last = -1 when it has none) and root[s] is its subtree's root
+ * node (~record when the scene has a single record).  scene_bits: bits the
+ * scene number takes at the top of the sort keys (written by the build). */
+typedef struct alb_scene_index {
+  int32_t n_scenes;
+  int32_t scene_bits;
+  int32_t* first;           /* [n_scenes] device                                   */
+  int32_t* last;            /* [n_scenes] device                                   */
+  int32_t* root;            /* [n_scenes] device                                   */
+} alb_scene_index_t;
+
 /* ---------------------------------------------------------------- entry points */
 
 int alb_abi_version(void);
@@ -211,6 +228,22 @@ int alb_trace(const alb_scene_t* scene, const alb_rng_t* rng, int32_t pass,
               const alb_queries_t* q, unsigned long long* counters,
               void* stream);
 
+/* Many scenes in one persistent launch: scenes (device array [n_scenes]),
+ * photon_offsets (device int64 [n_scenes + 1], prefix of photons per scene).
+ * Photon p of scene s uses Philox counter words {draw/2, p, p>>32,
+ * rng->stream + 2 s}; rng->stream must carry the pass in its low bit, and
+ * with rng->stream = pass scene 0 reproduces alb_trace.  Records / queries
+ * carry their scene in the sinks' scene arrays (optional) and the photon index
+ * within the scene in the tags.  work_counter: device uint64 set to 0 before
+ * the call; lanes whose photon has finished take new photons from it every
+ * refill_interval events.  block: 64, 128 or 256 threads.  Philox streams
+ * only.  No counterpart in the reference, which traces one scene per
+ * run_linear call (core/simulation.py:71-107). */
+int alb_trace_batched(const alb_scene_t* scenes, int32_t n_scenes, const int64_t* photon_offsets,
+                      const alb_rng_t* rng, int32_t pass, const alb_records_t* out,
+                      const alb_queries_t* q, unsigned long long* work_counter, int32_t block,
+                      int32_t refill_interval, unsigned long long* counters, void* stream);
+
 /* Workspace bytes needed to build a map of n records. */
 int alb_map_workspace_size(int64_t n, size_t* bytes);
 
@@ -220,6 +253,15 @@ int alb_map_workspace_size(int64_t n, size_t* bytes);
 int alb_map_build(const alb_records_t* in, int64_t n, int32_t dims,
                   alb_map_t* map, void* workspace, size_t workspace_bytes,
                   void* stream);
+
+/* alb_map_build for the records of many scenes (in->scene required): one
+ * sort by (scene, Morton key) and one tree, in which every scene is a subtree
+ * (index receives the per-scene record ranges and roots).  Same workspace
+ * size as alb_map_build.  No counterpart in the reference, which builds one
+ * KDTree per scene and type (photon_mapping/photon_map_index.py:17). */
+int alb_map_build_batched(const alb_records_t* in, int64_t n, int32_t dims, alb_map_t* map,
+                          alb_scene_index_t* index, void* workspace, size_t workspace_bytes,
+                          void* stream);
 
 /* Workspace bytes alb_gather needs for q_n queries (query sort buffers plus
  * the neighbour rows of at most 2^22 queries, reused round by round). */
@@ -260,6 +302,30 @@ int alb_contrib(const alb_scene_t* scene, const alb_map_t* map, int32_t photon_t
                 const alb_queries_t* q, int64_t q_begin, const int32_t* order, int64_t c0, int64_t cn,
                 int32_t rows_by_query, const int32_t* nbr, const double* knorm, double* waveform,
                 int64_t n_bins, int32_t win0, int32_t win_len, void* stream);
+
+/* Batched gather over a map built by alb_map_build_batched.  Each query
+ * (q->scene required) is answered from its own scene's records only, so scene
+ * s gets exactly the neighbour sets and waveform a one-scene gather of its
+ * records would give.  alb_gather_order_batched sorts queries by (scene,
+ * Morton key); alb_knn_batched writes rows in slot order (rows of scenes with
+ * fewer than k records are -1 / NaN and counted in ALB_CTR_SMALL_SCENE);
+ * alb_contrib_batched reads the scene table scenes[n_scenes] (device) and adds
+ * scene s's type-t waveform into waveforms[wf_offset[s] + t * 2 * steps_s ...],
+ * LDS-privatising bins [win0[s], win0[s] + win_len) (device arrays).  No
+ * counterpart in the reference (core/simulation.py:132-195 gathers one scene
+ * per Simulation). */
+int alb_gather_order_batched(const alb_map_t* map, const alb_scene_index_t* index,
+                             const alb_queries_t* q, int64_t q_n, int32_t* order,
+                             void* workspace, size_t workspace_bytes, void* stream);
+int alb_knn_batched(const alb_map_t* map, const alb_scene_index_t* index, int32_t photon_type,
+                    int32_t k, const alb_queries_t* q, const int32_t* order, int64_t c0,
+                    int64_t cn, int32_t* nbr, double* radius, double* knorm,
+                    unsigned long long* counters, void* stream);
+int alb_contrib_batched(const alb_scene_t* scenes, int32_t n_scenes, int32_t k,
+                        const alb_map_t* map, int32_t photon_type, const alb_queries_t* q,
+                        const int32_t* order, int64_t c0, int64_t cn, const int32_t* nbr,
+                        const double* knorm, double* waveforms, const int64_t* wf_offset,
+                        const int32_t* win0, int32_t win_len, void* stream);
 
 #ifdef __cplusplus
 }

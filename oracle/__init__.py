@@ -28,8 +28,9 @@ _lock = threading.Lock()
 
 def build(force: bool = False) -> str:
     """Compile alb_oracle.c with gcc (oracle/Makefile)."""
+    srcs = (os.path.join(HERE, "alb_oracle.c"), os.path.join(HERE, "..", "include", "alb_abi.h"))
     if force or not os.path.exists(LIB_PATH) or \
-            os.path.getmtime(LIB_PATH) < os.path.getmtime(os.path.join(HERE, "alb_oracle.c")):
+            os.path.getmtime(LIB_PATH) < max(os.path.getmtime(p) for p in srcs):
         subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
     return LIB_PATH
 

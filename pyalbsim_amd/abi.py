@@ -5,9 +5,9 @@ import ctypes as C
 
 import numpy as np
 
-ALB_ABI_VERSION = 1
+ALB_ABI_VERSION = 2
 N_TYPES = 4
-N_COUNTERS = 16
+N_COUNTERS = 17
 
 ALB_OK = 0
 ALB_ERR_ARG = -1
@@ -25,10 +25,10 @@ PASS_BACKWARD = 1
 
 CTR_PATHS, CTR_AIR_LEGS, CTR_WATER_STEPS, CTR_SCATTER, CTR_ENTER, CTR_EXIT, CTR_FLOOR, \
     CTR_REPLAY_EXHAUSTED, CTR_QUERIES, CTR_CANDIDATES, CTR_NODES, CTR_OVERFLOW, CTR_DEPTH_SOLVES, \
-    CTR_KNN_STEPS, CTR_KNN_SORTS, CTR_KNN_DESCENT = range(16)
+    CTR_KNN_STEPS, CTR_KNN_SORTS, CTR_KNN_DESCENT, CTR_SMALL_SCENE = range(17)
 COUNTER_NAMES = ["paths", "air_legs", "water_steps", "scatter", "enter", "exit", "floor",
                  "replay_exhausted", "queries", "candidates", "nodes", "overflow", "depth_solves",
-                 "knn_steps", "knn_sorts", "knn_descent"]
+                 "knn_steps", "knn_sorts", "knn_descent", "small_scene"]
 
 
 class Scene(C.Structure):
@@ -54,7 +54,7 @@ class Rng(C.Structure):
 class Records(C.Structure):
     _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("dx", C.c_void_p),
                 ("dy", C.c_void_p), ("dz", C.c_void_p), ("energy", C.c_void_p), ("time", C.c_void_p),
-                ("tag", C.c_void_p), ("capacity", C.c_int64), ("count", C.c_void_p)]
+                ("tag", C.c_void_p), ("capacity", C.c_int64), ("count", C.c_void_p), ("scene", C.c_void_p)]
 
 
 class Queries(C.Structure):
@@ -67,6 +67,11 @@ class Map(C.Structure):
                 ("dy", C.c_void_p), ("dz", C.c_void_p), ("energy", C.c_void_p), ("time", C.c_void_p),
                 ("perm", C.c_void_p), ("keys", C.c_void_p), ("meta", C.c_void_p),
                 ("nodes", C.c_void_p)]
+
+
+class SceneIndex(C.Structure):
+    _fields_ = [("n_scenes", C.c_int32), ("scene_bits", C.c_int32), ("first", C.c_void_p), ("last", C.c_void_p),
+                ("root", C.c_void_p)]
 
 
 def make_scene(model, theta_ptr: int, cdf_ptr: int, mass_ptr: int = 0) -> Scene:

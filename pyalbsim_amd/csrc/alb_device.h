@@ -325,6 +325,13 @@ __device__ __forceinline__ uint64_t quant(double p, double lo, double inv, doubl
 
 // Morton key of a point on the map grid described by meta (origin, inverse
 // cell width, dims, bits per axis); used by the map build and the gather
+// bits that hold scene numbers 0 .. n_scenes-1 in scene-major keys (0 for one scene)
+__host__ __device__ inline int scene_bits(int n_scenes) {
+  int b = 0;
+  while ((1 << b) < n_scenes) ++b;
+  return b;
+}
+
 __device__ __forceinline__ uint64_t morton_key(double px, double py, double pz, const double* meta) {
   const int dims = (int)meta[6];
   const double maxc = ldexp(1.0, (int)meta[7]) - 1.0;

@@ -63,6 +63,10 @@ struct KnnArgs {
   double* radius;    // [rows]
   double* knorm;     // [rows]
   unsigned long long* ctr;
+  // many scenes: each query searches its own scene's subtree (null for one scene)
+  const int32_t* sfirst;
+  const int32_t* slast;
+  const int32_t* sroot;
 };
 
 struct ContribArgs {
@@ -281,7 +285,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   const Node* __restrict__ nodes = (const Node*)m.nodes;
   const int64_t n = m.n;
   const int k = a.k;
-  unsigned long long c_q = 0, c_cand = 0, c_nodes = 0, c_ovf = 0, c_steps = 0, c_sorts = 0, c_desc = 0;
+  unsigned long long c_q = 0, c_cand = 0, c_nodes = 0, c_ovf = 0, c_steps = 0, c_sorts = 0, c_desc = 0, c_small = 0;
 
   for (int64_t si = (int64_t)blockIdx.x * kWaves + w; si < a.q_n; si += (int64_t)gridDim.x * kWaves) {
     const int64_t qrel = a.order[a.c0 + si];
@@ -294,9 +298,21 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     // box distance), so that stack + seed cover the whole map.  Stack entries
     // are node indices, or ~record for a single record.
     int sp = 0;
-    int32_t s0 = 0, s1 = (int32_t)n - 1;
-    if (n > kSeed) {
-      int32_t node = 0;
+    int32_t s0 = 0, s1 = (int32_t)n - 1, root = 0;
+    if (a.sroot) {
+      const int32_t sc = a.q.scene[qg];
+      s0 = a.sfirst[sc];
+      s1 = a.slast[sc];
+      root = a.sroot[sc];
+      if (s1 - s0 + 1 < k) {  // fewer records than neighbours: no estimate (counted, rows marked)
+        c_small++;
+        for (int t = (int)lane; t < k; t += 64) a.nbr[qi * k + t] = -1;
+        if (lane == 0) a.radius[qi] = __builtin_nan("");
+        continue;
+      }
+    }
+    if (s1 - s0 + 1 > kSeed) {
+      int32_t node = root;
       for (;;) {
         const Node nd = nodes[node];
         c_nodes++;
@@ -487,6 +503,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     if (c_steps) atomicAdd(&a.ctr[ALB_CTR_KNN_STEPS], c_steps);
     if (c_sorts) atomicAdd(&a.ctr[ALB_CTR_KNN_SORTS], c_sorts);
     if (c_desc) atomicAdd(&a.ctr[ALB_CTR_KNN_DESCENT], c_desc);
+    if (c_small) atomicAdd(&a.ctr[ALB_CTR_SMALL_SCENE], c_small);
   }
 }
 
@@ -550,12 +567,115 @@ __global__ __launch_bounds__(256) void contrib_kernel(ContribArgs a) {
     if (hist[t] != 0.0) unsafeAtomicAdd(&a.wf[a.win0 + t], hist[t]);
 }
 
+// Many scenes: slots are sorted by scene, and each block takes a contiguous
+// run of 256-item tiles, so a block meets few scene changes.  The LDS window
+// belongs to the scene of the block's current tile; it is flushed when the
+// tile scene changes.  Items of another scene than the tile's (a scene border
+// inside a tile) and bins outside the window go straight to global atomics.
+// Scene s's waveform of photon type t is wf[wf_off[s] + t * 2 * steps_s ...].
+struct ContribBatchedArgs {
+  const alb_scene_t* scenes;
+  alb_map_t map;
+  alb_queries_t q;
+  int64_t c0, q_n;
+  const int32_t* order;
+  int32_t type, k;
+  const int32_t* nbr;
+  const double* knorm;
+  double* wf;
+  const int64_t* wf_off;
+  const int32_t* win0;
+  int32_t win_len;
+};
+
+__device__ __forceinline__ void flush_window(double* hist, int win_len, double* dst) {
+  for (int t = threadIdx.x; t < win_len; t += blockDim.x) {
+    const double v = hist[t];
+    if (v != 0.0) {
+      unsafeAtomicAdd(&dst[t], v);
+      hist[t] = 0.0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void contrib_batched_kernel(ContribBatchedArgs a) {
+  extern __shared__ double hist[];  // [win_len]
+  for (int t = threadIdx.x; t < a.win_len; t += blockDim.x) hist[t] = 0.0;
+  __syncthreads();
+  const int k = a.k;
+  const int64_t total = a.q_n * (int64_t)k;
+  const int64_t tiles = (total + 255) / 256;
+  const int64_t per = (tiles + gridDim.x - 1) / gridDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * per;
+  const int64_t t1 = t0 + per < tiles ? t0 + per : tiles;
+  const alb_map_t& m = a.map;
+  int32_t cur = -1, cw0 = 0;
+  double* cdst = nullptr;  // global bins of the LDS window
+  for (int64_t tile = t0; tile < t1; ++tile) {
+    const int64_t it0 = tile * 256;
+    const int32_t ts = a.q.scene[a.order[a.c0 + it0 / k]];
+    if (ts != cur) {  // uniform: every thread read the same slot
+      __syncthreads();
+      if (cur >= 0) flush_window(hist, a.win_len, cdst);
+      __syncthreads();
+      cur = ts;
+      cw0 = a.win0[ts];
+      const int64_t nb = 2 * (int64_t)a.scenes[ts].steps;
+      cdst = a.wf + a.wf_off[ts] + (int64_t)a.type * nb + cw0;
+    }
+    const int64_t it = it0 + threadIdx.x;
+    if (it >= total) continue;
+    const int64_t qi = it / k;
+    const int64_t qg = a.order[a.c0 + qi];
+    const int32_t sc = a.q.scene[qg];
+    const alb_scene_t* S = a.scenes + sc;
+    const int32_t i = a.nbr[it];
+    const double knorm = a.knorm[qi];
+    if (i < 0) continue;  // scene with fewer records than neighbours (counted by the k-NN pass)
+    const double sdx = a.q.dx[qg], sdy = a.q.dy[qg], sdz = a.q.dz[qg];
+    double mul;
+    if (a.type == ALB_BOTTOM_REFLECTION) {
+      mul = (S->floor_albedo / ALB_PI) * fmax(0.0, -sdy);
+    } else if (a.type == ALB_SCATTER) {
+      mul = scatter_mass(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, S->phase_mass, S->phase_n);
+    } else if (a.type == ALB_SURFACE_REFLECTION) {
+      mul = (double)brdf_reflect(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, S->alpha_ggx, S->F0);
+    } else {
+      mul = btdf_transmit(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, S->alpha_ggx, S->n_water, S->F0,
+                          S->surface_albedo);
+    }
+    const float ee = a.q.energy[qg] * m.energy[i];
+    const double e = (double)ee * knorm * mul;
+    const float tt = a.q.time[qg] + m.time[i];
+    const int64_t bin = (int64_t)(tt / (float)S->mult);
+    if (sc == cur && bin >= cw0 && bin < (int64_t)cw0 + a.win_len) {
+      atomicAdd(&hist[bin - cw0], e);
+    } else {
+      const int64_t nb = 2 * (int64_t)S->steps;
+      if (bin >= 0 && bin < nb) unsafeAtomicAdd(&a.wf[a.wf_off[sc] + (int64_t)a.type * nb + bin], e);
+    }
+  }
+  __syncthreads();
+  if (cur >= 0) flush_window(hist, a.win_len, cdst);
+}
+
 __global__ __launch_bounds__(256) void query_key_kernel(alb_queries_t q, int64_t q_begin, int64_t q_n,
                                                         const double* __restrict__ meta, uint64_t* __restrict__ keys,
                                                         int32_t* __restrict__ vals) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= q_n) return;
   keys[i] = morton_key(q.x[q_begin + i], q.y[q_begin + i], q.z[q_begin + i], meta);
+  vals[i] = (int32_t)i;
+}
+
+__global__ __launch_bounds__(256) void query_key_batched_kernel(alb_queries_t q, int64_t q_n,
+                                                                const double* __restrict__ meta, int key_bits, int sb,
+                                                                uint64_t* __restrict__ keys,
+                                                                int32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= q_n) return;
+  const uint64_t mk = morton_key(q.x[i], q.y[i], q.z[i], meta);
+  keys[i] = ((uint64_t)(uint32_t)q.scene[i] << (key_bits - sb)) | (mk >> sb);
   vals[i] = (int32_t)i;
 }
 
@@ -664,7 +784,7 @@ extern "C" int alb_knn(const alb_map_t* map, int32_t photon_type, int32_t k, con
   int cus;
   if ((rc = device_cus(&cus)) != ALB_OK) return rc;
   KnnArgs ka{*map, *q, q_begin, c0, cn, order, rows_by_query ? 1 : 0, k, photon_type == ALB_SCATTER ? 1 : 0,
-             nbr, radius, knorm, counters};
+             nbr, radius, knorm, counters, nullptr, nullptr, nullptr};
   int64_t kb = (cn + kWaves - 1) / kWaves;
   if (kb > (int64_t)cus * 16) kb = (int64_t)cus * 16;
   hipLaunchKernelGGL(knn_kernel, dim3((unsigned)kb), dim3(64 * kWaves), 0, (hipStream_t)stream, ka);
@@ -743,4 +863,95 @@ extern "C" int alb_gather(const alb_scene_t* scene, const alb_map_t* map, int32_
     if (rc != ALB_OK) return rc;
   }
   return ALB_OK;
+}
+
+extern "C" int alb_gather_order_batched(const alb_map_t* map, const alb_scene_index_t* index,
+                                        const alb_queries_t* q, int64_t q_n, int32_t* order, void* workspace,
+                                        size_t workspace_bytes, void* stream) {
+  using namespace alb;
+  if (!map || !index || !q || !order) return set_error(ALB_ERR_ARG, "alb_gather_order_batched: null argument");
+  if (q_n < 0 || q_n > 0x7FFFFFF0) return set_error(ALB_ERR_ARG, "alb_gather_order_batched: bad query count");
+  if (q_n == 0) return ALB_OK;
+  if (!map->meta) return set_error(ALB_ERR_ARG, "alb_gather_order_batched: map not built");
+  if (!q->scene) return set_error(ALB_ERR_ARG, "alb_gather_order_batched: queries carry no scene array");
+  if (index->scene_bits != scene_bits(index->n_scenes))
+    return set_error(ALB_ERR_ARG, "alb_gather_order_batched: scene index not built by alb_map_build_batched");
+  GatherWs L;
+  int rc = gather_layout(q_n, 1, 0, &L);
+  if (rc != ALB_OK) return rc;
+  if (!workspace || workspace_bytes < L.total)
+    return set_error(ALB_ERR_WORKSPACE, "alb_gather_order_batched: workspace too small (%zu < %zu)", workspace_bytes,
+                     L.total);
+  char* ws = (char*)workspace;
+  hipStream_t s = (hipStream_t)stream;
+  uint64_t* keys_a = (uint64_t*)(ws + L.keys_a);
+  uint64_t* keys_b = (uint64_t*)(ws + L.keys_b);
+  int32_t* vals_a = (int32_t*)(ws + L.vals_a);
+  const int key_bits = map->dims == 2 ? 64 : 63;
+  const int sb = index->scene_bits;
+  const unsigned g = (unsigned)((q_n + 255) / 256);
+  if (sb > 0)
+    hipLaunchKernelGGL(query_key_batched_kernel, dim3(g), dim3(256), 0, s, *q, q_n, map->meta, key_bits, sb, keys_a,
+                       vals_a);
+  else
+    hipLaunchKernelGGL(query_key_kernel, dim3(g), dim3(256), 0, s, *q, (int64_t)0, q_n, map->meta, keys_a, vals_a);
+  size_t tb = L.temp_bytes;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs((void*)(ws + L.temp), tb, keys_a, keys_b, vals_a, order,
+                                                    (int)q_n, 0, key_bits, s);
+  if (e != hipSuccess)
+    return set_error(ALB_ERR_HIP, "alb_gather_order_batched: query sort failed: %s", hipGetErrorString(e));
+  return check_launch("alb_gather_order_batched");
+}
+
+extern "C" int alb_knn_batched(const alb_map_t* map, const alb_scene_index_t* index, int32_t photon_type, int32_t k,
+                               const alb_queries_t* q, const int32_t* order, int64_t c0, int64_t cn, int32_t* nbr,
+                               double* radius, double* knorm, unsigned long long* counters, void* stream) {
+  using namespace alb;
+  if (!map || !index || !q || !order || !nbr || !radius || !knorm)
+    return set_error(ALB_ERR_ARG, "alb_knn_batched: null argument");
+  if (map->n <= 0) return set_error(ALB_ERR_ARG, "alb_knn_batched: empty photon map");
+  if (k <= 0 || k > kMaxK) return set_error(ALB_ERR_UNSUPPORTED, "alb_knn_batched: k = %d outside [1, %d]", k, kMaxK);
+  if (!map->keys || !map->meta || !map->x || (map->n > 1 && !map->nodes))
+    return set_error(ALB_ERR_ARG, "alb_knn_batched: map not built");
+  if (!index->first || !index->last || !index->root || !q->scene)
+    return set_error(ALB_ERR_ARG, "alb_knn_batched: scene index or query scenes missing");
+  if (photon_type < 0 || photon_type >= ALB_N_TYPES) return set_error(ALB_ERR_ARG, "alb_knn_batched: bad photon type");
+  if (cn < 0 || c0 < 0) return set_error(ALB_ERR_ARG, "alb_knn_batched: bad slot range");
+  if (cn == 0) return ALB_OK;
+  int cus, rc;
+  if ((rc = device_cus(&cus)) != ALB_OK) return rc;
+  KnnArgs ka{*map, *q, 0, c0, cn, order, 0, k, photon_type == ALB_SCATTER ? 1 : 0,
+             nbr, radius, knorm, counters, index->first, index->last, index->root};
+  int64_t kb = (cn + kWaves - 1) / kWaves;
+  if (kb > (int64_t)cus * 16) kb = (int64_t)cus * 16;
+  hipLaunchKernelGGL(knn_kernel, dim3((unsigned)kb), dim3(64 * kWaves), 0, (hipStream_t)stream, ka);
+  hipLaunchKernelGGL(knorm_kernel, dim3((unsigned)((cn + 255) / 256)), dim3(256), 0, (hipStream_t)stream, radius,
+                     knorm, cn, photon_type == ALB_SCATTER ? 1 : 0);
+  return check_launch("alb_knn_batched");
+}
+
+extern "C" int alb_contrib_batched(const alb_scene_t* scenes, int32_t n_scenes, int32_t k, const alb_map_t* map,
+                                   int32_t photon_type, const alb_queries_t* q, const int32_t* order, int64_t c0,
+                                   int64_t cn, const int32_t* nbr, const double* knorm, double* waveforms,
+                                   const int64_t* wf_offset, const int32_t* win0, int32_t win_len, void* stream) {
+  using namespace alb;
+  if (!scenes || !map || !q || !order || !nbr || !knorm || !waveforms || !wf_offset || !win0)
+    return set_error(ALB_ERR_ARG, "alb_contrib_batched: null argument");
+  if (n_scenes < 1) return set_error(ALB_ERR_ARG, "alb_contrib_batched: bad scene count");
+  if (!q->scene) return set_error(ALB_ERR_ARG, "alb_contrib_batched: queries carry no scene array");
+  if (photon_type < 0 || photon_type >= ALB_N_TYPES) return set_error(ALB_ERR_ARG, "alb_contrib_batched: bad photon type");
+  if (k <= 0 || k > kMaxK) return set_error(ALB_ERR_UNSUPPORTED, "alb_contrib_batched: k = %d outside [1, %d]", k, kMaxK);
+  if (win_len < 0 || win_len > kMaxWindow)
+    return set_error(ALB_ERR_ARG, "alb_contrib_batched: window length %d outside [0, %d]", win_len, kMaxWindow);
+  if (cn <= 0) return ALB_OK;
+  int cus, rc;
+  if ((rc = device_cus(&cus)) != ALB_OK) return rc;
+  ContribBatchedArgs ca{scenes, *map, *q, c0, cn, order, photon_type, k, nbr, knorm, waveforms, wf_offset, win0,
+                        win_len};
+  const int64_t tiles = (cn * k + 255) / 256;
+  int64_t cb = (int64_t)cus * 8;
+  if (cb > tiles) cb = tiles;
+  hipLaunchKernelGGL(contrib_batched_kernel, dim3((unsigned)cb), dim3(256), sizeof(double) * (size_t)win_len,
+                     (hipStream_t)stream, ca);
+  return check_launch("alb_contrib_batched");
 }

@@ -220,6 +220,59 @@ __global__ __launch_bounds__(256) void box_level_kernel(alb_map_t m, const int32
   }
 }
 
+// Scene-major keys for maps of many scenes: the scene number in the top sb
+// bits, the Morton key (truncated by sb bits) below, so one sort groups the
+// records by scene and orders each scene along the Morton curve.
+__global__ __launch_bounds__(256) void key_batched_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                          const float* __restrict__ z,
+                                                          const int32_t* __restrict__ scene, int64_t n,
+                                                          const double* __restrict__ meta, int key_bits, int sb,
+                                                          uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t mk = morton_key((double)x[i], (double)y[i], (double)z[i], meta);
+  keys[i] = ((uint64_t)(uint32_t)scene[i] << (key_bits - sb)) | (mk >> sb);
+  vals[i] = (int32_t)i;
+}
+
+__global__ __launch_bounds__(256) void scene_range_init_kernel(int32_t* first, int32_t* last, int32_t* root, int ns) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  first[s] = 0;
+  last[s] = -1;
+  root[s] = 0;
+}
+
+__global__ __launch_bounds__(256) void scene_range_kernel(const uint64_t* __restrict__ keys, int64_t n, int shift,
+                                                          int32_t* first, int32_t* last) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t s = keys[i] >> shift;
+  if (i == 0 || (keys[i - 1] >> shift) != s) first[s] = (int32_t)i;
+  if (i == n - 1 || (keys[i + 1] >> shift) != s) last[s] = (int32_t)i;
+}
+
+// The records of one scene share the key prefix of the scene bits, so the
+// radix tree holds a node covering exactly [first, last]; Karras nodes cover
+// ranges that start or end at their own index, so it is node first or node last.
+__global__ void scene_range_one_kernel(int32_t* first, int32_t* last, int64_t n) {
+  first[0] = 0;
+  last[0] = (int32_t)(n - 1);
+}
+
+__global__ __launch_bounds__(256) void scene_root_kernel(const Node* __restrict__ nodes, const int32_t* first,
+                                                         const int32_t* last, int32_t* root, int ns) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const int32_t f = first[s], l = last[s];
+  if (l - f + 1 < 2) {
+    root[s] = l == f ? ~f : 0;
+    return;
+  }
+  const Node& a = nodes[f];
+  root[s] = (a.first == f && a.last == l) ? f : l;
+}
+
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct WsLayout {
@@ -258,25 +311,38 @@ extern "C" int alb_map_workspace_size(int64_t n, size_t* bytes) {
   return ALB_OK;
 }
 
-extern "C" int alb_map_build(const alb_records_t* in, int64_t n, int32_t dims, alb_map_t* map,
-                             void* workspace, size_t workspace_bytes, void* stream) {
+static int map_build(const alb_records_t* in, int64_t n, int32_t dims, alb_map_t* map, alb_scene_index_t* idx,
+                     void* workspace, size_t workspace_bytes, void* stream, const char* who) {
   using namespace alb;
-  if (!in || !map) return set_error(ALB_ERR_ARG, "alb_map_build: null argument");
-  if (dims != 2 && dims != 3) return set_error(ALB_ERR_ARG, "alb_map_build: dims must be 2 or 3");
-  if (n < 0 || n > 0x7FFFFFF0) return set_error(ALB_ERR_ARG, "alb_map_build: record count out of range");
+  if (!in || !map) return set_error(ALB_ERR_ARG, "%s: null argument", who);
+  if (dims != 2 && dims != 3) return set_error(ALB_ERR_ARG, "%s: dims must be 2 or 3", who);
+  if (n < 0 || n > 0x7FFFFFF0) return set_error(ALB_ERR_ARG, "%s: record count out of range", who);
+  int sb = 0;
+  if (idx) {
+    if (idx->n_scenes < 1 || idx->n_scenes > (1 << 20)) return set_error(ALB_ERR_ARG, "%s: bad scene count", who);
+    if (!idx->first || !idx->last || !idx->root) return set_error(ALB_ERR_ARG, "%s: scene index arrays missing", who);
+    if (n > 0 && !in->scene) return set_error(ALB_ERR_ARG, "%s: records carry no scene array", who);
+    sb = scene_bits(idx->n_scenes);
+    idx->scene_bits = sb;
+  }
   map->n = n;
   map->dims = dims;
-  if (n == 0) return ALB_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const int B = 256;
+  if (idx)
+    hipLaunchKernelGGL(scene_range_init_kernel, dim3((idx->n_scenes + B - 1) / B), dim3(B), 0, s, idx->first,
+                       idx->last, idx->root, idx->n_scenes);
+  if (n == 0) return check_launch(who);
   if (!map->x || !map->y || !map->z || !map->dx || !map->dy || !map->dz || !map->energy || !map->time ||
       !map->perm || !map->keys || !map->meta)
-    return set_error(ALB_ERR_ARG, "alb_map_build: map arrays missing");
+    return set_error(ALB_ERR_ARG, "%s: map arrays missing", who);
   if (n > 1 && !map->nodes)
-    return set_error(ALB_ERR_ARG, "alb_map_build: tree arrays missing");
+    return set_error(ALB_ERR_ARG, "%s: tree arrays missing", who);
   WsLayout L;
   int rc = layout(n, &L);
   if (rc != ALB_OK) return rc;
   if (!workspace || workspace_bytes < L.total)
-    return set_error(ALB_ERR_WORKSPACE, "alb_map_build: workspace too small (%zu < %zu)", workspace_bytes, L.total);
+    return set_error(ALB_ERR_WORKSPACE, "%s: workspace too small (%zu < %zu)", who, workspace_bytes, L.total);
   char* ws = (char*)workspace;
   uint64_t* keys_in = (uint64_t*)(ws + L.keys_in);
   int32_t* vals_in = (int32_t*)(ws + L.vals_in);
@@ -287,19 +353,22 @@ extern "C" int alb_map_build(const alb_records_t* in, int64_t n, int32_t dims, a
   uint32_t* hist = small + 8;          // kLevels
   uint32_t* offset = hist + kLevels;   // kLevels + 1
   uint32_t* cursor = offset + kLevels + 1;
-  hipStream_t s = (hipStream_t)stream;
-  const int B = 256;
   const unsigned grid = (unsigned)((n + B - 1) / B);
+  const int key_bits = dims == 2 ? 64 : 63;
 
   hipLaunchKernelGGL(bbox_init_kernel, dim3(1), dim3(128), 0, s, bb, hist);
   hipLaunchKernelGGL(bbox_kernel, dim3(grid < 2048 ? grid : 2048), dim3(B), 0, s, in->x, in->y, in->z, n, bb);
   hipLaunchKernelGGL(meta_kernel, dim3(1), dim3(1), 0, s, bb, map->meta, dims);
-  hipLaunchKernelGGL(key_kernel, dim3(grid), dim3(B), 0, s, in->x, in->y, in->z, n, map->meta, keys_in, vals_in);
-  if ((rc = check_launch("alb_map_build keys")) != ALB_OK) return rc;
+  if (sb > 0)
+    hipLaunchKernelGGL(key_batched_kernel, dim3(grid), dim3(B), 0, s, in->x, in->y, in->z, in->scene, n, map->meta,
+                       key_bits, sb, keys_in, vals_in);
+  else
+    hipLaunchKernelGGL(key_kernel, dim3(grid), dim3(B), 0, s, in->x, in->y, in->z, n, map->meta, keys_in, vals_in);
+  if ((rc = check_launch(who)) != ALB_OK) return rc;
   size_t tb = L.temp_bytes;
   hipError_t e = hipcub::DeviceRadixSort::SortPairs((void*)(ws + L.temp), tb, keys_in, map->keys, vals_in,
-                                                    map->perm, (int)n, 0, dims == 2 ? 64 : 63, s);
-  if (e != hipSuccess) return set_error(ALB_ERR_HIP, "alb_map_build: sort failed: %s", hipGetErrorString(e));
+                                                    map->perm, (int)n, 0, key_bits, s);
+  if (e != hipSuccess) return set_error(ALB_ERR_HIP, "%s: sort failed: %s", who, hipGetErrorString(e));
   alb_map_t m = *map;
   hipLaunchKernelGGL(permute_kernel, dim3(grid), dim3(B), 0, s, *in, m);
   if (n > 1) {
@@ -311,5 +380,27 @@ extern "C" int alb_map_build(const alb_records_t* in, int64_t n, int32_t dims, a
     for (int lv = kLevels - 1; lv >= 0; --lv)
       hipLaunchKernelGGL(box_level_kernel, dim3(g3), dim3(B), 0, s, m, order, offset, lv);
   }
-  return check_launch("alb_map_build");
+  if (idx) {
+    if (sb > 0) {
+      hipLaunchKernelGGL(scene_range_kernel, dim3(grid), dim3(B), 0, s, map->keys, n, key_bits - sb, idx->first,
+                         idx->last);
+    } else {
+      hipLaunchKernelGGL(scene_range_one_kernel, dim3(1), dim3(1), 0, s, idx->first, idx->last, n);
+    }
+    hipLaunchKernelGGL(scene_root_kernel, dim3((idx->n_scenes + B - 1) / B), dim3(B), 0, s,
+                       (const Node*)map->nodes, idx->first, idx->last, idx->root, idx->n_scenes);
+  }
+  return check_launch(who);
+}
+
+extern "C" int alb_map_build(const alb_records_t* in, int64_t n, int32_t dims, alb_map_t* map,
+                             void* workspace, size_t workspace_bytes, void* stream) {
+  return map_build(in, n, dims, map, nullptr, workspace, workspace_bytes, stream, "alb_map_build");
+}
+
+extern "C" int alb_map_build_batched(const alb_records_t* in, int64_t n, int32_t dims, alb_map_t* map,
+                                     alb_scene_index_t* index, void* workspace, size_t workspace_bytes,
+                                     void* stream) {
+  if (!index) return alb::set_error(ALB_ERR_ARG, "alb_map_build_batched: null scene index");
+  return map_build(in, n, dims, map, index, workspace, workspace_bytes, stream, "alb_map_build_batched");
 }

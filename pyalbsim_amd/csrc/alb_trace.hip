@@ -201,286 +201,404 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
+// ------------------------------------------------------------ per-photon engine
+
+struct Photon {
+  double px, py, pz, dx, dy, dz;
+  float E, tau, tau_t;
+  int64_t s;          // current sub-step
+  uint32_t event_no;  // events so far (record tag)
+  bool alive;
+};
+
+// record / query produced by one event
+struct Emit {
+  int et;  // photon type, -1 for none
+  double x, y, z, dx, dy, dz;
+  float E, t;
+  uint32_t eno;
+};
+
+struct Tally {
+  unsigned long long paths, air, ws, sc, en, ex, fl, exh, solve;
+};
+
+// Start a photon at the origin (core/simulation.py:75) with its emission
+// direction (physics/models/simulation.py:130-177).
+__device__ __forceinline__ void emit_photon(const alb_scene_t& sc, bool fwd, const alb_rng_t& rng, Stream& st,
+                                            Photon& ph, Tally& c) {
+  const double two_pi = 2.0 * ALB_PI;
+  ph.px = ph.py = ph.pz = 0.0;
+  if (fwd) {
+    // sample_directions_in_cone_gaussian (math/direction_cone.py:53-117)
+    double theta;
+    do {
+      const double u = uniform(rng, st);
+      theta = sc.rayleigh_sigma * sqrt(-2.0 * log(1.0 - u));
+    } while (theta > sc.divergence && !st.exhausted);
+    const double phi = two_pi * uniform(rng, st);
+    double s, cth, sp, cp;
+    sincos(theta, &s, &cth);
+    sincos(phi, &sp, &cp);
+    double w0, w1, w2;
+    rotate(sc.rot_laser, s * cp, s * sp, cth, w0, w1, w2);
+    ph.dx = (double)(float)w0;
+    ph.dy = (double)(float)w1;
+    ph.dz = (double)(float)w2;
+  } else {
+    // aperture disk point added to a cone-uniform direction
+    // (physics/models/simulation.py:157-177; math/disk_sampling.py:7-53)
+    const double xi1 = uniform(rng, st);
+    const double xi2 = uniform(rng, st);
+    const double r = sc.aperture_radius * sqrt(xi1);
+    const double th = two_pi * xi2;
+    double st_, ct_;
+    sincos(th, &st_, &ct_);
+    double k0, k1, k2;
+    rotate(sc.rot_laser, r * ct_, r * st_, 0.0, k0, k1, k2);
+    const double z = sc.cos_fov + (1.0 - sc.cos_fov) * uniform(rng, st);
+    const double phi = two_pi * uniform(rng, st);
+    const double sn = sqrt(1.0 - z * z);
+    double sp, cp;
+    sincos(phi, &sp, &cp);
+    double w0, w1, w2;
+    rotate(sc.rot_laser, sn * cp, sn * sp, z, w0, w1, w2);
+    ph.dx = k0 + (double)(float)w0;
+    ph.dy = k1 + (double)(float)w1;
+    ph.dz = k2 + (double)(float)w2;
+  }
+  ph.E = 1.0f;
+  ph.tau = 0.0f;
+  ph.tau_t = __builtin_inff();
+  ph.s = 0;
+  ph.event_no = 0;
+  ph.alive = true;
+  c.paths++;
+  if (st.exhausted) {
+    ph.alive = false;
+    c.exh++;
+  }
+}
+
+// Advance a live photon to its next event and apply it; fills e when the
+// event leaves a record (forward) or a query (backward).
+__device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, const alb_rng_t& rng, Stream& st,
+                                           Photon& ph, Emit& e, Tally& c) {
+  const double two_pi = 2.0 * ALB_PI;
+  const int64_t S = sc.n_substeps;
+  const double y_ws = sc.y_ws, y_sf = sc.y_sf;
+  const double alpha = (double)sc.alpha_l;
+  e.et = -1;
+  if (ph.py > y_ws) {
+    // ------------------------------------------------------------ air leg
+    if (ph.dy >= 0.0) {
+      ph.alive = false;  // above the surface, not descending: no more events
+      return;
+    }
+    const double Dx = (ph.dx * sc.v_air) * sc.dt, Dy = (ph.dy * sc.v_air) * sc.dt, Dz = (ph.dz * sc.v_air) * sc.dt;
+    const int64_t avail = S - ph.s;
+    const int64_t j = first_step_below(ph.py, Dy, y_ws, avail);
+    if (j >= avail) {
+      ph.alive = false;
+      return;
+    }
+    c.air++;
+    c.en++;
+    // handle_enter (core/simulation.py:375-431)
+    const double qx = ph.px + (double)j * Dx, qy = ph.py + (double)j * Dy, qz = ph.pz + (double)j * Dz;
+    const double nx = qx + Dx, ny = qy + Dy, nz = qz + Dz;
+    const double svx = nx - qx, svy = ny - qy, svz = nz - qz;
+    double f = (y_ws - qy) / svy;
+    f = clipd(f, 0.0, 1.0);
+    const double Px = qx + f * svx, Py = qy + f * svy, Pz = qz + f * svz;
+    const double R = schlick(-ph.dy, sc.F0);
+    const double T = 1.0 - R;
+    e.et = ALB_SURFACE_REFLECTION;
+    e.x = Px; e.y = Py; e.z = Pz; e.dx = ph.dx; e.dy = ph.dy; e.dz = ph.dz;
+    e.E = ph.E;
+    e.t = (float)(ph.s + j);
+    e.eno = ph.event_no++;
+    refract_y(ph.dx, ph.dy, ph.dz, 1.0, sc.eta);
+    ph.E = (float)((double)ph.E * T);
+    const double len = norm3(svx, svy, svz);
+    const double rem = (1.0 - f) * len;
+    ph.px = Px + ph.dx * rem;
+    ph.py = Py + ph.dy * rem;
+    ph.pz = Pz + ph.dz * rem;
+    ph.tau_t = tau_target(uniform(rng, st));
+    ph.s = ph.s + j + 1;
+  } else if (ph.py <= y_sf && ph.dy <= 0.0) {
+    ph.alive = false;  // below the floor and not rising: the reference finds no event
+    return;
+  } else {
+    // ---------------------------------------------------------- water leg
+    const double Dx = (ph.dx * sc.v_water) * sc.dt, Dy = (ph.dy * sc.v_water) * sc.dt,
+                 Dz = (ph.dz * sc.v_water) * sc.dt;
+    const int64_t avail = S - ph.s;
+    int64_t jb;
+    if (Dy < 0.0) jb = first_step_below(ph.py, Dy, y_sf, avail);
+    else if (Dy > 0.0) jb = first_step_above(ph.py, Dy, y_ws, avail);
+    else jb = avail;
+    const int64_t jmax = jb < avail ? jb : avail;
+    int64_t j = 0;
+    bool scattered = false;
+    const double len = norm3(Dx, Dy, Dz);
+    const double inc = alpha * len;
+    if (!__builtin_isinf(ph.tau_t)) {
+      bool hit = false;
+      const int64_t taken = advance_optical_depth(ph.tau, ph.tau_t, inc, jmax, &hit, c.solve);
+      scattered = hit;
+      j = hit ? taken - 1 : taken;  // index of the scattering sub-step
+    } else {
+      j = jmax;
+    }
+    c.ws += scattered ? (unsigned long long)(j + 1) : (unsigned long long)j;
+    if (scattered) {
+      // handle_water scatter branch (core/simulation.py:349-371)
+      c.sc++;
+      const double a1 = (double)(j + 1);
+      ph.px = ph.px + a1 * Dx;
+      ph.py = ph.py + a1 * Dy;
+      ph.pz = ph.pz + a1 * Dz;
+      e.et = ALB_SCATTER;
+      e.x = ph.px; e.y = ph.py; e.z = ph.pz; e.dx = ph.dx; e.dy = ph.dy; e.dz = ph.dz;
+      e.E = ph.E;
+      e.t = (float)(ph.s + j);
+      e.eno = ph.event_no++;
+      const double u = uniform(rng, st);
+      const double theta = interp(u, sc.phase_cdf, sc.phase_theta, sc.phase_n);
+      const double phi = two_pi * uniform(rng, st);
+      double sth, cth, sph, cph;
+      sincos(theta, &sth, &cth);
+      sincos(phi, &sph, &cph);
+      float Rm[9];
+      rotation_from_z(ph.dx, ph.dy, ph.dz, Rm);
+      const double l0 = sth * cph, l1 = sth * sph, l2 = cth;
+      double w0, w1, w2;
+      w0 = (double)Rm[0] * l0; w0 = w0 + (double)Rm[1] * l1; w0 = w0 + (double)Rm[2] * l2;
+      w1 = (double)Rm[3] * l0; w1 = w1 + (double)Rm[4] * l1; w1 = w1 + (double)Rm[5] * l2;
+      w2 = (double)Rm[6] * l0; w2 = w2 + (double)Rm[7] * l1; w2 = w2 + (double)Rm[8] * l2;
+      ph.dx = (double)(float)w0;
+      ph.dy = (double)(float)w1;
+      ph.dz = (double)(float)w2;
+      ph.E = ph.E * sc.omega0;
+      ph.tau = ph.tau - ph.tau_t;
+      ph.tau_t = tau_target(uniform(rng, st));
+      ph.s = ph.s + j + 1;
+    } else if (jb < avail) {
+      const double qx = ph.px + (double)jb * Dx, qy = ph.py + (double)jb * Dy, qz = ph.pz + (double)jb * Dz;
+      const double nx = qx + Dx, ny = qy + Dy, nz = qz + Dz;
+      const double svx = nx - qx, svy = ny - qy, svz = nz - qz;
+      const double slen = norm3(svx, svy, svz);
+      if (Dy < 0.0) {
+        // handle_seafloor (core/simulation.py:504-570)
+        c.fl++;
+        double f = (y_sf - qy) / svy;
+        f = clipd(f, 0.0, 1.0);
+        const double Px = qx + f * svx, Py = qy + f * svy, Pz = qz + f * svz;
+        e.et = ALB_BOTTOM_REFLECTION;
+        e.x = Px; e.y = Py; e.z = Pz; e.dx = ph.dx; e.dy = ph.dy; e.dz = ph.dz;
+        e.E = ph.E;
+        e.t = (float)(ph.s + jb);
+        e.eno = ph.event_no++;
+        const double u1 = uniform(rng, st);
+        const double u2 = uniform(rng, st);
+        const double sr = sqrt(u1);
+        double sp, cp;
+        sincos(two_pi * u2, &sp, &cp);
+        ph.dx = sr * cp;
+        ph.dy = sqrt(1.0 - u1);
+        ph.dz = sr * sp;
+        const double rem = (1.0 - f) * slen;
+        ph.px = Px + ph.dx * rem;
+        ph.py = Py + ph.dy * rem;
+        ph.pz = Pz + ph.dz * rem;
+        ph.tau = (float)((double)ph.tau + alpha * slen);
+        ph.E = ph.E * (float)sc.floor_albedo;
+      } else {
+        // handle_exit (core/simulation.py:433-502)
+        c.ex++;
+        double f = (y_ws - qy) / svy;
+        f = clipd(f, 0.0, 1.0);
+        const double Px = qx + f * svx, Py = qy + f * svy, Pz = qz + f * svz;
+        const double R = schlick(ph.dy, sc.F0);
+        const bool reflect = uniform(rng, st) < R;
+        const uint32_t no = ph.event_no++;
+        if (!reflect && fwd) {
+          e.et = ALB_SURFACE_TRANSMISSION_UP;
+          e.x = Px; e.y = Py; e.z = Pz; e.dx = ph.dx; e.dy = ph.dy; e.dz = ph.dz;
+          e.E = ph.E;
+          e.t = (float)(ph.s + jb);
+          e.eno = no;
+        }
+        if (reflect) {
+          ph.dy = -ph.dy;
+        } else {
+          refract_y(ph.dx, ph.dy, ph.dz, -1.0, sc.eta);
+        }
+        const double rem = (1.0 - f) * slen;
+        ph.px = Px + ph.dx * rem;
+        ph.py = Py + ph.dy * rem;
+        ph.pz = Pz + ph.dz * rem;
+        if (reflect) ph.tau = (float)((double)ph.tau + alpha * slen);
+      }
+      ph.s = ph.s + jb + 1;
+    } else {
+      ph.alive = false;  // time budget used up in water
+    }
+  }
+  if (st.exhausted) {
+    ph.alive = false;  // the record of this event still leaves, as in the oracle
+    c.exh++;
+  }
+}
+
+// Wave-aggregated emission: one atomic per photon type per wave.  A backward
+// surface entry goes to both surface types.
+template <bool FWD>
+__device__ __forceinline__ void emit_record(const Sinks& sk, const Emit& e, int64_t tag, int32_t scene,
+                                            uint32_t lane) {
+#pragma unroll
+  for (int t = 0; t < ALB_N_TYPES; ++t) {
+    const bool mine = (e.et == t) || (!FWD && t == ALB_SURFACE_TRANSMISSION_UP && e.et == ALB_SURFACE_REFLECTION);
+    const uint64_t m = ballot(mine);
+    if (m == 0) continue;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    unsigned long long* cnt = FWD ? sk.rec[t].count : sk.q[t].count;
+    if ((int)lane == leader) base = atomicAdd(cnt, (unsigned long long)__popcll(m));
+    base = bcast_u64(base, leader);
+    if (mine) {
+      const int64_t slot = (int64_t)(base + mask_rank(m));
+      if (FWD) {
+        const alb_records_t& r = sk.rec[t];
+        if (slot < r.capacity) {
+          r.x[slot] = (float)e.x; r.y[slot] = (float)e.y; r.z[slot] = (float)e.z;
+          r.dx[slot] = (float)e.dx; r.dy[slot] = (float)e.dy; r.dz[slot] = (float)e.dz;
+          r.energy[slot] = e.E;
+          r.time[slot] = e.t;
+          if (r.tag) r.tag[slot] = tag;
+          if (r.scene) r.scene[slot] = scene;
+        }
+      } else {
+        const alb_queries_t& r = sk.q[t];
+        if (slot < r.capacity) {
+          r.x[slot] = e.x; r.y[slot] = e.y; r.z[slot] = e.z;
+          r.dx[slot] = e.dx; r.dy[slot] = e.dy; r.dz[slot] = e.dz;
+          r.energy[slot] = e.E;
+          r.time[slot] = e.t;
+          if (r.tag) r.tag[slot] = tag;
+          if (r.scene) r.scene[slot] = scene;
+        }
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void flush_tally(const Tally& c, unsigned long long* ctr, uint32_t lane) {
+  if (!ctr) return;
+  const unsigned long long v[8] = {c.paths, c.air, c.ws, c.sc, c.en, c.ex, c.fl, c.exh};
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const unsigned long long tot = wave_sum(v[i]);
+    if (lane == 0 && tot) atomicAdd(&ctr[i], tot);
+  }
+  const unsigned long long ts = wave_sum(c.solve);
+  if (lane == 0 && ts) atomicAdd(&ctr[ALB_CTR_DEPTH_SOLVES], ts);
+}
+
+// One scene, one lane per photon.
 template <bool FWD>
 __global__ __launch_bounds__(256) void trace_kernel(alb_scene_t sc, alb_rng_t rng, int64_t begin,
                                                     int64_t n, Sinks sk, unsigned long long* ctr) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool alive = i < n;
-  Stream st{(uint64_t)(begin + i), i, 0u, false, 0.0};
-  const double two_pi = 2.0 * ALB_PI;
-  double px = 0.0, py = 0.0, pz = 0.0, dx = 0.0, dy = -1.0, dz = 0.0;
-  unsigned long long c_paths = alive ? 1 : 0, c_air = 0, c_ws = 0, c_sc = 0, c_en = 0, c_ex = 0,
-                     c_fl = 0, c_exh = 0, c_solve = 0;
-
-  if (alive) {
-    if (FWD) {
-      // sample_directions_in_cone_gaussian (math/direction_cone.py:53-117)
-      double theta;
-      do {
-        const double u = uniform(rng, st);
-        theta = sc.rayleigh_sigma * sqrt(-2.0 * log(1.0 - u));
-      } while (theta > sc.divergence && !st.exhausted);
-      const double phi = two_pi * uniform(rng, st);
-      double s, c, sp, cp;
-      sincos(theta, &s, &c);
-      sincos(phi, &sp, &cp);
-      double w0, w1, w2;
-      rotate(sc.rot_laser, s * cp, s * sp, c, w0, w1, w2);
-      dx = (double)(float)w0;
-      dy = (double)(float)w1;
-      dz = (double)(float)w2;
-    } else {
-      // aperture disk point added to a cone-uniform direction
-      // (physics/models/simulation.py:157-177; math/disk_sampling.py:7-53)
-      const double xi1 = uniform(rng, st);
-      const double xi2 = uniform(rng, st);
-      const double r = sc.aperture_radius * sqrt(xi1);
-      const double th = two_pi * xi2;
-      double st_, ct_;
-      sincos(th, &st_, &ct_);
-      double k0, k1, k2;
-      rotate(sc.rot_laser, r * ct_, r * st_, 0.0, k0, k1, k2);
-      const double z = sc.cos_fov + (1.0 - sc.cos_fov) * uniform(rng, st);
-      const double phi = two_pi * uniform(rng, st);
-      const double sn = sqrt(1.0 - z * z);
-      double sp, cp;
-      sincos(phi, &sp, &cp);
-      double w0, w1, w2;
-      rotate(sc.rot_laser, sn * cp, sn * sp, z, w0, w1, w2);
-      dx = k0 + (double)(float)w0;
-      dy = k1 + (double)(float)w1;
-      dz = k2 + (double)(float)w2;
-    }
-    if (st.exhausted) {
-      alive = false;
-      c_exh++;
-    }
-  }
-
-  float E = 1.0f, tau = 0.0f, tau_t = __builtin_inff();
-  int64_t s = 0;
-  uint32_t event_no = 0;
-  const int64_t S = sc.n_substeps;
-  const double y_ws = sc.y_ws, y_sf = sc.y_sf;
-  const double alpha = (double)sc.alpha_l;
   const uint32_t lane = lane_id();
+  Stream st{(uint64_t)(begin + i), i, 0u, false, 0.0};
+  Photon ph;
+  ph.alive = false;
+  Tally c{};
+  if (i < n) emit_photon(sc, FWD, rng, st, ph, c);
+  while (ballot(ph.alive)) {
+    Emit e{-1, 0, 0, 0, 0, 0, 0, 0.0f, 0.0f, 0u};
+    if (ph.alive) next_event(sc, FWD, rng, st, ph, e, c);
+    const int64_t tag = (int64_t)(((uint64_t)(begin + i) << 16) | (uint64_t)(e.eno & 0xFFFFu));
+    emit_record<FWD>(sk, e, tag, 0, lane);
+  }
+  flush_tally(c, ctr, lane);
+}
 
-  while (ballot(alive)) {
-    int et = -1;  // photon type of this iteration's record / query
-    double ex = 0, ey = 0, ez = 0, edx = 0, edy = 0, edz = 0;
-    float eE = 0.0f, etime = 0.0f;
-    uint32_t eno = 0;
-    if (alive) {
-      if (py > y_ws) {
-        // ---------------------------------------------------------- air leg
-        if (dy >= 0.0) {
-          alive = false;  // above the surface, not descending: no more events
-        } else {
-          const double Dx = (dx * sc.v_air) * sc.dt, Dy = (dy * sc.v_air) * sc.dt, Dz = (dz * sc.v_air) * sc.dt;
-          const int64_t avail = S - s;
-          const int64_t j = first_step_below(py, Dy, y_ws, avail);
-          if (j >= avail) {
-            alive = false;
+// Many scenes in one persistent launch.  Photons [offsets[s], offsets[s+1])
+// of the flattened index belong to scene s; scene s's photon p uses Philox
+// counter words {draw/2, p, p>>32, rng.stream + 2 s}, so with rng.stream =
+// pass scene 0 reproduces the single-scene trace.  Lanes whose photon has finished take the next index
+// from a global counter every refill_interval iterations, which keeps the
+// wave's lanes busy when path lengths differ.
+template <bool FWD>
+__global__ __launch_bounds__(256) void trace_batched_kernel(const alb_scene_t* __restrict__ scenes, int32_t n_scenes,
+                                                            const int64_t* __restrict__ offsets, alb_rng_t rng,
+                                                            Sinks sk, unsigned long long* work,
+                                                            int32_t refill_interval, unsigned long long* ctr) {
+  const uint32_t lane = lane_id();
+  const int64_t total = offsets[n_scenes];
+  Photon ph;
+  ph.alive = false;
+  ph.event_no = 0;
+  bool done = false;
+  int32_t sid = 0;
+  int64_t pidx = 0;
+  Stream st{0, 0, 0u, false, 0.0};
+  alb_rng_t lrng = rng;
+  Tally c{};
+  for (uint32_t it = 0;; ++it) {
+    if (it % (uint32_t)refill_interval == 0) {
+      const bool need = !ph.alive && !done;
+      const uint64_t m = ballot(need);
+      if (m) {
+        const int leader = __ffsll((long long)m) - 1;
+        unsigned long long base = 0;
+        if ((int)lane == leader) base = atomicAdd(work, (unsigned long long)__popcll(m));
+        base = bcast_u64(base, leader);
+        if (need) {
+          const int64_t g = (int64_t)(base + mask_rank(m));
+          if (g >= total) {
+            done = true;
           } else {
-            c_air++;
-            c_en++;
-            // handle_enter (core/simulation.py:375-431)
-            const double qx = px + (double)j * Dx, qy = py + (double)j * Dy, qz = pz + (double)j * Dz;
-            const double nx = qx + Dx, ny = qy + Dy, nz = qz + Dz;
-            const double svx = nx - qx, svy = ny - qy, svz = nz - qz;
-            double f = (y_ws - qy) / svy;
-            f = clipd(f, 0.0, 1.0);
-            const double Px = qx + f * svx, Py = qy + f * svy, Pz = qz + f * svz;
-            const double R = schlick(-dy, sc.F0);
-            const double T = 1.0 - R;
-            et = ALB_SURFACE_REFLECTION;
-            ex = Px; ey = Py; ez = Pz; edx = dx; edy = dy; edz = dz;
-            eE = E;
-            etime = (float)(s + j);
-            eno = event_no++;
-            refract_y(dx, dy, dz, 1.0, sc.eta);
-            E = (float)((double)E * T);
-            const double len = norm3(svx, svy, svz);
-            const double rem = (1.0 - f) * len;
-            px = Px + dx * rem;
-            py = Py + dy * rem;
-            pz = Pz + dz * rem;
-            tau_t = tau_target(uniform(rng, st));
-            s = s + j + 1;
+            int lo = 0, hi = n_scenes;  // offsets[lo] <= g < offsets[hi]
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (offsets[mid] <= g) lo = mid; else hi = mid;
+            }
+            sid = lo;
+            pidx = g - offsets[lo];
+            lrng.stream = rng.stream + ((uint32_t)sid << 1);
+            st = Stream{(uint64_t)pidx, 0, 0u, false, 0.0};
+            emit_photon(scenes[sid], FWD, lrng, st, ph, c);
           }
         }
-      } else if (py <= y_sf && dy <= 0.0) {
-        alive = false;  // below the floor and not rising: the reference finds no event
-      } else {
-        // -------------------------------------------------------- water leg
-        const double Dx = (dx * sc.v_water) * sc.dt, Dy = (dy * sc.v_water) * sc.dt,
-                     Dz = (dz * sc.v_water) * sc.dt;
-        const int64_t avail = S - s;
-        int64_t jb;
-        if (Dy < 0.0) jb = first_step_below(py, Dy, y_sf, avail);
-        else if (Dy > 0.0) jb = first_step_above(py, Dy, y_ws, avail);
-        else jb = avail;
-        const int64_t jmax = jb < avail ? jb : avail;
-        int64_t j = 0;
-        bool scattered = false;
-        const double len = norm3(Dx, Dy, Dz);
-        const double inc = alpha * len;
-        if (!__builtin_isinf(tau_t)) {
-          bool hit = false;
-          const int64_t taken = advance_optical_depth(tau, tau_t, inc, jmax, &hit, c_solve);
-          scattered = hit;
-          j = hit ? taken - 1 : taken;  // index of the scattering sub-step
-        } else {
-          j = jmax;
-        }
-        c_ws += scattered ? (unsigned long long)(j + 1) : (unsigned long long)j;
-        if (scattered) {
-          // handle_water scatter branch (core/simulation.py:349-371)
-          c_sc++;
-          const double a1 = (double)(j + 1);
-          px = px + a1 * Dx;
-          py = py + a1 * Dy;
-          pz = pz + a1 * Dz;
-          et = ALB_SCATTER;
-          ex = px; ey = py; ez = pz; edx = dx; edy = dy; edz = dz;
-          eE = E;
-          etime = (float)(s + j);
-          eno = event_no++;
-          const double u = uniform(rng, st);
-          const double theta = interp(u, sc.phase_cdf, sc.phase_theta, sc.phase_n);
-          const double phi = two_pi * uniform(rng, st);
-          double sth, cth, sph, cph;
-          sincos(theta, &sth, &cth);
-          sincos(phi, &sph, &cph);
-          float Rm[9];
-          rotation_from_z(dx, dy, dz, Rm);
-          const double l0 = sth * cph, l1 = sth * sph, l2 = cth;
-          double w0, w1, w2;
-          w0 = (double)Rm[0] * l0; w0 = w0 + (double)Rm[1] * l1; w0 = w0 + (double)Rm[2] * l2;
-          w1 = (double)Rm[3] * l0; w1 = w1 + (double)Rm[4] * l1; w1 = w1 + (double)Rm[5] * l2;
-          w2 = (double)Rm[6] * l0; w2 = w2 + (double)Rm[7] * l1; w2 = w2 + (double)Rm[8] * l2;
-          dx = (double)(float)w0;
-          dy = (double)(float)w1;
-          dz = (double)(float)w2;
-          E = E * sc.omega0;
-          tau = tau - tau_t;
-          tau_t = tau_target(uniform(rng, st));
-          s = s + j + 1;
-        } else if (jb < avail) {
-          const double qx = px + (double)jb * Dx, qy = py + (double)jb * Dy, qz = pz + (double)jb * Dz;
-          const double nx = qx + Dx, ny = qy + Dy, nz = qz + Dz;
-          const double svx = nx - qx, svy = ny - qy, svz = nz - qz;
-          const double slen = norm3(svx, svy, svz);
-          if (Dy < 0.0) {
-            // handle_seafloor (core/simulation.py:504-570)
-            c_fl++;
-            double f = (y_sf - qy) / svy;
-            f = clipd(f, 0.0, 1.0);
-            const double Px = qx + f * svx, Py = qy + f * svy, Pz = qz + f * svz;
-            et = ALB_BOTTOM_REFLECTION;
-            ex = Px; ey = Py; ez = Pz; edx = dx; edy = dy; edz = dz;
-            eE = E;
-            etime = (float)(s + jb);
-            eno = event_no++;
-            const double u1 = uniform(rng, st);
-            const double u2 = uniform(rng, st);
-            const double sr = sqrt(u1);
-            double sp, cp;
-            sincos(two_pi * u2, &sp, &cp);
-            dx = sr * cp;
-            dy = sqrt(1.0 - u1);
-            dz = sr * sp;
-            const double rem = (1.0 - f) * slen;
-            px = Px + dx * rem;
-            py = Py + dy * rem;
-            pz = Pz + dz * rem;
-            tau = (float)((double)tau + alpha * slen);
-            E = E * (float)sc.floor_albedo;
-          } else {
-            // handle_exit (core/simulation.py:433-502)
-            c_ex++;
-            double f = (y_ws - qy) / svy;
-            f = clipd(f, 0.0, 1.0);
-            const double Px = qx + f * svx, Py = qy + f * svy, Pz = qz + f * svz;
-            const double R = schlick(dy, sc.F0);
-            const bool reflect = uniform(rng, st) < R;
-            const uint32_t no = event_no++;
-            if (!reflect && FWD) {
-              et = ALB_SURFACE_TRANSMISSION_UP;
-              ex = Px; ey = Py; ez = Pz; edx = dx; edy = dy; edz = dz;
-              eE = E;
-              etime = (float)(s + jb);
-              eno = no;
-            }
-            if (reflect) {
-              dy = -dy;
-            } else {
-              refract_y(dx, dy, dz, -1.0, sc.eta);
-            }
-            const double rem = (1.0 - f) * slen;
-            px = Px + dx * rem;
-            py = Py + dy * rem;
-            pz = Pz + dz * rem;
-            if (reflect) tau = (float)((double)tau + alpha * slen);
-          }
-          s = s + jb + 1;
-        } else {
-          alive = false;  // time budget used up in water
-        }
-      }
-      if (st.exhausted) {
-        alive = false;  // the record of this event still leaves, as in the oracle
-        c_exh++;
       }
     }
+    if (!ballot(ph.alive || !done)) break;
+    Emit e{-1, 0, 0, 0, 0, 0, 0, 0.0f, 0.0f, 0u};
+    if (ph.alive) next_event(scenes[sid], FWD, lrng, st, ph, e, c);
+    const int64_t tag = (int64_t)(((uint64_t)pidx << 16) | (uint64_t)(e.eno & 0xFFFFu));
+    emit_record<FWD>(sk, e, tag, sid, lane);
+  }
+  flush_tally(c, ctr, lane);
+}
 
-    // ------------------------------------------------ wave-aggregated emission
-    const int64_t tag = (int64_t)(((uint64_t)(begin + i) << 16) | (uint64_t)(eno & 0xFFFFu));
-#pragma unroll
-    for (int t = 0; t < ALB_N_TYPES; ++t) {
-      const bool mine = (et == t) || (!FWD && t == ALB_SURFACE_TRANSMISSION_UP && et == ALB_SURFACE_REFLECTION);
-      const uint64_t m = ballot(mine);
-      if (m == 0) continue;
-      const int leader = __ffsll((long long)m) - 1;
-      unsigned long long base = 0;
-      unsigned long long* cnt = FWD ? sk.rec[t].count : sk.q[t].count;
-      if ((int)lane == leader) base = atomicAdd(cnt, (unsigned long long)__popcll(m));
-      base = bcast_u64(base, leader);
-      if (mine) {
-        const int64_t slot = (int64_t)(base + mask_rank(m));
-        if (FWD) {
-          const alb_records_t& r = sk.rec[t];
-          if (slot < r.capacity) {
-            r.x[slot] = (float)ex; r.y[slot] = (float)ey; r.z[slot] = (float)ez;
-            r.dx[slot] = (float)edx; r.dy[slot] = (float)edy; r.dz[slot] = (float)edz;
-            r.energy[slot] = eE;
-            r.time[slot] = etime;
-            if (r.tag) r.tag[slot] = tag;
-          }
-        } else {
-          const alb_queries_t& r = sk.q[t];
-          if (slot < r.capacity) {
-            r.x[slot] = ex; r.y[slot] = ey; r.z[slot] = ez;
-            r.dx[slot] = edx; r.dy[slot] = edy; r.dz[slot] = edz;
-            r.energy[slot] = eE;
-            r.time[slot] = etime;
-            if (r.tag) r.tag[slot] = tag;
-          }
-        }
-      }
+static int fill_sinks(int32_t pass, const alb_records_t* out, const alb_queries_t* q, Sinks* sk, const char* who) {
+  for (int t = 0; t < ALB_N_TYPES; ++t) {
+    if (pass == ALB_PASS_FORWARD) {
+      if (!out || !out[t].count) return set_error(ALB_ERR_ARG, "%s: forward pass needs 4 record sinks", who);
+      sk->rec[t] = out[t];
+      sk->q[t] = alb_queries_t{};
+    } else {
+      if (!q || !q[t].count) return set_error(ALB_ERR_ARG, "%s: backward pass needs 4 query sinks", who);
+      sk->q[t] = q[t];
+      sk->rec[t] = alb_records_t{};
     }
   }
-
-  if (ctr) {
-    const unsigned long long v[8] = {c_paths, c_air, c_ws, c_sc, c_en, c_ex, c_fl, c_exh};
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const unsigned long long tot = wave_sum(v[c]);
-      if (lane == 0 && tot) atomicAdd(&ctr[c], tot);
-    }
-    const unsigned long long ts = wave_sum(c_solve);
-    if (lane == 0 && ts) atomicAdd(&ctr[ALB_CTR_DEPTH_SOLVES], ts);
-  }
+  return ALB_OK;
 }
 
 }  // namespace alb
@@ -500,17 +618,8 @@ extern "C" int alb_trace(const alb_scene_t* scene, const alb_rng_t* rng, int32_t
   if (photon_begin + n > ((int64_t)1 << 47))
     return set_error(ALB_ERR_ARG, "alb_trace: photon index exceeds 2^47");
   Sinks sk;
-  for (int t = 0; t < ALB_N_TYPES; ++t) {
-    if (pass == ALB_PASS_FORWARD) {
-      if (!out || !out[t].count) return set_error(ALB_ERR_ARG, "alb_trace: forward pass needs 4 record sinks");
-      sk.rec[t] = out[t];
-      sk.q[t] = alb_queries_t{};
-    } else {
-      if (!q || !q[t].count) return set_error(ALB_ERR_ARG, "alb_trace: backward pass needs 4 query sinks");
-      sk.q[t] = q[t];
-      sk.rec[t] = alb_records_t{};
-    }
-  }
+  int rc = fill_sinks(pass, out, q, &sk, "alb_trace");
+  if (rc != ALB_OK) return rc;
   if (n == 0) return ALB_OK;
   const int block = 256;
   const int64_t grid = (n + block - 1) / block;
@@ -522,4 +631,41 @@ extern "C" int alb_trace(const alb_scene_t* scene, const alb_rng_t* rng, int32_t
     hipLaunchKernelGGL(trace_kernel<false>, dim3((unsigned)grid), dim3(block), 0, s, *scene, *rng,
                        photon_begin, n, sk, counters);
   return check_launch("alb_trace");
+}
+
+extern "C" int alb_trace_batched(const alb_scene_t* scenes, int32_t n_scenes, const int64_t* photon_offsets,
+                                 const alb_rng_t* rng, int32_t pass, const alb_records_t* out,
+                                 const alb_queries_t* q, unsigned long long* work_counter, int32_t block,
+                                 int32_t refill_interval, unsigned long long* counters, void* stream) {
+  using namespace alb;
+  if (!scenes || !photon_offsets || !rng || !work_counter)
+    return set_error(ALB_ERR_ARG, "alb_trace_batched: null argument");
+  if (n_scenes <= 0 || n_scenes > (1 << 20)) return set_error(ALB_ERR_ARG, "alb_trace_batched: bad scene count");
+  if (pass != ALB_PASS_FORWARD && pass != ALB_PASS_BACKWARD)
+    return set_error(ALB_ERR_ARG, "alb_trace_batched: pass must be 0 (forward) or 1 (backward)");
+  if (rng->mode != ALB_RNG_PHILOX) return set_error(ALB_ERR_UNSUPPORTED, "alb_trace_batched: Philox streams only");
+  if ((int32_t)(rng->stream & 1u) != pass)
+    return set_error(ALB_ERR_ARG, "alb_trace_batched: rng stream word must have the pass in its low bit");
+  if (block != 64 && block != 128 && block != 256)
+    return set_error(ALB_ERR_ARG, "alb_trace_batched: block must be 64, 128 or 256");
+  if (refill_interval < 1) return set_error(ALB_ERR_ARG, "alb_trace_batched: refill_interval must be >= 1");
+  Sinks sk;
+  int rc = fill_sinks(pass, out, q, &sk, "alb_trace_batched");
+  if (rc != ALB_OK) return rc;
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return set_error(ALB_ERR_HIP, "alb_trace_batched: no HIP device");
+  int per_cu = 1;
+  const void* fn = pass == ALB_PASS_FORWARD ? (const void*)trace_batched_kernel<true> : (const void*)trace_batched_kernel<false>;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  const unsigned grid = (unsigned)(cus * per_cu);
+  hipStream_t s = (hipStream_t)stream;
+  if (pass == ALB_PASS_FORWARD)
+    hipLaunchKernelGGL(trace_batched_kernel<true>, dim3(grid), dim3(block), 0, s, scenes, n_scenes, photon_offsets,
+                       *rng, sk, work_counter, refill_interval, counters);
+  else
+    hipLaunchKernelGGL(trace_batched_kernel<false>, dim3(grid), dim3(block), 0, s, scenes, n_scenes, photon_offsets,
+                       *rng, sk, work_counter, refill_interval, counters);
+  return check_launch("alb_trace_batched");
 }

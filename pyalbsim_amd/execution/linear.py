@@ -47,7 +47,7 @@ class SimulationResult:
 def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=None, keep=False,
              tags: bool = False, dscene: Optional[runtime.DeviceScene] = None, sync_timing: bool = False,
              strict: bool = True, events: Optional[dict] = None,
-             kernel_events: Optional[dict] = None) -> SimulationResult:
+             kernel_events: Optional[dict] = None, scene_id: int = 0) -> SimulationResult:
     """Forward trace -> map build -> backward trace -> gather, all on the GPU.
 
     events: optional dict; HIP events are recorded on the launch stream at the
@@ -57,6 +57,8 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
     strict: raise KeyError when a photon type has gather queries but an empty
     map and IndexError when k exceeds a map's size, as the reference does
     (core/simulation.py:134-153); strict=False gives such types zero signal.
+    scene_id: Philox stream word pass | scene_id << 1 (scene scene_id of a
+    batched run uses the same streams).
     """
     import torch
 
@@ -76,13 +78,13 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
     mark("start")
     cf = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device="cuda")
     cb = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device="cuda")
-    recs, rcounts = runtime.trace(ds, key, abi.PASS_FORWARD, n_forward, tags=tags, counters=cf)
+    recs, rcounts = runtime.trace(ds, key, abi.PASS_FORWARD, n_forward, tags=tags, counters=cf, scene_id=scene_id)
     mark("forward")
     maps = [runtime.build_map(recs[ti], rcounts[ti], runtime.MAP_DIMS[ti]) for ti in range(abi.N_TYPES)]
     if not keep:
         del recs
     mark("build")
-    qs, qcounts = runtime.trace(ds, key, abi.PASS_BACKWARD, n_backward, tags=tags, counters=cb)
+    qs, qcounts = runtime.trace(ds, key, abi.PASS_BACKWARD, n_backward, tags=tags, counters=cb, scene_id=scene_id)
     mark("backward")
     wf = torch.zeros((abi.N_TYPES, model.n_bins), dtype=torch.float64, device="cuda")
     keep_alive = []

@@ -96,6 +96,33 @@ def phase_bin_mass(theta: np.ndarray, cdf: np.ndarray) -> np.ndarray:
     return np.interp(hi, theta, cdf) - np.interp(lo, theta, cdf)
 
 
+def henyey_greenstein_table(g: float):
+    """(theta, p(theta), cdf) of the Henyey-Greenstein phase function on the
+    Fournier-Forand grid, with the same Riemann-sum CDF, so that sampling and
+    the gather's phase-mass lookup treat it exactly like the FF table.
+    p = (1 - g^2) / (4 pi (1 + g^2 - 2 g cos theta)^(3/2))."""
+    theta = np.linspace(1e-6, np.pi, SCATTER_DIVISIONS, dtype=np.float64)
+    pf = (1 - g * g) / (4 * np.pi * (1 + g * g - 2 * g * np.cos(theta)) ** 1.5)
+    cdf = np.cumsum(pf * np.sin(theta))
+    cdf /= cdf[-1]
+    return theta, pf, cdf
+
+
+def henyey_greenstein_backscatter(g: float) -> float:
+    """Backscatter fraction of HG: 2 pi int_{pi/2}^{pi} p sin = (1-g)/(2g) ((1+g)/sqrt(1+g^2) - 1)."""
+    if g == 0:
+        return 0.5
+    return float((1 - g) / (2 * g) * ((1 + g) / math.sqrt(1 + g * g) - 1))
+
+
+def cox_munk_roughness(wind_speed: float) -> float:
+    """Surface roughness r (GGX alpha = r^2) from wind speed (m/s): Cox-Munk
+    mean-square slope s^2 = 0.003 + 0.00512 W (clean surface, both directions);
+    alpha = sqrt(s^2), the RMS slope, so r = (s^2)^(1/4)."""
+    s2 = 0.003 + 0.00512 * float(wind_speed)
+    return float(s2 ** 0.25)
+
+
 def backscatter_fraction(junge_slope: float, m: float) -> float:
     """Closed-form FF backscatter fraction B_p."""
     v = (3 - junge_slope) / 2
@@ -185,7 +212,10 @@ class SceneModel:
         return g0, min(length, self.n_bins)
 
 
-def build_scene(cfg: SimulationConfig) -> SceneModel:
+def build_scene(cfg: SimulationConfig, hg_g: float | None = None) -> SceneModel:
+    """Scene constants of cfg.  hg_g: use a Henyey-Greenstein phase function
+    with asymmetry hg_g instead of the layer's Fournier-Forand one (its
+    backscatter fraction then also enters the lidar attenuation)."""
     layers = cfg.water.layers
     if len(layers) != 1:
         raise NotImplementedError("the traced path supports exactly one water layer")
@@ -204,8 +234,12 @@ def build_scene(cfg: SimulationConfig) -> SceneModel:
     distance = y_sf / d_l[1]
     steps = int(1.5 * (distance * round(sr)) / LIGHT_SPEED_AIR)
     ff = layer.fournier_forand_parameters
-    theta, pf, cdf = fournier_forand_table(ff.junge_slope, ff.refractive_index_ratio)
-    bpf = backscatter_fraction(ff.junge_slope, ff.refractive_index_ratio)
+    if hg_g is None:
+        theta, pf, cdf = fournier_forand_table(ff.junge_slope, ff.refractive_index_ratio)
+        bpf = backscatter_fraction(ff.junge_slope, ff.refractive_index_ratio)
+    else:
+        theta, pf, cdf = henyey_greenstein_table(float(hg_g))
+        bpf = henyey_greenstein_backscatter(float(hg_g))
     a = layer.absorption_coefficient
     b = layer.scattering_coefficient
     alpha = lidar_attenuation(a, b, layer.salinity, cfg.laser.wavelength, H, cfg.laser.divergence_angle, bpf)

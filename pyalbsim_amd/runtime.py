@@ -21,7 +21,8 @@ _lib = None
 # header entry points, checked by the ABI test
 EXPORTS = ["alb_abi_version", "alb_scene_size", "alb_last_error", "alb_trace", "alb_map_workspace_size",
            "alb_map_build", "alb_gather_workspace_size", "alb_gather", "alb_gather_order", "alb_knn",
-           "alb_contrib"]
+           "alb_contrib", "alb_trace_batched", "alb_map_build_batched", "alb_gather_order_batched",
+           "alb_knn_batched", "alb_contrib_batched"]
 
 
 class AlbError(RuntimeError):
@@ -58,6 +59,17 @@ def load_library(path: str = LIB_PATH):
     L.alb_contrib.argtypes = [P, P, C.c_int32, P, C.c_int64, P, C.c_int64, C.c_int64, C.c_int32, P, P, P, C.c_int64,
                               C.c_int32, C.c_int32, P]
     L.alb_contrib.restype = C.c_int
+    L.alb_trace_batched.argtypes = [P, C.c_int32, P, P, C.c_int32, P, P, P, C.c_int32, C.c_int32, P, P]
+    L.alb_trace_batched.restype = C.c_int
+    L.alb_map_build_batched.argtypes = [P, C.c_int64, C.c_int32, P, P, P, C.c_size_t, P]
+    L.alb_map_build_batched.restype = C.c_int
+    L.alb_gather_order_batched.argtypes = [P, P, P, C.c_int64, P, P, C.c_size_t, P]
+    L.alb_gather_order_batched.restype = C.c_int
+    L.alb_knn_batched.argtypes = [P, P, C.c_int32, C.c_int32, P, P, C.c_int64, C.c_int64, P, P, P, P, P]
+    L.alb_knn_batched.restype = C.c_int
+    L.alb_contrib_batched.argtypes = [P, C.c_int32, C.c_int32, P, C.c_int32, P, P, C.c_int64, C.c_int64, P, P, P,
+                                      P, P, C.c_int32, P]
+    L.alb_contrib_batched.restype = C.c_int
     if L.alb_abi_version() != abi.ALB_ABI_VERSION:
         raise AlbError("libalbsim_hip.so ABI version mismatch")
     if L.alb_scene_size() != C.sizeof(abi.Scene):
@@ -120,7 +132,7 @@ class Sink:
     struct: object = None
 
     @staticmethod
-    def alloc(capacity: int, query: bool, tags: bool, device="cuda") -> "Sink":
+    def alloc(capacity: int, query: bool, tags: bool, device="cuda", scenes: bool = False) -> "Sink":
         torch = _torch()
         capacity = max(int(capacity), 1)
         gdt = torch.float64 if query else torch.float32
@@ -128,17 +140,20 @@ class Sink:
         for f in REC_F32:
             t[f] = torch.empty(capacity, dtype=torch.float32 if f in ("energy", "time") else gdt, device=device)
         t["tag"] = torch.empty(capacity, dtype=torch.int64, device=device) if tags else None
+        t["scene"] = torch.empty(capacity, dtype=torch.int32, device=device) if scenes else None
         t["count"] = torch.zeros(1, dtype=torch.int64, device=device)
         st = abi.Queries() if query else abi.Records()
         for f in REC_F32:
             setattr(st, f, _p(t[f]))
         st.tag = _p(t["tag"])
+        st.scene = _p(t["scene"])
         st.capacity = capacity
         st.count = _p(t["count"])
         return Sink(t, capacity, query, st)
 
     def view(self, n: int) -> dict:
-        return {f: (self.tensors[f][:n] if self.tensors[f] is not None else None) for f in (*REC_F32, "tag")}
+        return {f: (self.tensors[f][:n] if self.tensors.get(f) is not None else None)
+                for f in (*REC_F32, "tag", "scene")}
 
 
 # expected records per photon and type (BOTTOM, SCATTER, SURF_REFL, SURF_TRANS) before a pilot
@@ -146,7 +161,7 @@ _DEFAULT_RATES = (1.5, 8.0, 1.0, 1.5)
 
 
 def trace(dscene: DeviceScene, key, pass_: int, n: int, begin: int = 0, replay=None, tags: bool = False,
-          rates=None, counters=None, stream=None, pilot: int = 65536):
+          rates=None, counters=None, stream=None, pilot: int = 65536, scene_id: int = 0):
     """Trace photons [begin, begin + n) of one pass on the GPU.
 
     Returns (sinks, counts): sinks[t] is a Sink holding counts[t] items.
@@ -159,10 +174,11 @@ def trace(dscene: DeviceScene, key, pass_: int, n: int, begin: int = 0, replay=N
     query = pass_ == abi.PASS_BACKWARD
     rp = _p(replay) if replay is not None else 0
     rlen = int(replay.shape[1]) if replay is not None else 0
-    rng = abi.make_rng(key, pass_, rp, rlen)
+    rng = abi.make_rng(key, pass_ | (int(scene_id) << 1), rp, rlen)
     s = stream if stream is not None else _stream_ptr()
     if rates is None and n > 4 * pilot and replay is None:
-        sinks, counts = trace(dscene, key, pass_, pilot, begin, None, False, _DEFAULT_RATES, None, stream, pilot)
+        sinks, counts = trace(dscene, key, pass_, pilot, begin, None, False, _DEFAULT_RATES, None, stream, pilot,
+                              scene_id)
         rates = [max(c / pilot, 0.05) for c in counts]
         del sinks
     rates = rates or _DEFAULT_RATES
@@ -196,6 +212,7 @@ class DeviceMap:
     dims: int
     tensors: dict = field(default_factory=dict)
     struct: object = None
+    index: object = None  # abi.SceneIndex of a map of many scenes
 
 
 def build_map(records: Sink, n: int, dims: int, stream=None) -> DeviceMap:
@@ -287,3 +304,185 @@ def gather(dscene: DeviceScene, dmap: DeviceMap, ptype: int, queries: Sink, q_n:
 
 
 MAP_DIMS = (2, 3, 2, 2)  # BOTTOM and the two surface maps are planar; SCATTER is volumetric
+
+
+# ----------------------------------------------------------------- many scenes
+
+class DeviceSceneTable:
+    """alb_scene_t[n_scenes] on the device; scenes with equal phase tables share them."""
+
+    def __init__(self, models, device="cuda"):
+        import hashlib
+
+        from .model import phase_bin_mass
+
+        torch = _torch()
+        self.models = list(models)
+        self.n = len(self.models)
+        if self.n == 0:
+            raise ValueError("no scenes")
+        ks = {m.k for m in self.models}
+        if len(ks) != 1:
+            raise ValueError("all scenes of a batch must use the same photon_mapping_k")
+        self.k = ks.pop()
+        self.tables = {}
+        structs = (abi.Scene * self.n)()
+        for i, m in enumerate(self.models):
+            h = hashlib.sha1(np.ascontiguousarray(m.phase_cdf).tobytes()
+                             + np.ascontiguousarray(m.phase_theta).tobytes()).hexdigest()
+            if h not in self.tables:
+                th = torch.as_tensor(np.ascontiguousarray(m.phase_theta), dtype=torch.float64, device=device)
+                cd = torch.as_tensor(np.ascontiguousarray(m.phase_cdf), dtype=torch.float64, device=device)
+                ms = torch.as_tensor(phase_bin_mass(m.phase_theta, m.phase_cdf), dtype=torch.float64, device=device)
+                self.tables[h] = (th, cd, ms)
+            th, cd, ms = self.tables[h]
+            structs[i] = abi.make_scene(m, _p(th), _p(cd), _p(ms))
+        self.host = structs
+        raw = np.frombuffer(bytes(structs), dtype=np.uint8).copy()
+        self.dev = torch.as_tensor(raw, device=device)
+        self.n_bins = np.array([m.n_bins for m in self.models], dtype=np.int64)
+        self.device = device
+
+
+def _dev_i64(values, device="cuda"):
+    torch = _torch()
+    return torch.as_tensor(np.asarray(values, dtype=np.int64), device=device)
+
+
+def trace_batched(table: DeviceSceneTable, key, pass_: int, counts, tags: bool = False, counters=None,
+                  block: int = 256, refill: int = 4, stream=None, pilot_fraction: float = 1 / 64,
+                  caps=None, stream_base: int = 0):
+    """Trace counts[s] photons of every scene s in one persistent launch.
+
+    Scene s's photon p uses the Philox stream word pass | (stream_base + s) << 1,
+    so its records equal those of trace(..., scene_id=stream_base + s) for the
+    same key.  Returns
+    (sinks, totals); sinks carry a scene array.  Capacities come from a pilot
+    batch of every scene's first photons; an overflow reruns with exact sizes.
+    """
+    torch = _torch()
+    L = load_library()
+    counts = np.asarray(counts, dtype=np.int64)
+    if len(counts) != table.n:
+        raise ValueError("one photon count per scene")
+    query = pass_ == abi.PASS_BACKWARD
+    rng = abi.make_rng(key, pass_ | (int(stream_base) << 1))
+    s = stream if stream is not None else _stream_ptr()
+    total = int(counts.sum())
+    if caps is None:
+        pc = np.minimum(counts, np.maximum(np.ceil(counts * pilot_fraction).astype(np.int64), 256))
+        if total > 8 * int(pc.sum()):
+            _, pcnt = trace_batched(table, key, pass_, pc, False, None, block, refill, stream,
+                                    caps=[int(r * pc.sum()) + 4096 for r in (4.0, 64.0, 1.0, 4.0)],
+                                    stream_base=stream_base)
+            frac = total / max(int(pc.sum()), 1)
+            caps = [int(c * frac * 1.08) + 65536 for c in pcnt]
+        else:
+            caps = [int(r * total) + 4096 for r in (4.0, 64.0, 1.0, 4.0)]
+    offsets = _dev_i64(np.concatenate([[0], np.cumsum(counts)]))
+    work = torch.zeros(1, dtype=torch.int64, device=table.device)
+    for attempt in range(2):
+        sinks = [Sink.alloc(caps[t], query, tags, scenes=True) for t in range(abi.N_TYPES)]
+        arr = ((abi.Queries if query else abi.Records) * 4)()
+        for t in range(4):
+            arr[t] = sinks[t].struct
+        if attempt == 1:
+            work.zero_()
+            if counters is not None:
+                counters.zero_()
+        rc = L.alb_trace_batched(_p(table.dev), table.n, _p(offsets), C.byref(rng), pass_,
+                                 None if query else C.cast(arr, C.c_void_p), C.cast(arr, C.c_void_p) if query else None,
+                                 _p(work), block, refill, _p(counters), C.c_void_p(s))
+        _check(rc, "alb_trace_batched")
+        got = [int(x) for x in torch.cat([sk.tensors["count"] for sk in sinks]).cpu().tolist()]
+        if all(got[t] <= caps[t] for t in range(4)):
+            return sinks, got
+        caps = [max(got[t], 1) for t in range(4)]
+    raise AlbError("alb_trace_batched: capacity overflow persisted after resizing")
+
+
+def build_map_batched(records: Sink, n: int, dims: int, n_scenes: int, stream=None) -> DeviceMap:
+    """Sort n records of many scenes by (scene, Morton key) into one map whose
+    tree holds every scene as a subtree (scene ranges and roots in m.index)."""
+    torch = _torch()
+    L = load_library()
+    dev = records.tensors["x"].device
+    m = DeviceMap(n=n, dims=dims)
+    t = m.tensors
+    for f in REC_F32:
+        t[f] = torch.empty(max(n, 1), dtype=torch.float32, device=dev)
+    t["perm"] = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    t["keys"] = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+    t["meta"] = torch.zeros(8, dtype=torch.float64, device=dev)
+    t["nodes"] = torch.empty((max(n - 1, 1), 16), dtype=torch.int32, device=dev)
+    for f in ("first", "last", "root"):
+        t[f] = torch.empty(n_scenes, dtype=torch.int32, device=dev)
+    st = abi.Map()
+    st.n = n
+    st.dims = dims
+    for f in ("x", "y", "z", "dx", "dy", "dz", "energy", "time", "perm", "keys", "meta", "nodes"):
+        setattr(st, f, _p(t[f]))
+    ix = abi.SceneIndex()
+    ix.n_scenes = n_scenes
+    ix.first, ix.last, ix.root = _p(t["first"]), _p(t["last"]), _p(t["root"])
+    m.struct = st
+    m.index = ix
+    ws_bytes = C.c_size_t(0)
+    _check(L.alb_map_workspace_size(n, C.byref(ws_bytes)), "alb_map_workspace_size")
+    ws = torch.empty(max(ws_bytes.value, 1), dtype=torch.uint8, device=dev)
+    s = stream if stream is not None else _stream_ptr()
+    _check(L.alb_map_build_batched(C.byref(records.struct), n, dims, C.byref(st), C.byref(ix), _p(ws),
+                                   ws_bytes.value, C.c_void_p(s)), "alb_map_build_batched")
+    t["_ws"] = ws
+    return m
+
+
+def gather_batched(table: DeviceSceneTable, dmap: DeviceMap, ptype: int, queries: Sink, q_n: int, waveforms,
+                   wf_offset, win0, win_len: int, counters=None, stream=None, events=None, nbr_out=None,
+                   radius_out=None):
+    """Batched k-NN gather of q_n queries (of many scenes) of one photon type.
+
+    waveforms: flat float64 device tensor; scene s's type-t waveform starts at
+    wf_offset[s] + t * n_bins_s.  win0: int32 device tensor of per-scene LDS
+    window starts.  nbr_out / radius_out (optional, [q_n, k] / [q_n]) receive
+    the rows in slot order; the slot order itself is returned.
+    """
+    torch = _torch()
+    L = load_library()
+    if q_n == 0:
+        return None
+    k = table.k
+    s = stream if stream is not None else _stream_ptr()
+    dev = waveforms.device
+    b = C.c_size_t(0)
+    _check(L.alb_gather_workspace_size(q_n, 1, C.byref(b)), "alb_gather_workspace_size")
+    ws = torch.empty(b.value, dtype=torch.uint8, device=dev)
+    order = torch.empty(q_n, dtype=torch.int32, device=dev)
+    _check(L.alb_gather_order_batched(C.byref(dmap.struct), C.byref(dmap.index), C.byref(queries.struct), q_n,
+                                      _p(order), _p(ws), b.value, C.c_void_p(s)), "alb_gather_order_batched")
+    direct = nbr_out is not None
+    chunk = q_n if direct else min(q_n, CHUNK)
+    nbr = nbr_out if direct else torch.empty((chunk, k), dtype=torch.int32, device=dev)
+    rad = radius_out if direct else torch.empty(chunk, dtype=torch.float64, device=dev)
+    knorm = torch.empty(chunk, dtype=torch.float64, device=dev)
+
+    def ev():
+        e = torch.cuda.Event(enable_timing=True)
+        e.record(torch.cuda.current_stream())
+        return e
+
+    for c0 in range(0, q_n, chunk):
+        cn = min(chunk, q_n - c0)
+        e0 = ev() if events is not None else None
+        _check(L.alb_knn_batched(C.byref(dmap.struct), C.byref(dmap.index), ptype, k, C.byref(queries.struct),
+                                 _p(order), c0, cn, _p(nbr), _p(rad), _p(knorm), _p(counters), C.c_void_p(s)),
+               "alb_knn_batched")
+        e1 = ev() if events is not None else None
+        _check(L.alb_contrib_batched(_p(table.dev), table.n, k, C.byref(dmap.struct), ptype, C.byref(queries.struct),
+                                     _p(order), c0, cn, _p(nbr), _p(knorm), _p(waveforms), _p(wf_offset), _p(win0),
+                                     int(win_len), C.c_void_p(s)), "alb_contrib_batched")
+        if events is not None:
+            e2 = ev()
+            events.setdefault("knn", []).append((e0, e1))
+            events.setdefault("contrib", []).append((e1, e2))
+    return (ws, order, knorm, nbr, rad)

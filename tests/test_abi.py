@@ -32,7 +32,8 @@ def test_layout_and_version():
     assert lib.alb_abi_version() == abi.ALB_ABI_VERSION
     assert lib.alb_scene_size() == C.sizeof(abi.Scene)
     assert C.sizeof(abi.Rng) == 32
-    assert C.sizeof(abi.Records) == 88 and C.sizeof(abi.Queries) == 88
+    assert C.sizeof(abi.Records) == 96 and C.sizeof(abi.Queries) == 96
+    assert C.sizeof(abi.SceneIndex) == 32
 
 
 def test_last_error_is_set_by_argument_checks():
@@ -41,6 +42,14 @@ def test_last_error_is_set_by_argument_checks():
     assert rc == abi.ALB_ERR_ARG
     assert b"null" in lib.alb_last_error()
     rc = lib.alb_map_build(None, 0, 2, None, None, 0, None)
+    assert rc == abi.ALB_ERR_ARG
+    rc = lib.alb_trace_batched(None, 1, None, None, 0, None, None, None, 256, 4, None, None)
+    assert rc == abi.ALB_ERR_ARG and b"null" in lib.alb_last_error()
+    rc = lib.alb_map_build_batched(None, 0, 2, None, None, None, 0, None)
+    assert rc == abi.ALB_ERR_ARG
+    rc = lib.alb_knn_batched(None, None, 1, 100, None, None, 0, 1, None, None, None, None, None)
+    assert rc == abi.ALB_ERR_ARG
+    rc = lib.alb_contrib_batched(None, 1, 100, None, 1, None, None, 0, 1, None, None, None, None, None, 0, None)
     assert rc == abi.ALB_ERR_ARG
 
 
