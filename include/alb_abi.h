@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define ALB_ABI_VERSION 2
+#define ALB_ABI_VERSION 3
 
 /* photon types, reference enum order (BOTTOM_REFLECTION = auto() = 1, ...) */
 enum alb_photon_type {
@@ -86,7 +86,8 @@ enum alb_counter {
   ALB_CTR_KNN_DESCENT = 15,  /* seed-descent levels                          */
   ALB_CTR_SMALL_SCENE = 16,  /* batched queries whose scene map holds fewer
                                 than k records (rows left at -1 / NaN)      */
-  ALB_N_COUNTERS = 17
+  ALB_CTR_REPLAY_MISSES = 17,/* replayed events whose record was not found   */
+  ALB_N_COUNTERS = 18
 };
 
 /* Scene constants, computed on the host in FP64 from SimulationConfig
@@ -125,6 +126,12 @@ typedef struct alb_scene {
   const double* phase_mass; /* [phase_n] CDF mass of each scattering-angle bin used
                                by the gather (bin i spans [i, i+1) * float32(pi -
                                eps)/phase_n), evaluated with np.interp semantics */
+  /* adjoint only (NULL otherwise): derivatives with respect to the phase
+   * function's parameter (Henyey-Greenstein g) */
+  const double* phase_score;     /* [phase_n] d log(sampling probability)/dg of the
+                                    inverse-CDF segment: [0] the atom below cdf[0],
+                                    [i] the interval [cdf[i-1], cdf[i])            */
+  const double* phase_dlog_mass; /* [phase_n] d log(phase_mass[i])/dg               */
 } alb_scene_t;
 
 typedef struct alb_rng {
@@ -326,6 +333,59 @@ int alb_contrib_batched(const alb_scene_t* scenes, int32_t n_scenes, int32_t k,
                         const int32_t* order, int64_t c0, int64_t cn, const int32_t* nbr,
                         const double* knorm, double* waveforms, const int64_t* wf_offset,
                         const int32_t* win0, int32_t win_len, void* stream);
+
+
+/* ------------------------------------------------------------------ adjoint
+ * Waveform-loss gradients by path replay (Vicini, Speierer and Jakob, "Path
+ * replay backpropagation", SIGGRAPH 2021) with detached sampling: the paths
+ * of the forward and backward traces are held fixed and every gather
+ * contribution c = E_q E_p K mult is differentiated through its log-weight,
+ * d log c = d log mult + S(path of p) + S(path of q), where for a path prefix
+ * with n scatters already weighted by omega0, a water length L whose optical
+ * depth was accumulated, m floor reflections and phase-sampling score sum G,
+ *   S = (n + [record is a scatter]) dlog(alpha_L) - L d(alpha_L)
+ *       + n dlog(omega0) + m dlog(albedo) + G dg.
+ * No counterpart in the reference (NumPy forward only). */
+
+/* Adjoint of alb_contrib_batched for one photon type: with G = dloss (same
+ * layout as the waveforms) and w = G[bin] * c for every (query, neighbour)
+ * contribution c, adds w to a_record[map.perm[neighbour]] and to
+ * a_query[query] (input slots of the forward / backward sinks), and per scene
+ * to direct[3 s + 0] the sum of w d log(phase_mass)/dg (scatter map),
+ * direct[3 s + 1] the sum of w d log(BRDF)/d(alpha^2) (surface reflection) and
+ * direct[3 s + 2] the sum of w over bottom contributions (d log / d albedo =
+ * 1 / albedo).  Same slot range, rows and scene table as alb_contrib_batched. */
+int alb_adjoint_contrib(const alb_scene_t* scenes, int32_t n_scenes, int32_t k,
+                        const alb_map_t* map, int32_t photon_type, const alb_queries_t* q,
+                        const int32_t* order, int64_t c0, int64_t cn, const int32_t* nbr,
+                        const double* knorm, const double* dloss, const int64_t* wf_offset,
+                        double* a_record, double* a_query, double* direct, void* stream);
+
+/* Index of one sink for the replay: keys ((photon_offsets[scene] + photon)
+ * << 16 | event ordinal), from the sink's tag and scene arrays, sorted with
+ * their slots. */
+typedef struct alb_replay_index {
+  const uint64_t* keys;     /* [n] sorted                                          */
+  const int32_t* slots;     /* [n] sink slot of each key                           */
+  int64_t n;
+  const double* weight;     /* [sink slots] adjoint weight of each record / query  */
+} alb_replay_index_t;
+
+int alb_replay_index_workspace_size(int64_t n, size_t* bytes);
+int alb_replay_index_build(const int64_t* tags, const int32_t* scenes, int64_t n,
+                           const int64_t* photon_offsets, uint64_t* keys_out, int32_t* slots_out,
+                           void* workspace, size_t workspace_bytes, void* stream);
+
+/* Replay the photons of alb_trace_batched (same scenes, offsets, rng, pass)
+ * with the path statistics above, look every record / query up in index[4]
+ * and add, per scene s, acc[5 s + ...] += weight * (n, [scatter record], L,
+ * m, G) of its prefix.  Nothing is stored per path.  work_counter as in
+ * alb_trace_batched; counters[ALB_CTR_REPLAY_MISSES] counts events whose
+ * record was not in the index (0 when the sinks did not overflow). */
+int alb_replay_batched(const alb_scene_t* scenes, int32_t n_scenes, const int64_t* photon_offsets,
+                       const alb_rng_t* rng, int32_t pass, const alb_replay_index_t* index,
+                       unsigned long long* work_counter, int32_t block, int32_t refill_interval,
+                       double* acc, unsigned long long* counters, void* stream);
 
 #ifdef __cplusplus
 }

@@ -47,6 +47,8 @@ def lib():
             L.orc_uniform_at.restype = C.c_double
             L.orc_trace.argtypes = [P, P, C.c_int32, C.c_int64, C.c_int64, P, P, P]
             L.orc_trace.restype = C.c_int64
+            L.orc_trace_stats.argtypes = [P, P, C.c_int32, C.c_int64, C.c_int64, P, P, P, P]
+            L.orc_trace_stats.restype = C.c_int64
             L.orc_knn_one.argtypes = [P, P, P, C.c_int64, C.c_double, C.c_double, C.c_double,
                                       C.c_int32, P, P]
             L.orc_knn_one.restype = C.c_int
@@ -174,11 +176,13 @@ def scatter_direction(model, din, u1, u2):
 class HostScene:
     """alb_scene_t with host phase tables, kept alive with the struct."""
 
-    def __init__(self, model):
+    def __init__(self, model, phase_score: np.ndarray | None = None):
         self.model = model
         self.theta = np.ascontiguousarray(model.phase_theta, dtype=np.float64)
         self.cdf = np.ascontiguousarray(model.phase_cdf, dtype=np.float64)
-        self.s = abi.make_scene(model, _ptr(self.theta), _ptr(self.cdf))
+        self.score = None if phase_score is None else np.ascontiguousarray(phase_score, dtype=np.float64)
+        self.s = abi.make_scene(model, _ptr(self.theta), _ptr(self.cdf), 0,
+                                _ptr(self.score) if self.score is not None else 0)
 
 
 _FIELDS_F32 = ("x", "y", "z", "dx", "dy", "dz", "energy", "time")
@@ -203,14 +207,22 @@ def _alloc_sink(capacity: int, query: bool):
     return st, arrs
 
 
+class _StatsSink(C.Structure):
+    _fields_ = [("lam", C.c_void_p), ("gs", C.c_void_p), ("nsc", C.c_void_p), ("nfl", C.c_void_p)]
+
+
 def trace(model, key, pass_: int, photon_begin: int, n: int, replay: np.ndarray | None = None,
-          stream: int | None = None, per_photon: int = 12):
+          stream: int | None = None, per_photon: int = 12, stats: bool = False,
+          phase_score: np.ndarray | None = None):
     """Trace n photons on the CPU; returns (sinks, counters, n_exhausted).
 
     sinks[t] is a dict of NumPy arrays (records for the forward pass, queries
-    for the backward pass) truncated to the emitted count.
+    for the backward pass) truncated to the emitted count.  stats: also return
+    each record's adjoint path statistics ("lam" water length whose optical
+    depth was accumulated, "gs" phase-sampling score sum with the per-segment
+    scores phase_score, "nsc" scatters, "nfl" floor reflections before it).
     """
-    hs = HostScene(model)
+    hs = HostScene(model, phase_score)
     rep = None
     if replay is not None:
         rep = np.ascontiguousarray(replay, dtype=np.float64)
@@ -232,8 +244,19 @@ def trace(model, key, pass_: int, photon_begin: int, n: int, replay: np.ndarray 
                 rec_arr[t] = sinks[t][0]
                 q_arr[t] = abi.Queries()
         ctr = np.zeros(abi.N_COUNTERS, dtype=np.uint64)
-        ex = lib().orc_trace(C.byref(hs.s), C.byref(rng), pass_, photon_begin, n,
-                             C.cast(rec_arr, C.c_void_p), C.cast(q_arr, C.c_void_p), _ptr(ctr))
+        if stats:
+            st_arrs = [{"lam": np.zeros(cap), "gs": np.zeros(cap), "nsc": np.zeros(cap, dtype=np.int32),
+                        "nfl": np.zeros(cap, dtype=np.int32)} for _ in range(4)]
+            ss = (_StatsSink * 4)()
+            for t in range(4):
+                for f in ("lam", "gs", "nsc", "nfl"):
+                    setattr(ss[t], f, _ptr(st_arrs[t][f]))
+            ex = lib().orc_trace_stats(C.byref(hs.s), C.byref(rng), pass_, photon_begin, n,
+                                       C.cast(rec_arr, C.c_void_p), C.cast(q_arr, C.c_void_p), _ptr(ctr),
+                                       C.cast(ss, C.c_void_p))
+        else:
+            ex = lib().orc_trace(C.byref(hs.s), C.byref(rng), pass_, photon_begin, n,
+                                 C.cast(rec_arr, C.c_void_p), C.cast(q_arr, C.c_void_p), _ptr(ctr))
         counts = [int(s[1]["count"][0]) for s in sinks]
         if max(counts) <= cap:
             break
@@ -242,7 +265,10 @@ def trace(model, key, pass_: int, photon_begin: int, n: int, replay: np.ndarray 
     for t in range(4):
         arrs = sinks[t][1]
         c = counts[t]
-        out.append({f: arrs[f][:c].copy() for f in (*_FIELDS_F32, "tag")})
+        d = {f: arrs[f][:c].copy() for f in (*_FIELDS_F32, "tag")}
+        if stats:
+            d.update({f: st_arrs[t][f][:c].copy() for f in ("lam", "gs", "nsc", "nfl")})
+        out.append(d)
     return out, ctr, int(ex)
 
 

@@ -219,6 +219,33 @@ int orc_refract(const double d[3], const double n[3], double eta, double out[3])
 
 /* ---------------------------------------------------------- record sinks */
 
+/* Path statistics of the adjoint (test of the replay trace): water length
+ * whose optical depth was accumulated, phase-sampling score sum, scatters
+ * weighted by omega0 and floor reflections so far. */
+typedef struct orc_path {
+  double lam, gs;
+  int32_t nsc, nfl;
+} orc_path;
+
+/* optional per-record statistics, parallel to one sink */
+typedef struct orc_stats_sink {
+  double* lam;
+  double* gs;
+  int32_t* nsc;
+  int32_t* nfl;
+} orc_stats_sink;
+
+static const orc_stats_sink* g_stats = NULL;  /* [4] while orc_trace_stats runs */
+static orc_path g_path;
+
+static void orc_emit_stats(int type, unsigned long long i, int64_t capacity) {
+  if (!g_stats || !g_stats[type].lam || (int64_t)i >= capacity) return;
+  g_stats[type].lam[i] = g_path.lam;
+  g_stats[type].gs[i] = g_path.gs;
+  g_stats[type].nsc[i] = g_path.nsc;
+  g_stats[type].nfl[i] = g_path.nfl;
+}
+
 static void orc_emit_record(const alb_records_t* r, const double p[3], const double d[3],
                             float E, float t, int64_t tag) {
   unsigned long long i = (*r->count)++;
@@ -312,6 +339,17 @@ void orc_lambert(double u1, double u2, double out[3]) {
 /* new direction after a scatter: theta by inverse CDF of the phase table,
  * phi uniform, rotated from +z onto the incoming direction with a float32
  * matrix, stored float32 (physics/scatter/direction.py:7-51) */
+/* inverse-CDF segment of u: 0 below cdf[0], i + 1 for cdf[i] <= u < cdf[i+1] */
+static int orc_phase_segment(double u, const alb_scene_t* sc) {
+  const double* c = sc->phase_cdf;
+  const int n = sc->phase_n;
+  if (u < c[0]) return 0;
+  if (u >= c[n - 1]) return n - 1;
+  int i = 0;
+  while (i + 1 < n && c[i + 1] <= u) ++i;
+  return i + 1;
+}
+
 void orc_scatter_direction(const alb_scene_t* sc, const double din[3], double u1, double u2, double out[3]) {
   double theta = orc_interp(u1, sc->phase_cdf, sc->phase_theta, sc->phase_n);
   double phi = 2.0 * ORC_PI * u2;
@@ -343,6 +381,10 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
 
   float E = 1.0f, tau = 0.0f, tau_t = INFINITY;
   int64_t event_no = 0;
+  g_path.lam = 0.0;
+  g_path.gs = 0.0;
+  g_path.nsc = 0;
+  g_path.nfl = 0;
   const double y_ws = sc->y_ws, y_sf = sc->y_sf;
   const double up[3] = {0.0, 1.0, 0.0}, down[3] = {0.0, -1.0, 0.0};
   const double alpha = (double)sc->alpha_l;
@@ -373,11 +415,15 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
       double len = orc_norm3(sv);
       tau = (float)((double)tau + alpha * len);
       int scatter = tau >= tau_t;
+      if (!scatter) g_path.lam += len;
       p[0] = nx[0]; p[1] = nx[1]; p[2] = nx[2];
       if (ctr) ctr[ALB_CTR_WATER_STEPS]++;
       if (scatter) {
+        g_path.lam += len;
         if (fwd) orc_emit_record(&out[ALB_SCATTER], p, d, E, t_evt, tag);
         else orc_emit_query(&q[ALB_SCATTER], p, d, E, t_evt, tag);
+        orc_emit_stats(ALB_SCATTER, (fwd ? *out[ALB_SCATTER].count : *q[ALB_SCATTER].count) - 1,
+                       fwd ? out[ALB_SCATTER].capacity : q[ALB_SCATTER].capacity);
         event_no++;
         if (ctr) ctr[ALB_CTR_SCATTER]++;
         /* sample_scattering_directions_batch (physics/scatter/direction.py:7-51) */
@@ -385,6 +431,8 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
         double u2 = orc_uniform(&st);
         double nd[3];
         orc_scatter_direction(sc, d, u1, u2, nd);
+        if (sc->phase_score) g_path.gs += sc->phase_score[orc_phase_segment(u1, sc)];
+        g_path.nsc++;
         d[0] = nd[0]; d[1] = nd[1]; d[2] = nd[2];
         E = E * sc->omega0;
         tau = tau - tau_t;
@@ -403,9 +451,15 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
       double T = 1.0 - Rf;
       if (fwd) {
         orc_emit_record(&out[ALB_SURFACE_REFLECTION], P, d, E, t_evt, tag);
+        orc_emit_stats(ALB_SURFACE_REFLECTION, *out[ALB_SURFACE_REFLECTION].count - 1,
+                       out[ALB_SURFACE_REFLECTION].capacity);
       } else {
         orc_emit_query(&q[ALB_SURFACE_REFLECTION], P, d, E, t_evt, tag);
         orc_emit_query(&q[ALB_SURFACE_TRANSMISSION_UP], P, d, E, t_evt, tag);
+        orc_emit_stats(ALB_SURFACE_REFLECTION, *q[ALB_SURFACE_REFLECTION].count - 1,
+                       q[ALB_SURFACE_REFLECTION].capacity);
+        orc_emit_stats(ALB_SURFACE_TRANSMISSION_UP, *q[ALB_SURFACE_TRANSMISSION_UP].count - 1,
+                       q[ALB_SURFACE_TRANSMISSION_UP].capacity);
       }
       event_no++;
       if (ctr) ctr[ALB_CTR_ENTER]++;
@@ -428,7 +482,11 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
       double ci = d[1];
       double Rf = orc_schlick(ci, sc->F0);
       int reflect = orc_uniform(&st) < Rf;
-      if (!reflect && fwd) orc_emit_record(&out[ALB_SURFACE_TRANSMISSION_UP], P, d, E, t_evt, tag);
+      if (!reflect && fwd) {
+        orc_emit_record(&out[ALB_SURFACE_TRANSMISSION_UP], P, d, E, t_evt, tag);
+        orc_emit_stats(ALB_SURFACE_TRANSMISSION_UP, *out[ALB_SURFACE_TRANSMISSION_UP].count - 1,
+                       out[ALB_SURFACE_TRANSMISSION_UP].capacity);
+      }
       event_no++;
       if (ctr) ctr[ALB_CTR_EXIT]++;
       double rd[3];
@@ -442,7 +500,10 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
       double len = orc_norm3(sv);
       double rem = (1.0 - f) * len;
       for (int i = 0; i < 3; ++i) p[i] = P[i] + d[i] * rem;
-      if (reflect) tau = (float)((double)tau + alpha * len);
+      if (reflect) {
+        tau = (float)((double)tau + alpha * len);
+        g_path.lam += len;
+      }
     } else { /* ST_FLOOR */
       /* handle_seafloor (core/simulation.py:504-570) */
       double sv[3] = {nx[0] - p[0], nx[1] - p[1], nx[2] - p[2]};
@@ -453,6 +514,9 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
       for (int i = 0; i < 3; ++i) P[i] = p[i] + f * sv[i];
       if (fwd) orc_emit_record(&out[ALB_BOTTOM_REFLECTION], P, d, E, t_evt, tag);
       else orc_emit_query(&q[ALB_BOTTOM_REFLECTION], P, d, E, t_evt, tag);
+      orc_emit_stats(ALB_BOTTOM_REFLECTION,
+                     (fwd ? *out[ALB_BOTTOM_REFLECTION].count : *q[ALB_BOTTOM_REFLECTION].count) - 1,
+                     fwd ? out[ALB_BOTTOM_REFLECTION].capacity : q[ALB_BOTTOM_REFLECTION].capacity);
       event_no++;
       if (ctr) ctr[ALB_CTR_FLOOR]++;
       /* cosine-weighted hemisphere about +y (physics/reflection/lambertian.py:7-27) */
@@ -463,6 +527,8 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
       double rem = (1.0 - f) * len;
       for (int i = 0; i < 3; ++i) p[i] = P[i] + d[i] * rem;
       tau = (float)((double)tau + alpha * len);
+      g_path.lam += len;
+      g_path.nfl++;
       E = E * (float)sc->floor_albedo;
     }
     if (st.exhausted) return 1;
@@ -485,6 +551,18 @@ int64_t orc_trace(const alb_scene_t* sc, const alb_rng_t* rng, int32_t pass,
     }
   }
   return exhausted;
+}
+
+/* orc_trace with the adjoint path statistics of every record / query
+ * written into stats[type] at the record's slot (test of the replay trace). */
+int64_t orc_trace_stats(const alb_scene_t* sc, const alb_rng_t* rng, int32_t pass,
+                        int64_t photon_begin, int64_t n, const alb_records_t* out,
+                        const alb_queries_t* q, unsigned long long* counters,
+                        const orc_stats_sink* stats) {
+  g_stats = stats;
+  int64_t ex = orc_trace(sc, rng, pass, photon_begin, n, out, q, counters);
+  g_stats = NULL;
+  return ex;
 }
 
 /* --------------------------------------------------------------- k-NN */

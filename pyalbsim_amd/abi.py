@@ -5,9 +5,9 @@ import ctypes as C
 
 import numpy as np
 
-ALB_ABI_VERSION = 2
+ALB_ABI_VERSION = 3
 N_TYPES = 4
-N_COUNTERS = 17
+N_COUNTERS = 18
 
 ALB_OK = 0
 ALB_ERR_ARG = -1
@@ -25,10 +25,10 @@ PASS_BACKWARD = 1
 
 CTR_PATHS, CTR_AIR_LEGS, CTR_WATER_STEPS, CTR_SCATTER, CTR_ENTER, CTR_EXIT, CTR_FLOOR, \
     CTR_REPLAY_EXHAUSTED, CTR_QUERIES, CTR_CANDIDATES, CTR_NODES, CTR_OVERFLOW, CTR_DEPTH_SOLVES, \
-    CTR_KNN_STEPS, CTR_KNN_SORTS, CTR_KNN_DESCENT, CTR_SMALL_SCENE = range(17)
+    CTR_KNN_STEPS, CTR_KNN_SORTS, CTR_KNN_DESCENT, CTR_SMALL_SCENE, CTR_REPLAY_MISSES = range(18)
 COUNTER_NAMES = ["paths", "air_legs", "water_steps", "scatter", "enter", "exit", "floor",
                  "replay_exhausted", "queries", "candidates", "nodes", "overflow", "depth_solves",
-                 "knn_steps", "knn_sorts", "knn_descent", "small_scene"]
+                 "knn_steps", "knn_sorts", "knn_descent", "small_scene", "replay_misses"]
 
 
 class Scene(C.Structure):
@@ -42,7 +42,7 @@ class Scene(C.Structure):
         ("absorption", C.c_double), ("alpha_l", C.c_float), ("omega0", C.c_float),
         ("n_substeps", C.c_int64), ("steps", C.c_int32), ("mult", C.c_int32), ("k", C.c_int32),
         ("phase_n", C.c_int32), ("phase_theta", C.c_void_p), ("phase_cdf", C.c_void_p),
-        ("phase_mass", C.c_void_p),
+        ("phase_mass", C.c_void_p), ("phase_score", C.c_void_p), ("phase_dlog_mass", C.c_void_p),
     ]
 
 
@@ -74,7 +74,12 @@ class SceneIndex(C.Structure):
                 ("root", C.c_void_p)]
 
 
-def make_scene(model, theta_ptr: int, cdf_ptr: int, mass_ptr: int = 0) -> Scene:
+class ReplayIndex(C.Structure):
+    _fields_ = [("keys", C.c_void_p), ("slots", C.c_void_p), ("n", C.c_int64), ("weight", C.c_void_p)]
+
+
+def make_scene(model, theta_ptr: int, cdf_ptr: int, mass_ptr: int = 0, score_ptr: int = 0,
+               dlog_mass_ptr: int = 0) -> Scene:
     """Pack a SceneModel; the phase-table pointers are device or host addresses."""
     s = Scene()
     s.y_ws = model.y_ws
@@ -108,6 +113,8 @@ def make_scene(model, theta_ptr: int, cdf_ptr: int, mass_ptr: int = 0) -> Scene:
     s.phase_theta = theta_ptr
     s.phase_cdf = cdf_ptr
     s.phase_mass = mass_ptr
+    s.phase_score = score_ptr
+    s.phase_dlog_mass = dlog_mass_ptr
     return s
 
 

@@ -1,0 +1,281 @@
+"""Waveform-loss gradients of the batched form by path replay.
+
+Parameters per scene, theta = (a, b, g, depth, albedo, roughness):
+absorption and scattering coefficients (1/m), Henyey-Greenstein asymmetry,
+water depth (m), floor albedo and surface roughness r (GGX alpha = r^2), the
+quantities the reference's SimulationConfig holds (config/water.py,
+sea_floor.py, sea_surface.py) plus the HG asymmetry of the sweep.
+
+Estimator.  The forward pass adds c = E_q E_p K mult for every (sensor-path
+vertex q, k-nearest map record p) into bin(t_q + t_p).  With the sampled paths
+held fixed (detached sampling; the k-NN selection and radius K are piecewise
+constant in theta and held fixed too),
+
+    d c / d theta = c (d log mult + S_p + S_q),
+
+where S of a path prefix with n scatters already weighted by omega0, a water
+length L whose optical depth was accumulated, m floor reflections and
+phase-sampling score sum G is (alpha = alpha_L, the lidar attenuation used as
+free-path rate; omega0 = (alpha - a) / alpha; [scatter] = 1 when the record is
+itself a scatter vertex, whose free-path density alpha e^{-alpha L} counts):
+
+    S = ([scatter] / alpha + n / (alpha - a) - L) d alpha
+        - n / (alpha - a) d a |alpha  +  m / albedo d albedo  +  G d g.
+
+alpha depends on (a, b, g) through the Churnside formula
+(physics/models/water.py:163-182, restated in model.lidar_attenuation; g
+enters through the HG backscatter fraction).  d log mult is the phase-mass
+derivative (scatter map, g), the GGX BRDF derivative (surface reflection,
+roughness) and 1 / albedo (bottom).
+
+The GPU work is: one adjoint-gather launch per photon type (the adjoint
+weights w = G[bin] c to records, queries and per-scene direct sums), one
+radix sort per sink to index records by (photon, event), and one replay trace
+per pass that recomputes every path from its Philox stream and reduces
+w * S per scene.  No path is stored.
+
+Depth moves the seafloor plane: the paths' support changes, which detached
+sampling does not cover.  Its derivative is taken by a central difference
+with common random numbers (two more batched forward passes), or left out
+(depth_grad="none").
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi, runtime
+from .model import (SCATTER_DIVISIONS, henyey_greenstein_backscatter, henyey_greenstein_table, lidar_attenuation,
+                    phase_bin_mass)
+from .sweep import shot_config
+
+PARAMS = ("a", "b", "g", "depth", "albedo", "roughness")
+
+
+def phase_g_tables(g: float, h: float = 1e-5):
+    """(score, dlog_mass) of the HG table at g: d log of the inverse-CDF
+    segment probabilities ([0] the atom cdf[0], [i] cdf[i] - cdf[i-1]) and of
+    the gather's per-bin phase masses, by central differences in g."""
+    def seg(gg):
+        th, _, cdf = henyey_greenstein_table(gg)
+        p = np.empty_like(cdf)
+        p[0] = cdf[0]
+        p[1:] = np.diff(cdf)
+        return th, cdf, p
+
+    thp, cdp, pp = seg(g + h)
+    thm, cdm, pm = seg(g - h)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        score = (np.log(pp) - np.log(pm)) / (2 * h)
+        mp = phase_bin_mass(thp, cdp)
+        mm = phase_bin_mass(thm, cdm)
+        dlm = (np.log(mp) - np.log(mm)) / (2 * h)
+    score[~np.isfinite(score)] = 0.0
+    dlm[~np.isfinite(dlm)] = 0.0
+    assert len(score) == SCATTER_DIVISIONS
+    return score, dlm
+
+
+@dataclass
+class SceneDerivs:
+    alpha: float     # float32 lidar attenuation used by the kernels
+    a: float
+    albedo: float
+    roughness: float
+    dalpha: np.ndarray  # d alpha / d (a, b, g)
+
+
+def scene_derivatives(model, theta) -> SceneDerivs:
+    """d alpha_L / d(a, b, g) by central differences of the Churnside formula."""
+    a, b, g, depth, albedo, rough = [float(v) for v in theta]
+    layer = model.config.water.layers[0]
+    H = model.config.scene.flying_height
+    lam = model.config.laser.wavelength
+    div = model.config.laser.divergence_angle
+
+    def al(aa, bb, gg):
+        return lidar_attenuation(aa, bb, layer.salinity, lam, H, div, henyey_greenstein_backscatter(gg))
+
+    d = np.zeros(3)
+    for i, (x, f) in enumerate(((a, lambda v: al(v, b, g)), (b, lambda v: al(a, v, g)),
+                                (g, lambda v: al(a, b, v)))):
+        h = 1e-6 * max(abs(x), 1e-3)
+        d[i] = (f(x + h) - f(x - h)) / (2 * h)
+    return SceneDerivs(model.alpha_l, a, albedo, rough, d)
+
+
+def model_from_theta(theta):
+    """SceneModel of the sweep geometry for theta = (a, b, g, depth, albedo, roughness)."""
+    from .model import build_scene
+
+    a, b, g, depth, albedo, rough = [float(v) for v in theta]
+    return build_scene(shot_config(depth, a, b, albedo, rough), hg_g=g)
+
+
+def _replay_index(torch, sink, count, offsets, dev):
+    L = runtime.load_library()
+    keys = torch.empty(max(count, 1), dtype=torch.int64, device=dev)
+    slots = torch.empty(max(count, 1), dtype=torch.int32, device=dev)
+    if count == 0:
+        return keys, slots, None
+    b = C.c_size_t(0)
+    runtime._check(L.alb_replay_index_workspace_size(count, C.byref(b)), "alb_replay_index_workspace_size")
+    ws = torch.empty(b.value, dtype=torch.uint8, device=dev)
+    runtime._check(L.alb_replay_index_build(runtime._p(sink.tensors["tag"]), runtime._p(sink.tensors["scene"]), count,
+                                            runtime._p(offsets), runtime._p(keys), runtime._p(slots), runtime._p(ws),
+                                            b.value, C.c_void_p(runtime._stream_ptr())), "alb_replay_index_build")
+    return keys, slots, ws
+
+
+def adjoint_sums(res, dloss):
+    """Per-scene sums from a batched forward result kept with adjoint=True.
+
+    dloss: flat float64 device tensor laid out like res.waveforms.  Returns
+    (acc [S, 5] = sums of w (n, [scatter], L, m, G) over records and queries,
+    direct [S, 3] = sums of w d log mult for (g, alpha^2, albedo * albedo)).
+    """
+    import torch
+
+    st = res.state
+    if not st:
+        raise ValueError("simulate_batched(..., adjoint=True) keeps the state the adjoint needs")
+    L = runtime.load_library()
+    tab = st["table"]
+    n = tab.n
+    dev = res.waveforms.device
+    s = C.c_void_p(runtime._stream_ptr())
+    direct = torch.zeros(3 * n, dtype=torch.float64, device=dev)
+    a_rec = [torch.zeros(max(st["rcounts"][t], 1), dtype=torch.float64, device=dev) for t in range(4)]
+    a_q = [torch.zeros(max(st["qcounts"][t], 1), dtype=torch.float64, device=dev) for t in range(4)]
+    dl = dloss.contiguous()
+    for t, rw in st["rows"].items():
+        m = st["maps"][t]
+        q = st["queries"][t]
+        runtime._check(L.alb_adjoint_contrib(runtime._p(tab.dev), n, tab.k, C.byref(m.struct), t, C.byref(q.struct),
+                                             runtime._p(rw["order"]), 0, st["qcounts"][t], runtime._p(rw["nbr"]),
+                                             runtime._p(rw["knorm"]), runtime._p(dl), runtime._p(st["wf_off"]),
+                                             runtime._p(a_rec[t]), runtime._p(a_q[t]), runtime._p(direct), s),
+                       "alb_adjoint_contrib")
+    acc = torch.zeros(5 * n, dtype=torch.float64, device=dev)
+    ctr = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device=dev)
+    keep = []
+    for pass_, sinks, counts, weights, nph in ((abi.PASS_FORWARD, st["records"], st["rcounts"], a_rec, st["nf"]),
+                                               (abi.PASS_BACKWARD, st["queries"], st["qcounts"], a_q, st["nb"])):
+        offsets = torch.as_tensor(np.concatenate([[0], np.cumsum(nph)]).astype(np.int64), device=dev)
+        idx = (abi.ReplayIndex * 4)()
+        for t in range(4):
+            keys, slots, ws = _replay_index(torch, sinks[t], counts[t], offsets, dev)
+            keep.append((keys, slots, ws))
+            idx[t].keys = runtime._p(keys)
+            idx[t].slots = runtime._p(slots)
+            idx[t].n = counts[t] if t in st["rows"] else 0
+            idx[t].weight = runtime._p(weights[t])
+        rng = abi.make_rng(st["key"], pass_ | (int(st["base"]) << 1))
+        work = torch.zeros(1, dtype=torch.int64, device=dev)
+        runtime._check(L.alb_replay_batched(runtime._p(tab.dev), n, runtime._p(offsets), C.byref(rng), pass_,
+                                            C.cast(idx, C.c_void_p), runtime._p(work), st["block"], st["refill"],
+                                            runtime._p(acc), runtime._p(ctr), s), "alb_replay_batched")
+        keep.append((offsets, work))
+    misses = int(ctr[abi.CTR_REPLAY_MISSES].item())
+    if misses:
+        raise runtime.AlbError(f"path replay did not find {misses} records")
+    del keep
+    return acc.view(n, 5).cpu().numpy(), direct.view(n, 3).cpu().numpy()
+
+
+def gradients_from_sums(acc, direct, derivs) -> np.ndarray:
+    """theta-gradients [S, 6] (depth column left at 0) from the per-scene sums."""
+    out = np.zeros((len(derivs), 6))
+    for s, d in enumerate(derivs):
+        c0, cs, c2, c3, c4 = acc[s]
+        alpha, a = d.alpha, d.a
+        dl_dalpha = cs / alpha + c0 / (alpha - a) - c2
+        out[s, 0] = dl_dalpha * d.dalpha[0] - c0 / (alpha - a)
+        out[s, 1] = dl_dalpha * d.dalpha[1]
+        out[s, 2] = dl_dalpha * d.dalpha[2] + c4 + direct[s, 0]
+        out[s, 4] = (c3 + direct[s, 2]) / d.albedo
+        out[s, 5] = direct[s, 1] * 4.0 * d.roughness ** 3  # alpha^2 = r^4
+    return out
+
+
+class BatchedWaveformFn:
+    """torch.autograd.Function over the batched form: theta [S, 6] (float64)
+    -> gated waveforms [S, 4, win_len] (float64, on the GPU)."""
+
+    @staticmethod
+    def build():
+        import torch
+
+        class _Fn(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, theta, spec):
+                from .execution.batched import simulate_batched
+
+                th = theta.detach().cpu().numpy().astype(np.float64)
+                models = [model_from_theta(row) for row in th]
+                res = simulate_batched(models, spec["n_forward"], spec["n_backward"], spec["key"],
+                                       win_len=spec.get("win_len", 1000), strict=False, adjoint=True,
+                                       block=spec.get("block", 256), refill=spec.get("refill", 4))
+                wl = spec.get("win_len", 1000)
+                out = torch.stack([res.gated(s, models[s].gate(wl)) for s in range(len(models))])
+                ctx.res = res
+                ctx.models = models
+                ctx.theta = th
+                ctx.spec = spec
+                ctx.device = theta.device
+                return out
+
+            @staticmethod
+            def backward(ctx, grad_out):
+                res, models, th, spec = ctx.res, ctx.models, ctx.theta, ctx.spec
+                wl = spec.get("win_len", 1000)
+                dloss = torch.zeros_like(res.waveforms)
+                go = grad_out.to(res.waveforms.device, torch.float64)
+                for s, m in enumerate(models):
+                    g0, gl = m.gate(wl)
+                    o = int(res.offsets[s])
+                    nb = m.n_bins
+                    dloss[o:o + 4 * nb].view(4, nb)[:, g0:g0 + gl] = go[s]
+                acc, direct = adjoint_sums(res, dloss)
+                derivs = [scene_derivatives(m, th[s]) for s, m in enumerate(models)]
+                grads = gradients_from_sums(acc, direct, derivs)
+                if spec.get("depth_grad", "fd") == "fd":
+                    grads[:, 3] = depth_fd(th, spec, go)
+                ctx.res = None
+                return torch.as_tensor(grads, dtype=torch.float64, device=ctx.device), None
+
+        return _Fn
+
+
+def depth_fd(th, spec, go, rel: float = 1e-3):
+    """d loss / d depth by central differences with common random numbers."""
+    import torch
+
+    from .execution.batched import simulate_batched
+
+    wl = spec.get("win_len", 1000)
+    outs = []
+    for sign in (1.0, -1.0):
+        t2 = th.copy()
+        t2[:, 3] = th[:, 3] * (1.0 + sign * rel)
+        models = [model_from_theta(row) for row in t2]
+        r = simulate_batched(models, spec["n_forward"], spec["n_backward"], spec["key"], win_len=wl, strict=False,
+                             block=spec.get("block", 256), refill=spec.get("refill", 4))
+        outs.append(torch.stack([r.gated(s, models[s].gate(wl)) for s in range(len(models))]))
+    d = (outs[0] - outs[1]) * go
+    return (d.sum(dim=(1, 2)) / torch.as_tensor(2.0 * rel * th[:, 3], device=d.device)).cpu().numpy()
+
+
+_FN = None
+
+
+def batched_waveforms(theta, n_forward, n_backward, key, **spec):
+    """Differentiable gated waveforms [S, 4, win_len] of the scenes theta [S, 6]."""
+    global _FN
+    if _FN is None:
+        _FN = BatchedWaveformFn.build()
+    spec = dict(spec, n_forward=n_forward, n_backward=n_backward, key=key)
+    return _FN.apply(theta, spec)

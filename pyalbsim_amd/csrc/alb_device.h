@@ -126,6 +126,35 @@ __device__ __forceinline__ double interp(double x, const double* __restrict__ xp
   return r;
 }
 
+// interp() that also reports where x fell: seg = 0 below xp[0] (the clamped
+// atom at fp[0]), seg = i + 1 inside [xp[i], xp[i+1]), n - 1 at or above xp[n-1].
+// Same arithmetic as interp(), so the returned value is identical.
+__device__ __forceinline__ double interp_seg(double x, const double* __restrict__ xp,
+                                             const double* __restrict__ fp, int n, int* seg) {
+  if (x < xp[0]) {
+    *seg = 0;
+    return fp[0];
+  }
+  if (x >= xp[n - 1]) {
+    *seg = n - 1;
+    return fp[n - 1];
+  }
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (xp[mid] <= x) lo = mid; else hi = mid;
+  }
+  *seg = lo + 1;
+  if (xp[lo] == x) return fp[lo];
+  const double slope = (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]);
+  double r = slope * (x - xp[lo]) + fp[lo];
+  if (isnan(r)) {
+    r = slope * (x - xp[lo + 1]) + fp[lo + 1];
+    if (isnan(r) && fp[lo] == fp[lo + 1]) r = fp[lo];
+  }
+  return r;
+}
+
 // np.interp on the theta grid itself (uniform spacing up to rounding): locate
 // the node by division and correct by one in either direction.
 __device__ __forceinline__ double interp_uniform_grid(double x, const double* __restrict__ xp,
@@ -219,6 +248,48 @@ __device__ __forceinline__ float brdf_reflect(float px, float py, float pz, doub
   const double G = smith_g1(ndwi, alpha) * smith_g1(ndwo, alpha);
   const double F = schlick(widh, F0);
   const double den = 4.0 * ndwi * ndwo + ALB_FLT_EPS;
+  return (float)((D * G * F) / den);
+}
+
+// brdf_reflect together with d log(BRDF) / d(alpha^2) (0 where the BRDF is 0):
+// log D = log a2 - log(pi u^2 + eps) with u = (a2 - 1) c^2 + 1, and
+// log G1(x) = log 2x - log(x + q) with q = sqrt(a2 + (1 - a2) x^2).
+__device__ __forceinline__ float brdf_reflect_dlog(float px, float py, float pz, double sx, double sy,
+                                                   double sz, double alpha, double F0, double* dlog_da2) {
+  float wx = -px, wy = -py, wz = -pz;
+  normalize_f32(wx, wy, wz);
+  double ox = -sx, oy = -sy, oz = -sz;
+  normalize_f64_keep_zero(ox, oy, oz);
+  double hx = (double)wx + ox, hy = (double)wy + oy, hz = (double)wz + oz;
+  {
+    double n = norm3(hx, hy, hz);
+    if (n == 0.0) n = 1.0;
+    hx = hx / n;
+    hy = hy / n;
+    hz = hz / n;
+  }
+  const double ndwi = clipd((double)wy, 0.0, 1.0);
+  const double ndwo = clipd(oy, 0.0, 1.0);
+  double widh = (double)wx * hx;
+  widh = widh + (double)wy * hy;
+  widh = widh + (double)wz * hz;
+  widh = clipd(widh, 0.0, 1.0);
+  const double ndh = clipd(hy, 0.0, 1.0);
+  *dlog_da2 = 0.0;
+  if (!(ndwi > 0.0 && ndwo > 0.0)) return 0.0f;
+  const double D = ggx_d(ndh, alpha);
+  const double G = smith_g1(ndwi, alpha) * smith_g1(ndwo, alpha);
+  const double F = schlick(widh, F0);
+  const double den = 4.0 * ndwi * ndwo + ALB_FLT_EPS;
+  const double a2 = alpha * alpha;
+  const double c2 = ndh * ndh;
+  const double u = (a2 - 1.0) * c2 + 1.0;
+  double dl = 1.0 / a2 - (2.0 * ALB_PI * u * c2) / (ALB_PI * u * u + ALB_FLT_EPS);
+  const double qi = sqrt(a2 + (1.0 - a2) * (ndwi * ndwi));
+  const double qo = sqrt(a2 + (1.0 - a2) * (ndwo * ndwo));
+  dl -= ((1.0 - ndwi * ndwi) / (2.0 * qi)) / (ndwi + qi);
+  dl -= ((1.0 - ndwo * ndwo) / (2.0 * qo)) / (ndwo + qo);
+  *dlog_da2 = dl;
   return (float)((D * G * F) / den);
 }
 
@@ -344,8 +415,7 @@ __device__ __forceinline__ uint64_t morton_key(double px, double py, double pz, 
 
 // Phase mass of the scattering-angle bin towards the sensor with the per-bin
 // masses tabulated on the host (same values as scatter_mult computes).
-__device__ __forceinline__ double scatter_mass(float px, float py, float pz, double sx, double sy, double sz,
-                                               const double* __restrict__ mass, int n) {
+__device__ __forceinline__ int scatter_bin(float px, float py, float pz, double sx, double sy, double sz, int n) {
   normalize_f32(px, py, pz);
   double ox = -sx, oy = -sy, oz = -sz;
   normalize_f64_keep_zero(ox, oy, oz);
@@ -358,7 +428,12 @@ __device__ __forceinline__ double scatter_mass(float px, float py, float pz, dou
   long long i = (long long)((ang - ALB_FLT_EPS) / (double)delta_f);
   if (i < 0) i = 0;
   if (i > n - 1) i = n - 1;
-  return mass[i];
+  return (int)i;
+}
+
+__device__ __forceinline__ double scatter_mass(float px, float py, float pz, double sx, double sy, double sz,
+                                               const double* __restrict__ mass, int n) {
+  return mass[scatter_bin(px, py, pz, sx, sy, sz, n)];
 }
 
 }  // namespace alb

@@ -209,6 +209,10 @@ struct Photon {
   int64_t s;          // current sub-step
   uint32_t event_no;  // events so far (record tag)
   bool alive;
+  // path statistics of the replay (adjoint) trace only
+  double lam;         // water path length whose optical depth was accumulated
+  double gs;          // sum of phase-sampling scores d log p / dg
+  int nsc, nfl;       // scatters (omega0 applied) and floor reflections so far
 };
 
 // record / query produced by one event
@@ -217,7 +221,20 @@ struct Emit {
   double x, y, z, dx, dy, dz;
   float E, t;
   uint32_t eno;
+  // replay trace: the path statistics of the record's prefix
+  double lam, gs;
+  int nsc, nfl;
 };
+
+template <bool STATS>
+__device__ __forceinline__ void snapshot(Emit& e, const Photon& ph) {
+  if constexpr (STATS) {
+    e.lam = ph.lam;
+    e.gs = ph.gs;
+    e.nsc = ph.nsc;
+    e.nfl = ph.nfl;
+  }
+}
 
 struct Tally {
   unsigned long long paths, air, ws, sc, en, ex, fl, exh, solve;
@@ -281,7 +298,10 @@ __device__ __forceinline__ void emit_photon(const alb_scene_t& sc, bool fwd, con
 }
 
 // Advance a live photon to its next event and apply it; fills e when the
-// event leaves a record (forward) or a query (backward).
+// event leaves a record (forward) or a query (backward).  STATS: also keep the
+// path statistics the adjoint needs (water length, scatter and floor counts,
+// phase-sampling score) and snapshot them into e; the physics is unchanged.
+template <bool STATS>
 __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, const alb_rng_t& rng, Stream& st,
                                            Photon& ph, Emit& e, Tally& c) {
   const double two_pi = 2.0 * ALB_PI;
@@ -318,6 +338,7 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
     e.E = ph.E;
     e.t = (float)(ph.s + j);
     e.eno = ph.event_no++;
+    snapshot<STATS>(e, ph);
     refract_y(ph.dx, ph.dy, ph.dz, 1.0, sc.eta);
     ph.E = (float)((double)ph.E * T);
     const double len = norm3(svx, svy, svz);
@@ -365,8 +386,18 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
       e.E = ph.E;
       e.t = (float)(ph.s + j);
       e.eno = ph.event_no++;
+      if constexpr (STATS) ph.lam += a1 * len;
+      snapshot<STATS>(e, ph);
       const double u = uniform(rng, st);
-      const double theta = interp(u, sc.phase_cdf, sc.phase_theta, sc.phase_n);
+      double theta;
+      if constexpr (STATS) {
+        int seg;
+        theta = interp_seg(u, sc.phase_cdf, sc.phase_theta, sc.phase_n, &seg);
+        if (sc.phase_score) ph.gs += sc.phase_score[seg];
+        ph.nsc++;
+      } else {
+        theta = interp(u, sc.phase_cdf, sc.phase_theta, sc.phase_n);
+      }
       const double phi = two_pi * uniform(rng, st);
       double sth, cth, sph, cph;
       sincos(theta, &sth, &cth);
@@ -390,6 +421,7 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
       const double nx = qx + Dx, ny = qy + Dy, nz = qz + Dz;
       const double svx = nx - qx, svy = ny - qy, svz = nz - qz;
       const double slen = norm3(svx, svy, svz);
+      if constexpr (STATS) ph.lam += (double)jb * len;
       if (Dy < 0.0) {
         // handle_seafloor (core/simulation.py:504-570)
         c.fl++;
@@ -401,6 +433,11 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
         e.E = ph.E;
         e.t = (float)(ph.s + jb);
         e.eno = ph.event_no++;
+        snapshot<STATS>(e, ph);
+        if constexpr (STATS) {
+          ph.lam += slen;
+          ph.nfl++;
+        }
         const double u1 = uniform(rng, st);
         const double u2 = uniform(rng, st);
         const double sr = sqrt(u1);
@@ -430,6 +467,10 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
           e.E = ph.E;
           e.t = (float)(ph.s + jb);
           e.eno = no;
+          snapshot<STATS>(e, ph);
+        }
+        if constexpr (STATS) {
+          if (reflect) ph.lam += slen;
         }
         if (reflect) {
           ph.dy = -ph.dy;
@@ -520,7 +561,7 @@ __global__ __launch_bounds__(256) void trace_kernel(alb_scene_t sc, alb_rng_t rn
   if (i < n) emit_photon(sc, FWD, rng, st, ph, c);
   while (ballot(ph.alive)) {
     Emit e{-1, 0, 0, 0, 0, 0, 0, 0.0f, 0.0f, 0u};
-    if (ph.alive) next_event(sc, FWD, rng, st, ph, e, c);
+    if (ph.alive) next_event<false>(sc, FWD, rng, st, ph, e, c);
     const int64_t tag = (int64_t)(((uint64_t)(begin + i) << 16) | (uint64_t)(e.eno & 0xFFFFu));
     emit_record<FWD>(sk, e, tag, 0, lane);
   }
@@ -579,11 +620,141 @@ __global__ __launch_bounds__(256) void trace_batched_kernel(const alb_scene_t* _
     }
     if (!ballot(ph.alive || !done)) break;
     Emit e{-1, 0, 0, 0, 0, 0, 0, 0.0f, 0.0f, 0u};
-    if (ph.alive) next_event(scenes[sid], FWD, lrng, st, ph, e, c);
+    if (ph.alive) next_event<false>(scenes[sid], FWD, lrng, st, ph, e, c);
     const int64_t tag = (int64_t)(((uint64_t)pidx << 16) | (uint64_t)(e.eno & 0xFFFFu));
     emit_record<FWD>(sk, e, tag, sid, lane);
   }
   flush_tally(c, ctr, lane);
+}
+
+// first position in keys[0, n) whose key is >= key
+__device__ __forceinline__ int64_t lower_bound_u64(const uint64_t* __restrict__ keys, int64_t n, uint64_t key) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+struct ReplayIdx {
+  alb_replay_index_t t[ALB_N_TYPES];
+};
+
+// Per-scene sums of weight * (n, [scatter], L, m, G) of the lane's records.
+struct ReplayAcc {
+  double v[5];
+  int32_t sid;
+};
+
+__device__ __forceinline__ void flush_acc(ReplayAcc& a, double* acc) {
+  if (a.sid >= 0) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+      if (a.v[c] != 0.0) unsafeAtomicAdd(&acc[5 * (int64_t)a.sid + c], a.v[c]);
+  }
+#pragma unroll
+  for (int c = 0; c < 5; ++c) a.v[c] = 0.0;
+}
+
+// Path replay for the adjoint: the photons of trace_batched_kernel, traced
+// again with the same streams (so the same events), now keeping the path
+// statistics; every record / query is found through the sorted key index
+// (the photon's records of one type are consecutive there, in event order,
+// so one binary search per photon and type places a cursor) and its adjoint
+// weight multiplies the prefix statistics.
+template <bool FWD>
+__global__ __launch_bounds__(256) void replay_batched_kernel(const alb_scene_t* __restrict__ scenes, int32_t n_scenes,
+                                                             const int64_t* __restrict__ offsets, alb_rng_t rng,
+                                                             ReplayIdx ix, unsigned long long* work,
+                                                             int32_t refill_interval, double* acc,
+                                                             unsigned long long* ctr) {
+  const uint32_t lane = lane_id();
+  const int64_t total = offsets[n_scenes];
+  Photon ph;
+  ph.alive = false;
+  ph.event_no = 0;
+  bool done = false;
+  int32_t sid = 0;
+  int64_t g = 0;
+  int64_t cur[ALB_N_TYPES] = {0, 0, 0, 0};
+  Stream st{0, 0, 0u, false, 0.0};
+  alb_rng_t lrng = rng;
+  Tally c{};
+  ReplayAcc a{{0.0, 0.0, 0.0, 0.0, 0.0}, -1};
+  unsigned long long misses = 0;
+  for (uint32_t it = 0;; ++it) {
+    if (it % (uint32_t)refill_interval == 0) {
+      const bool need = !ph.alive && !done;
+      const uint64_t m = ballot(need);
+      if (m) {
+        const int leader = __ffsll((long long)m) - 1;
+        unsigned long long base = 0;
+        if ((int)lane == leader) base = atomicAdd(work, (unsigned long long)__popcll(m));
+        base = bcast_u64(base, leader);
+        if (need) {
+          g = (int64_t)(base + mask_rank(m));
+          if (g >= total) {
+            done = true;
+          } else {
+            int lo = 0, hi = n_scenes;
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (offsets[mid] <= g) lo = mid; else hi = mid;
+            }
+            sid = lo;
+            if (a.sid != sid) {
+              flush_acc(a, acc);
+              a.sid = sid;
+            }
+            const int64_t pidx = g - offsets[lo];
+            lrng.stream = rng.stream + ((uint32_t)sid << 1);
+            st = Stream{(uint64_t)pidx, 0, 0u, false, 0.0};
+#pragma unroll
+            for (int t = 0; t < ALB_N_TYPES; ++t)
+              cur[t] = ix.t[t].n > 0 ? lower_bound_u64(ix.t[t].keys, ix.t[t].n, (uint64_t)g << 16) : 0;
+            emit_photon(scenes[sid], FWD, lrng, st, ph, c);
+            ph.lam = 0.0;
+            ph.gs = 0.0;
+            ph.nsc = 0;
+            ph.nfl = 0;
+          }
+        }
+      }
+    }
+    if (!ballot(ph.alive || !done)) break;
+    if (ph.alive) {
+      Emit e{-1, 0, 0, 0, 0, 0, 0, 0.0f, 0.0f, 0u, 0.0, 0.0, 0, 0};
+      next_event<true>(scenes[sid], FWD, lrng, st, ph, e, c);
+      if (e.et >= 0) {
+        const uint64_t key = ((uint64_t)g << 16) | (uint64_t)(e.eno & 0xFFFFu);
+#pragma unroll
+        for (int t = 0; t < ALB_N_TYPES; ++t) {
+          const bool mine = (e.et == t) || (!FWD && t == ALB_SURFACE_TRANSMISSION_UP && e.et == ALB_SURFACE_REFLECTION);
+          if (!mine || ix.t[t].n == 0) continue;
+          const int64_t p = cur[t];
+          if (p < ix.t[t].n && ix.t[t].keys[p] == key) {
+            cur[t] = p + 1;
+            const double A = ix.t[t].weight[ix.t[t].slots[p]];
+            if (A != 0.0) {
+              a.v[0] += A * (double)e.nsc;
+              if (t == ALB_SCATTER) a.v[1] += A;
+              a.v[2] += A * e.lam;
+              a.v[3] += A * (double)e.nfl;
+              a.v[4] += A * e.gs;
+            }
+          } else {
+            misses++;
+          }
+        }
+      }
+    }
+  }
+  flush_acc(a, acc);
+  if (ctr) {
+    const unsigned long long tm = wave_sum(misses);
+    if (lane == 0 && tm) atomicAdd(&ctr[ALB_CTR_REPLAY_MISSES], tm);
+  }
 }
 
 static int fill_sinks(int32_t pass, const alb_records_t* out, const alb_queries_t* q, Sinks* sk, const char* who) {
@@ -668,4 +839,45 @@ extern "C" int alb_trace_batched(const alb_scene_t* scenes, int32_t n_scenes, co
     hipLaunchKernelGGL(trace_batched_kernel<false>, dim3(grid), dim3(block), 0, s, scenes, n_scenes, photon_offsets,
                        *rng, sk, work_counter, refill_interval, counters);
   return check_launch("alb_trace_batched");
+}
+
+extern "C" int alb_replay_batched(const alb_scene_t* scenes, int32_t n_scenes, const int64_t* photon_offsets,
+                                  const alb_rng_t* rng, int32_t pass, const alb_replay_index_t* index,
+                                  unsigned long long* work_counter, int32_t block, int32_t refill_interval,
+                                  double* acc, unsigned long long* counters, void* stream) {
+  using namespace alb;
+  if (!scenes || !photon_offsets || !rng || !index || !work_counter || !acc)
+    return set_error(ALB_ERR_ARG, "alb_replay_batched: null argument");
+  if (n_scenes <= 0 || n_scenes > (1 << 20)) return set_error(ALB_ERR_ARG, "alb_replay_batched: bad scene count");
+  if (pass != ALB_PASS_FORWARD && pass != ALB_PASS_BACKWARD)
+    return set_error(ALB_ERR_ARG, "alb_replay_batched: pass must be 0 (forward) or 1 (backward)");
+  if (rng->mode != ALB_RNG_PHILOX) return set_error(ALB_ERR_UNSUPPORTED, "alb_replay_batched: Philox streams only");
+  if ((int32_t)(rng->stream & 1u) != pass)
+    return set_error(ALB_ERR_ARG, "alb_replay_batched: rng stream word must have the pass in its low bit");
+  if (block != 64 && block != 128 && block != 256)
+    return set_error(ALB_ERR_ARG, "alb_replay_batched: block must be 64, 128 or 256");
+  if (refill_interval < 1) return set_error(ALB_ERR_ARG, "alb_replay_batched: refill_interval must be >= 1");
+  ReplayIdx ix;
+  for (int t = 0; t < ALB_N_TYPES; ++t) {
+    ix.t[t] = index[t];
+    if (ix.t[t].n < 0 || (ix.t[t].n > 0 && (!ix.t[t].keys || !ix.t[t].slots || !ix.t[t].weight)))
+      return set_error(ALB_ERR_ARG, "alb_replay_batched: index %d incomplete", t);
+  }
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return set_error(ALB_ERR_HIP, "alb_replay_batched: no HIP device");
+  int per_cu = 1;
+  const void* fn = pass == ALB_PASS_FORWARD ? (const void*)replay_batched_kernel<true>
+                                            : (const void*)replay_batched_kernel<false>;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  const unsigned grid = (unsigned)(cus * per_cu);
+  hipStream_t s = (hipStream_t)stream;
+  if (pass == ALB_PASS_FORWARD)
+    hipLaunchKernelGGL(replay_batched_kernel<true>, dim3(grid), dim3(block), 0, s, scenes, n_scenes, photon_offsets,
+                       *rng, ix, work_counter, refill_interval, acc, counters);
+  else
+    hipLaunchKernelGGL(replay_batched_kernel<false>, dim3(grid), dim3(block), 0, s, scenes, n_scenes, photon_offsets,
+                       *rng, ix, work_counter, refill_interval, acc, counters);
+  return check_launch("alb_replay_batched");
 }

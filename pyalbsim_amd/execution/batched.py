@@ -45,6 +45,7 @@ class BatchedResult:
     timings: dict = field(default_factory=dict)
     maps: list = field(default_factory=list)
     queries: list = field(default_factory=list)
+    state: dict = field(default_factory=dict)
 
     def waveform(self, s: int):
         """Device view [4, n_bins_s] of scene s."""
@@ -60,8 +61,8 @@ class BatchedResult:
 def simulate_batched(models: Sequence[SceneModel], n_forward, n_backward, key, *, scene_ids=None,
                      win_len: int = 1024, block: int = 256, refill: int = 4, strict: bool = True,
                      keep: bool = False, tags: bool = False, events: Optional[dict] = None,
-                     kernel_events: Optional[dict] = None, table: Optional[runtime.DeviceSceneTable] = None
-                     ) -> BatchedResult:
+                     kernel_events: Optional[dict] = None, table: Optional[runtime.DeviceSceneTable] = None,
+                     adjoint: bool = False) -> BatchedResult:
     """Forward trace, map build, backward trace and gather of many scenes.
 
     n_forward / n_backward: photons per scene (sequences or scalars).
@@ -73,6 +74,9 @@ def simulate_batched(models: Sequence[SceneModel], n_forward, n_backward, key, *
     (core/simulation.py:134-153), when a scene has gather queries of a type
     whose map is empty or smaller than k; strict=False leaves those scenes'
     contributions of that type at zero.
+    adjoint: keep what pyalbsim_amd.adjoint needs afterwards (record and
+    query sinks with tags, the maps, each type's slot order, neighbour rows
+    and kernel normalisations) in res.state.
     """
     import torch
 
@@ -96,7 +100,9 @@ def simulate_batched(models: Sequence[SceneModel], n_forward, n_backward, key, *
         t[name] = time.perf_counter()
 
     mark("start")
-    tab = table or runtime.DeviceSceneTable(models)
+    if adjoint:
+        tags = keep = True
+    tab = table or runtime.DeviceSceneTable(models, adjoint=adjoint)
     cf = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device="cuda")
     cb = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device="cuda")
     recs, rcounts = runtime.trace_batched(tab, key, abi.PASS_FORWARD, nf, tags=tags, counters=cf, block=block,
@@ -124,6 +130,7 @@ def simulate_batched(models: Sequence[SceneModel], n_forward, n_backward, key, *
             l_ = maps[ti].tensors["last"].cpu().numpy().astype(np.int64)
             per_scene[:, ti] = l_ - f + 1
     keep_alive = []
+    rows = {}
     k = tab.k
     for ti in range(abi.N_TYPES):
         if qcounts[ti] == 0:
@@ -138,8 +145,16 @@ def simulate_batched(models: Sequence[SceneModel], n_forward, n_backward, key, *
                              f"map size {per_scene[s, ti]}")
         if rcounts[ti] == 0:
             continue
-        keep_alive.append(runtime.gather_batched(tab, maps[ti], ti, qs[ti], qcounts[ti], wf, wf_off, win0, wl,
-                                                 counters=cb, events=kernel_events))
+        if adjoint:
+            nbr = torch.empty((qcounts[ti], k), dtype=torch.int32, device="cuda")
+            rad = torch.empty(qcounts[ti], dtype=torch.float64, device="cuda")
+            kept = runtime.gather_batched(tab, maps[ti], ti, qs[ti], qcounts[ti], wf, wf_off, win0, wl, counters=cb,
+                                          events=kernel_events, nbr_out=nbr, radius_out=rad)
+            rows[ti] = {"order": kept[1], "knorm": kept[2], "nbr": nbr}
+            keep_alive.append(kept)
+        else:
+            keep_alive.append(runtime.gather_batched(tab, maps[ti], ti, qs[ti], qcounts[ti], wf, wf_off, win0, wl,
+                                                     counters=cb, events=kernel_events))
     mark("gather")
     ctr_f, ctr_b = cf.cpu().numpy().view(np.uint64), cb.cpu().numpy().view(np.uint64)
     if ctr_b[abi.CTR_OVERFLOW]:
@@ -149,4 +164,8 @@ def simulate_batched(models: Sequence[SceneModel], n_forward, n_backward, key, *
     if keep:
         res.maps = maps
         res.queries = qs
+    if adjoint:
+        res.state = {"table": tab, "nf": nf, "nb": nb, "key": key, "base": base, "block": block, "refill": refill,
+                     "records": recs, "rcounts": rcounts, "queries": qs, "qcounts": qcounts, "maps": maps,
+                     "rows": rows, "wf_off": wf_off}
     return res

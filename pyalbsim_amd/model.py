@@ -182,6 +182,7 @@ class SceneModel:
     phase_pf: np.ndarray
     phase_cdf: np.ndarray
     backscatter: float
+    phase_g: float | None = None  # Henyey-Greenstein asymmetry when the table is HG
 
     @property
     def n_substeps(self) -> int:
@@ -253,5 +254,5 @@ def build_scene(cfg: SimulationConfig, hg_g: float | None = None) -> SceneModel:
         surface_albedo=float(cfg.sea_surface.albedo), floor_albedo=float(cfg.sea_floor.albedo),
         absorption=float(a), scattering=float(b), alpha_l64=alpha, omega064=(alpha - a) / alpha,
         steps=steps, mult=int(mult), k=int(cfg.photon_mapping_k), phase_theta=theta, phase_pf=pf,
-        phase_cdf=cdf, backscatter=bpf,
+        phase_cdf=cdf, backscatter=bpf, phase_g=None if hg_g is None else float(hg_g),
     )

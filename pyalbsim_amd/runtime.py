@@ -22,7 +22,8 @@ _lib = None
 EXPORTS = ["alb_abi_version", "alb_scene_size", "alb_last_error", "alb_trace", "alb_map_workspace_size",
            "alb_map_build", "alb_gather_workspace_size", "alb_gather", "alb_gather_order", "alb_knn",
            "alb_contrib", "alb_trace_batched", "alb_map_build_batched", "alb_gather_order_batched",
-           "alb_knn_batched", "alb_contrib_batched"]
+           "alb_knn_batched", "alb_contrib_batched", "alb_adjoint_contrib", "alb_replay_index_workspace_size",
+           "alb_replay_index_build", "alb_replay_batched"]
 
 
 class AlbError(RuntimeError):
@@ -70,6 +71,15 @@ def load_library(path: str = LIB_PATH):
     L.alb_contrib_batched.argtypes = [P, C.c_int32, C.c_int32, P, C.c_int32, P, P, C.c_int64, C.c_int64, P, P, P,
                                       P, P, C.c_int32, P]
     L.alb_contrib_batched.restype = C.c_int
+    L.alb_adjoint_contrib.argtypes = [P, C.c_int32, C.c_int32, P, C.c_int32, P, P, C.c_int64, C.c_int64, P, P, P, P,
+                                      P, P, P, P]
+    L.alb_adjoint_contrib.restype = C.c_int
+    L.alb_replay_index_workspace_size.argtypes = [C.c_int64, C.POINTER(C.c_size_t)]
+    L.alb_replay_index_workspace_size.restype = C.c_int
+    L.alb_replay_index_build.argtypes = [P, P, C.c_int64, P, P, P, P, C.c_size_t, P]
+    L.alb_replay_index_build.restype = C.c_int
+    L.alb_replay_batched.argtypes = [P, C.c_int32, P, P, C.c_int32, P, P, C.c_int32, C.c_int32, P, P, P]
+    L.alb_replay_batched.restype = C.c_int
     if L.alb_abi_version() != abi.ALB_ABI_VERSION:
         raise AlbError("libalbsim_hip.so ABI version mismatch")
     if L.alb_scene_size() != C.sizeof(abi.Scene):
@@ -311,7 +321,10 @@ MAP_DIMS = (2, 3, 2, 2)  # BOTTOM and the two surface maps are planar; SCATTER i
 class DeviceSceneTable:
     """alb_scene_t[n_scenes] on the device; scenes with equal phase tables share them."""
 
-    def __init__(self, models, device="cuda"):
+    def __init__(self, models, device="cuda", adjoint: bool = False, gather_phase=None):
+        """adjoint: also build the phase-derivative tables of HG scenes.
+        gather_phase: optional models whose phase tables the gather's phase
+        mass is taken from (default: each scene's own)."""
         import hashlib
 
         from .model import phase_bin_mass
@@ -328,15 +341,24 @@ class DeviceSceneTable:
         self.tables = {}
         structs = (abi.Scene * self.n)()
         for i, m in enumerate(self.models):
+            gm = gather_phase[i] if gather_phase is not None else m
             h = hashlib.sha1(np.ascontiguousarray(m.phase_cdf).tobytes()
-                             + np.ascontiguousarray(m.phase_theta).tobytes()).hexdigest()
+                             + np.ascontiguousarray(m.phase_theta).tobytes()
+                             + np.ascontiguousarray(gm.phase_cdf).tobytes()).hexdigest()
             if h not in self.tables:
                 th = torch.as_tensor(np.ascontiguousarray(m.phase_theta), dtype=torch.float64, device=device)
                 cd = torch.as_tensor(np.ascontiguousarray(m.phase_cdf), dtype=torch.float64, device=device)
-                ms = torch.as_tensor(phase_bin_mass(m.phase_theta, m.phase_cdf), dtype=torch.float64, device=device)
-                self.tables[h] = (th, cd, ms)
-            th, cd, ms = self.tables[h]
-            structs[i] = abi.make_scene(m, _p(th), _p(cd), _p(ms))
+                ms = torch.as_tensor(phase_bin_mass(gm.phase_theta, gm.phase_cdf), dtype=torch.float64, device=device)
+                sc = dm = None
+                if adjoint and m.phase_g is not None:
+                    from .adjoint import phase_g_tables
+
+                    score, dlm = phase_g_tables(m.phase_g)
+                    sc = torch.as_tensor(score, dtype=torch.float64, device=device)
+                    dm = torch.as_tensor(dlm, dtype=torch.float64, device=device)
+                self.tables[h] = (th, cd, ms, sc, dm)
+            th, cd, ms, sc, dm = self.tables[h]
+            structs[i] = abi.make_scene(m, _p(th), _p(cd), _p(ms), _p(sc), _p(dm))
         self.host = structs
         raw = np.frombuffer(bytes(structs), dtype=np.uint8).copy()
         self.dev = torch.as_tensor(raw, device=device)
