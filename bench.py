@@ -8,11 +8,23 @@ N_b sensor paths, k-NN gather into FP64 waveforms.  Default workload: the
 
 Also reported: per-phase times (HIP events on the launch stream), the
 roofline position of the dominant kernel (algorithmic flops or bytes from the
-kernels' own event counters / its time) and a CPU baseline: the time-stepped
-C oracle on a bounded sample of the same scene, on this host's cores.
+kernels' own event counters / its time), its HBM traffic per launch when a
+measured summary is present (profiles/knn_kernel_traffic.json, from rocprofv3
+FETCH_SIZE / WRITE_SIZE runs scaled by a copy kernel's reading), and a CPU
+baseline: the time-stepped C oracle on a bounded sample of the same scene, on
+this host's cores.
+
+The two other forms of the metric, on bounded samples:
+  * batched: the first --sweep-scenes scenes of the Latin-hypercube sweep
+    (seed 1) at 1/100 of their photon counts, once as one batched run and
+    once as plain per-scene runs; paths/s of each and the ratio;
+  * adjoint: --adjoint-scenes sweep scenes at 1e5 forward / 2.5e4 backward
+    photons; wall time of forward + adjoint (path replay, five parameters)
+    over the forward alone, and with the depth difference (two more
+    forward passes).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--forward N]
-       [--backward N] [--no-cpu]
+       [--backward N] [--no-cpu] [--no-batched] [--no-adjoint]
 """
 from __future__ import annotations
 
@@ -90,6 +102,12 @@ def main(argv=None):
     ap.add_argument("--cpu-forward", type=int, default=16000)
     ap.add_argument("--cpu-backward", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--sweep-scenes", type=int, default=256)
+    ap.add_argument("--adjoint-scenes", type=int, default=16)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 2 GiB device copy at the end (reference reading for HBM byte counters)")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -102,10 +120,10 @@ def main(argv=None):
     from pyalbsim_amd import abi, runtime
     from pyalbsim_amd.execution.linear import simulate
     from pyalbsim_amd.model import build_scene
-    from tests.scenes import single_shot
+    from pyalbsim_amd.sweep import shot_config
 
     runtime.load_library()
-    model = build_scene(single_shot())
+    model = build_scene(shot_config(depth=5.0, a=0.1, b=0.2, albedo=0.15, roughness=0.05))
     n_f, n_b = int(args.forward), int(args.backward)
     ds = runtime.DeviceScene(model)
     key = (0x5EED, 0)
@@ -161,6 +179,12 @@ def main(argv=None):
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = None
+    tpath = os.path.join(ROOT, "profiles", "knn_kernel_traffic.json")
+    if roof["kernel"] == "knn_kernel" and os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        roof["traffic"] = tr.get("bytes_per_launch")
+        roof["traffic_source"] = tr.get("source")
     g_bytes = knn_bytes + (20 + 48) * k * nq
     t_gather = phases["gather"] / 1e3
 
@@ -194,8 +218,100 @@ def main(argv=None):
                                "sample": f"time-stepped C oracle, same scene, {args.cpu_forward} forward + "
                                          f"{args.cpu_backward} backward photons, brute-force k-NN, "
                                          f"{cwall:.1f} s wall"}
+    del res
+    if not args.no_batched:
+        out["batched"] = bench_batched(args.sweep_scenes)
+    if not args.no_adjoint:
+        out["adjoint"] = bench_adjoint(args.adjoint_scenes)
+    if args.copy_calibration:
+        x = torch.empty(1 << 28, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = {"read": x.numel() * 8, "write": y.numel() * 8}
     print(json.dumps(out), flush=True)
     return 0
+
+
+def bench_batched(n_scenes: int):
+    """Plain per-scene runs vs one batched run over the same sweep scenes."""
+    import torch
+
+    from pyalbsim_amd.execution.batched import simulate_batched
+    from pyalbsim_amd.execution.linear import simulate
+    from pyalbsim_amd.sweep import hypercube
+
+    sc = hypercube(1024, 1, (3.0, 5.0))[:n_scenes]
+    models = [x.model for x in sc]
+    nf = [x.n_forward for x in sc]
+    nb = [x.n_backward for x in sc]
+    paths = sum(nf) + sum(nb)
+    key = (0xBA7C, 1)
+    simulate_batched(models[:8], nf[:8], nb[:8], key, strict=False)  # warm-up
+    simulate(models[0], nf[0], nb[0], key, strict=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = simulate_batched(models, nf, nb, key, strict=False)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    counts = (res.map_counts, res.query_counts)
+    del res
+    print(f"batched {n_scenes} scenes done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for s in range(n_scenes):
+        simulate(models[s], nf[s], nb[s], key, strict=False, scene_id=s)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    print(f"plain {n_scenes} scenes done", file=sys.stderr, flush=True)
+    return {"workload": f"first {n_scenes} of 1024 Latin-hypercube sweep scenes (seed 1; depth U[1,30] m, "
+                        f"a U[0.02,0.5], b U[0.05,2], HG g U[0.8,0.95], albedo U[0.05,0.4], wind U[0,10] m/s), "
+                        f"photons 10^U[3,5] per scene (1/100 of 10^U[5,7]), backward = forward / 4",
+            "forward_photons": int(sum(nf)), "backward_photons": int(sum(nb)),
+            "map_records": counts[0], "gather_queries": counts[1],
+            "batched_paths_per_s": paths / tb, "plain_paths_per_s": paths / tp, "batched_s": tb, "plain_s": tp,
+            "speedup": tp / tb}
+
+
+def bench_adjoint(n_scenes: int):
+    """Forward vs forward + path-replay adjoint on sweep scenes."""
+    import numpy as np
+    import torch
+
+    from pyalbsim_amd import adjoint
+    from pyalbsim_amd.execution.batched import simulate_batched
+    from pyalbsim_amd.sweep import hypercube
+
+    sc = hypercube(256, 1)[:n_scenes]
+    th = np.array([[x.params[k] for k in adjoint.PARAMS] for x in sc])
+    models = [adjoint.model_from_theta(t) for t in th]
+    nf, nb = 100000, 25000
+    key = (0xAD70, 2)
+    simulate_batched(models[:2], 1000, 250, key, strict=False, adjoint=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    simulate_batched(models, nf, nb, key, win_len=1000, strict=False)
+    torch.cuda.synchronize()
+    tf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = simulate_batched(models, nf, nb, key, win_len=1000, strict=False, adjoint=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    d = torch.zeros_like(res.waveforms)
+    for s, m in enumerate(models):
+        g0, gl = m.gate(1000)
+        o = int(res.offsets[s])
+        d[o:o + 4 * m.n_bins].view(4, m.n_bins)[:, g0:g0 + gl] = res.waveform(s)[:, g0:g0 + gl]
+    acc, direct = adjoint.adjoint_sums(res, d)
+    derivs = [adjoint.scene_derivatives(m, th[s]) for s, m in enumerate(models)]
+    adjoint.gradients_from_sums(acc, direct, derivs)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    print(f"adjoint {n_scenes} scenes done", file=sys.stderr, flush=True)
+    return {"workload": f"first {n_scenes} Latin-hypercube sweep scenes (seed 1), {nf} forward + {nb} backward "
+                        f"photons each; loss 0.5 |W|^2 over the 1000-bin gate",
+            "forward_s": tf, "forward_kept_s": t1 - t0, "adjoint_s": t2 - t1,
+            "ratio": (t2 - t0) / tf, "ratio_with_depth_difference": (t2 - t0 + 2 * tf) / tf,
+            "parameters": list(adjoint.PARAMS)}
 
 
 if __name__ == "__main__":
