@@ -47,6 +47,7 @@ constexpr int kBucket = 64;       // nodes with <= kBucket records are scanned
 constexpr int kMaxWindow = 2048;  // LDS-privatised waveform bins
 constexpr int kMaxK = kSeed;      // the seed must hold k records
 constexpr int64_t kChunk = 1 << 22;  // queries per k-NN / contribution round
+constexpr int kXcds = 8;          // MI355X XCDs; workgroups are dealt to them round-robin
 
 // Query slots are processed in sorted order: slot si (c0 <= si < c0 + q_n)
 // is query q_begin + order[si].  Output rows are slot - c0, or the query's own
@@ -196,10 +197,60 @@ __device__ uint64_t wave_select(const double* d, const int32_t* idx, int n, int 
   return prefix;
 }
 
+// k-th smallest (1-based) of the first n buffer distances by quickselect in
+// registers: kBuf / 64 entries per lane, keys = FP64 bit patterns (ordered
+// like the non-negative values); each round takes the key of a pseudo-random
+// entry still in [lo, hi] as pivot and counts the entries below and equal to
+// it with ballots, with no LDS traffic.  Random pivots give ~2 ln(n) rounds.
+__device__ uint64_t wave_kth_key(const double* d, int n, int kk, uint32_t lane) {
+  constexpr int U = kBuf / 64;
+  uint64_t key[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int t = (int)lane + 64 * u;
+    key[u] = t < n ? (uint64_t)__double_as_longlong(d[t]) : ~0ull;
+  }
+  uint64_t lo = 0, hi = ~0ull - 1;  // invalid slots hold ~0 and never enter [lo, hi]
+  for (uint32_t round = 0;; ++round) {
+    // this lane's candidate pivot: its first entry inside [lo, hi], starting at a rotating slot
+    uint64_t pk = 0;
+    bool has = false;
+#pragma unroll
+    for (int v = 0; v < U; ++v) {
+      const int u = (v + (int)round) % U;
+      const bool in = key[u] >= lo && key[u] <= hi;
+      if (in && !has) {
+        pk = key[u];
+        has = true;
+      }
+    }
+    const uint64_t m = ballot(has);
+    const uint32_t r = (round * 37u + 11u) & 63u;
+    const uint64_t rot = (m >> r) | (r ? (m << (64 - r)) : 0ull);
+    const int L = (int)((__ffsll((long long)rot) - 1 + r) & 63);
+    const uint64_t pivot = bcast_u64(pk, L);
+    int lt = 0, eq = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool in = key[u] >= lo && key[u] <= hi;
+      lt += __popcll(ballot(in && key[u] < pivot));
+      eq += __popcll(ballot(in && key[u] == pivot));
+    }
+    if (kk <= lt) {
+      hi = pivot - 1;
+    } else if (kk <= lt + eq) {
+      return pivot;
+    } else {
+      kk -= lt + eq;
+      lo = pivot + 1;
+    }
+  }
+}
+
 // Keep the k smallest entries by (d, idx) at the front of the buffer (any
 // order) and return the k-th smallest d.
 __device__ double wave_keep_k(double* d, int32_t* idx, int n, int k, uint32_t* hist, uint32_t lane) {
-  const double th = __longlong_as_double((long long)wave_select(d, idx, n, k, false, 0.0, hist, lane));
+  const double th = __longlong_as_double((long long)wave_kth_key(d, n, k, lane));
   int lt = 0, eq = 0;
   for (int t = (int)lane; t < n; t += 64) {
     lt += d[t] < th;
@@ -287,7 +338,13 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   const int k = a.k;
   unsigned long long c_q = 0, c_cand = 0, c_nodes = 0, c_ovf = 0, c_steps = 0, c_sorts = 0, c_desc = 0, c_small = 0;
 
-  for (int64_t si = (int64_t)blockIdx.x * kWaves + w; si < a.q_n; si += (int64_t)gridDim.x * kWaves) {
+  // XCD-aware slot ranges: the grid is a multiple of kXcds blocks and block b
+  // runs on XCD b % kXcds, so XCD x walks its own contiguous 1/kXcds of the
+  // Morton-sorted slots and the records its queries visit stay in its L2.
+  const int xcd = (int)(blockIdx.x % kXcds);
+  const int64_t xblocks = gridDim.x / kXcds;
+  const int64_t s_lo = a.q_n * xcd / kXcds, s_hi = a.q_n * (xcd + 1) / kXcds;
+  for (int64_t si = s_lo + (int64_t)(blockIdx.x / kXcds) * kWaves + w; si < s_hi; si += xblocks * kWaves) {
     const int64_t qrel = a.order[a.c0 + si];
     const int64_t qg = a.q_begin + qrel;
     const int64_t qi = a.direct ? qrel : si;  // output row
@@ -537,8 +594,12 @@ __global__ __launch_bounds__(256) void contrib_kernel(ContribArgs a) {
   const int64_t total = a.q_n * (int64_t)k;
   const float fmult = (float)a.sc.mult;
   const alb_map_t& m = a.map;
-  for (int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; it < total;
-       it += (int64_t)gridDim.x * blockDim.x) {
+  // XCD-aware item ranges (as in knn_kernel)
+  const int xcd = (int)(blockIdx.x % kXcds);
+  const int64_t xblocks = gridDim.x / kXcds;
+  const int64_t i_lo = total * xcd / kXcds, i_hi = total * (xcd + 1) / kXcds;
+  for (int64_t it = i_lo + (int64_t)(blockIdx.x / kXcds) * blockDim.x + threadIdx.x; it < i_hi;
+       it += xblocks * blockDim.x) {
     const int64_t qi = it / k;  // row
     const int64_t qg = a.q_begin + (a.direct ? qi : (int64_t)a.order[a.c0 + qi]);
     const int32_t i = a.nbr[it];
@@ -787,6 +848,7 @@ extern "C" int alb_knn(const alb_map_t* map, int32_t photon_type, int32_t k, con
              nbr, radius, knorm, counters, nullptr, nullptr, nullptr};
   int64_t kb = (cn + kWaves - 1) / kWaves;
   if (kb > (int64_t)cus * 16) kb = (int64_t)cus * 16;
+  kb = (kb + kXcds - 1) / kXcds * kXcds;
   hipLaunchKernelGGL(knn_kernel, dim3((unsigned)kb), dim3(64 * kWaves), 0, (hipStream_t)stream, ka);
   // rows written: [0, cn) in slot order, or the query offsets of the slots when rows follow queries
   if (rows_by_query) {
@@ -821,6 +883,7 @@ extern "C" int alb_contrib(const alb_scene_t* scene, const alb_map_t* map, int32
                  waveform, n_bins, win0, win_len};
   int64_t cb = (cn * scene->k + 255) / 256;
   if (cb > (int64_t)cus * 8) cb = (int64_t)cus * 8;
+  cb = (cb + kXcds - 1) / kXcds * kXcds;
   hipLaunchKernelGGL(contrib_kernel, dim3((unsigned)cb), dim3(256), sizeof(double) * (size_t)win_len,
                      (hipStream_t)stream, ca);
   return check_launch("alb_contrib");
@@ -924,6 +987,7 @@ extern "C" int alb_knn_batched(const alb_map_t* map, const alb_scene_index_t* in
              nbr, radius, knorm, counters, index->first, index->last, index->root};
   int64_t kb = (cn + kWaves - 1) / kWaves;
   if (kb > (int64_t)cus * 16) kb = (int64_t)cus * 16;
+  kb = (kb + kXcds - 1) / kXcds * kXcds;
   hipLaunchKernelGGL(knn_kernel, dim3((unsigned)kb), dim3(64 * kWaves), 0, (hipStream_t)stream, ka);
   hipLaunchKernelGGL(knorm_kernel, dim3((unsigned)((cn + 255) / 256)), dim3(256), 0, (hipStream_t)stream, radius,
                      knorm, cn, photon_type == ALB_SCATTER ? 1 : 0);

@@ -59,7 +59,7 @@ class BatchedResult:
 
 
 def simulate_batched(models: Sequence[SceneModel], n_forward, n_backward, key, *, scene_ids=None,
-                     win_len: int = 1024, block: int = 256, refill: int = 4, strict: bool = True,
+                     win_len: int = 1024, block=256, refill: int = 4, strict: bool = True,
                      keep: bool = False, tags: bool = False, events: Optional[dict] = None,
                      kernel_events: Optional[dict] = None, table: Optional[runtime.DeviceSceneTable] = None,
                      adjoint: bool = False) -> BatchedResult:
@@ -70,6 +70,8 @@ def simulate_batched(models: Sequence[SceneModel], n_forward, n_backward, key, *
     0..n-1).  Only contiguous ids are supported by the trace kernel, so
     scene_ids must be range(base, base + n).
     win_len: LDS-privatised bins per scene, starting at model.gate(win_len).
+    block / refill: launch parameters of the batched trace; block="auto" picks
+    both with pyalbsim_amd.autotune on a pilot sample (cached per workload).
     strict: raise KeyError / IndexError, as the reference does per scene
     (core/simulation.py:134-153), when a scene has gather queries of a type
     whose map is empty or smaller than k; strict=False leaves those scenes'
@@ -103,6 +105,10 @@ def simulate_batched(models: Sequence[SceneModel], n_forward, n_backward, key, *
     if adjoint:
         tags = keep = True
     tab = table or runtime.DeviceSceneTable(models, adjoint=adjoint)
+    if block == "auto":
+        from ..autotune import tune_batched
+
+        block, refill, _ = tune_batched(models, nf, key, table=tab)
     cf = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device="cuda")
     cb = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device="cuda")
     recs, rcounts = runtime.trace_batched(tab, key, abi.PASS_FORWARD, nf, tags=tags, counters=cf, block=block,

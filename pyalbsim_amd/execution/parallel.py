@@ -1,0 +1,72 @@
+"""run_parallel — the reference's multiprocess driver (src/alb_sim/execution/
+parallel.py:186-284), same signature and return contract, on the GPU.
+
+The reference splits the forward and backward batches over ``processes``
+worker processes, merges the workers' photon storages into PhotonMapData and
+sums the workers' waveforms.  Here the GPU's lanes are the workers: one
+forward trace, one map build per type, one backward trace and one gather,
+exactly as run_linear, and ``processes`` is ignored.  Returned:
+
+  * waveform: {PhotonType: float32 [2*steps]} (the reference sums float32
+    worker arrays);
+  * photon_maps_data: {PhotonType: PhotonMapData} of the forward records in
+    the map's Morton order (the reference's order is batch order; the records
+    are the same set).  first_water_interaction / seafloor_interaction are
+    NaN: they feed only the heat maps, which are not computed here;
+  * sampled heat maps: None (not computed).
+
+``seed`` seeds numpy.random.default_rng, whose two draws make the Philox key
+(None: a fresh secrets-seeded generator, as the reference).
+"""
+from __future__ import annotations
+
+import secrets
+import time
+from typing import Optional, Union
+
+import numpy as np
+
+from .. import abi
+from ..config.run import RunConfig
+from ..config.simulation import SimulationConfig
+from ..model import build_scene
+from ..photon_mapping.photon_map_data import PhotonMapData
+from ..photon_mapping.photon_type import PhotonType
+from .linear import photon_map_stats_table, simulate
+
+
+def photon_maps_from_result(res) -> dict:
+    """PhotonMapData per type from a simulate(..., keep=True) result (device -> host copy)."""
+    out = {}
+    for pt in PhotonType:
+        t = pt.abi_index
+        n = res.map_counts[t]
+        m = res.maps[t].tensors
+        pos = np.stack([m[f][:n].cpu().numpy() for f in ("x", "y", "z")], 1).astype(np.float32)
+        dirs = np.stack([m[f][:n].cpu().numpy() for f in ("dx", "dy", "dz")], 1).astype(np.float32)
+        nan = np.full((n, 3), np.nan, dtype=np.float32)
+        out[pt] = PhotonMapData(positions=pos, directions=dirs, energies=m["energy"][:n].cpu().numpy().copy(),
+                                times=m["time"][:n].cpu().numpy().copy(), first_water_interaction=nan,
+                                seafloor_interaction=nan.copy())
+    return out
+
+
+def run_parallel(simulation_config: SimulationConfig, run_config: RunConfig,
+                 seed: Union[int, None] = None, *, verbose: bool = True):
+    """(waveform, photon_maps_data, None) for one scene, computed on the GPU."""
+    rng = np.random.default_rng(secrets.randbits(128) if seed is None else seed)
+    key = abi.key_from_generator(rng)
+    model = build_scene(simulation_config)
+    n_f = run_config.photons_per_batch_forward * run_config.batches_forward
+    n_b = run_config.photons_per_batch_backward * run_config.batches_backward
+    if verbose:
+        print("Starting forwards pass")
+    t0 = time.perf_counter()
+    res = simulate(model, n_f, n_b, key, keep=True)
+    maps = photon_maps_from_result(res)
+    if verbose:
+        print("\n" + photon_map_stats_table(res.map_counts) + "\n")
+        print(f"Finished {n_f} + {n_b} photon paths in {time.perf_counter() - t0:.2f} s")
+    wf = res.waveforms.cpu().numpy()
+    waveform = {pt: wf[pt.abi_index].astype(np.float32) for pt in PhotonType}
+    return waveform, maps, None

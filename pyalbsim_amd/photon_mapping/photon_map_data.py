@@ -19,3 +19,25 @@ class PhotonMapData:
 
     def __len__(self) -> int:
         return int(self.positions.shape[0])
+
+
+_FIELDS = ("positions", "directions", "energies", "times", "first_water_interaction", "seafloor_interaction")
+
+
+def save_photon_maps(path: str, maps: dict) -> None:
+    """Write {PhotonType: PhotonMapData} as one .npz of float32 arrays named
+    "<TYPE NAME>/<field>" (no pickled objects)."""
+    arrays = {}
+    for pt, m in maps.items():
+        for f in _FIELDS:
+            arrays[f"{pt.name}/{f}"] = np.ascontiguousarray(getattr(m, f), dtype=np.float32)
+    np.savez(path, **arrays)
+
+
+def load_photon_maps(path: str) -> dict:
+    """Inverse of save_photon_maps (loaded with allow_pickle=False)."""
+    from .photon_type import PhotonType
+
+    with np.load(path, allow_pickle=False) as z:
+        names = {k.split("/")[0] for k in z.files}
+        return {PhotonType[n]: PhotonMapData(**{f: z[f"{n}/{f}"] for f in _FIELDS}) for n in sorted(names)}

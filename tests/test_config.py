@@ -62,3 +62,35 @@ def test_dropin_module_paths():
     assert W is not None and callable(run_linear)
     assert [p.name for p in PhotonType] == ["BOTTOM_REFLECTION", "SCATTER", "SURFACE_REFLECTION",
                                             "SURFACE_TRANSMISSION_UP"]
+
+
+def test_photon_map_npz_round_trip(tmp_path):
+    import numpy as np
+
+    from alb_sim.photon_mapping.photon_map_data import PhotonMapData, load_photon_maps, save_photon_maps
+    from alb_sim.photon_mapping.photon_type import PhotonType
+
+    rng = np.random.default_rng(0)
+    maps = {}
+    for i, pt in enumerate(PhotonType):
+        n = 5 + i
+        maps[pt] = PhotonMapData(rng.normal(size=(n, 3)).astype(np.float32), rng.normal(size=(n, 3)).astype(np.float32),
+                                 rng.uniform(size=n).astype(np.float32), rng.uniform(size=n).astype(np.float32),
+                                 np.full((n, 3), np.nan, np.float32), np.full((n, 3), np.nan, np.float32))
+    path = str(tmp_path / "maps.npz")
+    save_photon_maps(path, maps)
+    back = load_photon_maps(path)
+    assert set(back) == set(maps)
+    for pt in PhotonType:
+        for f in ("positions", "directions", "energies", "times"):
+            assert np.array_equal(getattr(back[pt], f), getattr(maps[pt], f))
+        assert np.isnan(back[pt].first_water_interaction).all()
+
+
+def test_run_parallel_signature_matches_reference():
+    import inspect
+
+    from alb_sim.execution.parallel import run_parallel
+
+    params = list(inspect.signature(run_parallel).parameters)
+    assert params[:3] == ["simulation_config", "run_config", "seed"]

@@ -206,3 +206,19 @@ def test_hypercube_batch_runs(gpu):
     assert res.map_counts[abi.SURFACE_REFLECTION] == sum(x.n_forward for x in sc)
     assert res.query_counts[abi.SURFACE_REFLECTION] == sum(x.n_backward for x in sc)
     assert int(res.counters_fwd[abi.CTR_PATHS]) == sum(x.n_forward for x in sc)
+
+
+def test_autotuned_launch_gives_same_waveforms(gpu):
+    """Block size and refill interval change speed only: the autotuned run
+    equals a fixed-parameter run up to FP64 summation order."""
+    from pyalbsim_amd.autotune import tune_batched
+
+    models = _scenes()
+    nf, nb = [8000, 5000, 6000], [1500, 1000, 1200]
+    b, r, times = tune_batched(models, nf, (3, 4))
+    assert b in (64, 128, 256) and r in (1, 2, 4, 8, 16) and len(times) == 15
+    a = simulate_batched(models, nf, nb, (3, 4), block="auto")
+    c = simulate_batched(models, nf, nb, (3, 4), block=64, refill=1)
+    for s in range(3):
+        x, y = a.waveform(s).cpu().numpy(), c.waveform(s).cpu().numpy()
+        assert np.allclose(x, y, rtol=1e-9, atol=1e-12 * float(np.abs(y).max()))

@@ -141,3 +141,27 @@ def test_full_size_invariants(gpu):
     assert bool(torch.isfinite(wf).all()) and bool((wf >= 0).all())
     assert float(wf.sum()) > 0
     assert int(res.counters_fwd[abi.CTR_PATHS]) == n_f and int(res.counters_bwd[abi.CTR_PATHS]) == n_b
+
+
+def test_run_parallel_contract(gpu):
+    """run_parallel returns (waveforms, photon maps, None); the same seed gives
+    the same result, and the maps hold the forward records."""
+    from alb_sim.config.run import RunConfig
+    from alb_sim.execution.parallel import run_parallel
+
+    cfg = single_shot()
+    rc = RunConfig(photons_per_batch_forward=3000, batches_forward=2, photons_per_batch_backward=800,
+                   batches_backward=2, processes=4)
+    wf, maps, heat = run_parallel(cfg, rc, seed=11, verbose=False)
+    assert heat is None
+    model = build_scene(cfg)
+    assert list(wf) == list(PhotonType)
+    for pt in PhotonType:
+        assert wf[pt].dtype == np.float32 and wf[pt].shape == (model.n_bins,)
+        m = maps[pt]
+        assert m.positions.shape == (len(m.energies), 3) and m.positions.dtype == np.float32
+    assert len(maps[PhotonType.SURFACE_REFLECTION]) == 6000
+    assert np.allclose(maps[PhotonType.SURFACE_REFLECTION].positions[:, 1], np.float32(model.y_ws))
+    wf2, _, _ = run_parallel(cfg, rc, seed=11, verbose=False)
+    for pt in PhotonType:
+        assert np.allclose(wf[pt], wf2[pt], rtol=1e-5, atol=0)
