@@ -53,22 +53,51 @@ constexpr int kXcds = 8;          // MI355X XCDs; workgroups are dealt to them r
 // is query q_begin + order[si].  Output rows are slot - c0, or the query's own
 // index when direct (caller-provided neighbour arrays).
 struct KnnArgs {
-  alb_map_t map;
-  alb_queries_t q;
+  // map (sorted positions, tree) and queries: only the arrays the k-NN reads,
+  // so the kernel's scalar registers hold few pointers
+  const float* x;
+  const float* y;
+  const float* z;
+  const Node* nodes;
+  int64_t n;
+  const double* qx;
+  const double* qy;
+  const double* qz;
+  const int32_t* qscene;
   int64_t q_begin, c0, q_n;
   const int32_t* order;
   int32_t direct;
   int32_t k;
-  int32_t volume;    // kernel normalisation by volume (scatter map) instead of area
   int32_t* nbr;      // [rows * k]
   double* radius;    // [rows]
-  double* knorm;     // [rows]
   unsigned long long* ctr;
   // many scenes: each query searches its own scene's subtree (null for one scene)
   const int32_t* sfirst;
   const int32_t* slast;
   const int32_t* sroot;
 };
+
+static KnnArgs knn_args(const alb_map_t& m, const alb_queries_t& q, int64_t q_begin, int64_t c0, int64_t q_n,
+                        const int32_t* order, int direct, int k, int32_t* nbr, double* radius,
+                        unsigned long long* ctr, const alb_scene_index_t* idx) {
+  KnnArgs a;
+  a.x = m.x; a.y = m.y; a.z = m.z;
+  a.nodes = (const Node*)m.nodes;
+  a.n = m.n;
+  a.qx = q.x; a.qy = q.y; a.qz = q.z;
+  a.qscene = q.scene;
+  a.q_begin = q_begin; a.c0 = c0; a.q_n = q_n;
+  a.order = order;
+  a.direct = direct;
+  a.k = k;
+  a.nbr = nbr;
+  a.radius = radius;
+  a.ctr = ctr;
+  a.sfirst = idx ? idx->first : nullptr;
+  a.slast = idx ? idx->last : nullptr;
+  a.sroot = idx ? idx->root : nullptr;
+  return a;
+}
 
 struct ContribArgs {
   alb_scene_t sc;
@@ -91,10 +120,10 @@ __device__ __forceinline__ double dist2(double qx, double qy, double qz, float x
   return s + dz * dz;
 }
 
+// distance from q to [lo, hi] along one axis; the same values as the branchy
+// form (lo - q below, q - hi above, 0 inside) without divergent branches
 __device__ __forceinline__ double axis_gap(double q, float lo, float hi) {
-  if (q < (double)lo) return (double)lo - q;
-  if (q > (double)hi) return q - (double)hi;
-  return 0.0;
+  return fmax(fmax((double)lo - q, q - (double)hi), 0.0);
 }
 
 __device__ __forceinline__ double box_dist2(double qx, double qy, double qz, const float* __restrict__ b) {
@@ -332,9 +361,11 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   int32_t* rlo = rlo_s[w];
   int32_t* rhi = rhi_s[w];
   uint32_t* hist = hist_s[w];
-  const alb_map_t& m = a.map;
-  const Node* __restrict__ nodes = (const Node*)m.nodes;
-  const int64_t n = m.n;
+  const Node* __restrict__ nodes = a.nodes;
+  const float* __restrict__ mx = a.x;
+  const float* __restrict__ my = a.y;
+  const float* __restrict__ mz = a.z;
+  const int64_t n = a.n;
   const int k = a.k;
   unsigned long long c_q = 0, c_cand = 0, c_nodes = 0, c_ovf = 0, c_steps = 0, c_sorts = 0, c_desc = 0, c_small = 0;
 
@@ -348,7 +379,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     const int64_t qrel = a.order[a.c0 + si];
     const int64_t qg = a.q_begin + qrel;
     const int64_t qi = a.direct ? qrel : si;  // output row
-    const double qx = a.q.x[qg], qy = a.q.y[qg], qz = a.q.z[qg];
+    const double qx = a.qx[qg], qy = a.qy[qg], qz = a.qz[qg];
     c_q++;
 
     // ---- 1. seed descent; every child not taken goes on the stack (with its
@@ -357,7 +388,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     int sp = 0;
     int32_t s0 = 0, s1 = (int32_t)n - 1, root = 0;
     if (a.sroot) {
-      const int32_t sc = a.q.scene[qg];
+      const int32_t sc = a.qscene[qg];
       s0 = a.sfirst[sc];
       s1 = a.slast[sc];
       root = a.sroot[sc];
@@ -412,7 +443,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     const int W = s1 - s0 + 1;
     for (int t = (int)lane; t < W; t += 64) {
       const int32_t r = s0 + t;
-      bd[t] = dist2(qx, qy, qz, m.x[r], m.y[r], m.z[r]);
+      bd[t] = dist2(qx, qy, qz, mx[r], my[r], mz[r]);
       bi[t] = r;
     }
     c_cand += (unsigned long long)W;
@@ -516,7 +547,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
             ix[u] = idx;
             c_cand += (unsigned long long)(rh - rl + 1);
             if (idx <= rh && !(idx >= s0 && idx <= s1)) {
-              d2v[u] = dist2(qx, qy, qz, m.x[idx], m.y[idx], m.z[idx]);
+              d2v[u] = dist2(qx, qy, qz, mx[idx], my[idx], mz[idx]);
               tk[u] = d2v[u] <= theta;
             }
           }
@@ -844,8 +875,7 @@ extern "C" int alb_knn(const alb_map_t* map, int32_t photon_type, int32_t k, con
   if (cn == 0) return ALB_OK;
   int cus;
   if ((rc = device_cus(&cus)) != ALB_OK) return rc;
-  KnnArgs ka{*map, *q, q_begin, c0, cn, order, rows_by_query ? 1 : 0, k, photon_type == ALB_SCATTER ? 1 : 0,
-             nbr, radius, knorm, counters, nullptr, nullptr, nullptr};
+  KnnArgs ka = knn_args(*map, *q, q_begin, c0, cn, order, rows_by_query ? 1 : 0, k, nbr, radius, counters, nullptr);
   int64_t kb = (cn + kWaves - 1) / kWaves;
   if (kb > (int64_t)cus * 16) kb = (int64_t)cus * 16;
   kb = (kb + kXcds - 1) / kXcds * kXcds;
@@ -983,8 +1013,7 @@ extern "C" int alb_knn_batched(const alb_map_t* map, const alb_scene_index_t* in
   if (cn == 0) return ALB_OK;
   int cus, rc;
   if ((rc = device_cus(&cus)) != ALB_OK) return rc;
-  KnnArgs ka{*map, *q, 0, c0, cn, order, 0, k, photon_type == ALB_SCATTER ? 1 : 0,
-             nbr, radius, knorm, counters, index->first, index->last, index->root};
+  KnnArgs ka = knn_args(*map, *q, 0, c0, cn, order, 0, k, nbr, radius, counters, index);
   int64_t kb = (cn + kWaves - 1) / kWaves;
   if (kb > (int64_t)cus * 16) kb = (int64_t)cus * 16;
   kb = (kb + kXcds - 1) / kXcds * kXcds;
