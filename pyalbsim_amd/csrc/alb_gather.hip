@@ -531,26 +531,32 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
         }
       }
       wave_sync();
-      // scan the ranges four at a time: all twelve coordinate loads issue together
+      // scan the ranges four at a time.  The twelve coordinate loads are
+      // unconditional (lanes past a range's end read its first record) so that
+      // they issue back to back before one wait, instead of one round trip per
+      // range behind a branch.
       for (int r0 = 0; r0 < nr; r0 += 4) {
         double d2v[4];
         bool tk[4];
         int32_t ix[4];
+        float xs[4], ys[4], zs[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          tk[u] = false;
-          d2v[u] = 0.0;
-          ix[u] = 0;
-          if (r0 + u < nr) {
-            const int32_t rl = rlo[r0 + u], rh = rhi[r0 + u];
-            const int32_t idx = rl + (int32_t)lane;
-            ix[u] = idx;
-            c_cand += (unsigned long long)(rh - rl + 1);
-            if (idx <= rh && !(idx >= s0 && idx <= s1)) {
-              d2v[u] = dist2(qx, qy, qz, mx[idx], my[idx], mz[idx]);
-              tk[u] = d2v[u] <= theta;
-            }
-          }
+          const bool vr = r0 + u < nr;
+          const int32_t rl = vr ? rlo[r0 + u] : 0, rh = vr ? rhi[r0 + u] : -1;
+          const int32_t idx = rl + (int32_t)lane;
+          ix[u] = idx;
+          c_cand += (unsigned long long)(rh - rl + 1);
+          tk[u] = idx <= rh && !(idx >= s0 && idx <= s1);
+          const int32_t li = idx <= rh ? idx : rl;
+          xs[u] = mx[li];
+          ys[u] = my[li];
+          zs[u] = mz[li];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          d2v[u] = dist2(qx, qy, qz, xs[u], ys[u], zs[u]);
+          tk[u] = tk[u] && d2v[u] <= theta;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
