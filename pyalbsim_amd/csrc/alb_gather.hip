@@ -367,7 +367,9 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   const float* __restrict__ mz = a.z;
   const int64_t n = a.n;
   const int k = a.k;
-  unsigned long long c_q = 0, c_cand = 0, c_nodes = 0, c_ovf = 0, c_steps = 0, c_sorts = 0, c_desc = 0, c_small = 0;
+  // per-wave event counts of one launch (at most 2^22 queries per launch spread
+  // over thousands of waves, so 32 bits hold them)
+  uint32_t c_q = 0, c_cand = 0, c_nodes = 0, c_ovf = 0, c_steps = 0, c_sorts = 0, c_desc = 0, c_small = 0;
 
   // XCD-aware slot ranges: the grid is a multiple of kXcds blocks and block b
   // runs on XCD b % kXcds, so XCD x walks its own contiguous 1/kXcds of the
@@ -441,12 +443,28 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
       }
     }
     const int W = s1 - s0 + 1;
-    for (int t = (int)lane; t < W; t += 64) {
-      const int32_t r = s0 + t;
-      bd[t] = dist2(qx, qy, qz, mx[r], my[r], mz[r]);
-      bi[t] = r;
+    {
+      // W <= kSeed: the loads of all slots issue before one wait
+      constexpr int SU = (kSeed + 63) / 64;
+      float xs[SU], ys[SU], zs[SU];
+#pragma unroll
+      for (int u = 0; u < SU; ++u) {
+        const int t = (int)lane + 64 * u;
+        const int32_t r = s0 + (t < W ? t : 0);
+        xs[u] = mx[r];
+        ys[u] = my[r];
+        zs[u] = mz[r];
+      }
+#pragma unroll
+      for (int u = 0; u < SU; ++u) {
+        const int t = (int)lane + 64 * u;
+        if (t < W) {
+          bd[t] = dist2(qx, qy, qz, xs[u], ys[u], zs[u]);
+          bi[t] = s0 + t;
+        }
+      }
     }
-    c_cand += (unsigned long long)W;
+    c_cand += (uint32_t)W;
     wave_sync();
     double theta = wave_keep_k(bd, bi, W, k, hist, lane);
     int buf_n = k;
@@ -496,7 +514,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
           }
         }
       }
-      c_nodes += (unsigned long long)__popcll(ballot(have));
+      c_nodes += (uint32_t)__popcll(ballot(have));
       const uint64_t pa = ballot(kind0 == 1), pb = ballot(kind1 == 1);
       const int npush = __popcll(pa) + __popcll(pb);
       if (sp + npush > kStack) {  // excluded by the pop policy; refuse rather than corrupt
@@ -546,7 +564,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
           const int32_t rl = vr ? rlo[r0 + u] : 0, rh = vr ? rhi[r0 + u] : -1;
           const int32_t idx = rl + (int32_t)lane;
           ix[u] = idx;
-          c_cand += (unsigned long long)(rh - rl + 1);
+          c_cand += (uint32_t)(rh - rl + 1);
           tk[u] = idx <= rh && !(idx >= s0 && idx <= s1);
           const int32_t li = idx <= rh ? idx : rl;
           xs[u] = mx[li];
@@ -590,14 +608,14 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     wave_sync();
   }
   if (a.ctr && lane == 0) {
-    if (c_q) atomicAdd(&a.ctr[ALB_CTR_QUERIES], c_q);
-    if (c_cand) atomicAdd(&a.ctr[ALB_CTR_CANDIDATES], c_cand);
-    if (c_nodes) atomicAdd(&a.ctr[ALB_CTR_NODES], c_nodes);
-    if (c_ovf) atomicAdd(&a.ctr[ALB_CTR_OVERFLOW], c_ovf);
-    if (c_steps) atomicAdd(&a.ctr[ALB_CTR_KNN_STEPS], c_steps);
-    if (c_sorts) atomicAdd(&a.ctr[ALB_CTR_KNN_SORTS], c_sorts);
-    if (c_desc) atomicAdd(&a.ctr[ALB_CTR_KNN_DESCENT], c_desc);
-    if (c_small) atomicAdd(&a.ctr[ALB_CTR_SMALL_SCENE], c_small);
+    if (c_q) atomicAdd(&a.ctr[ALB_CTR_QUERIES], (unsigned long long)c_q);
+    if (c_cand) atomicAdd(&a.ctr[ALB_CTR_CANDIDATES], (unsigned long long)c_cand);
+    if (c_nodes) atomicAdd(&a.ctr[ALB_CTR_NODES], (unsigned long long)c_nodes);
+    if (c_ovf) atomicAdd(&a.ctr[ALB_CTR_OVERFLOW], (unsigned long long)c_ovf);
+    if (c_steps) atomicAdd(&a.ctr[ALB_CTR_KNN_STEPS], (unsigned long long)c_steps);
+    if (c_sorts) atomicAdd(&a.ctr[ALB_CTR_KNN_SORTS], (unsigned long long)c_sorts);
+    if (c_desc) atomicAdd(&a.ctr[ALB_CTR_KNN_DESCENT], (unsigned long long)c_desc);
+    if (c_small) atomicAdd(&a.ctr[ALB_CTR_SMALL_SCENE], (unsigned long long)c_small);
   }
 }
 
