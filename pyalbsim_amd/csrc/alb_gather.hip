@@ -231,7 +231,8 @@ __device__ uint64_t wave_select(const double* d, const int32_t* idx, int n, int 
 // like the non-negative values); each round takes the key of a pseudo-random
 // entry still in [lo, hi] as pivot and counts the entries below and equal to
 // it with ballots, with no LDS traffic.  Random pivots give ~2 ln(n) rounds.
-__device__ uint64_t wave_kth_key(const double* d, int n, int kk, uint32_t lane) {
+// *lt_out / *eq_out receive the counts of entries below and equal to the result.
+__device__ uint64_t wave_kth_key(const double* d, int n, int kk, uint32_t lane, int* lt_out, int* eq_out) {
   constexpr int U = kBuf / 64;
   uint64_t key[U];
 #pragma unroll
@@ -240,6 +241,7 @@ __device__ uint64_t wave_kth_key(const double* d, int n, int kk, uint32_t lane) 
     key[u] = t < n ? (uint64_t)__double_as_longlong(d[t]) : ~0ull;
   }
   uint64_t lo = 0, hi = ~0ull - 1;  // invalid slots hold ~0 and never enter [lo, hi]
+  int below = 0;                    // entries below lo
   for (uint32_t round = 0;; ++round) {
     // this lane's candidate pivot: its first entry inside [lo, hi], starting at a rotating slot
     uint64_t pk = 0;
@@ -268,9 +270,12 @@ __device__ uint64_t wave_kth_key(const double* d, int n, int kk, uint32_t lane) 
     if (kk <= lt) {
       hi = pivot - 1;
     } else if (kk <= lt + eq) {
+      *lt_out = below + lt;
+      *eq_out = eq;
       return pivot;
     } else {
       kk -= lt + eq;
+      below += lt + eq;
       lo = pivot + 1;
     }
   }
@@ -279,17 +284,8 @@ __device__ uint64_t wave_kth_key(const double* d, int n, int kk, uint32_t lane) 
 // Keep the k smallest entries by (d, idx) at the front of the buffer (any
 // order) and return the k-th smallest d.
 __device__ double wave_keep_k(double* d, int32_t* idx, int n, int k, uint32_t* hist, uint32_t lane) {
-  const double th = __longlong_as_double((long long)wave_kth_key(d, n, k, lane));
   int lt = 0, eq = 0;
-  for (int t = (int)lane; t < n; t += 64) {
-    lt += d[t] < th;
-    eq += d[t] == th;
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    lt += __shfl_xor(lt, off);
-    eq += __shfl_xor(eq, off);
-  }
+  const double th = __longlong_as_double((long long)wave_kth_key(d, n, k, lane, &lt, &eq));
   int32_t icut = INT_MAX;  // entries equal to th are kept up to this index
   if (lt + eq > k) icut = (int32_t)wave_select(d, idx, n, k - lt, true, th, hist, lane);
   double vd[kBuf / 64];
