@@ -226,59 +226,69 @@ __device__ uint64_t wave_select(const double* d, const int32_t* idx, int n, int 
   return prefix;
 }
 
-// k-th smallest (1-based) of the first n buffer distances by quickselect in
-// registers: kBuf / 64 entries per lane, keys = FP64 bit patterns (ordered
-// like the non-negative values); each round takes the key of a pseudo-random
-// entry still in [lo, hi] as pivot and counts the entries below and equal to
-// it with ballots, with no LDS traffic.  Random pivots give ~2 ln(n) rounds.
-// *lt_out / *eq_out receive the counts of entries below and equal to the result.
+// k-th smallest (1-based) of the first n buffer distances, bit by bit: the
+// keys (FP64 bit patterns, ordered like the non-negative values) stay in
+// registers, kBuf / 64 per lane.  The bits on which all keys agree are found
+// with one wave OR / AND reduction and skipped; for every other bit, from the
+// top, one ballot per slot counts the still-active keys with a 0 there, and the
+// side holding rank kk stays active.  It stops when one key (or a run of equal
+// keys) is left.  *lt_out / *eq_out receive the counts of entries below and
+// equal to the result.
 __device__ uint64_t wave_kth_key(const double* d, int n, int kk, uint32_t lane, int* lt_out, int* eq_out) {
   constexpr int U = kBuf / 64;
   uint64_t key[U];
+  bool act[U];
+  uint32_t olo = 0u, ohi = 0u, alo = ~0u, ahi = ~0u;
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int t = (int)lane + 64 * u;
-    key[u] = t < n ? (uint64_t)__double_as_longlong(d[t]) : ~0ull;
+    act[u] = t < n;
+    key[u] = act[u] ? (uint64_t)__double_as_longlong(d[t]) : 0ull;
+    if (act[u]) {
+      olo |= (uint32_t)key[u];
+      ohi |= (uint32_t)(key[u] >> 32);
+      alo &= (uint32_t)key[u];
+      ahi &= (uint32_t)(key[u] >> 32);
+    }
   }
-  uint64_t lo = 0, hi = ~0ull - 1;  // invalid slots hold ~0 and never enter [lo, hi]
-  int below = 0;                    // entries below lo
-  for (uint32_t round = 0;; ++round) {
-    // this lane's candidate pivot: its first entry inside [lo, hi], starting at a rotating slot
-    uint64_t pk = 0;
-    bool has = false;
 #pragma unroll
-    for (int v = 0; v < U; ++v) {
-      const int u = (v + (int)round) % U;
-      const bool in = key[u] >= lo && key[u] <= hi;
-      if (in && !has) {
-        pk = key[u];
-        has = true;
-      }
-    }
-    const uint64_t m = ballot(has);
-    const uint32_t r = (round * 37u + 11u) & 63u;
-    const uint64_t rot = (m >> r) | (r ? (m << (64 - r)) : 0ull);
-    const int L = (int)((__ffsll((long long)rot) - 1 + r) & 63);
-    const uint64_t pivot = bcast_u64(pk, L);
-    int lt = 0, eq = 0;
+  for (int off = 32; off > 0; off >>= 1) {
+    olo |= (uint32_t)__shfl_xor((int)olo, off);
+    ohi |= (uint32_t)__shfl_xor((int)ohi, off);
+    alo &= (uint32_t)__shfl_xor((int)alo, off);
+    ahi &= (uint32_t)__shfl_xor((int)ahi, off);
+  }
+  const uint64_t diff = ((((uint64_t)ohi) << 32) | olo) ^ ((((uint64_t)ahi) << 32) | alo);
+  int m = n, below = 0;
+  for (int b = diff ? 63 - __clzll((long long)diff) : -1; b >= 0 && m > 1; --b) {
+    if (!((diff >> b) & 1ull)) continue;  // all keys share this bit
+    int c0 = 0;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const bool in = key[u] >= lo && key[u] <= hi;
-      lt += __popcll(ballot(in && key[u] < pivot));
-      eq += __popcll(ballot(in && key[u] == pivot));
-    }
-    if (kk <= lt) {
-      hi = pivot - 1;
-    } else if (kk <= lt + eq) {
-      *lt_out = below + lt;
-      *eq_out = eq;
-      return pivot;
+    for (int u = 0; u < U; ++u) c0 += __popcll(ballot(act[u] && !((key[u] >> b) & 1ull)));
+    const bool one = kk > c0;  // rank kk lies among the keys with a 1 here
+    if (one) {
+      kk -= c0;
+      below += c0;
+      m -= c0;
     } else {
-      kk -= lt + eq;
-      below += lt + eq;
-      lo = pivot + 1;
+      m = c0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) act[u] = act[u] && (((key[u] >> b) & 1ull) != 0ull) == one;
+  }
+  // every key still active equals the result
+  uint64_t pk = 0;
+  bool has = false;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    if (act[u] && !has) {
+      pk = key[u];
+      has = true;
     }
   }
+  *lt_out = below;
+  *eq_out = m;
+  return bcast_u64(pk, __ffsll((long long)ballot(has)) - 1);
 }
 
 // Keep the k smallest entries by (d, idx) at the front of the buffer (any
