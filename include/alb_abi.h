@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define ALB_ABI_VERSION 3
+#define ALB_ABI_VERSION 4
 
 /* photon types, reference enum order (BOTTOM_REFLECTION = auto() = 1, ...) */
 enum alb_photon_type {
@@ -299,7 +299,12 @@ int alb_gather(const alb_scene_t* scene, const alb_map_t* map,
  *   knorm[r] the kernel normalisation 1/(pi r^2), or 1/(4/3 pi r^3) for
  *   photon_type ALB_SCATTER.  Neighbours are in (distance, index) order when
  *   rows_by_query != 0, else in no particular order.
- * alb_contrib: adds the energies of those rows to the waveform. */
+ * alb_contrib: adds the energies of those rows to the waveform.  Bins outside
+ *   the LDS window go through n_replicas copies of the waveform (replicas:
+ *   device double[n_replicas * n_bins], contents need not be zero), so that
+ *   concurrent FP64 atomics on one bin are spread over that many addresses;
+ *   the copies are summed into waveform before the call returns (on the
+ *   stream).  n_replicas = 0 adds straight into waveform. */
 int alb_gather_order(const alb_map_t* map, const alb_queries_t* q, int64_t q_begin, int64_t q_n,
                      int32_t* order, void* workspace, size_t workspace_bytes, void* stream);
 int alb_knn(const alb_map_t* map, int32_t photon_type, int32_t k, const alb_queries_t* q,
@@ -308,7 +313,8 @@ int alb_knn(const alb_map_t* map, int32_t photon_type, int32_t k, const alb_quer
 int alb_contrib(const alb_scene_t* scene, const alb_map_t* map, int32_t photon_type,
                 const alb_queries_t* q, int64_t q_begin, const int32_t* order, int64_t c0, int64_t cn,
                 int32_t rows_by_query, const int32_t* nbr, const double* knorm, double* waveform,
-                int64_t n_bins, int32_t win0, int32_t win_len, void* stream);
+                int64_t n_bins, int32_t win0, int32_t win_len, double* replicas,
+                int32_t n_replicas, void* stream);
 
 /* Batched gather over a map built by alb_map_build_batched.  Each query
  * (q->scene required) is answered from its own scene's records only, so scene

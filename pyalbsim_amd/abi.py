@@ -5,7 +5,7 @@ import ctypes as C
 
 import numpy as np
 
-ALB_ABI_VERSION = 3
+ALB_ABI_VERSION = 4
 N_TYPES = 4
 N_COUNTERS = 18
 

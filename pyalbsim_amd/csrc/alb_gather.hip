@@ -112,6 +112,8 @@ struct ContribArgs {
   double* wf;
   int64_t n_bins;
   int32_t win0, win_len;
+  double* rep;     // [n_rep][n_bins] copies for the bins outside the window, or null
+  int32_t n_rep;
 };
 
 __device__ __forceinline__ double dist2(double qx, double qy, double qz, float x, float y, float z) {
@@ -682,7 +684,8 @@ __global__ __launch_bounds__(256) void contrib_kernel(ContribArgs a) {
     const float tt = a.q.time[qg] + m.time[i];
     const int64_t bin = (int64_t)(tt / fmult);
     if (bin >= a.win0 && bin < (int64_t)a.win0 + a.win_len) atomicAdd(&hist[bin - a.win0], e);
-    else if (bin >= 0 && bin < a.n_bins) unsafeAtomicAdd(&a.wf[bin], e);
+    else if (bin >= 0 && bin < a.n_bins)
+      unsafeAtomicAdd(a.rep ? &a.rep[(int64_t)(blockIdx.x % a.n_rep) * a.n_bins + bin] : &a.wf[bin], e);
   }
   __syncthreads();
   for (int t = threadIdx.x; t < a.win_len; t += blockDim.x)
@@ -779,6 +782,21 @@ __global__ __launch_bounds__(256) void contrib_batched_kernel(ContribBatchedArgs
   }
   __syncthreads();
   if (cur >= 0) flush_window(hist, a.win_len, cdst);
+}
+
+__global__ __launch_bounds__(256) void fill_zero_kernel(double* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+// waveform[b] += sum of the replicas' bin b, in replica order
+__global__ __launch_bounds__(256) void replica_reduce_kernel(const double* __restrict__ rep, int32_t n_rep,
+                                                             int64_t n_bins, double* __restrict__ wf) {
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n_bins; b += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int r = 0; r < n_rep; ++r) s += rep[(int64_t)r * n_bins + b];
+    if (s != 0.0) wf[b] += s;
+  }
 }
 
 __global__ __launch_bounds__(256) void query_key_kernel(alb_queries_t q, int64_t q_begin, int64_t q_n,
@@ -925,7 +943,8 @@ extern "C" int alb_knn(const alb_map_t* map, int32_t photon_type, int32_t k, con
 extern "C" int alb_contrib(const alb_scene_t* scene, const alb_map_t* map, int32_t photon_type,
                            const alb_queries_t* q, int64_t q_begin, const int32_t* order, int64_t c0, int64_t cn,
                            int32_t rows_by_query, const int32_t* nbr, const double* knorm, double* waveform,
-                           int64_t n_bins, int32_t win0, int32_t win_len, void* stream) {
+                           int64_t n_bins, int32_t win0, int32_t win_len, double* replicas, int32_t n_replicas,
+                           void* stream) {
   using namespace alb;
   if (!scene || !q || !nbr || !knorm || !waveform || !order) return set_error(ALB_ERR_ARG, "alb_contrib: null argument");
   int rc = check_map(map, scene->k, "alb_contrib");
@@ -936,16 +955,25 @@ extern "C" int alb_contrib(const alb_scene_t* scene, const alb_map_t* map, int32
   if (win_len < 0 || win_len > kMaxWindow || win0 < 0 || (int64_t)win0 + win_len > n_bins)
     return set_error(ALB_ERR_ARG, "alb_contrib: window [%d, %d) invalid (max length %d, %lld bins)", win0,
                      win0 + win_len, kMaxWindow, (long long)n_bins);
+  if (n_replicas < 0 || (n_replicas > 0 && !replicas))
+    return set_error(ALB_ERR_ARG, "alb_contrib: replicas missing");
   if (cn <= 0) return ALB_OK;
   int cus;
   if ((rc = device_cus(&cus)) != ALB_OK) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t rn = (int64_t)n_replicas * n_bins;
+  if (n_replicas > 0)
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((rn + 255) / 256 < 4096 ? (rn + 255) / 256 : 4096)),
+                       dim3(256), 0, s, replicas, rn);
   ContribArgs ca{*scene, *map, *q, q_begin, c0, cn, order, rows_by_query ? 1 : 0, photon_type, nbr, knorm,
-                 waveform, n_bins, win0, win_len};
+                 waveform, n_bins, win0, win_len, n_replicas > 0 ? replicas : nullptr, n_replicas > 0 ? n_replicas : 1};
   int64_t cb = (cn * scene->k + 255) / 256;
   if (cb > (int64_t)cus * 8) cb = (int64_t)cus * 8;
   cb = (cb + kXcds - 1) / kXcds * kXcds;
-  hipLaunchKernelGGL(contrib_kernel, dim3((unsigned)cb), dim3(256), sizeof(double) * (size_t)win_len,
-                     (hipStream_t)stream, ca);
+  hipLaunchKernelGGL(contrib_kernel, dim3((unsigned)cb), dim3(256), sizeof(double) * (size_t)win_len, s, ca);
+  if (n_replicas > 0)
+    hipLaunchKernelGGL(replica_reduce_kernel, dim3((unsigned)((n_bins + 255) / 256)), dim3(256), 0, s, replicas,
+                       n_replicas, n_bins, waveform);
   return check_launch("alb_contrib");
 }
 
@@ -982,7 +1010,7 @@ extern "C" int alb_gather(const alb_scene_t* scene, const alb_map_t* map, int32_
     rc = alb_knn(map, photon_type, k, q, q_begin, order, c0, cn, direct ? 1 : 0, nbr, rad, kn, counters, stream);
     if (rc != ALB_OK) return rc;
     rc = alb_contrib(scene, map, photon_type, q, q_begin, order, c0, cn, direct ? 1 : 0, nbr, kn, waveform, n_bins,
-                     win0, win_len, stream);
+                     win0, win_len, nullptr, 0, stream);
     if (rc != ALB_OK) return rc;
   }
   return ALB_OK;

@@ -58,7 +58,7 @@ def load_library(path: str = LIB_PATH):
     L.alb_knn.argtypes = [P, C.c_int32, C.c_int32, P, C.c_int64, P, C.c_int64, C.c_int64, C.c_int32, P, P, P, P, P]
     L.alb_knn.restype = C.c_int
     L.alb_contrib.argtypes = [P, P, C.c_int32, P, C.c_int64, P, C.c_int64, C.c_int64, C.c_int32, P, P, P, C.c_int64,
-                              C.c_int32, C.c_int32, P]
+                              C.c_int32, C.c_int32, P, C.c_int32, P]
     L.alb_contrib.restype = C.c_int
     L.alb_trace_batched.argtypes = [P, C.c_int32, P, P, C.c_int32, P, P, P, C.c_int32, C.c_int32, P, P]
     L.alb_trace_batched.restype = C.c_int
@@ -258,6 +258,7 @@ def build_map(records: Sink, n: int, dims: int, stream=None) -> DeviceMap:
 
 
 CHUNK = 1 << 22  # queries per k-NN / contribution round
+REPLICAS = 64    # waveform copies that spread the global FP64 atomics of one gather
 
 
 def gather(dscene: DeviceScene, dmap: DeviceMap, ptype: int, queries: Sink, q_n: int, waveform, window,
@@ -289,6 +290,8 @@ def gather(dscene: DeviceScene, dmap: DeviceMap, ptype: int, queries: Sink, q_n:
                               b.value, C.c_void_p(s)), "alb_gather_order")
     chunk = q_n if direct else min(q_n, CHUNK)
     knorm = torch.empty(q_n if direct else chunk, dtype=torch.float64, device=dev)
+    n_rep = REPLICAS
+    rep = torch.empty(n_rep * int(waveform.numel()), dtype=torch.float64, device=dev) if n_rep else None
     nbr = nbr_out if direct else torch.empty((chunk, k), dtype=torch.int32, device=dev)
     rad = radius_out if direct else torch.empty(chunk, dtype=torch.float64, device=dev)
 
@@ -305,12 +308,13 @@ def gather(dscene: DeviceScene, dmap: DeviceMap, ptype: int, queries: Sink, q_n:
         e1 = ev() if events is not None else None
         _check(L.alb_contrib(C.byref(dscene.s), C.byref(dmap.struct), ptype, C.byref(queries.struct), q_begin,
                              _p(order), c0, cn, 1 if direct else 0, _p(nbr), _p(knorm), _p(waveform),
-                             int(waveform.numel()), int(win0), int(win_len), C.c_void_p(s)), "alb_contrib")
+                             int(waveform.numel()), int(win0), int(win_len), _p(rep), n_rep, C.c_void_p(s)),
+               "alb_contrib")
         if events is not None:
             e2 = ev()
             events.setdefault("knn", []).append((e0, e1))
             events.setdefault("contrib", []).append((e1, e2))
-    return (ws, order, knorm, nbr, rad)  # the caller keeps these alive until the stream has run
+    return (ws, order, knorm, nbr, rad, rep)  # the caller keeps these alive until the stream has run
 
 
 MAP_DIMS = (2, 3, 2, 2)  # BOTTOM and the two surface maps are planar; SCATTER is volumetric
