@@ -324,7 +324,10 @@ int alb_contrib(const alb_scene_t* scene, const alb_map_t* map, int32_t photon_t
  * fewer than k records are -1 / NaN and counted in ALB_CTR_SMALL_SCENE);
  * alb_contrib_batched reads the scene table scenes[n_scenes] (device) and adds
  * scene s's type-t waveform into waveforms[wf_offset[s] + t * 2 * steps_s ...],
- * LDS-privatising bins [win0[s], win0[s] + win_len) (device arrays).  No
+ * LDS-privatising bins [win0[s], win0[s] + win_len) (device arrays); other
+ * bins go through n_replicas copies (replicas: device double[n_replicas *
+ * n_scenes * nb_max], nb_max >= every scene's 2 * steps), summed into the
+ * waveforms before the call returns (n_replicas = 0: straight atomics).  No
  * counterpart in the reference (core/simulation.py:132-195 gathers one scene
  * per Simulation). */
 int alb_gather_order_batched(const alb_map_t* map, const alb_scene_index_t* index,
@@ -338,7 +341,8 @@ int alb_contrib_batched(const alb_scene_t* scenes, int32_t n_scenes, int32_t k,
                         const alb_map_t* map, int32_t photon_type, const alb_queries_t* q,
                         const int32_t* order, int64_t c0, int64_t cn, const int32_t* nbr,
                         const double* knorm, double* waveforms, const int64_t* wf_offset,
-                        const int32_t* win0, int32_t win_len, void* stream);
+                        const int32_t* win0, int32_t win_len, double* replicas,
+                        int32_t n_replicas, int64_t nb_max, void* stream);
 
 
 /* ------------------------------------------------------------------ adjoint

@@ -711,6 +711,10 @@ struct ContribBatchedArgs {
   const int64_t* wf_off;
   const int32_t* win0;
   int32_t win_len;
+  // replicas of this type's waveforms, [n_rep][n_scenes][nb_max], or null
+  double* rep;
+  int32_t n_rep, n_scenes;
+  int64_t nb_max;
 };
 
 __device__ __forceinline__ void flush_window(double* hist, int win_len, double* dst) {
@@ -777,7 +781,10 @@ __global__ __launch_bounds__(256) void contrib_batched_kernel(ContribBatchedArgs
       atomicAdd(&hist[bin - cw0], e);
     } else {
       const int64_t nb = 2 * (int64_t)S->steps;
-      if (bin >= 0 && bin < nb) unsafeAtomicAdd(&a.wf[a.wf_off[sc] + (int64_t)a.type * nb + bin], e);
+      if (bin >= 0 && bin < nb)
+        unsafeAtomicAdd(a.rep ? &a.rep[((int64_t)(blockIdx.x % a.n_rep) * a.n_scenes + sc) * a.nb_max + bin]
+                              : &a.wf[a.wf_off[sc] + (int64_t)a.type * nb + bin],
+                        e);
     }
   }
   __syncthreads();
@@ -806,6 +813,23 @@ __global__ __launch_bounds__(256) void query_key_kernel(alb_queries_t q, int64_t
   if (i >= q_n) return;
   keys[i] = morton_key(q.x[q_begin + i], q.y[q_begin + i], q.z[q_begin + i], meta);
   vals[i] = (int32_t)i;
+}
+
+// waveform of scene s, type t, bin b += sum of the replicas' (s, b), in replica order
+__global__ __launch_bounds__(256) void replica_reduce_batched_kernel(const double* __restrict__ rep, int32_t n_rep,
+                                                                     int32_t n_scenes, int64_t nb_max,
+                                                                     const alb_scene_t* __restrict__ scenes,
+                                                                     const int64_t* __restrict__ wf_off, int32_t type,
+                                                                     double* __restrict__ wf) {
+  const int64_t total = (int64_t)n_scenes * nb_max;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t sc = i / nb_max, b = i - sc * nb_max;
+    const int64_t nb = 2 * (int64_t)scenes[sc].steps;
+    if (b >= nb) continue;
+    double s = 0.0;
+    for (int r = 0; r < n_rep; ++r) s += rep[(int64_t)r * total + i];
+    if (s != 0.0) wf[wf_off[sc] + (int64_t)type * nb + b] += s;
+  }
 }
 
 __global__ __launch_bounds__(256) void query_key_batched_kernel(alb_queries_t q, int64_t q_n,
@@ -1084,7 +1108,8 @@ extern "C" int alb_knn_batched(const alb_map_t* map, const alb_scene_index_t* in
 extern "C" int alb_contrib_batched(const alb_scene_t* scenes, int32_t n_scenes, int32_t k, const alb_map_t* map,
                                    int32_t photon_type, const alb_queries_t* q, const int32_t* order, int64_t c0,
                                    int64_t cn, const int32_t* nbr, const double* knorm, double* waveforms,
-                                   const int64_t* wf_offset, const int32_t* win0, int32_t win_len, void* stream) {
+                                   const int64_t* wf_offset, const int32_t* win0, int32_t win_len, double* replicas,
+                                   int32_t n_replicas, int64_t nb_max, void* stream) {
   using namespace alb;
   if (!scenes || !map || !q || !order || !nbr || !knorm || !waveforms || !wf_offset || !win0)
     return set_error(ALB_ERR_ARG, "alb_contrib_batched: null argument");
@@ -1094,15 +1119,28 @@ extern "C" int alb_contrib_batched(const alb_scene_t* scenes, int32_t n_scenes, 
   if (k <= 0 || k > kMaxK) return set_error(ALB_ERR_UNSUPPORTED, "alb_contrib_batched: k = %d outside [1, %d]", k, kMaxK);
   if (win_len < 0 || win_len > kMaxWindow)
     return set_error(ALB_ERR_ARG, "alb_contrib_batched: window length %d outside [0, %d]", win_len, kMaxWindow);
+  if (n_replicas < 0 || (n_replicas > 0 && (!replicas || nb_max <= 0)))
+    return set_error(ALB_ERR_ARG, "alb_contrib_batched: replicas missing");
   if (cn <= 0) return ALB_OK;
   int cus, rc;
   if ((rc = device_cus(&cus)) != ALB_OK) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t rn = (int64_t)n_replicas * n_scenes * nb_max;
+  if (n_replicas > 0)
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((rn + 255) / 256 < 8192 ? (rn + 255) / 256 : 8192)),
+                       dim3(256), 0, s, replicas, rn);
   ContribBatchedArgs ca{scenes, *map, *q, c0, cn, order, photon_type, k, nbr, knorm, waveforms, wf_offset, win0,
-                        win_len};
+                        win_len, n_replicas > 0 ? replicas : nullptr, n_replicas > 0 ? n_replicas : 1, n_scenes,
+                        nb_max};
   const int64_t tiles = (cn * k + 255) / 256;
   int64_t cb = (int64_t)cus * 8;
   if (cb > tiles) cb = tiles;
-  hipLaunchKernelGGL(contrib_batched_kernel, dim3((unsigned)cb), dim3(256), sizeof(double) * (size_t)win_len,
-                     (hipStream_t)stream, ca);
+  hipLaunchKernelGGL(contrib_batched_kernel, dim3((unsigned)cb), dim3(256), sizeof(double) * (size_t)win_len, s, ca);
+  if (n_replicas > 0) {
+    const int64_t tot = (int64_t)n_scenes * nb_max;
+    hipLaunchKernelGGL(replica_reduce_batched_kernel, dim3((unsigned)((tot + 255) / 256 < 8192 ? (tot + 255) / 256 : 8192)),
+                       dim3(256), 0, s, replicas, n_replicas, n_scenes, nb_max, scenes, wf_offset, photon_type,
+                       waveforms);
+  }
   return check_launch("alb_contrib_batched");
 }

@@ -69,7 +69,7 @@ def load_library(path: str = LIB_PATH):
     L.alb_knn_batched.argtypes = [P, P, C.c_int32, C.c_int32, P, P, C.c_int64, C.c_int64, P, P, P, P, P]
     L.alb_knn_batched.restype = C.c_int
     L.alb_contrib_batched.argtypes = [P, C.c_int32, C.c_int32, P, C.c_int32, P, P, C.c_int64, C.c_int64, P, P, P,
-                                      P, P, C.c_int32, P]
+                                      P, P, C.c_int32, P, C.c_int32, C.c_int64, P]
     L.alb_contrib_batched.restype = C.c_int
     L.alb_adjoint_contrib.argtypes = [P, C.c_int32, C.c_int32, P, C.c_int32, P, P, C.c_int64, C.c_int64, P, P, P, P,
                                       P, P, P, P]
@@ -259,6 +259,7 @@ def build_map(records: Sink, n: int, dims: int, stream=None) -> DeviceMap:
 
 CHUNK = 1 << 22  # queries per k-NN / contribution round
 REPLICAS = 64    # waveform copies that spread the global FP64 atomics of one gather
+BATCHED_REPLICAS = 0   # the same for a batched gather (copies of one type's waveforms of every scene)
 
 
 def gather(dscene: DeviceScene, dmap: DeviceMap, ptype: int, queries: Sink, q_n: int, waveform, window,
@@ -491,6 +492,9 @@ def gather_batched(table: DeviceSceneTable, dmap: DeviceMap, ptype: int, queries
     nbr = nbr_out if direct else torch.empty((chunk, k), dtype=torch.int32, device=dev)
     rad = radius_out if direct else torch.empty(chunk, dtype=torch.float64, device=dev)
     knorm = torch.empty(chunk, dtype=torch.float64, device=dev)
+    nb_max = int(table.n_bins.max())
+    n_rep = BATCHED_REPLICAS
+    rep = torch.empty(n_rep * table.n * nb_max, dtype=torch.float64, device=dev) if n_rep else None
 
     def ev():
         e = torch.cuda.Event(enable_timing=True)
@@ -506,9 +510,9 @@ def gather_batched(table: DeviceSceneTable, dmap: DeviceMap, ptype: int, queries
         e1 = ev() if events is not None else None
         _check(L.alb_contrib_batched(_p(table.dev), table.n, k, C.byref(dmap.struct), ptype, C.byref(queries.struct),
                                      _p(order), c0, cn, _p(nbr), _p(knorm), _p(waveforms), _p(wf_offset), _p(win0),
-                                     int(win_len), C.c_void_p(s)), "alb_contrib_batched")
+                                     int(win_len), _p(rep), n_rep, nb_max, C.c_void_p(s)), "alb_contrib_batched")
         if events is not None:
             e2 = ev()
             events.setdefault("knn", []).append((e0, e1))
             events.setdefault("contrib", []).append((e1, e2))
-    return (ws, order, knorm, nbr, rad)
+    return (ws, order, knorm, nbr, rad, rep)
