@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define ALB_ABI_VERSION 4
+#define ALB_ABI_VERSION 5
 
 /* photon types, reference enum order (BOTTOM_REFLECTION = auto() = 1, ...) */
 enum alb_photon_type {
@@ -90,6 +90,32 @@ enum alb_counter {
   ALB_N_COUNTERS = 18
 };
 
+/* One turbidity layer of a layered or depth-profiled water column
+ * (src/alb_sim/physics/models/water.py:39-416).  A photon at depth d (= -(y -
+ * y_ws)) is in the first layer whose bottom exceeds d (searchsorted right,
+ * clamped to the last layer) and at local depth d - top. */
+typedef struct alb_layer {
+  double top;               /* float32 cumulative height of the layers above       */
+  double bottom;            /* FP64 cumulative height including this layer         */
+  double height;
+  int32_t profiled;         /* 0: constant properties, 1: depth profiles           */
+  int32_t lut_n;            /* profiled: entries of the attenuation table          */
+  float alpha;              /* constant: float32 lidar attenuation (1/m)           */
+  float omega0;             /* constant: float32 single-scattering albedo          */
+  double v;                 /* constant: float32(c / n)                            */
+  int32_t a_kind;           /* absorption: 0 constant, 1 linear, 2 exponential     */
+  int32_t n_kind;           /* refractive index: same codes                        */
+  double a0, a1;            /* absorption at the top / bottom (profile start, end) */
+  double n0, n1;            /* refractive index at the top / bottom                */
+  const float* z_lut;       /* profiled: [lut_n] float32 local depths              */
+  const double* alpha_lut;  /* profiled: [lut_n] attenuation at z_lut              */
+  const double* phase_theta;/* [phase_n] phase-function tables of this layer       */
+  const double* phase_cdf;
+  const double* phase_mass;
+  int32_t phase_n;
+  int32_t pad0;
+} alb_layer_t;
+
 /* Scene constants, computed on the host in FP64 from SimulationConfig
  * (src/alb_sim/physics/models/simulation.py:21-74, water.py:117-245,
  *  sea_surface.py:30-33, laser.py:22-33).  Pointer members are device pointers
@@ -132,6 +158,12 @@ typedef struct alb_scene {
                                     inverse-CDF segment: [0] the atom below cdf[0],
                                     [i] the interval [cdf[i-1], cdf[i])            */
   const double* phase_dlog_mass; /* [phase_n] d log(phase_mass[i])/dg               */
+  /* layered column (0 / NULL: one constant layer described by the fields
+   * above, traced with closed-form water legs); otherwise water sub-steps are
+   * taken one at a time with per-depth properties */
+  int32_t n_layers;
+  int32_t pad1;
+  const alb_layer_t* layers;     /* [n_layers] (device pointer for the HIP library)  */
 } alb_scene_t;
 
 typedef struct alb_rng {

@@ -67,6 +67,7 @@ def lib():
             L.orc_scatter_mult.argtypes = [P, P, P, P, C.c_int]
             L.orc_scatter_mult.restype = C.c_double
             L.orc_emission_dir.argtypes = [P, P, C.c_int32, C.c_uint64, P]
+            L.orc_layer_lookup.argtypes = [P, C.c_double, P]
             L.orc_lambert.argtypes = [C.c_double, C.c_double, P]
             L.orc_scatter_direction.argtypes = [P, P, C.c_double, C.c_double, P]
             for name in ("orc_ggx_d_x", "orc_g1_x", "orc_schlick_x"):
@@ -183,6 +184,27 @@ class HostScene:
         self.score = None if phase_score is None else np.ascontiguousarray(phase_score, dtype=np.float64)
         self.s = abi.make_scene(model, _ptr(self.theta), _ptr(self.cdf), 0,
                                 _ptr(self.score) if self.score is not None else 0)
+        self.layer_arrays = self.layer_structs = None
+        if model.layers:
+            from pyalbsim_amd.model import layer_arrays
+
+            self.layer_arrays = layer_arrays(model)
+            ptrs = [{nm: _ptr(a[nm]) for nm in abi.LAYER_TABLES} for a in self.layer_arrays]
+            self.layer_structs = abi.make_layers(model, ptrs)
+            self.s.n_layers = len(model.layers)
+            self.s.layers = C.addressof(self.layer_structs)
+
+
+def layer_lookup(model, depths) -> np.ndarray:
+    """(layer, velocity, float32 attenuation, float32 albedo) at each depth of a
+    layered scene, from the oracle's per-depth lookups."""
+    hs = HostScene(model)
+    out = np.zeros((len(depths), 4))
+    buf = np.zeros(4)
+    for i, d in enumerate(np.asarray(depths, dtype=np.float64)):
+        lib().orc_layer_lookup(C.byref(hs.s), float(d), _ptr(buf))
+        out[i] = buf
+    return out
 
 
 _FIELDS_F32 = ("x", "y", "z", "dx", "dy", "dz", "energy", "time")

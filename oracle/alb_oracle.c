@@ -195,6 +195,88 @@ double orc_interp(double x, const double* xp, const double* fp, int n) {
   return r;
 }
 
+/* ------------------------------------------------------- layered water
+ * Per-depth lookups of WaterModel (src/alb_sim/physics/models/water.py:253-416)
+ * and TurbidityLayerModel (water.py:39-250), with NumPy's dtype rules. */
+
+/* layer_index (water.py:283-286): searchsorted(cumsum, d, side="right"),
+ * clamped to the last layer */
+static int orc_layer_index(const alb_scene_t* sc, double d) {
+  int idx = 0;
+  for (int i = 0; i < sc->n_layers; ++i)
+    if (sc->layers[i].bottom <= d) idx = i + 1;
+  return idx < sc->n_layers - 1 ? idx : sc->n_layers - 1;
+}
+
+/* _eval of a profile at t = z / h (utils/parameter_profile.py:47-99):
+ * LinearParameter casts t to float32 and computes
+ * float32(start) + t * float32(end - start) in float32; ExponentialParameter
+ * exp(log s + t (log e - log s)) in FP64 */
+static double orc_profile(int kind, double s, double e, double z, double h) {
+  if (kind == 0) return s;
+  double t = z / h;
+  if (kind == 1) {
+    float tf = (float)t;
+    float span = (float)(e - s);
+    float prod = tf * span;
+    float r = (float)s + prod;
+    return (double)r;
+  }
+  double ls = log(s), le = log(e);
+  return exp(ls + t * (le - ls));
+}
+
+/* np.interp at float32(z) over the float32 z grid (water.py:210-229) */
+static double orc_layer_alpha64(const alb_layer_t* L, double z) {
+  const int n = L->lut_n;
+  const double x = (double)(float)z;
+  if (x < (double)L->z_lut[0]) return L->alpha_lut[0];
+  if (x >= (double)L->z_lut[n - 1]) return L->alpha_lut[n - 1];
+  int lo = 0;
+  while (lo + 1 < n - 1 && (double)L->z_lut[lo + 1] <= x) ++lo;
+  const double x0 = (double)L->z_lut[lo], x1 = (double)L->z_lut[lo + 1];
+  if (x == x0) return L->alpha_lut[lo];
+  const double slope = (L->alpha_lut[lo + 1] - L->alpha_lut[lo]) / (x1 - x0);
+  return slope * (x - x0) + L->alpha_lut[lo];
+}
+
+/* Lookups at depth d below the surface: out = {layer, velocity (float32
+ * value), float32 attenuation, float32 single-scattering albedo}.
+ *   velocity: velocities_by_position stores c / n into a float32 array
+ *     (physics/models/simulation.py:214-242); a float32 n (linear profile)
+ *     divides in float32 (water.py:198-201), an FP64 n in FP64;
+ *   attenuation: a float32 array of the LUT value (water.py:330-349), or of
+ *     the FP64 constant for a constant layer;
+ *   albedo: (alpha - a) / alpha in FP64 from the FP64 LUT value, stored
+ *     float32 (water.py:235-245, 351-370). */
+void orc_layer_lookup(const alb_scene_t* sc, double d, double out[4]) {
+  const int li = orc_layer_index(sc, d);
+  const alb_layer_t* L = &sc->layers[li];
+  const double z = d - L->top;  /* layer_local_depth: float32 tops (water.py:294-299) */
+  double v, a32, w32;
+  if (!L->profiled) {
+    v = L->v;
+    a32 = (double)L->alpha;
+    w32 = (double)L->omega0;
+  } else {
+    if (L->n_kind == 1) {
+      float n32 = (float)orc_profile(1, L->n0, L->n1, z, L->height);
+      float c32 = (float)2.998e8;
+      v = (double)(c32 / n32);
+    } else {
+      v = (double)(float)(2.998e8 / orc_profile(L->n_kind, L->n0, L->n1, z, L->height));
+    }
+    const double al = orc_layer_alpha64(L, z);
+    a32 = (double)(float)al;
+    const double a = orc_profile(L->a_kind, L->a0, L->a1, z, L->height);
+    w32 = (double)(float)((al - a) / al);
+  }
+  out[0] = (double)li;
+  out[1] = v;
+  out[2] = a32;
+  out[3] = w32;
+}
+
 /* Schlick Fresnel (src/alb_sim/physics/reflection/fresnel.py:14-29) */
 static inline double orc_schlick(double cos_t, double F0) {
   return F0 + (1.0 - F0) * pow(1.0 - cos_t, 5.0);
@@ -350,8 +432,9 @@ static int orc_phase_segment(double u, const alb_scene_t* sc) {
   return i + 1;
 }
 
-void orc_scatter_direction(const alb_scene_t* sc, const double din[3], double u1, double u2, double out[3]) {
-  double theta = orc_interp(u1, sc->phase_cdf, sc->phase_theta, sc->phase_n);
+static void orc_scatter_direction_tab(const double* theta_tab, const double* cdf, int n, const double din[3],
+                                      double u1, double u2, double out[3]) {
+  double theta = orc_interp(u1, cdf, theta_tab, n);
   double phi = 2.0 * ORC_PI * u2;
   double l[3] = {sin(theta) * cos(phi), sin(theta) * sin(phi), cos(theta)};
   float R[9];
@@ -362,6 +445,10 @@ void orc_scatter_direction(const alb_scene_t* sc, const double din[3], double u1
     acc = acc + (double)R[3 * i + 2] * l[2];
     out[i] = (double)(float)acc;
   }
+}
+
+void orc_scatter_direction(const alb_scene_t* sc, const double din[3], double u1, double u2, double out[3]) {
+  orc_scatter_direction_tab(sc->phase_theta, sc->phase_cdf, sc->phase_n, din, u1, u2, out);
 }
 
 /* Trace one photon through all S sub-steps.  Returns 0, or 1 if its replay
@@ -388,6 +475,7 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
   const double y_ws = sc->y_ws, y_sf = sc->y_sf;
   const double up[3] = {0.0, 1.0, 0.0}, down[3] = {0.0, -1.0, 0.0};
   const double alpha = (double)sc->alpha_l;
+  const int layered = sc->n_layers > 0 && sc->layers != NULL;
 
   for (int64_t s = 0; s < sc->n_substeps; ++s) {
     double y = p[1];
@@ -395,6 +483,11 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
     if (y > y_ws && d[1] >= 0.0) break;
     /* velocities_by_position (physics/models/simulation.py:214-242) */
     double v = (y > y_ws) ? sc->v_air : sc->v_water;
+    double look[4];
+    if (layered && y <= y_ws) {
+      orc_layer_lookup(sc, (y - y_ws) * -1.0, look);
+      v = look[1];
+    }
     double nx[3];
     for (int i = 0; i < 3; ++i) nx[i] = p[i] + (d[i] * v) * sc->dt;
     double yn = nx[1];
@@ -413,7 +506,8 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
       /* handle_water (core/simulation.py:328-373) */
       double sv[3] = {nx[0] - p[0], nx[1] - p[1], nx[2] - p[2]};
       double len = orc_norm3(sv);
-      tau = (float)((double)tau + alpha * len);
+      /* attenuation at the pre-step position (the lookup above) */
+      tau = (float)((double)tau + (layered ? look[2] : alpha) * len);
       int scatter = tau >= tau_t;
       if (!scatter) g_path.lam += len;
       p[0] = nx[0]; p[1] = nx[1]; p[2] = nx[2];
@@ -430,11 +524,22 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
         double u1 = orc_uniform(&st);
         double u2 = orc_uniform(&st);
         double nd[3];
-        orc_scatter_direction(sc, d, u1, u2, nd);
-        if (sc->phase_score) g_path.gs += sc->phase_score[orc_phase_segment(u1, sc)];
+        float w0 = sc->omega0;
+        if (layered) {
+          /* phase function and albedo of the layer at the new position
+           * (core/simulation.py:362-368, water.py:372-403) */
+          double lk[4];
+          orc_layer_lookup(sc, (p[1] - y_ws) * -1.0, lk);
+          const alb_layer_t* L = &sc->layers[(int)lk[0]];
+          orc_scatter_direction_tab(L->phase_theta, L->phase_cdf, L->phase_n, d, u1, u2, nd);
+          w0 = (float)lk[3];
+        } else {
+          orc_scatter_direction(sc, d, u1, u2, nd);
+          if (sc->phase_score) g_path.gs += sc->phase_score[orc_phase_segment(u1, sc)];
+        }
         g_path.nsc++;
         d[0] = nd[0]; d[1] = nd[1]; d[2] = nd[2];
-        E = E * sc->omega0;
+        E = E * w0;
         tau = tau - tau_t;
         tau_t = orc_tau_target(orc_uniform(&st));
       }
@@ -501,7 +606,14 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
       double rem = (1.0 - f) * len;
       for (int i = 0; i < 3; ++i) p[i] = P[i] + d[i] * rem;
       if (reflect) {
-        tau = (float)((double)tau + alpha * len);
+        /* attenuation at the position after the reflection (core/simulation.py:495-499) */
+        double a_post = alpha;
+        if (layered) {
+          double lk[4];
+          orc_layer_lookup(sc, (p[1] - y_ws) * -1.0, lk);
+          a_post = lk[2];
+        }
+        tau = (float)((double)tau + a_post * len);
         g_path.lam += len;
       }
     } else { /* ST_FLOOR */
@@ -526,7 +638,14 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
       double len = orc_norm3(sv);
       double rem = (1.0 - f) * len;
       for (int i = 0; i < 3; ++i) p[i] = P[i] + d[i] * rem;
-      tau = (float)((double)tau + alpha * len);
+      /* attenuation at the position after the reflection (core/simulation.py:563-566) */
+      double a_post = alpha;
+      if (layered) {
+        double lk[4];
+        orc_layer_lookup(sc, (p[1] - y_ws) * -1.0, lk);
+        a_post = lk[2];
+      }
+      tau = (float)((double)tau + a_post * len);
       g_path.lam += len;
       g_path.nfl++;
       E = E * (float)sc->floor_albedo;
@@ -780,12 +899,23 @@ int orc_gather(const alb_scene_t* sc, int32_t type, const float* x, const float*
     double m_bottom = (sc->floor_albedo / ORC_PI) * fmax(0.0, -ds[1]);
     if (nbr_out) memcpy(nbr_out + j * k, idx, sizeof(int32_t) * (size_t)k);
     if (radius_out) radius_out[j] = r;
+    /* scatter_energy: phase function of the layer at the query vertex depth
+     * (physics/models/water.py:418-447) */
+    const double* th = sc->phase_theta;
+    const double* cdf = sc->phase_cdf;
+    int pn = sc->phase_n;
+    if (sc->n_layers > 0 && sc->layers) {
+      const alb_layer_t* L = &sc->layers[orc_layer_index(sc, (qy + (-sc->y_ws)) * -1.0)];
+      th = L->phase_theta;
+      cdf = L->phase_cdf;
+      pn = L->phase_n;
+    }
     for (int32_t t = 0; t < k; ++t) {
       int32_t i = idx[t];
       float dp[3] = {dx[i], dy[i], dz[i]};
       double m;
       if (type == ALB_BOTTOM_REFLECTION) m = m_bottom;
-      else if (type == ALB_SCATTER) m = orc_scatter_mult(dp, ds, sc->phase_theta, sc->phase_cdf, sc->phase_n);
+      else if (type == ALB_SCATTER) m = orc_scatter_mult(dp, ds, th, cdf, pn);
       else if (type == ALB_SURFACE_REFLECTION) m = (double)orc_brdf_reflect(dp, ds, sc->alpha_ggx, sc->F0);
       else m = orc_btdf_transmit(dp, ds, sc->alpha_ggx, sc->n_water, sc->F0, sc->surface_albedo);
       float ee = q->energy[j] * energy[i];

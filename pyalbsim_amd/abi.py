@@ -5,7 +5,7 @@ import ctypes as C
 
 import numpy as np
 
-ALB_ABI_VERSION = 4
+ALB_ABI_VERSION = 5
 N_TYPES = 4
 N_COUNTERS = 18
 
@@ -43,7 +43,49 @@ class Scene(C.Structure):
         ("n_substeps", C.c_int64), ("steps", C.c_int32), ("mult", C.c_int32), ("k", C.c_int32),
         ("phase_n", C.c_int32), ("phase_theta", C.c_void_p), ("phase_cdf", C.c_void_p),
         ("phase_mass", C.c_void_p), ("phase_score", C.c_void_p), ("phase_dlog_mass", C.c_void_p),
+        ("n_layers", C.c_int32), ("pad1", C.c_int32), ("layers", C.c_void_p),
     ]
+
+
+class Layer(C.Structure):
+    _fields_ = [
+        ("top", C.c_double), ("bottom", C.c_double), ("height", C.c_double), ("profiled", C.c_int32),
+        ("lut_n", C.c_int32), ("alpha", C.c_float), ("omega0", C.c_float), ("v", C.c_double),
+        ("a_kind", C.c_int32), ("n_kind", C.c_int32), ("a0", C.c_double), ("a1", C.c_double),
+        ("n0", C.c_double), ("n1", C.c_double), ("z_lut", C.c_void_p), ("alpha_lut", C.c_void_p),
+        ("phase_theta", C.c_void_p), ("phase_cdf", C.c_void_p), ("phase_mass", C.c_void_p),
+        ("phase_n", C.c_int32), ("pad0", C.c_int32),
+    ]
+
+
+def make_layer(lm, z_lut_ptr: int, alpha_lut_ptr: int, theta_ptr: int, cdf_ptr: int, mass_ptr: int) -> Layer:
+    """Pack a model.LayerModel; the table pointers are device or host addresses."""
+    L = Layer()
+    L.top, L.bottom, L.height = lm.top, lm.bottom, lm.height
+    L.profiled = 1 if lm.profiled else 0
+    L.lut_n = len(lm.z_lut)
+    L.alpha = float(np.float32(lm.alpha64)) if not lm.profiled else 0.0
+    L.omega0 = float(np.float32(lm.omega064)) if not lm.profiled else 0.0
+    L.v = lm.v if not lm.profiled else 0.0
+    L.a_kind, L.a0, L.a1 = lm.a_spec
+    L.n_kind, L.n0, L.n1 = lm.n_spec
+    L.z_lut, L.alpha_lut = z_lut_ptr, alpha_lut_ptr
+    L.phase_theta, L.phase_cdf, L.phase_mass = theta_ptr, cdf_ptr, mass_ptr
+    L.phase_n = len(lm.phase_theta)
+    return L
+
+
+LAYER_TABLES = ("z_lut", "alpha_lut", "theta", "cdf", "mass")
+
+
+def make_layers(model, ptrs) -> C.Array:
+    """alb_layer_t[n_layers] of a layered SceneModel; ptrs[i] maps the names in
+    LAYER_TABLES to layer i's table addresses (device or host)."""
+    arr = (Layer * len(model.layers))()
+    for i, lm in enumerate(model.layers):
+        p = ptrs[i]
+        arr[i] = make_layer(lm, p["z_lut"], p["alpha_lut"], p["theta"], p["cdf"], p["mass"])
+    return arr
 
 
 class Rng(C.Structure):

@@ -155,6 +155,93 @@ __device__ __forceinline__ double interp_seg(double x, const double* __restrict_
   return r;
 }
 
+// np.interp with a float32 grid (converted to FP64 as NumPy does)
+__device__ __forceinline__ double interp_f32grid(double x, const float* __restrict__ xp,
+                                                 const double* __restrict__ fp, int n) {
+  if (x < (double)xp[0]) return fp[0];
+  if (x >= (double)xp[n - 1]) return fp[n - 1];
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if ((double)xp[mid] <= x) lo = mid; else hi = mid;
+  }
+  if ((double)xp[lo] == x) return fp[lo];
+  const double slope = (fp[lo + 1] - fp[lo]) / ((double)xp[lo + 1] - (double)xp[lo]);
+  double r = slope * (x - (double)xp[lo]) + fp[lo];
+  if (isnan(r)) {
+    r = slope * (x - (double)xp[lo + 1]) + fp[lo + 1];
+    if (isnan(r) && fp[lo] == fp[lo + 1]) r = fp[lo];
+  }
+  return r;
+}
+
+// ------------------------------------------------------------ layered water
+// (src/alb_sim/physics/models/water.py:253-416, physics/models/simulation.py:214-314)
+
+// layer of depth d: the number of layer bottoms <= d (np.searchsorted
+// side="right"), clamped to the last layer
+__device__ __forceinline__ int layer_of(const alb_layer_t* __restrict__ L, int n, double d) {
+  int i = 0;
+  while (i < n - 1 && L[i].bottom <= d) ++i;
+  return i;
+}
+
+// a layer property at local depth z (TurbidityLayerModel._eval with NumPy's
+// dtype rules): a linear profile evaluates in float32 with t = float32(z / h),
+// an exponential one in FP64 (parameter_profile.py:47-99)
+__device__ __forceinline__ double layer_profile(int kind, double s, double e, double z, double h) {
+  if (kind == 0) return s;
+  if (kind == 1) {
+    const float t = (float)(z / h);
+    const float r = (float)s + t * (float)(e - s);
+    return (double)r;
+  }
+  const double ls = log(s), le = log(e);
+  return exp(ls + (z / h) * (le - ls));
+}
+
+// speed (float32 value) at local depth z: c / n, in float32 arithmetic when n
+// is a float32 profile value, else in FP64 then rounded to float32
+__device__ __forceinline__ double layer_velocity(const alb_layer_t& L, double z) {
+  if (!L.profiled) return L.v;
+  if (L.n_kind == 1) {
+    const float n32 = (float)layer_profile(1, L.n0, L.n1, z, L.height);
+    return (double)(2.998e8f / n32);
+  }
+  return (double)(float)(2.998e8 / layer_profile(L.n_kind, L.n0, L.n1, z, L.height));
+}
+
+// lidar attenuation at local depth z: FP64 table value (np.interp at float32(z))
+__device__ __forceinline__ double layer_alpha64(const alb_layer_t& L, double z) {
+  return interp_f32grid((double)(float)z, L.z_lut, L.alpha_lut, L.lut_n);
+}
+
+// float32 attenuation used for the optical depth
+__device__ __forceinline__ float layer_alpha32(const alb_layer_t& L, double z) {
+  return L.profiled ? (float)layer_alpha64(L, z) : L.alpha;
+}
+
+// float32 single-scattering albedo (alpha - a) / alpha at local depth z
+__device__ __forceinline__ float layer_omega0(const alb_layer_t& L, double z) {
+  if (!L.profiled) return L.omega0;
+  const double al = layer_alpha64(L, z);
+  const double a = layer_profile(L.a_kind, L.a0, L.a1, z, L.height);
+  return (float)((al - a) / al);
+}
+
+// Phase-bin mass table of the gather's scatter multiplier: the layer of the
+// query vertex depth (y + flying height) * -1 in a layered column
+// (physics/models/water.py:418-447), else the scene's own table.
+__device__ __forceinline__ const double* gather_phase_mass(const alb_scene_t& sc, double qy, int* n) {
+  if (!sc.layers) {
+    *n = sc.phase_n;
+    return sc.phase_mass;
+  }
+  const alb_layer_t& L = sc.layers[layer_of(sc.layers, sc.n_layers, (qy - sc.y_ws) * -1.0)];
+  *n = L.phase_n;
+  return L.phase_mass;
+}
+
 // np.interp on the theta grid itself (uniform spacing up to rounding): locate
 // the node by division and correct by one in either direction.
 __device__ __forceinline__ double interp_uniform_grid(double x, const double* __restrict__ xp,

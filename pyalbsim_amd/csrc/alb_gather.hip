@@ -672,7 +672,9 @@ __global__ __launch_bounds__(256) void contrib_kernel(ContribArgs a) {
     if (a.type == ALB_BOTTOM_REFLECTION) {
       mul = (a.sc.floor_albedo / ALB_PI) * fmax(0.0, -sdy);
     } else if (a.type == ALB_SCATTER) {
-      mul = scatter_mass(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, a.sc.phase_mass, a.sc.phase_n);
+      int pn;
+      const double* pm = gather_phase_mass(a.sc, a.q.y[qg], &pn);
+      mul = scatter_mass(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, pm, pn);
     } else if (a.type == ALB_SURFACE_REFLECTION) {
       mul = (double)brdf_reflect(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, a.sc.alpha_ggx, a.sc.F0);
     } else {
@@ -766,7 +768,9 @@ __global__ __launch_bounds__(256) void contrib_batched_kernel(ContribBatchedArgs
     if (a.type == ALB_BOTTOM_REFLECTION) {
       mul = (S->floor_albedo / ALB_PI) * fmax(0.0, -sdy);
     } else if (a.type == ALB_SCATTER) {
-      mul = scatter_mass(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, S->phase_mass, S->phase_n);
+      int pn;
+      const double* pm = gather_phase_mass(*S, a.q.y[qg], &pn);
+      mul = scatter_mass(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, pm, pn);
     } else if (a.type == ALB_SURFACE_REFLECTION) {
       mul = (double)brdf_reflect(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, S->alpha_ggx, S->F0);
     } else {

@@ -297,6 +297,167 @@ __device__ __forceinline__ void emit_photon(const alb_scene_t& sc, bool fwd, con
   }
 }
 
+// Layered or depth-profiled water column: sub-steps are taken one at a time,
+// as Simulation.simulate_photon_step does (core/simulation.py:246-322): the
+// speed and the attenuation come from the layer of the pre-step position
+// (physics/models/simulation.py:214-265), the phase table and the
+// single-scattering albedo of a scatter from the layer of the new position
+// (core/simulation.py:362-368), and the optical depth added after a floor or
+// surface reflection from the layer of the position after the event
+// (core/simulation.py:495-499, 563-566).  Positions are updated by the same
+// repeated additions, so they match the reference's bit for bit.
+template <bool STATS>
+__device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, const alb_rng_t& rng, Stream& st,
+                                              Photon& ph, Emit& e, Tally& c) {
+  const double two_pi = 2.0 * ALB_PI;
+  const int64_t S = sc.n_substeps;
+  const double y_ws = sc.y_ws, y_sf = sc.y_sf;
+  const alb_layer_t* __restrict__ Ls = sc.layers;
+  const int nl = sc.n_layers;
+  for (;;) {
+    if (ph.s >= S) {
+      ph.alive = false;  // time budget used up in water
+      return;
+    }
+    const double y = ph.py;
+    const double dep = (y - y_ws) * -1.0;
+    const alb_layer_t& L = Ls[layer_of(Ls, nl, dep)];
+    const double zl = dep - L.top;
+    const double v = layer_velocity(L, zl);
+    const double nx = ph.px + (ph.dx * v) * sc.dt, ny = ph.py + (ph.dy * v) * sc.dt, nz = ph.pz + (ph.dz * v) * sc.dt;
+    // evaluate_state (core/simulation.py:220-244) for a position at or below the surface
+    int state = 0;
+    if (ny > y_sf) state = 1;                // in water
+    if (y > y_sf && ny <= y_sf) state = 2;   // hitting the floor
+    if (ny > y_ws) state = 3;                // leaving the water
+    if (state == 0) {
+      ph.alive = false;  // below the floor and not rising: no event can follow
+      return;
+    }
+    const double svx = nx - ph.px, svy = ny - ph.py, svz = nz - ph.pz;
+    const double slen = norm3(svx, svy, svz);
+    if (state == 1) {
+      // handle_water (core/simulation.py:328-373)
+      const float a32 = layer_alpha32(L, zl);
+      ph.tau = (float)((double)ph.tau + (double)a32 * slen);
+      c.ws++;
+      if constexpr (STATS) ph.lam += slen;
+      const bool scat = ph.tau >= ph.tau_t;
+      ph.px = nx;
+      ph.py = ny;
+      ph.pz = nz;
+      if (!scat) {
+        ph.s++;
+        continue;
+      }
+      c.sc++;
+      e.et = ALB_SCATTER;
+      e.x = ph.px; e.y = ph.py; e.z = ph.pz; e.dx = ph.dx; e.dy = ph.dy; e.dz = ph.dz;
+      e.E = ph.E;
+      e.t = (float)ph.s;
+      e.eno = ph.event_no++;
+      snapshot<STATS>(e, ph);
+      const double dep2 = (ph.py - y_ws) * -1.0;
+      const alb_layer_t& L2 = Ls[layer_of(Ls, nl, dep2)];
+      const double zl2 = dep2 - L2.top;
+      const double u = uniform(rng, st);
+      const double theta = interp(u, L2.phase_cdf, L2.phase_theta, L2.phase_n);
+      const double phi = two_pi * uniform(rng, st);
+      double sth, cth, sph, cph;
+      sincos(theta, &sth, &cth);
+      sincos(phi, &sph, &cph);
+      float Rm[9];
+      rotation_from_z(ph.dx, ph.dy, ph.dz, Rm);
+      const double l0 = sth * cph, l1 = sth * sph, l2 = cth;
+      double w0, w1, w2;
+      w0 = (double)Rm[0] * l0; w0 = w0 + (double)Rm[1] * l1; w0 = w0 + (double)Rm[2] * l2;
+      w1 = (double)Rm[3] * l0; w1 = w1 + (double)Rm[4] * l1; w1 = w1 + (double)Rm[5] * l2;
+      w2 = (double)Rm[6] * l0; w2 = w2 + (double)Rm[7] * l1; w2 = w2 + (double)Rm[8] * l2;
+      ph.dx = (double)(float)w0;
+      ph.dy = (double)(float)w1;
+      ph.dz = (double)(float)w2;
+      if constexpr (STATS) ph.nsc++;
+      ph.E = ph.E * layer_omega0(L2, zl2);
+      ph.tau = ph.tau - ph.tau_t;
+      ph.tau_t = tau_target(uniform(rng, st));
+      ph.s++;
+      break;
+    }
+    if (state == 2) {
+      // handle_seafloor (core/simulation.py:504-570)
+      c.fl++;
+      double f = (y_sf - ph.py) / svy;
+      f = clipd(f, 0.0, 1.0);
+      const double Px = ph.px + f * svx, Py = ph.py + f * svy, Pz = ph.pz + f * svz;
+      e.et = ALB_BOTTOM_REFLECTION;
+      e.x = Px; e.y = Py; e.z = Pz; e.dx = ph.dx; e.dy = ph.dy; e.dz = ph.dz;
+      e.E = ph.E;
+      e.t = (float)ph.s;
+      e.eno = ph.event_no++;
+      snapshot<STATS>(e, ph);
+      const double u1 = uniform(rng, st);
+      const double u2 = uniform(rng, st);
+      const double sr = sqrt(u1);
+      double sp, cp;
+      sincos(two_pi * u2, &sp, &cp);
+      ph.dx = sr * cp;
+      ph.dy = sqrt(1.0 - u1);
+      ph.dz = sr * sp;
+      const double rem = (1.0 - f) * slen;
+      ph.px = Px + ph.dx * rem;
+      ph.py = Py + ph.dy * rem;
+      ph.pz = Pz + ph.dz * rem;
+      const double dep2 = (ph.py - y_ws) * -1.0;
+      const alb_layer_t& L2 = Ls[layer_of(Ls, nl, dep2)];
+      ph.tau = (float)((double)ph.tau + (double)layer_alpha32(L2, dep2 - L2.top) * slen);
+      if constexpr (STATS) {
+        ph.lam += slen;
+        ph.nfl++;
+      }
+      ph.E = ph.E * (float)sc.floor_albedo;
+      ph.s++;
+      break;
+    }
+    // handle_exit (core/simulation.py:433-502)
+    c.ex++;
+    double f = (y_ws - ph.py) / svy;
+    f = clipd(f, 0.0, 1.0);
+    const double Px = ph.px + f * svx, Py = ph.py + f * svy, Pz = ph.pz + f * svz;
+    const double R = schlick(ph.dy, sc.F0);
+    const bool reflect = uniform(rng, st) < R;
+    const uint32_t no = ph.event_no++;
+    if (!reflect && fwd) {
+      e.et = ALB_SURFACE_TRANSMISSION_UP;
+      e.x = Px; e.y = Py; e.z = Pz; e.dx = ph.dx; e.dy = ph.dy; e.dz = ph.dz;
+      e.E = ph.E;
+      e.t = (float)ph.s;
+      e.eno = no;
+      snapshot<STATS>(e, ph);
+    }
+    if (reflect) {
+      ph.dy = -ph.dy;
+    } else {
+      refract_y(ph.dx, ph.dy, ph.dz, -1.0, sc.eta);
+    }
+    const double rem = (1.0 - f) * slen;
+    ph.px = Px + ph.dx * rem;
+    ph.py = Py + ph.dy * rem;
+    ph.pz = Pz + ph.dz * rem;
+    if (reflect) {
+      const double dep2 = (ph.py - y_ws) * -1.0;
+      const alb_layer_t& L2 = Ls[layer_of(Ls, nl, dep2)];
+      ph.tau = (float)((double)ph.tau + (double)layer_alpha32(L2, dep2 - L2.top) * slen);
+      if constexpr (STATS) ph.lam += slen;
+    }
+    ph.s++;
+    break;
+  }
+  if (st.exhausted) {
+    ph.alive = false;
+    c.exh++;
+  }
+}
+
 // Advance a live photon to its next event and apply it; fills e when the
 // event leaves a record (forward) or a query (backward).  STATS: also keep the
 // path statistics the adjoint needs (water length, scatter and floor counts,
@@ -350,6 +511,9 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
     ph.s = ph.s + j + 1;
   } else if (ph.py <= y_sf && ph.dy <= 0.0) {
     ph.alive = false;  // below the floor and not rising: the reference finds no event
+    return;
+  } else if (sc.layers) {
+    water_stepped<STATS>(sc, fwd, rng, st, ph, e, c);
     return;
   } else {
     // ---------------------------------------------------------- water leg
@@ -786,6 +950,8 @@ extern "C" int alb_trace(const alb_scene_t* scene, const alb_rng_t* rng, int32_t
     return set_error(ALB_ERR_ARG, "alb_trace: replay mode needs a replay buffer");
   if (!scene->phase_cdf || !scene->phase_theta || scene->phase_n < 2)
     return set_error(ALB_ERR_ARG, "alb_trace: missing phase tables");
+  if (scene->n_layers < 0 || (scene->n_layers > 0) != (scene->layers != nullptr))
+    return set_error(ALB_ERR_ARG, "alb_trace: n_layers and layers disagree");
   if (photon_begin + n > ((int64_t)1 << 47))
     return set_error(ALB_ERR_ARG, "alb_trace: photon index exceeds 2^47");
   Sinks sk;

@@ -12,8 +12,8 @@ that is computed once per scene rather than per photon:
 * +z rotation to a vector  math/rotation_matrix.py:39-102
 
 The traced path (HIP) and the oracle both consume the resulting ``alb_scene_t``.
-Only one constant water layer is supported on the traced path; depth profiles
-and layer stacks raise ``NotImplementedError`` here.
+Layer stacks and linear / exponential depth profiles are supported
+(LayerModel); other profile types raise ``NotImplementedError``.
 """
 from __future__ import annotations
 
@@ -150,6 +150,66 @@ def base_reflectance(n: float) -> float:
     return ((1 - n) / (1 + n)) ** 2
 
 
+PROFILE_CONST, PROFILE_LINEAR, PROFILE_EXPONENTIAL = 0, 1, 2
+
+
+def profile_spec(value) -> tuple:
+    """(kind, start, end) of a layer property: a float or a Linear/Exponential
+    parameter (utils/parameter_profile.py:47-99)."""
+    from .utils.parameter_profile import ExponentialParameter, LinearParameter
+
+    if isinstance(value, float):
+        return PROFILE_CONST, value, value
+    if isinstance(value, LinearParameter):
+        return PROFILE_LINEAR, float(value.start), float(value.end)
+    if isinstance(value, ExponentialParameter):
+        return PROFILE_EXPONENTIAL, float(value.start), float(value.end)
+    raise NotImplementedError(f"depth profile {type(value).__name__} is not supported on the traced path")
+
+
+def eval_profile(value, z_local, height: float):
+    """TurbidityLayerModel._eval (physics/models/water.py:103-109) with NumPy's
+    dtype rules: floats pass through, profiles see t = z_local / height."""
+    if isinstance(value, float):
+        return value
+    return value.at(z_local / height)
+
+
+def lidar_attenuation_profile(layer, z_local, wavelength_um: float, flying_height: float, divergence: float,
+                              bp_frac: float):
+    """_calculate_lidar_attenuation_coefficient (physics/models/water.py:117-184)
+    at local depths z_local, in the same operation order and dtypes."""
+    h = layer.height
+    bw = (1 + 0.008027 * eval_profile(layer.salinity, z_local, h)) * 0.00012 * wavelength_um ** (-4.24)
+    bp = eval_profile(layer.scattering_coefficient, z_local, h) - bw
+    spot = 2 * flying_height * math.tan(divergence / 2)
+    a = eval_profile(layer.absorption_coefficient, z_local, h)
+    kd = a + 4.18 * bp * bp_frac * (1 - 0.52 * np.exp(-10.8 * a))
+    b = eval_profile(layer.scattering_coefficient, z_local, h)
+    return kd + (bp - 4.18 * bp * bp_frac * (1 - 0.52 * np.exp(-10.8 * b))) * np.exp(-0.85 * spot * (a + bp))
+
+
+@dataclass
+class LayerModel:
+    """One turbidity layer of a layered water column (physics/models/water.py:39-250)."""
+
+    top: float          # depth of the top: float32 cumulative height, as layer_local_depth uses
+    bottom: float       # FP64 cumulative height: depths >= bottom belong to deeper layers
+    height: float
+    profiled: bool
+    alpha64: float      # constant layers: lidar attenuation (FP64)
+    omega064: float     # constant layers: single-scattering albedo (FP64)
+    v: float            # constant layers: float32(c / n)
+    a_spec: tuple       # (kind, start, end) of the absorption coefficient
+    n_spec: tuple       # (kind, start, end) of the refractive index
+    z_lut: np.ndarray   # profiled layers: float32 z grid (num_sublayers)
+    alpha_lut: np.ndarray  # profiled layers: attenuation at z_lut (as computed, float32 or float64)
+    phase_theta: np.ndarray
+    phase_pf: np.ndarray
+    phase_cdf: np.ndarray
+    backscatter: float
+
+
 @dataclass
 class SceneModel:
     """All per-scene constants; arrays stay alive as long as the model."""
@@ -183,6 +243,7 @@ class SceneModel:
     phase_cdf: np.ndarray
     backscatter: float
     phase_g: float | None = None  # Henyey-Greenstein asymmetry when the table is HG
+    layers: list = None           # LayerModel per layer when the column is layered or depth-profiled
 
     @property
     def n_substeps(self) -> int:
@@ -213,18 +274,75 @@ class SceneModel:
         return g0, min(length, self.n_bins)
 
 
+def _layer_model(layer, top32: float, bottom: float, cfg, hg_g):
+    ff = layer.fournier_forand_parameters
+    if hg_g is None:
+        theta, pf, cdf = fournier_forand_table(ff.junge_slope, ff.refractive_index_ratio)
+        bpf = backscatter_fraction(ff.junge_slope, ff.refractive_index_ratio)
+    else:
+        theta, pf, cdf = henyey_greenstein_table(float(hg_g))
+        bpf = henyey_greenstein_backscatter(float(hg_g))
+    names = ("scattering_coefficient", "absorption_coefficient", "salinity", "refractive_index")
+    profiled = not all(isinstance(getattr(layer, nm), float) for nm in names)
+    for nm in names:
+        profile_spec(getattr(layer, nm))  # rejects unsupported profile types
+    H = cfg.scene.flying_height
+    lam, div = cfg.laser.wavelength, cfg.laser.divergence_angle
+    a_spec, n_spec = profile_spec(layer.absorption_coefficient), profile_spec(layer.refractive_index)
+    if profiled:
+        if int(layer.num_sublayers) < 1:
+            raise ValueError("a depth-profiled layer needs num_sublayers >= 1")
+        z_lut = np.linspace(0.0, layer.height, int(layer.num_sublayers), dtype=np.float32)
+        # the reference's backscatter fraction is a NumPy float64 scalar, which takes part in dtype promotion
+        lut = np.asarray(lidar_attenuation_profile(layer, z_lut, lam, H, div, np.float64(bpf)))
+        if lut.ndim == 0:
+            lut = np.full(z_lut.shape, lut)
+        alpha64 = omega064 = v = float("nan")
+    else:
+        z_lut = np.zeros(0, dtype=np.float32)
+        lut = np.zeros(0)
+        a = layer.absorption_coefficient
+        alpha64 = lidar_attenuation(a, layer.scattering_coefficient, layer.salinity, lam, H, div, bpf)
+        omega064 = (alpha64 - a) / alpha64
+        v = float(np.float32(LIGHT_SPEED_AIR / layer.refractive_index))
+    return LayerModel(top=float(top32), bottom=float(bottom), height=float(layer.height), profiled=profiled,
+                      alpha64=float(alpha64), omega064=float(omega064), v=v, a_spec=a_spec, n_spec=n_spec,
+                      z_lut=z_lut, alpha_lut=lut, phase_theta=theta, phase_pf=pf, phase_cdf=cdf, backscatter=bpf)
+
+
+def layer_arrays(model: "SceneModel") -> list:
+    """Contiguous per-layer tables of a layered scene, in the dtypes alb_layer_t
+    points at: z grid (float32), attenuation at the grid (FP64), phase theta,
+    CDF and phase-bin mass (FP64)."""
+    out = []
+    for lm in model.layers or ():
+        out.append({
+            "z_lut": np.ascontiguousarray(lm.z_lut, dtype=np.float32),
+            "alpha_lut": np.ascontiguousarray(lm.alpha_lut, dtype=np.float64),
+            "theta": np.ascontiguousarray(lm.phase_theta, dtype=np.float64),
+            "cdf": np.ascontiguousarray(lm.phase_cdf, dtype=np.float64),
+            "mass": np.ascontiguousarray(phase_bin_mass(lm.phase_theta, lm.phase_cdf), dtype=np.float64),
+        })
+    return out
+
+
 def build_scene(cfg: SimulationConfig, hg_g: float | None = None) -> SceneModel:
     """Scene constants of cfg.  hg_g: use a Henyey-Greenstein phase function
-    with asymmetry hg_g instead of the layer's Fournier-Forand one (its
-    backscatter fraction then also enters the lidar attenuation)."""
+    with asymmetry hg_g instead of the layers' Fournier-Forand ones (its
+    backscatter fraction then also enters the lidar attenuation).
+
+    A column of one constant layer uses the closed-form water legs; several
+    layers or depth profiles (physics/models/water.py:253-416) are traced one
+    sub-step at a time with the per-depth lookups (SceneModel.layers)."""
     layers = cfg.water.layers
-    if len(layers) != 1:
-        raise NotImplementedError("the traced path supports exactly one water layer")
+    heights = [layer.height for layer in layers]
+    cum = np.cumsum(heights)
+    depth = float(cum[-1])
+    tops32 = np.concatenate(([0.0], cum[:-1])).astype(np.float32)
+    lmodels = [_layer_model(layer, tops32[i], cum[i], cfg, hg_g) for i, layer in enumerate(layers)]
+    layered = len(layers) > 1 or lmodels[0].profiled
     layer = layers[0]
-    for name in ("scattering_coefficient", "absorption_coefficient", "salinity", "refractive_index"):
-        if not isinstance(getattr(layer, name), float):
-            raise NotImplementedError(f"depth-profiled {name} is not supported on the traced path")
-    depth = float(np.cumsum([layer.height])[-1])
+    l0 = lmodels[0]
     H = cfg.scene.flying_height
     d_l = laser_direction(cfg.laser.nadir_angle, cfg.laser.azimuth_angle)
     y_ws = -H
@@ -234,25 +352,20 @@ def build_scene(cfg: SimulationConfig, hg_g: float | None = None) -> SceneModel:
     dt = 1 / (mult * sr)
     distance = y_sf / d_l[1]
     steps = int(1.5 * (distance * round(sr)) / LIGHT_SPEED_AIR)
-    ff = layer.fournier_forand_parameters
-    if hg_g is None:
-        theta, pf, cdf = fournier_forand_table(ff.junge_slope, ff.refractive_index_ratio)
-        bpf = backscatter_fraction(ff.junge_slope, ff.refractive_index_ratio)
-    else:
-        theta, pf, cdf = henyey_greenstein_table(float(hg_g))
-        bpf = henyey_greenstein_backscatter(float(hg_g))
-    a = layer.absorption_coefficient
-    b = layer.scattering_coefficient
-    alpha = lidar_attenuation(a, b, layer.salinity, cfg.laser.wavelength, H, cfg.laser.divergence_angle, bpf)
-    n = layer.refractive_index
+    # sea surface: layer 0's refractive index at z = 0 (physics/models/sea_surface.py:30-33)
+    n = float(eval_profile(layer.refractive_index, 0.0, layer.height))
+    a0 = float(eval_profile(layer.absorption_coefficient, 0.0, layer.height))
+    b0 = float(eval_profile(layer.scattering_coefficient, 0.0, layer.height))
+    alpha = l0.alpha64 if not l0.profiled else float("nan")
     return SceneModel(
         config=cfg, y_ws=float(y_ws), y_sf=float(y_sf), depth=depth, laser_dir=d_l,
         rot_laser=rotation_from_z(d_l), divergence=float(cfg.laser.divergence_angle),
         fov=float(cfg.sensor.field_of_view), aperture_radius=float(cfg.sensor.aperture_radius),
-        v_air=float(np.float32(LIGHT_SPEED_AIR)), v_water=float(np.float32(LIGHT_SPEED_AIR / n)),
-        dt=dt, n_water=float(n), F0=base_reflectance(n), roughness=float(cfg.sea_surface.roughness),
+        v_air=float(np.float32(LIGHT_SPEED_AIR)), v_water=l0.v if not l0.profiled else float("nan"),
+        dt=dt, n_water=n, F0=base_reflectance(n), roughness=float(cfg.sea_surface.roughness),
         surface_albedo=float(cfg.sea_surface.albedo), floor_albedo=float(cfg.sea_floor.albedo),
-        absorption=float(a), scattering=float(b), alpha_l64=alpha, omega064=(alpha - a) / alpha,
-        steps=steps, mult=int(mult), k=int(cfg.photon_mapping_k), phase_theta=theta, phase_pf=pf,
-        phase_cdf=cdf, backscatter=bpf, phase_g=None if hg_g is None else float(hg_g),
+        absorption=a0, scattering=b0, alpha_l64=alpha, omega064=(alpha - a0) / alpha if not l0.profiled else float("nan"),
+        steps=steps, mult=int(mult), k=int(cfg.photon_mapping_k), phase_theta=l0.phase_theta, phase_pf=l0.phase_pf,
+        phase_cdf=l0.phase_cdf, backscatter=l0.backscatter, phase_g=None if hg_g is None else float(hg_g),
+        layers=lmodels if layered else None,
     )

@@ -113,6 +113,25 @@ def _p(t) -> int:
     return int(t.data_ptr()) if t is not None else 0
 
 
+def _upload_layers(model: SceneModel, device):
+    """Layer tables and the alb_layer_t array of a layered scene on the device:
+    (device struct tensor or None, tensors to keep alive)."""
+    from .model import layer_arrays
+
+    if not model.layers:
+        return None, []
+    torch = _torch()
+    keep, ptrs = [], []
+    for arrs in layer_arrays(model):
+        tens = {nm: torch.as_tensor(arrs[nm], device=device) for nm in abi.LAYER_TABLES}
+        keep.extend(tens.values())
+        ptrs.append({nm: _p(t) for nm, t in tens.items()})
+    structs = abi.make_layers(model, ptrs)
+    dev = torch.as_tensor(np.frombuffer(bytes(structs), dtype=np.uint8).copy(), device=device)
+    keep.append(dev)
+    return dev, keep
+
+
 class DeviceScene:
     """Scene constants with the phase tables resident on the device."""
 
@@ -126,6 +145,10 @@ class DeviceScene:
         self.mass = torch.as_tensor(phase_bin_mass(model.phase_theta, model.phase_cdf), dtype=torch.float64,
                                     device=device)
         self.s = abi.make_scene(model, _p(self.theta), _p(self.cdf), _p(self.mass))
+        self.layers, self._layer_keep = _upload_layers(model, device)
+        if self.layers is not None:
+            self.s.n_layers = len(model.layers)
+            self.s.layers = _p(self.layers)
         self.device = device
 
 
@@ -344,6 +367,7 @@ class DeviceSceneTable:
             raise ValueError("all scenes of a batch must use the same photon_mapping_k")
         self.k = ks.pop()
         self.tables = {}
+        self.layer_tables = {}
         structs = (abi.Scene * self.n)()
         for i, m in enumerate(self.models):
             gm = gather_phase[i] if gather_phase is not None else m
@@ -364,6 +388,13 @@ class DeviceSceneTable:
                 self.tables[h] = (th, cd, ms, sc, dm)
             th, cd, ms, sc, dm = self.tables[h]
             structs[i] = abi.make_scene(m, _p(th), _p(cd), _p(ms), _p(sc), _p(dm))
+            if m.layers:
+                if adjoint:
+                    raise NotImplementedError("the adjoint covers single constant-layer scenes only")
+                if id(m) not in self.layer_tables:
+                    self.layer_tables[id(m)] = _upload_layers(m, device)
+                structs[i].n_layers = len(m.layers)
+                structs[i].layers = _p(self.layer_tables[id(m)][0])
         self.host = structs
         raw = np.frombuffer(bytes(structs), dtype=np.uint8).copy()
         self.dev = torch.as_tensor(raw, device=device)

@@ -24,3 +24,45 @@ def run_linear_script_scene():
         water=WaterConfig(layers=(TurbidityLayerConfig(height=7, absorption_coefficient=0.01,
                                                        scattering_coefficient=3.5),)),
         sea_floor=SeaFloorConfig(albedo=0.1))
+
+
+def layered_scene(n_profile=None, k=100, flying_height=400.0, nadir=0.0):
+    """Three-layer column, 6 m: a clear constant top layer, a depth-profiled
+    middle layer (linear scattering, exponential absorption, linear refractive
+    index unless n_profile is given) with its own Fournier-Forand particles,
+    and a turbid constant bottom layer."""
+    from pyalbsim_amd.config import FournierForandConfig
+    from pyalbsim_amd.utils.parameter_profile import ExponentialParameter, LinearParameter
+
+    n_mid = LinearParameter(1.34, 1.345) if n_profile is None else n_profile
+    layers = (
+        TurbidityLayerConfig(height=1.5, absorption_coefficient=0.08, scattering_coefficient=0.25,
+                             refractive_index=1.34),
+        TurbidityLayerConfig(height=2.0, scattering_coefficient=LinearParameter(0.3, 1.2),
+                             absorption_coefficient=ExponentialParameter(0.05, 0.2), salinity=35.0,
+                             refractive_index=n_mid, num_sublayers=40,
+                             fournier_forand_parameters=FournierForandConfig(junge_slope=3.8,
+                                                                             refractive_index_ratio=1.05)),
+        TurbidityLayerConfig(height=2.5, absorption_coefficient=0.15, scattering_coefficient=0.6,
+                             refractive_index=1.34,
+                             fournier_forand_parameters=FournierForandConfig(junge_slope=3.4,
+                                                                             refractive_index_ratio=1.15)),
+    )
+    return SimulationConfig(
+        scene=SceneConfig(flying_height=flying_height),
+        laser=LaserConfig(nadir_angle=nadir, divergence_angle=1e-3, wavelength=0.532),
+        sensor=SensorConfig(field_of_view=20e-3, sample_rate=int(1e10), aperture_radius=0.1),
+        sea_surface=SeaSurfaceConfig(roughness=0.05),
+        water=WaterConfig(layers=layers),
+        sea_floor=SeaFloorConfig(albedo=0.15), sample_multiplier=10, photon_mapping_k=k,
+        heatmap=HeatmapConfig(enabled=False))
+
+
+def stacked_copies(cfg, n: int):
+    """cfg with its single constant layer split into n identical layers of
+    the same total height."""
+    import dataclasses
+
+    (layer,) = cfg.water.layers
+    part = dataclasses.replace(layer, height=layer.height / n)
+    return dataclasses.replace(cfg, water=WaterConfig(layers=(part,) * n))

@@ -34,6 +34,7 @@ def test_layout_and_version():
     assert C.sizeof(abi.Rng) == 32
     assert C.sizeof(abi.Records) == 96 and C.sizeof(abi.Queries) == 96
     assert C.sizeof(abi.SceneIndex) == 32
+    assert C.sizeof(abi.Layer) == 136 and abi.Layer.z_lut.offset == 88 and abi.Layer.phase_n.offset == 128
 
 
 def test_last_error_is_set_by_argument_checks():

@@ -1,0 +1,152 @@
+"""Layered and depth-profiled water columns in the oracle (CPU).
+
+The oracle's per-depth lookups (layer index, local depth, speed, attenuation,
+single-scattering albedo) are written in C with explicit float32 / FP64
+casts.  Here they are checked against the same quantities computed the way
+the reference computes them, with NumPy's own routines and dtype promotion
+(np.searchsorted side="right", np.interp on a float32 grid, float32 arrays
+receiving FP64 values, the profile objects' at()): a wrong searchsorted side,
+FP64 layer tops, a linear profile evaluated in FP64 or an albedo taken from
+the float32 attenuation all fail here.  Depths include every layer boundary
+exactly, values just above and below them, and depths outside the column.
+
+Invariant: a constant layer split into identical layers of the same total
+height traces exactly like the single layer (same records, bit for bit).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from pyalbsim_amd import abi
+from pyalbsim_amd.model import build_scene, lidar_attenuation, lidar_attenuation_profile
+from pyalbsim_amd.utils.parameter_profile import LinearParameter
+from tests.scenes import layered_scene, single_shot, stacked_copies
+
+C_AIR = 2.998e8
+
+
+def _numpy_lookup(cfg, model, depths):
+    """velocities_by_depth, lidar_attenuation_coefficients_by_depth and
+    single_scattering_albedos_by_depth (physics/models/water.py:302-370)
+    evaluated with NumPy, as the reference evaluates them."""
+    layers = cfg.water.layers
+    cum = np.cumsum([layer.height for layer in layers])
+    idx = np.minimum(np.searchsorted(cum, depths, side="right"), len(layers) - 1)
+    z0 = np.concatenate(([0.0], cum[:-1])).astype(np.float32)
+    z = depths - z0[idx]
+    eps = np.finfo(np.float32).eps
+    vel = np.full(len(depths), eps)                       # float32 (velocities_by_position)
+    alpha = np.full_like(depths, eps, dtype=np.float32)
+    albedo = np.full_like(depths, eps, dtype=np.float32)
+    for li, layer in enumerate(layers):
+        m = idx == li
+        if not np.any(m):
+            continue
+        zl = z[m]
+        h = layer.height
+
+        def ev(v):
+            return v if isinstance(v, float) else v.at(zl / h)
+
+        vel[m] = C_AIR / ev(layer.refractive_index)
+        lm = model.layers[li]
+        if lm.profiled:
+            a64 = np.interp(np.asarray(zl, dtype=np.float32), lm.z_lut, lm.alpha_lut,
+                            left=lm.alpha_lut[0], right=lm.alpha_lut[-1])
+        else:
+            a64 = np.full(zl.shape, lm.alpha64)
+        alpha[m] = a64
+        albedo[m] = (a64 - ev(layer.absorption_coefficient)) / a64
+    return idx, vel.astype(np.float64), alpha, albedo
+
+
+def _depths(cfg):
+    cum = np.cumsum([layer.height for layer in cfg.water.layers])
+    rs = np.random.default_rng(3)
+    pts = [rs.uniform(-0.5, cum[-1] + 0.5, 4000), [0.0, -1e-9, cum[-1] + 3.0]]
+    for c in cum:
+        pts.append([c, np.nextafter(c, -np.inf), np.nextafter(c, np.inf), c - 1e-6, c + 1e-6])
+        pts.append([float(np.float32(c)), float(np.nextafter(np.float32(c), np.float32(-np.inf)))])
+    return np.concatenate([np.asarray(p, dtype=np.float64) for p in pts])
+
+
+@pytest.mark.parametrize("n_profile", [None, 1.338, "exp"])
+def test_layer_lookups_follow_numpy_semantics(n_profile):
+    from pyalbsim_amd.utils.parameter_profile import ExponentialParameter
+
+    prof = ExponentialParameter(1.34, 1.35) if n_profile == "exp" else n_profile
+    cfg = layered_scene(n_profile=prof)
+    model = build_scene(cfg)
+    d = _depths(cfg)
+    got = oracle.layer_lookup(model, d)
+    idx, vel, alpha, albedo = _numpy_lookup(cfg, model, d)
+    assert np.array_equal(got[:, 0].astype(int), idx)
+    assert np.array_equal(got[:, 1], vel)
+    assert np.array_equal(got[:, 2], alpha.astype(np.float64))
+    assert np.array_equal(got[:, 3], albedo.astype(np.float64))
+    # the profiled layer really varies with depth
+    mid = idx == 1
+    assert np.ptp(alpha[mid]) > 0.1 and np.ptp(albedo[mid]) > 0.01
+    if n_profile is None or n_profile == "exp":
+        assert np.ptp(vel[mid]) > 0
+
+
+def test_linear_profile_rounds_in_float32():
+    """LinearParameter.at: start + float32(t) * (end - start) in float32."""
+    p = LinearParameter(1.34, 1.345)
+    t = np.float32(0.3)
+    expect = np.float32(np.float32(1.34) + np.float32(t * np.float32(1.345 - 1.34)))
+    assert p.at(0.3) == float(expect)
+    assert float(expect) != 1.34 + 0.3 * (1.345 - 1.34)
+
+
+def test_constant_layer_lut_equals_closed_form():
+    """A profiled layer whose properties are all floats except the LUT grid
+    (evaluated as arrays) gives the single-layer attenuation at every node."""
+    cfg = single_shot()
+    layer = cfg.water.layers[0]
+    z = np.linspace(0.0, layer.height, 11, dtype=np.float32)
+    model = build_scene(cfg)
+    lut = np.asarray(lidar_attenuation_profile(layer, z, cfg.laser.wavelength, cfg.scene.flying_height,
+                                               cfg.laser.divergence_angle, np.float64(model.backscatter)))
+    ref = lidar_attenuation(layer.absorption_coefficient, layer.scattering_coefficient, layer.salinity,
+                            cfg.laser.wavelength, cfg.scene.flying_height, cfg.laser.divergence_angle,
+                            model.backscatter)
+    assert np.all(lut == ref)
+
+
+@pytest.mark.parametrize("pass_", [abi.PASS_FORWARD, abi.PASS_BACKWARD])
+def test_split_constant_layer_traces_like_one_layer(pass_):
+    cfg = single_shot(a=0.1, b=0.6)
+    one = build_scene(cfg)
+    many = build_scene(stacked_copies(cfg, 3))
+    assert many.layers is not None and len(many.layers) == 3
+    a, ca, _ = oracle.trace(one, (11, 12), pass_, 0, 300)
+    b, cb, _ = oracle.trace(many, (11, 12), pass_, 0, 300)
+    assert sum(len(a[t]["x"]) for t in range(4)) > 300
+    for t in range(4):
+        for f in a[t]:
+            assert np.array_equal(a[t][f], b[t][f]), (t, f)
+    assert np.array_equal(ca, cb)
+
+
+def test_layered_trace_uses_per_layer_properties():
+    """The middle layer's properties change the records: a column with the
+    profiled middle layer replaced by a constant one traces differently, and
+    scatter records are found in all three layers."""
+    cfg = layered_scene()
+    model = build_scene(cfg)
+    recs, _, _ = oracle.trace(model, (5, 6), abi.PASS_FORWARD, 0, 3000)
+    sc = recs[abi.SCATTER]
+    depth = -(sc["y"].astype(np.float64) - model.y_ws)
+    cum = np.cumsum([layer.height for layer in cfg.water.layers])
+    li = np.minimum(np.searchsorted(cum, depth, side="right"), 2)
+    assert all(np.sum(li == i) > 20 for i in range(3)), np.bincount(li)
+    import dataclasses
+
+    flat = dataclasses.replace(cfg.water.layers[1], scattering_coefficient=0.3, absorption_coefficient=0.05,
+                               refractive_index=1.34, salinity=37.0)
+    cfg2 = dataclasses.replace(cfg, water=dataclasses.replace(
+        cfg.water, layers=(cfg.water.layers[0], flat, cfg.water.layers[2])))
+    recs2, _, _ = oracle.trace(build_scene(cfg2), (5, 6), abi.PASS_FORWARD, 0, 3000)
+    assert len(recs2[abi.SCATTER]["x"]) != len(sc["x"])
