@@ -423,7 +423,9 @@ int alb_replay_index_build(const int64_t* tags, const int32_t* scenes, int64_t n
  * and add, per scene s, acc[5 s + ...] += weight * (n, [scatter record], L,
  * m, G) of its prefix.  Nothing is stored per path.  work_counter as in
  * alb_trace_batched; counters[ALB_CTR_REPLAY_MISSES] counts events whose
- * record was not in the index (0 when the sinks did not overflow). */
+ * record was not in the index (0 when the sinks did not overflow).
+ * Single constant-layer scenes only (layers == NULL): the statistics are
+ * those of one homogeneous water column. */
 int alb_replay_batched(const alb_scene_t* scenes, int32_t n_scenes, const int64_t* photon_offsets,
                        const alb_rng_t* rng, int32_t pass, const alb_replay_index_t* index,
                        unsigned long long* work_counter, int32_t block, int32_t refill_interval,

@@ -462,7 +462,9 @@ __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, c
 // event leaves a record (forward) or a query (backward).  STATS: also keep the
 // path statistics the adjoint needs (water length, scatter and floor counts,
 // phase-sampling score) and snapshot them into e; the physics is unchanged.
-template <bool STATS>
+// LAYERED: the scene may be a layered column (checked at run time); without
+// it the kernel carries only the closed-form water legs.
+template <bool STATS, bool LAYERED = true>
 __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, const alb_rng_t& rng, Stream& st,
                                            Photon& ph, Emit& e, Tally& c) {
   const double two_pi = 2.0 * ALB_PI;
@@ -512,7 +514,7 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
   } else if (ph.py <= y_sf && ph.dy <= 0.0) {
     ph.alive = false;  // below the floor and not rising: the reference finds no event
     return;
-  } else if (sc.layers) {
+  } else if (LAYERED && sc.layers) {
     water_stepped<STATS>(sc, fwd, rng, st, ph, e, c);
     return;
   } else {
@@ -712,8 +714,9 @@ __device__ __forceinline__ void flush_tally(const Tally& c, unsigned long long* 
   if (lane == 0 && ts) atomicAdd(&ctr[ALB_CTR_DEPTH_SOLVES], ts);
 }
 
-// One scene, one lane per photon.
-template <bool FWD>
+// One scene, one lane per photon; LAYERED selects the stepped water legs of
+// a layered column.
+template <bool FWD, bool LAYERED>
 __global__ __launch_bounds__(256) void trace_kernel(alb_scene_t sc, alb_rng_t rng, int64_t begin,
                                                     int64_t n, Sinks sk, unsigned long long* ctr) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -725,7 +728,7 @@ __global__ __launch_bounds__(256) void trace_kernel(alb_scene_t sc, alb_rng_t rn
   if (i < n) emit_photon(sc, FWD, rng, st, ph, c);
   while (ballot(ph.alive)) {
     Emit e{-1, 0, 0, 0, 0, 0, 0, 0.0f, 0.0f, 0u};
-    if (ph.alive) next_event<false>(sc, FWD, rng, st, ph, e, c);
+    if (ph.alive) next_event<false, LAYERED>(sc, FWD, rng, st, ph, e, c);
     const int64_t tag = (int64_t)(((uint64_t)(begin + i) << 16) | (uint64_t)(e.eno & 0xFFFFu));
     emit_record<FWD>(sk, e, tag, 0, lane);
   }
@@ -889,7 +892,7 @@ __global__ __launch_bounds__(256) void replay_batched_kernel(const alb_scene_t* 
     if (!ballot(ph.alive || !done)) break;
     if (ph.alive) {
       Emit e{-1, 0, 0, 0, 0, 0, 0, 0.0f, 0.0f, 0u, 0.0, 0.0, 0, 0};
-      next_event<true>(scenes[sid], FWD, lrng, st, ph, e, c);
+      next_event<true, false>(scenes[sid], FWD, lrng, st, ph, e, c);
       if (e.et >= 0) {
         const uint64_t key = ((uint64_t)g << 16) | (uint64_t)(e.eno & 0xFFFFu);
 #pragma unroll
@@ -961,12 +964,10 @@ extern "C" int alb_trace(const alb_scene_t* scene, const alb_rng_t* rng, int32_t
   const int block = 256;
   const int64_t grid = (n + block - 1) / block;
   hipStream_t s = (hipStream_t)stream;
-  if (pass == ALB_PASS_FORWARD)
-    hipLaunchKernelGGL(trace_kernel<true>, dim3((unsigned)grid), dim3(block), 0, s, *scene, *rng,
-                       photon_begin, n, sk, counters);
-  else
-    hipLaunchKernelGGL(trace_kernel<false>, dim3((unsigned)grid), dim3(block), 0, s, *scene, *rng,
-                       photon_begin, n, sk, counters);
+  const bool layered = scene->layers != nullptr;
+  auto kern = pass == ALB_PASS_FORWARD ? (layered ? trace_kernel<true, true> : trace_kernel<true, false>)
+                                       : (layered ? trace_kernel<false, true> : trace_kernel<false, false>);
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), 0, s, *scene, *rng, photon_begin, n, sk, counters);
   return check_launch("alb_trace");
 }
 
