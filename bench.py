@@ -14,6 +14,12 @@ FETCH_SIZE / WRITE_SIZE runs scaled by a copy kernel's reading), and a CPU
 baseline: the time-stepped C oracle on a bounded sample of the same scene, on
 this host's cores.
 
+Radius mode (cfg of the 1e8 shot): the fixed-radius gather of a 256 x 256
+footprint grid on the seafloor plane (1 cm pitch, r = 0.05 m) over the BOTTOM
+and SCATTER maps of one more 1e8-photon forward pass; per map the gather
+time (HIP events), records taken and examined per query, and algorithmic
+GB/s.
+
 The two other forms of the metric, on bounded samples:
   * batched: the first --sweep-scenes scenes of the Latin-hypercube sweep
     (seed 1) at 1/100 of their photon counts, once as one batched run and
@@ -24,7 +30,7 @@ The two other forms of the metric, on bounded samples:
     forward passes).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--forward N]
-       [--backward N] [--no-cpu] [--no-batched] [--no-adjoint]
+       [--backward N] [--no-cpu] [--no-batched] [--no-adjoint] [--no-radius]
 """
 from __future__ import annotations
 
@@ -104,6 +110,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-radius", action="store_true")
+    ap.add_argument("--radius-grid", type=int, default=256)
     ap.add_argument("--sweep-scenes", type=int, default=256)
     ap.add_argument("--adjoint-scenes", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -219,6 +227,8 @@ def main(argv=None):
                                          f"{args.cpu_backward} backward photons, brute-force k-NN, "
                                          f"{cwall:.1f} s wall"}
     del res
+    if not args.no_radius:
+        out["radius_mode"] = bench_radius(model, ds, key, n_f, args.radius_grid)
     if not args.no_batched:
         out["batched"] = bench_batched(args.sweep_scenes)
     if not args.no_adjoint:
@@ -231,6 +241,54 @@ def main(argv=None):
         out["copy_calibration_bytes"] = {"read": x.numel() * 8, "write": y.numel() * 8}
     print(json.dumps(out), flush=True)
     return 0
+
+
+def bench_radius(model, ds, key, n_f: int, grid: int, radius: float = 0.05, pitch: float = 0.01,
+                 repeats: int = 3):
+    """Fixed-radius gather of a footprint grid over the BOTTOM and SCATTER maps."""
+    import torch
+
+    from pyalbsim_amd import abi, runtime
+    from pyalbsim_amd.footprint import footprint_queries, upload_queries
+
+    recs, rc = runtime.trace(ds, key, abi.PASS_FORWARD, n_f)
+    types = (abi.BOTTOM_REFLECTION, abi.SCATTER)
+    maps = {t: runtime.build_map(recs[t], rc[t], runtime.MAP_DIMS[t]) for t in types}
+    del recs
+    sink = upload_queries(footprint_queries(model, grid, pitch))
+    nq = grid * grid
+    out = {"workload": f"{grid}x{grid} footprint grid on the seafloor plane around the beam's floor point, "
+                       f"{pitch * 100:g} cm pitch, r = {radius} m, over the maps of a {n_f:.3g}-photon forward pass",
+           "queries": nq, "radius_m": radius, "types": {}}
+    for t in types:
+        wf = torch.zeros(model.n_bins, dtype=torch.float64, device="cuda")
+        runtime.radius_gather(ds, maps[t], t, sink, nq, radius, wf, model.gate())  # warm-up
+        ctr = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device="cuda")
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(repeats):
+            wf.zero_()
+            runtime.radius_gather(ds, maps[t], t, sink, nq, radius, wf, model.gate(), counters=ctr, check=False)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / repeats
+        c = ctr.cpu().numpy().view("uint64").astype(float) / repeats
+        if c[abi.CTR_OVERFLOW]:
+            raise runtime.AlbError("radius gather stack overflow")
+        taken, cand, nodes = c[abi.CTR_NEIGHBOURS], c[abi.CTR_CANDIDATES], c[abi.CTR_NODES]
+        # bytes: coordinates of examined records (12 B), payload of taken ones
+        # (dir 12 B for SCATTER, energy 4 B, time 4 B), node lines (64 B)
+        pay = 20 if t == abi.SCATTER else 8
+        gbytes = 12 * cand + pay * taken + 64 * nodes
+        out["types"][abi.TYPE_NAMES[t]] = {
+            "map_records": int(rc[t]), "ms": ms, "queries_per_s": nq / (ms / 1e3),
+            "taken_per_query": taken / nq, "examined_per_query": cand / nq, "nodes_per_query": nodes / nq,
+            "records_per_s": taken / (ms / 1e3), "algorithmic_gbs": gbytes / (ms / 1e3) / 1e9,
+            "waveform_sum": float(wf.sum().item())}
+    del maps
+    print("radius mode done", file=sys.stderr, flush=True)
+    return out
 
 
 def bench_batched(n_scenes: int):

@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define ALB_ABI_VERSION 5
+#define ALB_ABI_VERSION 6
 
 /* photon types, reference enum order (BOTTOM_REFLECTION = auto() = 1, ...) */
 enum alb_photon_type {
@@ -87,7 +87,8 @@ enum alb_counter {
   ALB_CTR_SMALL_SCENE = 16,  /* batched queries whose scene map holds fewer
                                 than k records (rows left at -1 / NaN)      */
   ALB_CTR_REPLAY_MISSES = 17,/* replayed events whose record was not found   */
-  ALB_N_COUNTERS = 18
+  ALB_CTR_NEIGHBOURS = 18,   /* records taken by the fixed-radius gather     */
+  ALB_N_COUNTERS = 19
 };
 
 /* One turbidity layer of a layered or depth-profiled water column
@@ -347,6 +348,24 @@ int alb_contrib(const alb_scene_t* scene, const alb_map_t* map, int32_t photon_t
                 int32_t rows_by_query, const int32_t* nbr, const double* knorm, double* waveform,
                 int64_t n_bins, int32_t win0, int32_t win_len, double* replicas,
                 int32_t n_replicas, void* stream);
+
+/* Fixed-radius gather: for queries q_begin + [0, q_n) (in the slot order of
+ * order[], e.g. from alb_gather_order, or in index order when order is NULL),
+ * every record of the built map within distance radius (FP64 squared distance
+ * <= radius^2, the brute-force set exactly) adds
+ * E_q E_p mult(type) / (pi r^2)  [1 / ((4/3) pi r^3) for ALB_SCATTER]
+ * into waveform[(int)((t_q + t_p) / sample_multiplier)], with the multipliers
+ * of alb_gather.  The reference's gather is the k-NN form
+ * (src/alb_sim/core/simulation.py:132-195); this is its fixed-radius variant
+ * for footprint-grid queries.  count_out / index_sum_out (optional, [q_n],
+ * indexed by query offset): records taken and the sum of their map indices.
+ * counters[ALB_CTR_OVERFLOW] > 0 means a query's traversal stack overflowed
+ * and its sum is incomplete.  Needs no workspace. */
+int alb_radius_gather(const alb_scene_t* scene, const alb_map_t* map, int32_t photon_type,
+                      const alb_queries_t* q, int64_t q_begin, int64_t q_n, const int32_t* order,
+                      double radius, double* waveform, int64_t n_bins, int32_t win0, int32_t win_len,
+                      int64_t* count_out, int64_t* index_sum_out, unsigned long long* counters,
+                      void* stream);
 
 /* Batched gather over a map built by alb_map_build_batched.  Each query
  * (q->scene required) is answered from its own scene's records only, so scene

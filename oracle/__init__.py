@@ -55,6 +55,9 @@ def lib():
             L.orc_gather.argtypes = [P, C.c_int32, P, P, P, P, P, P, P, P, C.c_int64, P,
                                      C.c_int64, P, C.c_int64, P, P]
             L.orc_gather.restype = C.c_int
+            L.orc_radius_gather.argtypes = [P, C.c_int32, P, P, P, P, P, P, P, P, C.c_int64, P,
+                                            C.c_int64, C.c_double, P, C.c_int64, P, P]
+            L.orc_radius_gather.restype = C.c_int
             L.orc_rotation_from_z.argtypes = [P, P]
             L.orc_interp.argtypes = [C.c_double, P, P, C.c_int]
             L.orc_interp.restype = C.c_double
@@ -338,6 +341,27 @@ def gather(model, ptype: int, mp: dict, qs: dict, n_bins: int | None = None,
     if want_nbrs:
         return wf, nbr, rad
     return wf
+
+
+def radius_gather(model, ptype: int, mp: dict, qs: dict, radius: float, n_bins: int | None = None):
+    """Fixed-radius gather, brute force: (FP64 waveform, per-query record
+    counts, per-query sums of record indices)."""
+    hs = HostScene(model)
+    n_bins = model.n_bins if n_bins is None else n_bins
+    m = {f: np.ascontiguousarray(mp[f], dtype=np.float32) for f in _FIELDS_F32}
+    nq = len(qs["x"])
+    qa = {f: np.ascontiguousarray(qs[f], dtype=np.float64 if f not in ("energy", "time") else np.float32)
+          for f in _FIELDS_F32}
+    qst = abi.Queries()
+    for f in _FIELDS_F32:
+        setattr(qst, f, _ptr(qa[f]))
+    qst.capacity = nq
+    wf = np.zeros(n_bins, dtype=np.float64)
+    cnt = np.zeros(nq, dtype=np.int64)
+    isum = np.zeros(nq, dtype=np.int64)
+    lib().orc_radius_gather(C.byref(hs.s), ptype, *[_ptr(m[f]) for f in _FIELDS_F32], len(m["x"]),
+                            C.byref(qst), nq, float(radius), _ptr(wf), n_bins, _ptr(cnt), _ptr(isum))
+    return wf, cnt, isum
 
 
 def run(model, n_forward: int, n_backward: int, key, replay_fwd=None, replay_bwd=None):

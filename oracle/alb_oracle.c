@@ -877,6 +877,43 @@ double orc_schlick_x(double c, double F0) { return orc_schlick(c, F0); }
  * (core/simulation.py:132-195), brute-force k-NN, FP64 accumulation in query
  * order then neighbour order.  nbr_out (optional) [nq*k]; radius_out
  * (optional) [nq].  Returns 0, or -1 (k > n or empty map). */
+/* scatter_energy's phase table: that of the layer at the query vertex depth
+ * (physics/models/water.py:418-447), else the scene's */
+typedef struct {
+  const double* theta;
+  const double* cdf;
+  int n;
+} orc_phase_tab;
+
+static orc_phase_tab orc_gather_phase(const alb_scene_t* sc, double qy) {
+  orc_phase_tab t = {sc->phase_theta, sc->phase_cdf, sc->phase_n};
+  if (sc->n_layers > 0 && sc->layers) {
+    const alb_layer_t* L = &sc->layers[orc_layer_index(sc, (qy + (-sc->y_ws)) * -1.0)];
+    t.theta = L->phase_theta;
+    t.cdf = L->phase_cdf;
+    t.n = L->phase_n;
+  }
+  return t;
+}
+
+/* one (query, record) contribution of sample_photons (core/simulation.py:155-195) */
+static void orc_contribute(const alb_scene_t* sc, int32_t type, orc_phase_tab tab, double m_bottom,
+                           const double ds[3], float q_energy, float q_time, int32_t i, const float* dx,
+                           const float* dy, const float* dz, const float* energy, const float* time,
+                           double knorm, double* waveform, int64_t n_bins) {
+  float dp[3] = {dx[i], dy[i], dz[i]};
+  double m;
+  if (type == ALB_BOTTOM_REFLECTION) m = m_bottom;
+  else if (type == ALB_SCATTER) m = orc_scatter_mult(dp, ds, tab.theta, tab.cdf, tab.n);
+  else if (type == ALB_SURFACE_REFLECTION) m = (double)orc_brdf_reflect(dp, ds, sc->alpha_ggx, sc->F0);
+  else m = orc_btdf_transmit(dp, ds, sc->alpha_ggx, sc->n_water, sc->F0, sc->surface_albedo);
+  float ee = q_energy * energy[i];
+  double e = (double)ee * knorm * m;
+  float tt = q_time + time[i];
+  int64_t bin = (int64_t)(tt / (float)sc->mult);
+  if (bin >= 0 && bin < n_bins) waveform[bin] += e;
+}
+
 int orc_gather(const alb_scene_t* sc, int32_t type, const float* x, const float* y,
                const float* z, const float* dx, const float* dy, const float* dz,
                const float* energy, const float* time, int64_t n, const alb_queries_t* q,
@@ -899,33 +936,45 @@ int orc_gather(const alb_scene_t* sc, int32_t type, const float* x, const float*
     double m_bottom = (sc->floor_albedo / ORC_PI) * fmax(0.0, -ds[1]);
     if (nbr_out) memcpy(nbr_out + j * k, idx, sizeof(int32_t) * (size_t)k);
     if (radius_out) radius_out[j] = r;
-    /* scatter_energy: phase function of the layer at the query vertex depth
-     * (physics/models/water.py:418-447) */
-    const double* th = sc->phase_theta;
-    const double* cdf = sc->phase_cdf;
-    int pn = sc->phase_n;
-    if (sc->n_layers > 0 && sc->layers) {
-      const alb_layer_t* L = &sc->layers[orc_layer_index(sc, (qy + (-sc->y_ws)) * -1.0)];
-      th = L->phase_theta;
-      cdf = L->phase_cdf;
-      pn = L->phase_n;
-    }
-    for (int32_t t = 0; t < k; ++t) {
-      int32_t i = idx[t];
-      float dp[3] = {dx[i], dy[i], dz[i]};
-      double m;
-      if (type == ALB_BOTTOM_REFLECTION) m = m_bottom;
-      else if (type == ALB_SCATTER) m = orc_scatter_mult(dp, ds, th, cdf, pn);
-      else if (type == ALB_SURFACE_REFLECTION) m = (double)orc_brdf_reflect(dp, ds, sc->alpha_ggx, sc->F0);
-      else m = orc_btdf_transmit(dp, ds, sc->alpha_ggx, sc->n_water, sc->F0, sc->surface_albedo);
-      float ee = q->energy[j] * energy[i];
-      double e = (double)ee * knorm * m;
-      float tt = q->time[j] + time[i];
-      int64_t bin = (int64_t)(tt / (float)sc->mult);
-      if (bin >= 0 && bin < n_bins) waveform[bin] += e;
-    }
+    const orc_phase_tab tab = orc_gather_phase(sc, qy);
+    for (int32_t t = 0; t < k; ++t)
+      orc_contribute(sc, type, tab, m_bottom, ds, q->energy[j], q->time[j], idx[t], dx, dy, dz, energy, time,
+                     knorm, waveform, n_bins);
   }
   free(idx);
   free(d2);
+  return 0;
+}
+
+/* Fixed-radius gather: every record within distance r of a query (squared
+ * distance as orc_d2, d2 <= r*r), brute force over the map in index order;
+ * kernel 1/(pi r^2), or 1/((4/3) pi r^3) for the scatter map, from the given r;
+ * multiplier, energy product and time bin as orc_gather
+ * (core/simulation.py:155-195).  count_out / isum_out (optional): per query
+ * the number of records taken and the sum of their indices. */
+int orc_radius_gather(const alb_scene_t* sc, int32_t type, const float* x, const float* y,
+                      const float* z, const float* dx, const float* dy, const float* dz,
+                      const float* energy, const float* time, int64_t n, const alb_queries_t* q,
+                      int64_t nq, double r, double* waveform, int64_t n_bins, int64_t* count_out,
+                      int64_t* isum_out) {
+  const double r2 = r * r;
+  const double ksize = (type == ALB_SCATTER) ? (4.0 / 3.0) * ORC_PI * pow(r, 3.0) : ORC_PI * (r * r);
+  const double knorm = 1.0 / ksize;
+  for (int64_t j = 0; j < nq; ++j) {
+    const double qx = q->x[j], qy = q->y[j], qz = q->z[j];
+    const double ds[3] = {q->dx[j], q->dy[j], q->dz[j]};
+    const double m_bottom = (sc->floor_albedo / ORC_PI) * fmax(0.0, -ds[1]);
+    const orc_phase_tab tab = orc_gather_phase(sc, qy);
+    int64_t cnt = 0, isum = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      if (orc_d2(qx, qy, qz, x[i], y[i], z[i]) > r2) continue;
+      cnt++;
+      isum += i;
+      orc_contribute(sc, type, tab, m_bottom, ds, q->energy[j], q->time[j], (int32_t)i, dx, dy, dz, energy,
+                     time, knorm, waveform, n_bins);
+    }
+    if (count_out) count_out[j] = cnt;
+    if (isum_out) isum_out[j] = isum;
+  }
   return 0;
 }

@@ -5,9 +5,9 @@ import ctypes as C
 
 import numpy as np
 
-ALB_ABI_VERSION = 5
+ALB_ABI_VERSION = 6
 N_TYPES = 4
-N_COUNTERS = 18
+N_COUNTERS = 19
 
 ALB_OK = 0
 ALB_ERR_ARG = -1
@@ -17,6 +17,7 @@ ALB_ERR_UNSUPPORTED = -4
 ALB_ERR_WORKSPACE = -5
 
 BOTTOM_REFLECTION, SCATTER, SURFACE_REFLECTION, SURFACE_TRANSMISSION_UP = range(4)
+TYPE_NAMES = ("BOTTOM_REFLECTION", "SCATTER", "SURFACE_REFLECTION", "SURFACE_TRANSMISSION_UP")
 
 RNG_PHILOX = 0
 RNG_REPLAY = 1
@@ -25,10 +26,10 @@ PASS_BACKWARD = 1
 
 CTR_PATHS, CTR_AIR_LEGS, CTR_WATER_STEPS, CTR_SCATTER, CTR_ENTER, CTR_EXIT, CTR_FLOOR, \
     CTR_REPLAY_EXHAUSTED, CTR_QUERIES, CTR_CANDIDATES, CTR_NODES, CTR_OVERFLOW, CTR_DEPTH_SOLVES, \
-    CTR_KNN_STEPS, CTR_KNN_SORTS, CTR_KNN_DESCENT, CTR_SMALL_SCENE, CTR_REPLAY_MISSES = range(18)
+    CTR_KNN_STEPS, CTR_KNN_SORTS, CTR_KNN_DESCENT, CTR_SMALL_SCENE, CTR_REPLAY_MISSES, CTR_NEIGHBOURS = range(19)
 COUNTER_NAMES = ["paths", "air_legs", "water_steps", "scatter", "enter", "exit", "floor",
                  "replay_exhausted", "queries", "candidates", "nodes", "overflow", "depth_solves",
-                 "knn_steps", "knn_sorts", "knn_descent", "small_scene", "replay_misses"]
+                 "knn_steps", "knn_sorts", "knn_descent", "small_scene", "replay_misses", "neighbours"]
 
 
 class Scene(C.Structure):

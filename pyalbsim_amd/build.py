@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libalbsim_hip.so")
-SOURCES = ["alb_common.hip", "alb_trace.hip", "alb_map.hip", "alb_gather.hip", "alb_adjoint.hip"]
+SOURCES = ["alb_common.hip", "alb_trace.hip", "alb_map.hip", "alb_gather.hip", "alb_adjoint.hip", "alb_radius.hip"]
 ARCH = os.environ.get("ALB_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

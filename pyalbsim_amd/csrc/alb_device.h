@@ -454,6 +454,40 @@ struct __attribute__((aligned(16))) Node {
 };
 static_assert(sizeof(Node) == 64, "tree node must be one 64-byte line");
 
+// squared distance in FP64 from a query to a float32 record, terms summed x,
+// y, z in this order (the order the oracle and the neighbour-set proofs use)
+__device__ __forceinline__ double dist2(double qx, double qy, double qz, float x, float y, float z) {
+  const double dx = (double)x - qx, dy = (double)y - qy, dz = (double)z - qz;
+  double s = dx * dx + dy * dy;
+  return s + dz * dz;
+}
+
+// distance from q to [lo, hi] along one axis; the same values as the branchy
+// form (lo - q below, q - hi above, 0 inside) without divergent branches
+__device__ __forceinline__ double axis_gap(double q, float lo, float hi) {
+  return fmax(fmax((double)lo - q, q - (double)hi), 0.0);
+}
+
+// squared distance from q to a float32 box (min x,y,z, max x,y,z); never larger
+// than dist2 of a record inside the box (monotone rounding)
+__device__ __forceinline__ double box_dist2(double qx, double qy, double qz, const float* __restrict__ b) {
+  const double gx = axis_gap(qx, b[0], b[3]);
+  const double gy = axis_gap(qy, b[1], b[4]);
+  const double gz = axis_gap(qz, b[2], b[5]);
+  double s = gx * gx + gy * gy;
+  return s + gz * gz;
+}
+
+// squared distance from q to the farthest corner of a box; never smaller than
+// dist2 of a record inside the box (monotone rounding)
+__device__ __forceinline__ double box_far2(double qx, double qy, double qz, const float* __restrict__ b) {
+  const double gx = fmax(fabs((double)b[0] - qx), fabs((double)b[3] - qx));
+  const double gy = fmax(fabs((double)b[1] - qy), fabs((double)b[4] - qy));
+  const double gz = fmax(fabs((double)b[2] - qz), fabs((double)b[5] - qz));
+  double s = gx * gx + gy * gy;
+  return s + gz * gz;
+}
+
 // ----------------------------------------------------------- Morton keys
 __device__ __forceinline__ uint64_t spread2(uint64_t v) {  // 32 bits -> even bit positions
   v &= 0xFFFFFFFFull;

@@ -116,26 +116,6 @@ struct ContribArgs {
   int32_t n_rep;
 };
 
-__device__ __forceinline__ double dist2(double qx, double qy, double qz, float x, float y, float z) {
-  const double dx = (double)x - qx, dy = (double)y - qy, dz = (double)z - qz;
-  double s = dx * dx + dy * dy;
-  return s + dz * dz;
-}
-
-// distance from q to [lo, hi] along one axis; the same values as the branchy
-// form (lo - q below, q - hi above, 0 inside) without divergent branches
-__device__ __forceinline__ double axis_gap(double q, float lo, float hi) {
-  return fmax(fmax((double)lo - q, q - (double)hi), 0.0);
-}
-
-__device__ __forceinline__ double box_dist2(double qx, double qy, double qz, const float* __restrict__ b) {
-  const double gx = axis_gap(qx, b[0], b[3]);
-  const double gy = axis_gap(qy, b[1], b[4]);
-  const double gz = axis_gap(qz, b[2], b[5]);
-  double s = gx * gx + gy * gy;
-  return s + gz * gz;
-}
-
 __device__ __forceinline__ bool before(double da, int32_t ia, double db, int32_t ib) {
   return da < db || (da == db && ia < ib);
 }

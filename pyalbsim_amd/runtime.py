@@ -23,7 +23,7 @@ EXPORTS = ["alb_abi_version", "alb_scene_size", "alb_last_error", "alb_trace", "
            "alb_map_build", "alb_gather_workspace_size", "alb_gather", "alb_gather_order", "alb_knn",
            "alb_contrib", "alb_trace_batched", "alb_map_build_batched", "alb_gather_order_batched",
            "alb_knn_batched", "alb_contrib_batched", "alb_adjoint_contrib", "alb_replay_index_workspace_size",
-           "alb_replay_index_build", "alb_replay_batched"]
+           "alb_replay_index_build", "alb_replay_batched", "alb_radius_gather"]
 
 
 class AlbError(RuntimeError):
@@ -54,6 +54,9 @@ def load_library(path: str = LIB_PATH):
                              P, P, P, C.c_size_t, P, P]
     L.alb_gather.restype = C.c_int
     L.alb_gather_order.argtypes = [P, P, C.c_int64, C.c_int64, P, P, C.c_size_t, P]
+    L.alb_radius_gather.argtypes = [P, P, C.c_int32, P, C.c_int64, C.c_int64, P, C.c_double, P, C.c_int64,
+                                    C.c_int32, C.c_int32, P, P, P, P]
+    L.alb_radius_gather.restype = C.c_int
     L.alb_gather_order.restype = C.c_int
     L.alb_knn.argtypes = [P, C.c_int32, C.c_int32, P, C.c_int64, P, C.c_int64, C.c_int64, C.c_int32, P, P, P, P, P]
     L.alb_knn.restype = C.c_int
@@ -342,6 +345,44 @@ def gather(dscene: DeviceScene, dmap: DeviceMap, ptype: int, queries: Sink, q_n:
 
 
 MAP_DIMS = (2, 3, 2, 2)  # BOTTOM and the two surface maps are planar; SCATTER is volumetric
+
+
+def radius_gather(dscene: DeviceScene, dmap: DeviceMap, ptype: int, queries: Sink, q_n: int, radius: float,
+                  waveform, window, q_begin: int = 0, sort: bool = True, counts: bool = False, counters=None,
+                  stream=None, check: bool = True):
+    """Accumulate the fixed-radius gather of q_n queries of one type into
+    waveform (float64 device tensor): every map record within radius of a
+    query contributes with the kernel 1/(pi r^2) (1/((4/3) pi r^3) for the
+    scatter map).  sort: visit the queries in Morton order (alb_gather_order).
+    counts: also return per-query record counts and index sums (int64 device
+    tensors).  check: synchronise and raise AlbError when a traversal
+    overflowed (its sum would be incomplete).  Returns (count, index_sum,
+    keep), keep holding the buffers the launch uses."""
+    torch = _torch()
+    L = load_library()
+    win0, win_len = window
+    s = stream if stream is not None else _stream_ptr()
+    dev = waveform.device
+    order = ws = None
+    if sort and q_n > 0:
+        b = C.c_size_t(0)
+        _check(L.alb_gather_workspace_size(q_n, 1, C.byref(b)), "alb_gather_workspace_size")
+        ws = torch.empty(b.value, dtype=torch.uint8, device=dev)
+        order = torch.empty(q_n, dtype=torch.int32, device=dev)
+        _check(L.alb_gather_order(C.byref(dmap.struct), C.byref(queries.struct), q_begin, q_n, _p(order), _p(ws),
+                                  b.value, C.c_void_p(s)), "alb_gather_order")
+    cnt = torch.zeros(max(q_n, 1), dtype=torch.int64, device=dev) if counts else None
+    isum = torch.zeros(max(q_n, 1), dtype=torch.int64, device=dev) if counts else None
+    ctr = counters if counters is not None else (torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device=dev)
+                                                 if check else None)
+    ovf0 = int(ctr[abi.CTR_OVERFLOW].item()) if (check and counters is not None) else 0
+    _check(L.alb_radius_gather(C.byref(dscene.s), C.byref(dmap.struct), ptype, C.byref(queries.struct), q_begin,
+                               q_n, _p(order), float(radius), _p(waveform), int(waveform.numel()), int(win0),
+                               int(win_len), _p(cnt), _p(isum), _p(ctr), C.c_void_p(s)), "alb_radius_gather")
+    if check and q_n > 0:
+        if int(ctr[abi.CTR_OVERFLOW].item()) != ovf0:
+            raise AlbError("alb_radius_gather: a query's traversal stack overflowed")
+    return cnt, isum, (order, ws, ctr)
 
 
 # ----------------------------------------------------------------- many scenes

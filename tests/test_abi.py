@@ -53,6 +53,8 @@ def test_last_error_is_set_by_argument_checks():
     rc = lib.alb_contrib_batched(None, 1, 100, None, 1, None, None, 0, 1, None, None, None, None, None, 0, None, 0, 0,
                                  None)
     assert rc == abi.ALB_ERR_ARG
+    rc = lib.alb_radius_gather(None, None, 0, None, 0, 1, None, 0.05, None, 10, 0, 0, None, None, None, None)
+    assert rc == abi.ALB_ERR_ARG and b"null" in lib.alb_last_error()
 
 
 def test_product_path_fails_loudly_without_gpu():
