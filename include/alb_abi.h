@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define ALB_ABI_VERSION 6
+#define ALB_ABI_VERSION 7
 
 /* photon types, reference enum order (BOTTOM_REFLECTION = auto() = 1, ...) */
 enum alb_photon_type {
@@ -165,6 +165,23 @@ typedef struct alb_scene {
   int32_t n_layers;
   int32_t pad1;
   const alb_layer_t* layers;     /* [n_layers] (device pointer for the HIP library)  */
+  /* sampled heat maps (src/alb_sim/math/accumulate_heatmap.py:6-124, hooked at
+   * core/simulation.py:197-214, 523-537); heat_bins = 0: none.  alb_trace
+   * (forward) counts every seafloor hit into heat_radius[bins x bins] at
+   * (surface radius / water extent, floor radius / floor extent) * bins;
+   * alb_gather / alb_contrib add each contribution into heat_water and
+   * heat_floor [bins x bins] at the record's first water / seafloor
+   * interaction, (x - centre + extent) / (2 extent / bins) along x and z.
+   * One-scene entry points only. */
+  int32_t heat_bins;
+  int32_t pad2;
+  double heat_water_extent;      /* half-width around the surface centre (m)        */
+  double heat_floor_extent;      /* half-width around the floor centre (m)          */
+  double heat_water_center[3];   /* laser / water-surface intersection              */
+  double heat_floor_center[3];   /* refracted beam / seafloor intersection          */
+  unsigned int* heat_radius;     /* [bins * bins] counts                            */
+  double* heat_water;            /* [bins * bins] energy                            */
+  double* heat_floor;            /* [bins * bins] energy                            */
 } alb_scene_t;
 
 typedef struct alb_rng {
@@ -191,6 +208,16 @@ typedef struct alb_records {
   int64_t capacity;
   unsigned long long* count;/* device counter                                     */
   int32_t* scene;           /* optional: scene of each record (batched traces)     */
+  /* optional (heat maps, alb_trace only): the photon's first water
+   * interaction (surface entry point) and latest seafloor interaction at the
+   * event, float32, NaN before the interaction
+   * (core/simulation.py:85-86, 393, 521) */
+  float* fwi_x;
+  float* fwi_y;
+  float* fwi_z;
+  float* sfi_x;
+  float* sfi_y;
+  float* sfi_z;
 } alb_records_t;
 
 /* Backward-pass query sink (sensor-path vertices), one per photon type. */
@@ -207,6 +234,12 @@ typedef struct alb_queries {
   int64_t capacity;
   unsigned long long* count;
   int32_t* scene;           /* optional: scene of each query (batched traces)      */
+  float* fwi_x;             /* unused for queries; same layout as alb_records_t    */
+  float* fwi_y;
+  float* fwi_z;
+  float* sfi_x;
+  float* sfi_y;
+  float* sfi_z;
 } alb_queries_t;
 
 /* A built photon map: records sorted along a Morton curve plus a binary radix
@@ -236,6 +269,14 @@ typedef struct alb_map {
   uint64_t* keys;           /* [n] sorted Morton keys                              */
   double* meta;             /* [8] grid origin x,y,z, inverse cell width x,y,z     */
   void* nodes;              /* [n-1] 64-byte internal nodes (layout above)         */
+  /* optional sorted interaction points (heat maps): permuted from the records'
+   * arrays by alb_map_build when both are set */
+  float* fwi_x;
+  float* fwi_y;
+  float* fwi_z;
+  float* sfi_x;
+  float* sfi_y;
+  float* sfi_z;
 } alb_map_t;
 
 /* Scene index of a map (or query set) holding records of many scenes: after

@@ -71,6 +71,7 @@ def lib():
             L.orc_scatter_mult.restype = C.c_double
             L.orc_emission_dir.argtypes = [P, P, C.c_int32, C.c_uint64, P]
             L.orc_layer_lookup.argtypes = [P, C.c_double, P]
+            L.orc_set_map_points.argtypes = [P, P, P, P]
             L.orc_lambert.argtypes = [C.c_double, C.c_double, P]
             L.orc_scatter_direction.argtypes = [P, P, C.c_double, C.c_double, P]
             for name in ("orc_ggx_d_x", "orc_g1_x", "orc_schlick_x"):
@@ -180,7 +181,10 @@ def scatter_direction(model, din, u1, u2):
 class HostScene:
     """alb_scene_t with host phase tables, kept alive with the struct."""
 
-    def __init__(self, model, phase_score: np.ndarray | None = None):
+    def __init__(self, model, phase_score: np.ndarray | None = None, heat: dict | None = None):
+        """heat: optional {"bins", "water_extent", "seafloor_extent"}: host heat
+        maps ("radius" uint32, "water" and "floor" FP64, [bins * bins]) in
+        self.heat, centred as the HIP path centres them."""
         self.model = model
         self.theta = np.ascontiguousarray(model.phase_theta, dtype=np.float64)
         self.cdf = np.ascontiguousarray(model.phase_cdf, dtype=np.float64)
@@ -196,6 +200,23 @@ class HostScene:
             self.layer_structs = abi.make_layers(model, ptrs)
             self.s.n_layers = len(model.layers)
             self.s.layers = C.addressof(self.layer_structs)
+        self.heat = None
+        if heat is not None:
+            from pyalbsim_amd.footprint import seafloor_intersection, water_surface_intersection
+
+            b = int(heat["bins"])
+            self.heat = {"bins": b, "radius": np.zeros(b * b, dtype=np.uint32), "water": np.zeros(b * b),
+                         "floor": np.zeros(b * b)}
+            self.s.heat_bins = b
+            self.s.heat_water_extent = float(heat["water_extent"])
+            self.s.heat_floor_extent = float(heat["seafloor_extent"])
+            wc, fc = water_surface_intersection(model), seafloor_intersection(model)
+            for i in range(3):
+                self.s.heat_water_center[i] = float(wc[i])
+                self.s.heat_floor_center[i] = float(fc[i])
+            self.s.heat_radius = _ptr(self.heat["radius"])
+            self.s.heat_water = _ptr(self.heat["water"])
+            self.s.heat_floor = _ptr(self.heat["floor"])
 
 
 def layer_lookup(model, depths) -> np.ndarray:
@@ -213,7 +234,7 @@ def layer_lookup(model, depths) -> np.ndarray:
 _FIELDS_F32 = ("x", "y", "z", "dx", "dy", "dz", "energy", "time")
 
 
-def _alloc_sink(capacity: int, query: bool):
+def _alloc_sink(capacity: int, query: bool, points: bool = False):
     dt = np.float64 if query else np.float32
     arrs = {}
     for f in _FIELDS_F32:
@@ -229,6 +250,10 @@ def _alloc_sink(capacity: int, query: bool):
     st.tag = _ptr(arrs["tag"])
     st.capacity = capacity
     st.count = _ptr(arrs["count"])
+    if points and not query:
+        for f in abi.INTERACTION_FIELDS:
+            arrs[f] = np.zeros(capacity, dtype=np.float32)
+            setattr(st, f, _ptr(arrs[f]))
     return st, arrs
 
 
@@ -238,7 +263,7 @@ class _StatsSink(C.Structure):
 
 def trace(model, key, pass_: int, photon_begin: int, n: int, replay: np.ndarray | None = None,
           stream: int | None = None, per_photon: int = 12, stats: bool = False,
-          phase_score: np.ndarray | None = None):
+          phase_score: np.ndarray | None = None, points: bool = False, heat: dict | None = None):
     """Trace n photons on the CPU; returns (sinks, counters, n_exhausted).
 
     sinks[t] is a dict of NumPy arrays (records for the forward pass, queries
@@ -246,8 +271,11 @@ def trace(model, key, pass_: int, photon_begin: int, n: int, replay: np.ndarray 
     each record's adjoint path statistics ("lam" water length whose optical
     depth was accumulated, "gs" phase-sampling score sum with the per-segment
     scores phase_score, "nsc" scatters, "nfl" floor reflections before it).
+    points: forward records also carry the interaction points (fwi_*, sfi_*).
+    heat: optional {"bins", "water_extent", "seafloor_extent"}; receives the
+    forward scatter-radius counts under "radius" ([bins, bins]).
     """
-    hs = HostScene(model, phase_score)
+    hs = HostScene(model, phase_score, heat)
     rep = None
     if replay is not None:
         rep = np.ascontiguousarray(replay, dtype=np.float64)
@@ -258,7 +286,9 @@ def trace(model, key, pass_: int, photon_begin: int, n: int, replay: np.ndarray 
     cap = max(16, per_photon * n)
     while True:
         query = pass_ == abi.PASS_BACKWARD
-        sinks = [_alloc_sink(cap, query) for _ in range(abi.N_TYPES)]
+        sinks = [_alloc_sink(cap, query, points) for _ in range(abi.N_TYPES)]
+        if hs.heat is not None:
+            hs.heat["radius"][:] = 0
         rec_arr = (abi.Records * 4)()
         q_arr = (abi.Queries * 4)()
         for t in range(4):
@@ -291,9 +321,14 @@ def trace(model, key, pass_: int, photon_begin: int, n: int, replay: np.ndarray 
         arrs = sinks[t][1]
         c = counts[t]
         d = {f: arrs[f][:c].copy() for f in (*_FIELDS_F32, "tag")}
+        if points and not query:
+            d.update({f: arrs[f][:c].copy() for f in abi.INTERACTION_FIELDS})
         if stats:
             d.update({f: st_arrs[t][f][:c].copy() for f in ("lam", "gs", "nsc", "nfl")})
         out.append(d)
+    if heat is not None:
+        b = hs.heat["bins"]
+        heat["radius"] = hs.heat["radius"].reshape(b, b).copy()
     return out, ctr, int(ex)
 
 
@@ -316,9 +351,16 @@ def knn(x, y, z, queries: np.ndarray, k: int):
 
 
 def gather(model, ptype: int, mp: dict, qs: dict, n_bins: int | None = None,
-           want_nbrs: bool = False):
-    """FP64 waveform of one photon type from map arrays mp and queries qs."""
-    hs = HostScene(model)
+           want_nbrs: bool = False, heat: dict | None = None):
+    """FP64 waveform of one photon type from map arrays mp and queries qs.
+    heat: optional {"bins", "water_extent", "seafloor_extent"} (mp must hold
+    the interaction points); receives this gather's "water" and "floor" heat
+    maps ([bins, bins], FP64)."""
+    hs = HostScene(model, None, heat)
+    pts = None
+    if heat is not None:
+        pts = [np.ascontiguousarray(mp[f], dtype=np.float32) for f in ("fwi_x", "fwi_z", "sfi_x", "sfi_z")]
+        lib().orc_set_map_points(*[_ptr(a) for a in pts])
     n_bins = model.n_bins if n_bins is None else n_bins
     m = {f: np.ascontiguousarray(mp[f], dtype=np.float32) for f in _FIELDS_F32}
     nq = len(qs["x"])
@@ -331,13 +373,21 @@ def gather(model, ptype: int, mp: dict, qs: dict, n_bins: int | None = None,
     wf = np.zeros(n_bins, dtype=np.float64)
     nbr = np.zeros((nq, model.k), dtype=np.int32) if want_nbrs else None
     rad = np.zeros(nq, dtype=np.float64) if want_nbrs else None
-    rc = lib().orc_gather(C.byref(hs.s), ptype, *[_ptr(m[f]) for f in _FIELDS_F32], len(m["x"]),
-                          C.byref(qst), nq, _ptr(wf), n_bins,
-                          _ptr(nbr) if want_nbrs else None, _ptr(rad) if want_nbrs else None)
+    try:
+        rc = lib().orc_gather(C.byref(hs.s), ptype, *[_ptr(m[f]) for f in _FIELDS_F32], len(m["x"]),
+                              C.byref(qst), nq, _ptr(wf), n_bins,
+                              _ptr(nbr) if want_nbrs else None, _ptr(rad) if want_nbrs else None)
+    finally:
+        if heat is not None:
+            lib().orc_set_map_points(None, None, None, None)
     if rc != 0:
         if len(m["x"]) == 0:
             raise KeyError(ptype)
         raise IndexError("k exceeds the photon map size")
+    if heat is not None:
+        b = hs.heat["bins"]
+        heat["water"] = hs.heat["water"].reshape(b, b).copy()
+        heat["floor"] = hs.heat["floor"].reshape(b, b).copy()
     if want_nbrs:
         return wf, nbr, rad
     return wf

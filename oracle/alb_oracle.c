@@ -328,6 +328,11 @@ static void orc_emit_stats(int type, unsigned long long i, int64_t capacity) {
   g_stats[type].nfl[i] = g_path.nfl;
 }
 
+/* the traced photon's first water interaction and latest seafloor
+ * interaction, float32, NaN before them (core/simulation.py:85-86, 393, 521);
+ * stored with every forward record whose sink holds them */
+static float g_fwi[3], g_sfi[3];
+
 static void orc_emit_record(const alb_records_t* r, const double p[3], const double d[3],
                             float E, float t, int64_t tag) {
   unsigned long long i = (*r->count)++;
@@ -337,6 +342,26 @@ static void orc_emit_record(const alb_records_t* r, const double p[3], const dou
   r->energy[i] = E;
   r->time[i] = t;
   if (r->tag) r->tag[i] = tag;
+  if (r->fwi_x) {
+    r->fwi_x[i] = g_fwi[0]; r->fwi_y[i] = g_fwi[1]; r->fwi_z[i] = g_fwi[2];
+    r->sfi_x[i] = g_sfi[0]; r->sfi_y[i] = g_sfi[1]; r->sfi_z[i] = g_sfi[2];
+  }
+}
+
+/* accumulate_scatter_radius_heatmap for one forward seafloor hit
+ * (math/accumulate_heatmap.py:64-124): surface radius from the float32 first
+ * water interaction, floor radius from the FP64 hit point, each over its
+ * extent times bins, truncated; photons without water entry are skipped */
+static void orc_count_radius(const alb_scene_t* sc, const double P[3]) {
+  if (sc->heat_bins <= 0 || !sc->heat_radius || isnan(g_fwi[0])) return;
+  const double sx = (double)g_fwi[0] - sc->heat_water_center[0];
+  const double sz = (double)g_fwi[2] - sc->heat_water_center[2];
+  const double fx = P[0] - sc->heat_floor_center[0], fz = P[2] - sc->heat_floor_center[2];
+  const double rs = sqrt(sx * sx + sz * sz), rf = sqrt(fx * fx + fz * fz);
+  const int64_t bins = sc->heat_bins;
+  const int64_t ix = (int64_t)((rs / sc->heat_water_extent) * (double)bins);
+  const int64_t iy = (int64_t)((rf / sc->heat_floor_extent) * (double)bins);
+  if (ix >= 0 && ix < bins && iy >= 0 && iy < bins) sc->heat_radius[ix * bins + iy] += 1u;
 }
 
 static void orc_emit_query(const alb_queries_t* q, const double p[3], const double d[3],
@@ -468,6 +493,7 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
 
   float E = 1.0f, tau = 0.0f, tau_t = INFINITY;
   int64_t event_no = 0;
+  for (int i = 0; i < 3; ++i) g_fwi[i] = g_sfi[i] = NAN;
   g_path.lam = 0.0;
   g_path.gs = 0.0;
   g_path.nsc = 0;
@@ -554,6 +580,7 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
       double ci = -d[1];
       double Rf = orc_schlick(ci, sc->F0);
       double T = 1.0 - Rf;
+      for (int i = 0; i < 3; ++i) g_fwi[i] = (float)P[i];  /* core/simulation.py:393 */
       if (fwd) {
         orc_emit_record(&out[ALB_SURFACE_REFLECTION], P, d, E, t_evt, tag);
         orc_emit_stats(ALB_SURFACE_REFLECTION, *out[ALB_SURFACE_REFLECTION].count - 1,
@@ -624,6 +651,8 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
       if (f > 1.0) f = 1.0;
       double P[3];
       for (int i = 0; i < 3; ++i) P[i] = p[i] + f * sv[i];
+      for (int i = 0; i < 3; ++i) g_sfi[i] = (float)P[i];  /* core/simulation.py:521 */
+      if (fwd) orc_count_radius(sc, P);                   /* core/simulation.py:523-537 */
       if (fwd) orc_emit_record(&out[ALB_BOTTOM_REFLECTION], P, d, E, t_evt, tag);
       else orc_emit_query(&q[ALB_BOTTOM_REFLECTION], P, d, E, t_evt, tag);
       orc_emit_stats(ALB_BOTTOM_REFLECTION,
@@ -896,6 +925,35 @@ static orc_phase_tab orc_gather_phase(const alb_scene_t* sc, double qy) {
   return t;
 }
 
+/* interaction points of the map records being gathered (heat maps), set by
+ * orc_set_map_points; NULL: no heat maps */
+static struct {
+  const float* fwi_x;
+  const float* fwi_z;
+  const float* sfi_x;
+  const float* sfi_z;
+} g_map_pts;
+
+void orc_set_map_points(const float* fwi_x, const float* fwi_z, const float* sfi_x, const float* sfi_z) {
+  g_map_pts.fwi_x = fwi_x;
+  g_map_pts.fwi_z = fwi_z;
+  g_map_pts.sfi_x = sfi_x;
+  g_map_pts.sfi_z = sfi_z;
+}
+
+/* accumulate_to_heatmap (math/accumulate_heatmap.py:6-61) for one point:
+ * bins of (x - centre + extent) / (2 extent / bins) along x and z, truncated
+ * toward zero (astype(int64)); NaN points and bins outside [0, bins) skipped */
+static void orc_heat_add(double* h, float px, float pz, const double c[3], double extent, int bins, double e) {
+  if (isnan(px)) return;
+  const double bs = 2.0 * extent / (double)bins;
+  const double xc = (double)px - c[0], zc = (double)pz - c[2];
+  const double vx = (xc + extent) / bs, vz = (zc + extent) / bs;
+  if (!(vx > -1.0 && vx < (double)bins && vz > -1.0 && vz < (double)bins)) return;
+  const int64_t ix = (int64_t)vx, iz = (int64_t)vz;
+  h[ix * bins + iz] += e;
+}
+
 /* one (query, record) contribution of sample_photons (core/simulation.py:155-195) */
 static void orc_contribute(const alb_scene_t* sc, int32_t type, orc_phase_tab tab, double m_bottom,
                            const double ds[3], float q_energy, float q_time, int32_t i, const float* dx,
@@ -912,6 +970,12 @@ static void orc_contribute(const alb_scene_t* sc, int32_t type, orc_phase_tab ta
   float tt = q_time + time[i];
   int64_t bin = (int64_t)(tt / (float)sc->mult);
   if (bin >= 0 && bin < n_bins) waveform[bin] += e;
+  if (sc->heat_bins > 0 && sc->heat_water && g_map_pts.fwi_x) {
+    orc_heat_add(sc->heat_water, g_map_pts.fwi_x[i], g_map_pts.fwi_z[i], sc->heat_water_center,
+                 sc->heat_water_extent, sc->heat_bins, e);
+    orc_heat_add(sc->heat_floor, g_map_pts.sfi_x[i], g_map_pts.sfi_z[i], sc->heat_floor_center,
+                 sc->heat_floor_extent, sc->heat_bins, e);
+  }
 }
 
 int orc_gather(const alb_scene_t* sc, int32_t type, const float* x, const float* y,

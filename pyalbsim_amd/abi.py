@@ -5,7 +5,7 @@ import ctypes as C
 
 import numpy as np
 
-ALB_ABI_VERSION = 6
+ALB_ABI_VERSION = 7
 N_TYPES = 4
 N_COUNTERS = 19
 
@@ -17,6 +17,7 @@ ALB_ERR_UNSUPPORTED = -4
 ALB_ERR_WORKSPACE = -5
 
 BOTTOM_REFLECTION, SCATTER, SURFACE_REFLECTION, SURFACE_TRANSMISSION_UP = range(4)
+INTERACTION_FIELDS = ("fwi_x", "fwi_y", "fwi_z", "sfi_x", "sfi_y", "sfi_z")
 TYPE_NAMES = ("BOTTOM_REFLECTION", "SCATTER", "SURFACE_REFLECTION", "SURFACE_TRANSMISSION_UP")
 
 RNG_PHILOX = 0
@@ -45,6 +46,10 @@ class Scene(C.Structure):
         ("phase_n", C.c_int32), ("phase_theta", C.c_void_p), ("phase_cdf", C.c_void_p),
         ("phase_mass", C.c_void_p), ("phase_score", C.c_void_p), ("phase_dlog_mass", C.c_void_p),
         ("n_layers", C.c_int32), ("pad1", C.c_int32), ("layers", C.c_void_p),
+        ("heat_bins", C.c_int32), ("pad2", C.c_int32), ("heat_water_extent", C.c_double),
+        ("heat_floor_extent", C.c_double), ("heat_water_center", C.c_double * 3),
+        ("heat_floor_center", C.c_double * 3), ("heat_radius", C.c_void_p), ("heat_water", C.c_void_p),
+        ("heat_floor", C.c_void_p),
     ]
 
 
@@ -97,7 +102,8 @@ class Rng(C.Structure):
 class Records(C.Structure):
     _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("dx", C.c_void_p),
                 ("dy", C.c_void_p), ("dz", C.c_void_p), ("energy", C.c_void_p), ("time", C.c_void_p),
-                ("tag", C.c_void_p), ("capacity", C.c_int64), ("count", C.c_void_p), ("scene", C.c_void_p)]
+                ("tag", C.c_void_p), ("capacity", C.c_int64), ("count", C.c_void_p), ("scene", C.c_void_p)] + \
+        [(f, C.c_void_p) for f in ("fwi_x", "fwi_y", "fwi_z", "sfi_x", "sfi_y", "sfi_z")]
 
 
 class Queries(C.Structure):
@@ -109,7 +115,8 @@ class Map(C.Structure):
                 ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("dx", C.c_void_p),
                 ("dy", C.c_void_p), ("dz", C.c_void_p), ("energy", C.c_void_p), ("time", C.c_void_p),
                 ("perm", C.c_void_p), ("keys", C.c_void_p), ("meta", C.c_void_p),
-                ("nodes", C.c_void_p)]
+                ("nodes", C.c_void_p)] + [(f, C.c_void_p) for f in ("fwi_x", "fwi_y", "fwi_z", "sfi_x", "sfi_y",
+                                                                     "sfi_z")]
 
 
 class SceneIndex(C.Structure):

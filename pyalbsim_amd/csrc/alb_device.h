@@ -445,6 +445,22 @@ __device__ __forceinline__ double scatter_mult(float px, float py, float pz, dou
   return interp_uniform_grid(hi, theta, cdf, n) - interp_uniform_grid(lo, theta, cdf, n);
 }
 
+// ------------------------------------------------------------- heat maps
+// accumulate_to_heatmap (src/alb_sim/math/accumulate_heatmap.py:6-61) for one
+// point: bins of (x - centre + extent) / (2 extent / bins) along x and z,
+// truncated toward zero as astype(int64) does; NaN points and bins outside
+// [0, bins) are skipped
+__device__ __forceinline__ void heat_add(double* __restrict__ h, float px, float pz, const double* c, double extent,
+                                         int bins, double e) {
+  if (isnan(px)) return;
+  const double bs = 2.0 * extent / (double)bins;
+  const double vx = (((double)px - c[0]) + extent) / bs;
+  const double vz = (((double)pz - c[2]) + extent) / bs;
+  if (!(vx > -1.0 && vx < (double)bins && vz > -1.0 && vz < (double)bins)) return;
+  const int64_t ix = (int64_t)vx, iz = (int64_t)vz;
+  unsafeAtomicAdd(&h[ix * bins + iz], e);
+}
+
 // ------------------------------------------------------------- tree nodes
 // 64-byte internal node of the photon-map radix tree (layout in alb_abi.h)
 struct __attribute__((aligned(16))) Node {

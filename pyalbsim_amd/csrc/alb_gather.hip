@@ -629,6 +629,9 @@ __global__ __launch_bounds__(256) void knorm_perm_kernel(const double* __restric
   knorm[r] = 1.0 / ksize;
 }
 
+// HEAT: also add every contribution into the scene's water and seafloor heat
+// maps at the record's interaction points (core/simulation.py:197-214)
+template <bool HEAT>
 __global__ __launch_bounds__(256) void contrib_kernel(ContribArgs a) {
   extern __shared__ double hist[];  // [win_len]
   for (int t = threadIdx.x; t < a.win_len; t += blockDim.x) hist[t] = 0.0;
@@ -668,6 +671,12 @@ __global__ __launch_bounds__(256) void contrib_kernel(ContribArgs a) {
     if (bin >= a.win0 && bin < (int64_t)a.win0 + a.win_len) atomicAdd(&hist[bin - a.win0], e);
     else if (bin >= 0 && bin < a.n_bins)
       unsafeAtomicAdd(a.rep ? &a.rep[(int64_t)(blockIdx.x % a.n_rep) * a.n_bins + bin] : &a.wf[bin], e);
+    if constexpr (HEAT) {
+      heat_add(a.sc.heat_water, m.fwi_x[i], m.fwi_z[i], a.sc.heat_water_center, a.sc.heat_water_extent,
+               a.sc.heat_bins, e);
+      heat_add(a.sc.heat_floor, m.sfi_x[i], m.sfi_z[i], a.sc.heat_floor_center, a.sc.heat_floor_extent,
+               a.sc.heat_bins, e);
+    }
   }
   __syncthreads();
   for (int t = threadIdx.x; t < a.win_len; t += blockDim.x)
@@ -978,7 +987,9 @@ extern "C" int alb_contrib(const alb_scene_t* scene, const alb_map_t* map, int32
   int64_t cb = (cn * scene->k + 255) / 256;
   if (cb > (int64_t)cus * 8) cb = (int64_t)cus * 8;
   cb = (cb + kXcds - 1) / kXcds * kXcds;
-  hipLaunchKernelGGL(contrib_kernel, dim3((unsigned)cb), dim3(256), sizeof(double) * (size_t)win_len, s, ca);
+  const bool heat = scene->heat_bins > 0 && scene->heat_water && scene->heat_floor && map->fwi_x && map->sfi_x;
+  hipLaunchKernelGGL(heat ? contrib_kernel<true> : contrib_kernel<false>, dim3((unsigned)cb), dim3(256),
+                     sizeof(double) * (size_t)win_len, s, ca);
   if (n_replicas > 0)
     hipLaunchKernelGGL(replica_reduce_kernel, dim3((unsigned)((n_bins + 255) / 256)), dim3(256), 0, s, replicas,
                        n_replicas, n_bins, waveform);

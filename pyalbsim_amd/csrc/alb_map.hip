@@ -102,6 +102,15 @@ __global__ __launch_bounds__(256) void permute_kernel(alb_records_t in, alb_map_
   m.time[i] = in.time[j];
 }
 
+// interaction points (heat maps) follow the records' permutation
+__global__ __launch_bounds__(256) void permute_points_kernel(alb_records_t in, alb_map_t m) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.n) return;
+  const int32_t j = m.perm[i];
+  m.fwi_x[i] = in.fwi_x[j]; m.fwi_y[i] = in.fwi_y[j]; m.fwi_z[i] = in.fwi_z[j];
+  m.sfi_x[i] = in.sfi_x[j]; m.sfi_y[i] = in.sfi_y[j]; m.sfi_z[i] = in.sfi_z[j];
+}
+
 // common-prefix length of keys i and j; equal keys continue with the index bits
 __device__ __forceinline__ int delta(const uint64_t* __restrict__ k, int64_t n, int64_t i, int64_t j) {
   if (j < 0 || j >= n) return -1;
@@ -371,6 +380,7 @@ static int map_build(const alb_records_t* in, int64_t n, int32_t dims, alb_map_t
   if (e != hipSuccess) return set_error(ALB_ERR_HIP, "%s: sort failed: %s", who, hipGetErrorString(e));
   alb_map_t m = *map;
   hipLaunchKernelGGL(permute_kernel, dim3(grid), dim3(B), 0, s, *in, m);
+  if (in->fwi_x && m.fwi_x) hipLaunchKernelGGL(permute_points_kernel, dim3(grid), dim3(B), 0, s, *in, m);
   if (n > 1) {
     const unsigned g2 = (unsigned)((n - 1 + B - 1) / B);
     hipLaunchKernelGGL(radix_tree_kernel, dim3(g2), dim3(B), 0, s, map->keys, n, (Node*)map->nodes, level, hist);

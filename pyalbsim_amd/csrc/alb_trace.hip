@@ -213,6 +213,9 @@ struct Photon {
   double lam;         // water path length whose optical depth was accumulated
   double gs;          // sum of phase-sampling scores d log p / dg
   int nsc, nfl;       // scatters (omega0 applied) and floor reflections so far
+  // heat maps only: first water interaction and latest seafloor interaction
+  // (float32, NaN before the interaction; core/simulation.py:85-86, 393, 521)
+  float fx, fy, fz, bx, by, bz;
 };
 
 // record / query produced by one event
@@ -224,9 +227,11 @@ struct Emit {
   // replay trace: the path statistics of the record's prefix
   double lam, gs;
   int nsc, nfl;
+  // heat maps: the photon's interaction points at the event
+  float fx, fy, fz, bx, by, bz;
 };
 
-template <bool STATS>
+template <bool STATS, bool HEAT = false>
 __device__ __forceinline__ void snapshot(Emit& e, const Photon& ph) {
   if constexpr (STATS) {
     e.lam = ph.lam;
@@ -234,6 +239,27 @@ __device__ __forceinline__ void snapshot(Emit& e, const Photon& ph) {
     e.nsc = ph.nsc;
     e.nfl = ph.nfl;
   }
+  if constexpr (HEAT) {
+    e.fx = ph.fx; e.fy = ph.fy; e.fz = ph.fz;
+    e.bx = ph.bx; e.by = ph.by; e.bz = ph.bz;
+  }
+}
+
+// forward scatter-radius histogram at a seafloor hit: surface radius from the
+// float32 first water interaction, floor radius from the FP64 hit point
+// (math/accumulate_heatmap.py:64-124, hooked at core/simulation.py:523-537);
+// photons without a water entry (NaN) and radii outside the extents are not
+// counted
+__device__ __forceinline__ void count_radius(const alb_scene_t& sc, const Photon& ph, double Px, double Pz) {
+  if (sc.heat_bins <= 0 || isnan(ph.fx)) return;
+  const double sx = (double)ph.fx - sc.heat_water_center[0], sz = (double)ph.fz - sc.heat_water_center[2];
+  const double fxd = Px - sc.heat_floor_center[0], fzd = Pz - sc.heat_floor_center[2];
+  const double rs = sqrt(sx * sx + sz * sz);
+  const double rf = sqrt(fxd * fxd + fzd * fzd);
+  const int64_t bins = sc.heat_bins;
+  const int64_t ix = (int64_t)((rs / sc.heat_water_extent) * (double)bins);
+  const int64_t iy = (int64_t)((rf / sc.heat_floor_extent) * (double)bins);
+  if (ix >= 0 && ix < bins && iy >= 0 && iy < bins) atomicAdd(&sc.heat_radius[ix * bins + iy], 1u);
 }
 
 struct Tally {
@@ -290,6 +316,7 @@ __device__ __forceinline__ void emit_photon(const alb_scene_t& sc, bool fwd, con
   ph.s = 0;
   ph.event_no = 0;
   ph.alive = true;
+  ph.fx = ph.fy = ph.fz = ph.bx = ph.by = ph.bz = __builtin_nanf("");
   c.paths++;
   if (st.exhausted) {
     ph.alive = false;
@@ -306,7 +333,7 @@ __device__ __forceinline__ void emit_photon(const alb_scene_t& sc, bool fwd, con
 // surface reflection from the layer of the position after the event
 // (core/simulation.py:495-499, 563-566).  Positions are updated by the same
 // repeated additions, so they match the reference's bit for bit.
-template <bool STATS>
+template <bool STATS, bool HEAT = false>
 __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, const alb_rng_t& rng, Stream& st,
                                               Photon& ph, Emit& e, Tally& c) {
   const double two_pi = 2.0 * ALB_PI;
@@ -356,7 +383,7 @@ __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, c
       e.E = ph.E;
       e.t = (float)ph.s;
       e.eno = ph.event_no++;
-      snapshot<STATS>(e, ph);
+      snapshot<STATS, HEAT>(e, ph);
       const double dep2 = (ph.py - y_ws) * -1.0;
       const alb_layer_t& L2 = Ls[layer_of(Ls, nl, dep2)];
       const double zl2 = dep2 - L2.top;
@@ -389,12 +416,18 @@ __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, c
       double f = (y_sf - ph.py) / svy;
       f = clipd(f, 0.0, 1.0);
       const double Px = ph.px + f * svx, Py = ph.py + f * svy, Pz = ph.pz + f * svz;
+      if constexpr (HEAT) {  // seafloor interaction, set before the record (core/simulation.py:521)
+        ph.bx = (float)Px;
+        ph.by = (float)Py;
+        ph.bz = (float)Pz;
+        if (fwd && sc.heat_radius) count_radius(sc, ph, Px, Pz);
+      }
       e.et = ALB_BOTTOM_REFLECTION;
       e.x = Px; e.y = Py; e.z = Pz; e.dx = ph.dx; e.dy = ph.dy; e.dz = ph.dz;
       e.E = ph.E;
       e.t = (float)ph.s;
       e.eno = ph.event_no++;
-      snapshot<STATS>(e, ph);
+      snapshot<STATS, HEAT>(e, ph);
       const double u1 = uniform(rng, st);
       const double u2 = uniform(rng, st);
       const double sr = sqrt(u1);
@@ -432,7 +465,7 @@ __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, c
       e.E = ph.E;
       e.t = (float)ph.s;
       e.eno = no;
-      snapshot<STATS>(e, ph);
+      snapshot<STATS, HEAT>(e, ph);
     }
     if (reflect) {
       ph.dy = -ph.dy;
@@ -464,7 +497,7 @@ __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, c
 // phase-sampling score) and snapshot them into e; the physics is unchanged.
 // LAYERED: the scene may be a layered column (checked at run time); without
 // it the kernel carries only the closed-form water legs.
-template <bool STATS, bool LAYERED = true>
+template <bool STATS, bool LAYERED = true, bool HEAT = false>
 __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, const alb_rng_t& rng, Stream& st,
                                            Photon& ph, Emit& e, Tally& c) {
   const double two_pi = 2.0 * ALB_PI;
@@ -494,6 +527,11 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
     double f = (y_ws - qy) / svy;
     f = clipd(f, 0.0, 1.0);
     const double Px = qx + f * svx, Py = qy + f * svy, Pz = qz + f * svz;
+    if constexpr (HEAT) {  // first water interaction, set before the record (core/simulation.py:393)
+      ph.fx = (float)Px;
+      ph.fy = (float)Py;
+      ph.fz = (float)Pz;
+    }
     const double R = schlick(-ph.dy, sc.F0);
     const double T = 1.0 - R;
     e.et = ALB_SURFACE_REFLECTION;
@@ -501,7 +539,7 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
     e.E = ph.E;
     e.t = (float)(ph.s + j);
     e.eno = ph.event_no++;
-    snapshot<STATS>(e, ph);
+    snapshot<STATS, HEAT>(e, ph);
     refract_y(ph.dx, ph.dy, ph.dz, 1.0, sc.eta);
     ph.E = (float)((double)ph.E * T);
     const double len = norm3(svx, svy, svz);
@@ -515,7 +553,7 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
     ph.alive = false;  // below the floor and not rising: the reference finds no event
     return;
   } else if (LAYERED && sc.layers) {
-    water_stepped<STATS>(sc, fwd, rng, st, ph, e, c);
+    water_stepped<STATS, HEAT>(sc, fwd, rng, st, ph, e, c);
     return;
   } else {
     // ---------------------------------------------------------- water leg
@@ -553,7 +591,7 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
       e.t = (float)(ph.s + j);
       e.eno = ph.event_no++;
       if constexpr (STATS) ph.lam += a1 * len;
-      snapshot<STATS>(e, ph);
+      snapshot<STATS, HEAT>(e, ph);
       const double u = uniform(rng, st);
       double theta;
       if constexpr (STATS) {
@@ -594,12 +632,18 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
         double f = (y_sf - qy) / svy;
         f = clipd(f, 0.0, 1.0);
         const double Px = qx + f * svx, Py = qy + f * svy, Pz = qz + f * svz;
+        if constexpr (HEAT) {  // seafloor interaction, set before the record (core/simulation.py:521)
+          ph.bx = (float)Px;
+          ph.by = (float)Py;
+          ph.bz = (float)Pz;
+          if (fwd && sc.heat_radius) count_radius(sc, ph, Px, Pz);
+        }
         e.et = ALB_BOTTOM_REFLECTION;
         e.x = Px; e.y = Py; e.z = Pz; e.dx = ph.dx; e.dy = ph.dy; e.dz = ph.dz;
         e.E = ph.E;
         e.t = (float)(ph.s + jb);
         e.eno = ph.event_no++;
-        snapshot<STATS>(e, ph);
+        snapshot<STATS, HEAT>(e, ph);
         if constexpr (STATS) {
           ph.lam += slen;
           ph.nfl++;
@@ -633,7 +677,7 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
           e.E = ph.E;
           e.t = (float)(ph.s + jb);
           e.eno = no;
-          snapshot<STATS>(e, ph);
+          snapshot<STATS, HEAT>(e, ph);
         }
         if constexpr (STATS) {
           if (reflect) ph.lam += slen;
@@ -662,7 +706,7 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
 
 // Wave-aggregated emission: one atomic per photon type per wave.  A backward
 // surface entry goes to both surface types.
-template <bool FWD>
+template <bool FWD, bool HEAT = false>
 __device__ __forceinline__ void emit_record(const Sinks& sk, const Emit& e, int64_t tag, int32_t scene,
                                             uint32_t lane) {
 #pragma unroll
@@ -686,6 +730,12 @@ __device__ __forceinline__ void emit_record(const Sinks& sk, const Emit& e, int6
           r.time[slot] = e.t;
           if (r.tag) r.tag[slot] = tag;
           if (r.scene) r.scene[slot] = scene;
+          if constexpr (HEAT) {
+            if (r.fwi_x) {
+              r.fwi_x[slot] = e.fx; r.fwi_y[slot] = e.fy; r.fwi_z[slot] = e.fz;
+              r.sfi_x[slot] = e.bx; r.sfi_y[slot] = e.by; r.sfi_z[slot] = e.bz;
+            }
+          }
         }
       } else {
         const alb_queries_t& r = sk.q[t];
@@ -715,8 +765,9 @@ __device__ __forceinline__ void flush_tally(const Tally& c, unsigned long long* 
 }
 
 // One scene, one lane per photon; LAYERED selects the stepped water legs of
-// a layered column.
-template <bool FWD, bool LAYERED>
+// a layered column, HEAT the interaction points and radius histogram of the
+// heat maps.
+template <bool FWD, bool LAYERED, bool HEAT>
 __global__ __launch_bounds__(256) void trace_kernel(alb_scene_t sc, alb_rng_t rng, int64_t begin,
                                                     int64_t n, Sinks sk, unsigned long long* ctr) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -728,9 +779,9 @@ __global__ __launch_bounds__(256) void trace_kernel(alb_scene_t sc, alb_rng_t rn
   if (i < n) emit_photon(sc, FWD, rng, st, ph, c);
   while (ballot(ph.alive)) {
     Emit e{-1, 0, 0, 0, 0, 0, 0, 0.0f, 0.0f, 0u};
-    if (ph.alive) next_event<false, LAYERED>(sc, FWD, rng, st, ph, e, c);
+    if (ph.alive) next_event<false, LAYERED, HEAT>(sc, FWD, rng, st, ph, e, c);
     const int64_t tag = (int64_t)(((uint64_t)(begin + i) << 16) | (uint64_t)(e.eno & 0xFFFFu));
-    emit_record<FWD>(sk, e, tag, 0, lane);
+    emit_record<FWD, HEAT>(sk, e, tag, 0, lane);
   }
   flush_tally(c, ctr, lane);
 }
@@ -955,6 +1006,8 @@ extern "C" int alb_trace(const alb_scene_t* scene, const alb_rng_t* rng, int32_t
     return set_error(ALB_ERR_ARG, "alb_trace: missing phase tables");
   if (scene->n_layers < 0 || (scene->n_layers > 0) != (scene->layers != nullptr))
     return set_error(ALB_ERR_ARG, "alb_trace: n_layers and layers disagree");
+  if (scene->heat_bins < 0 || (scene->heat_bins > 0 && !(scene->heat_water_extent > 0.0 && scene->heat_floor_extent > 0.0)))
+    return set_error(ALB_ERR_ARG, "alb_trace: heat maps need bins >= 0 and positive extents");
   if (photon_begin + n > ((int64_t)1 << 47))
     return set_error(ALB_ERR_ARG, "alb_trace: photon index exceeds 2^47");
   Sinks sk;
@@ -965,8 +1018,13 @@ extern "C" int alb_trace(const alb_scene_t* scene, const alb_rng_t* rng, int32_t
   const int64_t grid = (n + block - 1) / block;
   hipStream_t s = (hipStream_t)stream;
   const bool layered = scene->layers != nullptr;
-  auto kern = pass == ALB_PASS_FORWARD ? (layered ? trace_kernel<true, true> : trace_kernel<true, false>)
-                                       : (layered ? trace_kernel<false, true> : trace_kernel<false, false>);
+  // interaction points: written into forward records whose sinks hold them;
+  // floor hits fill the radius histogram when the scene has one
+  const bool heat = pass == ALB_PASS_FORWARD && ((scene->heat_bins > 0 && scene->heat_radius) || out[0].fwi_x);
+  auto kern = pass == ALB_PASS_FORWARD
+                  ? (heat ? (layered ? trace_kernel<true, true, true> : trace_kernel<true, false, true>)
+                          : (layered ? trace_kernel<true, true, false> : trace_kernel<true, false, false>))
+                  : (layered ? trace_kernel<false, true, false> : trace_kernel<false, false, false>);
   hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), 0, s, *scene, *rng, photon_begin, n, sk, counters);
   return check_launch("alb_trace");
 }

@@ -42,12 +42,14 @@ class SimulationResult:
     maps: list = field(default_factory=list)
     queries: list = field(default_factory=list)
     timings: dict = field(default_factory=dict)
+    heatmaps: Optional[dict] = None  # water / floor energy and radius counts [bins, bins], when computed
 
 
 def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=None, keep=False,
              tags: bool = False, dscene: Optional[runtime.DeviceScene] = None, sync_timing: bool = False,
              strict: bool = True, events: Optional[dict] = None,
-             kernel_events: Optional[dict] = None, scene_id: int = 0) -> SimulationResult:
+             kernel_events: Optional[dict] = None, scene_id: int = 0, heatmaps=None,
+             points: bool = False) -> SimulationResult:
     """Forward trace -> map build -> backward trace -> gather, all on the GPU.
 
     events: optional dict; HIP events are recorded on the launch stream at the
@@ -59,10 +61,21 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
     (core/simulation.py:134-153); strict=False gives such types zero signal.
     scene_id: Philox stream word pass | scene_id << 1 (scene scene_id of a
     batched run uses the same streams).
+    heatmaps: a HeatmapConfig (or None): also compute the sampled heat maps
+    (core/simulation.py:51-69, 197-214, 523-537) into res.heatmaps.
+    points: keep the records' first water / seafloor interaction points in
+    the maps (implied by heatmaps).
     """
     import torch
 
     ds = dscene or runtime.DeviceScene(model)
+    if heatmaps is not None and heatmaps.enabled:
+        if ds.heat is None or ds.heat["bins"] != int(heatmaps.bins):
+            ds.enable_heatmaps(heatmaps.bins, heatmaps.water_extent, heatmaps.seafloor_extent)
+        ds.clear_heatmaps()
+        points = True
+    elif ds.heat is not None:
+        ds = runtime.DeviceScene(model)  # a scene without heat maps for this run
     win = window or model.gate()
     t = {}
 
@@ -78,7 +91,8 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
     mark("start")
     cf = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device="cuda")
     cb = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device="cuda")
-    recs, rcounts = runtime.trace(ds, key, abi.PASS_FORWARD, n_forward, tags=tags, counters=cf, scene_id=scene_id)
+    recs, rcounts = runtime.trace(ds, key, abi.PASS_FORWARD, n_forward, tags=tags, counters=cf, scene_id=scene_id,
+                                  points=points)
     mark("forward")
     maps = [runtime.build_map(recs[ti], rcounts[ti], runtime.MAP_DIMS[ti]) for ti in range(abi.N_TYPES)]
     if not keep:
@@ -107,6 +121,8 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
         raise runtime.AlbError("gather node stack overflow")
     res.counters_fwd, res.counters_bwd = ctr_f, ctr_b
     del keep_alive  # the counter read-back above synchronised the stream
+    if ds.heat is not None:
+        res.heatmaps = ds.heatmaps()
     if keep:
         res.maps = maps
         res.queries = qs

@@ -11,9 +11,11 @@ exactly as run_linear, and ``processes`` is ignored.  Returned:
     worker arrays);
   * photon_maps_data: {PhotonType: PhotonMapData} of the forward records in
     the map's Morton order (the reference's order is batch order; the records
-    are the same set).  first_water_interaction / seafloor_interaction are
-    NaN: they feed only the heat maps, which are not computed here;
-  * sampled heat maps: None (not computed).
+    are the same set), with each record's first water and seafloor
+    interaction points (NaN before the interaction);
+  * sampled heat maps: SampledHeatmapsResult (water and seafloor energy,
+    scatter-radius counts; float32 like the reference, summed in FP64) when
+    simulation_config.heatmap.enabled, else None.
 
 ``seed`` seeds numpy.random.default_rng, whose two draws make the Philox key
 (None: a fresh secrets-seeded generator, as the reference).
@@ -44,16 +46,21 @@ def photon_maps_from_result(res) -> dict:
         m = res.maps[t].tensors
         pos = np.stack([m[f][:n].cpu().numpy() for f in ("x", "y", "z")], 1).astype(np.float32)
         dirs = np.stack([m[f][:n].cpu().numpy() for f in ("dx", "dy", "dz")], 1).astype(np.float32)
-        nan = np.full((n, 3), np.nan, dtype=np.float32)
+        if m.get("fwi_x") is not None:
+            fwi = np.stack([m[f][:n].cpu().numpy() for f in ("fwi_x", "fwi_y", "fwi_z")], 1).astype(np.float32)
+            sfi = np.stack([m[f][:n].cpu().numpy() for f in ("sfi_x", "sfi_y", "sfi_z")], 1).astype(np.float32)
+        else:
+            fwi = np.full((n, 3), np.nan, dtype=np.float32)
+            sfi = fwi.copy()
         out[pt] = PhotonMapData(positions=pos, directions=dirs, energies=m["energy"][:n].cpu().numpy().copy(),
-                                times=m["time"][:n].cpu().numpy().copy(), first_water_interaction=nan,
-                                seafloor_interaction=nan.copy())
+                                times=m["time"][:n].cpu().numpy().copy(), first_water_interaction=fwi,
+                                seafloor_interaction=sfi)
     return out
 
 
 def run_parallel(simulation_config: SimulationConfig, run_config: RunConfig,
                  seed: Union[int, None] = None, *, verbose: bool = True):
-    """(waveform, photon_maps_data, None) for one scene, computed on the GPU."""
+    """(waveform, photon_maps_data, sampled heat maps or None) for one scene, computed on the GPU."""
     rng = np.random.default_rng(secrets.randbits(128) if seed is None else seed)
     key = abi.key_from_generator(rng)
     model = build_scene(simulation_config)
@@ -62,11 +69,20 @@ def run_parallel(simulation_config: SimulationConfig, run_config: RunConfig,
     if verbose:
         print("Starting forwards pass")
     t0 = time.perf_counter()
-    res = simulate(model, n_f, n_b, key, keep=True)
+    hc = simulation_config.heatmap
+    res = simulate(model, n_f, n_b, key, keep=True, points=True, heatmaps=hc if hc.enabled else None)
     maps = photon_maps_from_result(res)
     if verbose:
         print("\n" + photon_map_stats_table(res.map_counts) + "\n")
         print(f"Finished {n_f} + {n_b} photon paths in {time.perf_counter() - t0:.2f} s")
     wf = res.waveforms.cpu().numpy()
     waveform = {pt: wf[pt.abi_index].astype(np.float32) for pt in PhotonType}
-    return waveform, maps, None
+    heat = None
+    if res.heatmaps is not None:
+        from ..utils.heatmap_result import SampledHeatmapsResult
+
+        h = res.heatmaps
+        heat = SampledHeatmapsResult(water_heatmap=h["water"].astype(np.float32),
+                                     seafloor_heatmap=h["floor"].astype(np.float32),
+                                     scatter_radius_heatmap=h["radius"].astype(np.float32))
+    return waveform, maps, heat

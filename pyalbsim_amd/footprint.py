@@ -17,6 +17,13 @@ import numpy as np
 from .model import SceneModel, eval_profile, LIGHT_SPEED_AIR
 
 
+def water_surface_intersection(model: SceneModel) -> np.ndarray:
+    """Point where the laser boresight meets the water surface
+    (SimulationModel.water_surface_intersection, physics/models/simulation.py:90-101)."""
+    d = np.asarray(model.laser_dir, dtype=np.float64)
+    return (model.y_ws / d[1]) * d
+
+
 def seafloor_intersection(model: SceneModel) -> np.ndarray:
     """Point where the laser boresight, refracted at the surface, meets the floor."""
     d = np.asarray(model.laser_dir, dtype=np.float64)

@@ -148,11 +148,49 @@ class DeviceScene:
         self.mass = torch.as_tensor(phase_bin_mass(model.phase_theta, model.phase_cdf), dtype=torch.float64,
                                     device=device)
         self.s = abi.make_scene(model, _p(self.theta), _p(self.cdf), _p(self.mass))
+        self.heat = None
         self.layers, self._layer_keep = _upload_layers(model, device)
         if self.layers is not None:
             self.s.n_layers = len(model.layers)
             self.s.layers = _p(self.layers)
         self.device = device
+
+    def enable_heatmaps(self, bins: int, water_extent: float, seafloor_extent: float):
+        """Allocate the sampled heat maps (core/simulation.py:51-69): water and
+        seafloor energy [bins, bins] (FP64) and the forward scatter-radius
+        counts [bins, bins], centred on the laser's water-surface and seafloor
+        intersections (physics/models/simulation.py:90-124)."""
+        from .footprint import seafloor_intersection, water_surface_intersection
+
+        torch = _torch()
+        b = int(bins)
+        if b <= 0 or not (water_extent > 0 and seafloor_extent > 0):
+            raise ValueError("heat maps need bins > 0 and positive extents")
+        self.heat = {"bins": b,
+                     "radius": torch.zeros(b * b, dtype=torch.int32, device=self.device),
+                     "water": torch.zeros(b * b, dtype=torch.float64, device=self.device),
+                     "floor": torch.zeros(b * b, dtype=torch.float64, device=self.device)}
+        s = self.s
+        s.heat_bins = b
+        s.heat_water_extent = float(water_extent)
+        s.heat_floor_extent = float(seafloor_extent)
+        wc, fc = water_surface_intersection(self.model), seafloor_intersection(self.model)
+        for i in range(3):
+            s.heat_water_center[i] = float(wc[i])
+            s.heat_floor_center[i] = float(fc[i])
+        s.heat_radius = _p(self.heat["radius"])
+        s.heat_water = _p(self.heat["water"])
+        s.heat_floor = _p(self.heat["floor"])
+
+    def clear_heatmaps(self):
+        if self.heat:
+            for f in ("radius", "water", "floor"):
+                self.heat[f].zero_()
+
+    def heatmaps(self) -> dict:
+        """Host copies: water and seafloor energy, scatter-radius counts, each [bins, bins]."""
+        b = self.heat["bins"]
+        return {f: self.heat[f].cpu().numpy().reshape(b, b) for f in ("water", "floor", "radius")}
 
 
 REC_F32 = ("x", "y", "z", "dx", "dy", "dz", "energy", "time")
@@ -168,7 +206,8 @@ class Sink:
     struct: object = None
 
     @staticmethod
-    def alloc(capacity: int, query: bool, tags: bool, device="cuda", scenes: bool = False) -> "Sink":
+    def alloc(capacity: int, query: bool, tags: bool, device="cuda", scenes: bool = False,
+              points: bool = False) -> "Sink":
         torch = _torch()
         capacity = max(int(capacity), 1)
         gdt = torch.float64 if query else torch.float32
@@ -177,19 +216,23 @@ class Sink:
             t[f] = torch.empty(capacity, dtype=torch.float32 if f in ("energy", "time") else gdt, device=device)
         t["tag"] = torch.empty(capacity, dtype=torch.int64, device=device) if tags else None
         t["scene"] = torch.empty(capacity, dtype=torch.int32, device=device) if scenes else None
+        for f in abi.INTERACTION_FIELDS:  # first water / seafloor interaction points (heat maps)
+            t[f] = torch.empty(capacity, dtype=torch.float32, device=device) if points and not query else None
         t["count"] = torch.zeros(1, dtype=torch.int64, device=device)
         st = abi.Queries() if query else abi.Records()
         for f in REC_F32:
             setattr(st, f, _p(t[f]))
         st.tag = _p(t["tag"])
         st.scene = _p(t["scene"])
+        for f in abi.INTERACTION_FIELDS:
+            setattr(st, f, _p(t[f]))
         st.capacity = capacity
         st.count = _p(t["count"])
         return Sink(t, capacity, query, st)
 
     def view(self, n: int) -> dict:
         return {f: (self.tensors[f][:n] if self.tensors.get(f) is not None else None)
-                for f in (*REC_F32, "tag", "scene")}
+                for f in (*REC_F32, "tag", "scene", *abi.INTERACTION_FIELDS)}
 
 
 # expected records per photon and type (BOTTOM, SCATTER, SURF_REFL, SURF_TRANS) before a pilot
@@ -197,13 +240,18 @@ _DEFAULT_RATES = (1.5, 8.0, 1.0, 1.5)
 
 
 def trace(dscene: DeviceScene, key, pass_: int, n: int, begin: int = 0, replay=None, tags: bool = False,
-          rates=None, counters=None, stream=None, pilot: int = 65536, scene_id: int = 0):
+          rates=None, counters=None, stream=None, pilot: int = 65536, scene_id: int = 0, points: bool = False,
+          _count_heat: bool = True):
     """Trace photons [begin, begin + n) of one pass on the GPU.
 
     Returns (sinks, counts): sinks[t] is a Sink holding counts[t] items.
     Capacities come from a pilot run over the first photons (same counters, so
     it does not change the result); an overflow reruns the whole range with
     exact capacities.  replay: device float64 tensor [n, L] of uniforms.
+    points: forward records also carry the photons' first water and seafloor
+    interaction points (heat maps, PhotonMapData).  A scene with heat maps
+    counts the forward seafloor hits of this range into its radius histogram
+    once (not for the pilot, and not twice on a rerun).
     """
     torch = _torch()
     L = load_library()
@@ -214,13 +262,18 @@ def trace(dscene: DeviceScene, key, pass_: int, n: int, begin: int = 0, replay=N
     s = stream if stream is not None else _stream_ptr()
     if rates is None and n > 4 * pilot and replay is None:
         sinks, counts = trace(dscene, key, pass_, pilot, begin, None, False, _DEFAULT_RATES, None, stream, pilot,
-                              scene_id)
+                              scene_id, _count_heat=False)
         rates = [max(c / pilot, 0.05) for c in counts]
         del sinks
     rates = rates or _DEFAULT_RATES
     caps = [int(r * n * 1.05) + 1024 for r in rates]
+    scene = dscene.s
+    if dscene.heat is not None and not (_count_heat and pass_ == abi.PASS_FORWARD):
+        scene = abi.Scene.from_buffer_copy(dscene.s)
+        scene.heat_radius = None
+    radius0 = dscene.heat["radius"].clone() if (dscene.heat is not None and scene is dscene.s) else None
     for attempt in range(2):
-        sinks = [Sink.alloc(caps[t], query, tags) for t in range(abi.N_TYPES)]
+        sinks = [Sink.alloc(caps[t], query, tags, points=points) for t in range(abi.N_TYPES)]
         arr_r = (abi.Records * 4)()
         arr_q = (abi.Queries * 4)()
         for t in range(4):
@@ -231,7 +284,9 @@ def trace(dscene: DeviceScene, key, pass_: int, n: int, begin: int = 0, replay=N
         ctr_t = counters if counters is not None else None
         if ctr_t is not None and attempt == 1:
             ctr_t.zero_()
-        rc = L.alb_trace(C.byref(dscene.s), C.byref(rng), pass_, begin, n,
+        if radius0 is not None and attempt == 1:
+            dscene.heat["radius"].copy_(radius0)
+        rc = L.alb_trace(C.byref(scene), C.byref(rng), pass_, begin, n,
                          C.cast(arr_r, C.c_void_p) if not query else None,
                          C.cast(arr_q, C.c_void_p) if query else None, _p(ctr_t), C.c_void_p(s))
         _check(rc, "alb_trace")
@@ -270,6 +325,10 @@ def build_map(records: Sink, n: int, dims: int, stream=None) -> DeviceMap:
     st.dims = dims
     for f in ("x", "y", "z", "dx", "dy", "dz", "energy", "time", "perm", "keys", "meta", "nodes"):
         setattr(st, f, _p(t[f]))
+    if records.tensors.get("fwi_x") is not None:  # interaction points follow the sort
+        for f in abi.INTERACTION_FIELDS:
+            t[f] = torch.empty(max(n, 1), dtype=torch.float32, device=dev)
+            setattr(st, f, _p(t[f]))
     m.struct = st
     if n == 0:
         return m

@@ -32,7 +32,7 @@ def test_layout_and_version():
     assert lib.alb_abi_version() == abi.ALB_ABI_VERSION
     assert lib.alb_scene_size() == C.sizeof(abi.Scene)
     assert C.sizeof(abi.Rng) == 32
-    assert C.sizeof(abi.Records) == 96 and C.sizeof(abi.Queries) == 96
+    assert C.sizeof(abi.Records) == 144 and C.sizeof(abi.Queries) == 144 and C.sizeof(abi.Map) == 160
     assert C.sizeof(abi.SceneIndex) == 32
     assert C.sizeof(abi.Layer) == 136 and abi.Layer.z_lut.offset == 88 and abi.Layer.phase_n.offset == 128
 
