@@ -20,6 +20,10 @@ and SCATTER maps of one more 1e8-photon forward pass; per map the gather
 time (HIP events), records taken and examined per query, and algorithmic
 GB/s.
 
+Inversion: Adam over the batched adjoint on --inversion-scenes synthetic
+scenes (unknown depth, b, albedo) for --inversion-iters iterations; depth
+RMSE and seconds per iteration, batched form vs plain.
+
 The two other forms of the metric, on bounded samples:
   * batched: the first --sweep-scenes scenes of the Latin-hypercube sweep
     (seed 1) at 1/100 of their photon counts, once as one batched run and
@@ -111,6 +115,9 @@ def main(argv=None):
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-radius", action="store_true")
+    ap.add_argument("--no-inversion", action="store_true")
+    ap.add_argument("--inversion-scenes", type=int, default=32)
+    ap.add_argument("--inversion-iters", type=int, default=10)
     ap.add_argument("--radius-grid", type=int, default=256)
     ap.add_argument("--sweep-scenes", type=int, default=256)
     ap.add_argument("--adjoint-scenes", type=int, default=16)
@@ -233,6 +240,8 @@ def main(argv=None):
         out["batched"] = bench_batched(args.sweep_scenes)
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(args.adjoint_scenes)
+    if not args.no_inversion:
+        out["inversion"] = bench_inversion(args.inversion_scenes, args.inversion_iters)
     if args.copy_calibration:
         x = torch.empty(1 << 28, dtype=torch.float64, device="cuda")
         y = torch.empty_like(x)
@@ -289,6 +298,29 @@ def bench_radius(model, ds, key, n_f: int, grid: int, radius: float = 0.05, pitc
     del maps
     print("radius mode done", file=sys.stderr, flush=True)
     return out
+
+
+def bench_inversion(n_scenes: int, iters: int, n_forward: int = 20000):
+    """Adam over the batched adjoint on synthetic inversion scenes: depth RMSE
+    and seconds per iteration, batched form vs plain (one scene per launch)."""
+    import numpy as np
+
+    from pyalbsim_amd.inversion import invert, inversion_scenes
+
+    th = inversion_scenes(n_scenes, seed=3)
+    rb = invert(th, n_forward, n_forward // 4, iterations=iters, lr=0.02)
+    print("inversion batched done", file=sys.stderr, flush=True)
+    rp = invert(th, n_forward, n_forward // 4, iterations=2, lr=0.02, form="plain")
+    print("inversion plain done", file=sys.stderr, flush=True)
+    sb = float(np.median(rb.seconds_per_iteration[1:] or rb.seconds_per_iteration))
+    sp = float(np.median(rp.seconds_per_iteration))
+    return {"workload": f"{n_scenes} scenes, depth U[2,40] m, b U[0.1,1.5] /m, albedo U[0.05,0.4] unknown "
+                        f"(a = 0.1 /m, g = 0.9 known; roughness from wind U[0,10] m/s; flat floor), "
+                        f"{n_forward} forward + {n_forward // 4} backward photons per scene per pass, "
+                        f"Adam lr 0.02 on [0,1]-scaled unknowns, {iters} iterations",
+            "depth_rmse_start_m": rb.depth_rmse[0], "depth_rmse_end_m": rb.depth_rmse[-1],
+            "loss_start": rb.loss[0], "loss_end": rb.loss[-1],
+            "batched_s_per_iteration": sb, "plain_s_per_iteration": sp, "speedup": sp / sb}
 
 
 def bench_batched(n_scenes: int):

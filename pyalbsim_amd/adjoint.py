@@ -107,12 +107,14 @@ def scene_derivatives(model, theta) -> SceneDerivs:
     return SceneDerivs(model.alpha_l, a, albedo, rough, d)
 
 
-def model_from_theta(theta):
-    """SceneModel of the sweep geometry for theta = (a, b, g, depth, albedo, roughness)."""
+def model_from_theta(theta, sample_rate=None):
+    """SceneModel of the sweep geometry for theta = (a, b, g, depth, albedo,
+    roughness); sample_rate: receiver sample rate (default 1e10 /s)."""
     from .model import build_scene
 
     a, b, g, depth, albedo, rough = [float(v) for v in theta]
-    return build_scene(shot_config(depth, a, b, albedo, rough), hg_g=g)
+    sr = int(1e10) if sample_rate is None else int(sample_rate)
+    return build_scene(shot_config(depth, a, b, albedo, rough, sample_rate=sr), hg_g=g)
 
 
 def _replay_index(torch, sink, count, offsets, dev):
@@ -215,10 +217,12 @@ class BatchedWaveformFn:
                 from .execution.batched import simulate_batched
 
                 th = theta.detach().cpu().numpy().astype(np.float64)
-                models = [model_from_theta(row) for row in th]
+                models = [model_from_theta(row, spec.get("sample_rate")) for row in th]
+                gr = spec.get("graphs")
                 res = simulate_batched(models, spec["n_forward"], spec["n_backward"], spec["key"],
                                        win_len=spec.get("win_len", 1000), strict=False, adjoint=True,
-                                       block=spec.get("block", 256), refill=spec.get("refill", 4))
+                                       block=spec.get("block", 256), refill=spec.get("refill", 4),
+                                       graphs=None if gr is None else gr.setdefault("forward", {}))
                 wl = spec.get("win_len", 1000)
                 out = torch.stack([res.gated(s, models[s].gate(wl)) for s in range(len(models))])
                 ctx.res = res
@@ -243,7 +247,7 @@ class BatchedWaveformFn:
                 derivs = [scene_derivatives(m, th[s]) for s, m in enumerate(models)]
                 grads = gradients_from_sums(acc, direct, derivs)
                 if spec.get("depth_grad", "fd") == "fd":
-                    grads[:, 3] = depth_fd(th, spec, go)
+                    grads[:, 3] = depth_fd(th, spec, go, spec.get("depth_rel", 1e-3))
                 ctx.res = None
                 return torch.as_tensor(grads, dtype=torch.float64, device=ctx.device), None
 
@@ -261,9 +265,11 @@ def depth_fd(th, spec, go, rel: float = 1e-3):
     for sign in (1.0, -1.0):
         t2 = th.copy()
         t2[:, 3] = th[:, 3] * (1.0 + sign * rel)
-        models = [model_from_theta(row) for row in t2]
+        models = [model_from_theta(row, spec.get("sample_rate")) for row in t2]
+        gr = spec.get("graphs")
         r = simulate_batched(models, spec["n_forward"], spec["n_backward"], spec["key"], win_len=wl, strict=False,
-                             block=spec.get("block", 256), refill=spec.get("refill", 4))
+                             block=spec.get("block", 256), refill=spec.get("refill", 4),
+                             graphs=None if gr is None else gr.setdefault("depth_difference", {}))
         outs.append(torch.stack([r.gated(s, models[s].gate(wl)) for s in range(len(models))]))
     d = (outs[0] - outs[1]) * go
     return (d.sum(dim=(1, 2)) / torch.as_tensor(2.0 * rel * th[:, 3], device=d.device)).cpu().numpy()

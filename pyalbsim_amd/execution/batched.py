@@ -62,7 +62,7 @@ def simulate_batched(models: Sequence[SceneModel], n_forward, n_backward, key, *
                      win_len: int = 1024, block=256, refill: int = 4, strict: bool = True,
                      keep: bool = False, tags: bool = False, events: Optional[dict] = None,
                      kernel_events: Optional[dict] = None, table: Optional[runtime.DeviceSceneTable] = None,
-                     adjoint: bool = False) -> BatchedResult:
+                     adjoint: bool = False, graphs: Optional[dict] = None) -> BatchedResult:
     """Forward trace, map build, backward trace and gather of many scenes.
 
     n_forward / n_backward: photons per scene (sequences or scalars).
@@ -79,6 +79,12 @@ def simulate_batched(models: Sequence[SceneModel], n_forward, n_backward, key, *
     adjoint: keep what pyalbsim_amd.adjoint needs afterwards (record and
     query sinks with tags, the maps, each type's slot order, neighbour rows
     and kernel normalisations) in res.state.
+    graphs: a dict the caller keeps across calls with the same scene count
+    and photon counts (an optimisation loop, repeated sweeps): the scene
+    table is then updated in place and both trace passes run as replays of
+    HIP graphs captured on the first call (runtime.BatchedTraceGraph).  The
+    sinks of a replayed pass belong to the graph: a kept result is valid
+    until the next call with the same dict.
     """
     import torch
 
@@ -104,22 +110,49 @@ def simulate_batched(models: Sequence[SceneModel], n_forward, n_backward, key, *
     mark("start")
     if adjoint:
         tags = keep = True
-    tab = table or runtime.DeviceSceneTable(models, adjoint=adjoint)
+    if graphs is not None and table is None:
+        tab = graphs.get("table")
+        if tab is not None and tab.n == n and tab.adjoint == adjoint:
+            tab.update(models)
+        else:
+            tab = runtime.DeviceSceneTable(models, adjoint=adjoint)
+            graphs.clear()
+            graphs["table"] = tab
+    else:
+        tab = table or runtime.DeviceSceneTable(models, adjoint=adjoint)
     if block == "auto":
         from ..autotune import tune_batched
 
         block, refill, _ = tune_batched(models, nf, key, table=tab)
     cf = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device="cuda")
     cb = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device="cuda")
-    recs, rcounts = runtime.trace_batched(tab, key, abi.PASS_FORWARD, nf, tags=tags, counters=cf, block=block,
-                                          refill=refill, stream_base=base)
+    def trace_pass(pass_, counts, ctr):
+        if graphs is None:
+            return runtime.trace_batched(tab, key, pass_, counts, tags=tags, counters=ctr, block=block,
+                                         refill=refill, stream_base=base)
+        gk = (pass_, tuple(int(c) for c in counts), bool(tags), block, refill, base, tuple(key))
+        g = graphs.get(gk)
+        if g is not None:
+            try:
+                sinks, got = g.replay()
+                ctr.copy_(g.counters)
+                return sinks, got
+            except MemoryError:
+                pass  # more records than the graph's sinks hold: re-plan below
+        sinks, got = runtime.trace_batched(tab, key, pass_, counts, tags=tags, counters=ctr, block=block,
+                                           refill=refill, stream_base=base)
+        caps = [int(c * 1.25) + 4096 for c in got]
+        graphs[gk] = runtime.BatchedTraceGraph(tab, key, pass_, counts, caps, tags=tags, block=block,
+                                               refill=refill, stream_base=base)
+        return sinks, got
+
+    recs, rcounts = trace_pass(abi.PASS_FORWARD, nf, cf)
     mark("forward")
     maps = [runtime.build_map_batched(recs[ti], rcounts[ti], runtime.MAP_DIMS[ti], n) for ti in range(abi.N_TYPES)]
     if not keep:
         del recs
     mark("build")
-    qs, qcounts = runtime.trace_batched(tab, key, abi.PASS_BACKWARD, nb, tags=tags, counters=cb, block=block,
-                                        refill=refill, stream_base=base)
+    qs, qcounts = trace_pass(abi.PASS_BACKWARD, nb, cb)
     mark("backward")
     sizes = np.array([4 * m.n_bins for m in models], dtype=np.int64)
     offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)

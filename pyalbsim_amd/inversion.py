@@ -1,0 +1,151 @@
+"""Batched waveform inversion: Adam over the path-replay adjoint.
+
+Synthetic inversion workload: scenes of the single-shot geometry (400 m,
+nadir, 0.5 ns bins so that a 1000-bin gate holds floor returns down to about
+50 m of water) with unknown water depth U[2, 40] m, scattering coefficient
+b U[0.1, 1.5] /m and floor albedo U[0.05, 0.4], drawn with
+scipy.stats.qmc.LatinHypercube(d=4, seed); absorption a = 0.1 /m and the
+Henyey-Greenstein asymmetry g = 0.9 are known, and the surface roughness is
+fixed per scene from a wind speed U[0, 10] m/s (Cox-Munk, the fourth column).
+Target waveforms are the forward model at the true parameters with their own
+Philox key.
+
+Each Adam iteration (torch.optim.Adam on the unknowns scaled to [0, 1])
+runs the batched forward pass plus the path-replay adjoint
+(adjoint.batched_waveforms, a torch.autograd.Function; the depth derivative
+by a common-random-number central difference, two more forward passes) at
+a fixed photon count per scene.  The loss has two parts per scene: the
+squared difference of the cumulative gated waveforms (sum over photon types)
+each normalised by its own total (the time shape, weighted 1000: its
+gradient in depth persists while the predicted and target floor returns do
+not overlap yet), plus the squared difference of the log totals (the
+amplitude).  Depth is identifiable only where the floor return survives the
+water column; in deep turbid scenes the estimate drifts.
+
+form="batched" runs all scenes per launch; form="plain" runs one scene per
+launch with the same code path, for the plain-vs-batched time comparison.
+The reference has no inversion, and its seafloor is flat: the tilted floor
+of the extended workload is not modelled.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .model import cox_munk_roughness
+
+UNKNOWN = ("depth", "b", "albedo")
+RANGES = {"depth": (2.0, 40.0), "b": (0.1, 1.5), "albedo": (0.05, 0.4), "wind": (0.0, 10.0)}
+KNOWN = {"a": 0.1, "g": 0.9}
+SAMPLE_RATE = int(2e9)
+COLS = {"a": 0, "b": 1, "g": 2, "depth": 3, "albedo": 4, "roughness": 5}
+
+
+def inversion_scenes(n: int, seed: int = 3) -> np.ndarray:
+    """True theta [n, 6] = (a, b, g, depth, albedo, roughness) of n scenes."""
+    from scipy.stats import qmc
+
+    u = qmc.LatinHypercube(d=4, seed=seed).random(n)
+    th = np.zeros((n, 6))
+    th[:, COLS["a"]] = KNOWN["a"]
+    th[:, COLS["g"]] = KNOWN["g"]
+    for i, c in enumerate(("depth", "b", "albedo", "wind")):
+        lo, hi = RANGES[c]
+        v = lo + u[:, i] * (hi - lo)
+        if c == "wind":
+            th[:, COLS["roughness"]] = [cox_munk_roughness(w) for w in v]
+        else:
+            th[:, COLS[c]] = v
+    return th
+
+
+@dataclass
+class InversionResult:
+    theta_true: np.ndarray
+    theta: np.ndarray                      # final estimates [S, 6]
+    depth_rmse: list = field(default_factory=list)   # per iteration (before the step)
+    loss: list = field(default_factory=list)
+    seconds_per_iteration: list = field(default_factory=list)
+
+
+def _cumulative(w):
+    """[S, 4, L] gated waveforms -> [S, L] cumulative sum over time of all types."""
+    return w.sum(dim=1).cumsum(dim=1)
+
+
+def waveform_loss(w, target, shape_weight: float = 1000.0):
+    """Per-scene loss [S]: shape_weight x the mean squared difference of the
+    self-normalised cumulative waveforms plus the squared difference of the
+    log totals."""
+    import torch
+
+    cw, ct = _cumulative(w), _cumulative(target)
+    tw, tt = cw[:, -1:].clamp_min(1e-300), ct[:, -1:].clamp_min(1e-300)
+    d = cw / tw - ct / tt
+    return shape_weight * (d * d).mean(dim=1) + (torch.log(tw[:, 0]) - torch.log(tt[:, 0])) ** 2
+
+
+def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: int = 200, lr: float = 1e-2,
+           key=(0x1A7E, 5), target_key=(0x7A26, 6), form: str = "batched", win_len: int = 1000,
+           depth_rel: float = 1e-2, x0=None, log=None, shape_weight: float = 1000.0) -> InversionResult:
+    """Adam on (depth, b, albedo) scaled to [0, 1]; returns the estimates and
+    the depth RMSE and loss of every iteration."""
+    import torch
+
+    from .adjoint import batched_waveforms
+
+    S = theta_true.shape[0]
+    dev = "cuda"
+    # batched form: the scene table is updated in place and the trace passes
+    # replay HIP graphs captured in the first iteration
+    spec = dict(win_len=win_len, sample_rate=SAMPLE_RATE, depth_rel=depth_rel,
+                graphs={} if form == "batched" else None)
+    lo = np.array([RANGES[c][0] for c in UNKNOWN])
+    hi = np.array([RANGES[c][1] for c in UNKNOWN])
+    idx = [COLS[c] for c in UNKNOWN]
+
+    def run(theta_t, k):
+        if form == "batched":
+            return batched_waveforms(theta_t, n_forward, n_backward, k, **spec)
+        return torch.cat([batched_waveforms(theta_t[s:s + 1], n_forward, n_backward, k, **spec) for s in range(S)])
+
+    with torch.no_grad():
+        tspec = dict(spec, graphs=None)
+        th_t = torch.as_tensor(theta_true, dtype=torch.float64, device=dev)
+        target = (batched_waveforms(th_t, n_forward, n_backward, target_key, **tspec) if form == "batched" else
+                  torch.cat([batched_waveforms(th_t[s:s + 1], n_forward, n_backward, target_key, **tspec)
+                             for s in range(S)]))
+    x = torch.tensor(np.full((S, len(UNKNOWN)), 0.5) if x0 is None else x0, dtype=torch.float64, device=dev,
+                     requires_grad=True)
+    opt = torch.optim.Adam([x], lr=lr)
+    base = torch.as_tensor(theta_true, dtype=torch.float64, device=dev).clone()
+    lo_t = torch.as_tensor(lo, device=dev)
+    span_t = torch.as_tensor(hi - lo, device=dev)
+    out = InversionResult(theta_true, theta_true.copy())
+    for it in range(iterations):
+        t0 = time.perf_counter()
+        opt.zero_grad()
+        theta = base.clone()
+        theta[:, idx] = lo_t + x * span_t
+        w = run(theta, key)
+        loss = waveform_loss(w, target, shape_weight).sum()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            x.clamp_(0.0, 1.0)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        th_np = theta.detach().cpu().numpy()
+        rmse = float(np.sqrt(np.mean((th_np[:, COLS["depth"]] - theta_true[:, COLS["depth"]]) ** 2)))
+        out.depth_rmse.append(rmse)
+        out.loss.append(float(loss.item()))
+        out.seconds_per_iteration.append(dt)
+        if log is not None:
+            log(f"iteration {it + 1}/{iterations}: loss {out.loss[-1]:.4g}, depth RMSE {rmse:.3f} m, {dt:.2f} s")
+    theta = base.clone()
+    theta[:, idx] = lo_t + x.detach() * span_t
+    out.theta = theta.cpu().numpy()
+    out.depth_rmse.append(float(np.sqrt(np.mean((out.theta[:, COLS["depth"]] - theta_true[:, COLS["depth"]]) ** 2))))
+    return out

@@ -453,23 +453,33 @@ class DeviceSceneTable:
         """adjoint: also build the phase-derivative tables of HG scenes.
         gather_phase: optional models whose phase tables the gather's phase
         mass is taken from (default: each scene's own)."""
-        import hashlib
-
-        from .model import phase_bin_mass
-
         torch = _torch()
         self.models = list(models)
         self.n = len(self.models)
         if self.n == 0:
             raise ValueError("no scenes")
-        ks = {m.k for m in self.models}
+        self.adjoint = adjoint
+        self.device = device
+        self.tables = {}
+        self.layer_tables = {}
+        raw = self._pack(self.models, gather_phase)
+        self.dev = torch.as_tensor(raw, device=device)
+
+    def _pack(self, models, gather_phase=None) -> np.ndarray:
+        """alb_scene_t bytes of models; phase and layer tables are uploaded
+        once per distinct table and kept by the table object."""
+        import hashlib
+
+        from .model import phase_bin_mass
+
+        torch = _torch()
+        device = self.device
+        ks = {m.k for m in models}
         if len(ks) != 1:
             raise ValueError("all scenes of a batch must use the same photon_mapping_k")
         self.k = ks.pop()
-        self.tables = {}
-        self.layer_tables = {}
-        structs = (abi.Scene * self.n)()
-        for i, m in enumerate(self.models):
+        structs = (abi.Scene * len(models))()
+        for i, m in enumerate(models):
             gm = gather_phase[i] if gather_phase is not None else m
             h = hashlib.sha1(np.ascontiguousarray(m.phase_cdf).tobytes()
                              + np.ascontiguousarray(m.phase_theta).tobytes()
@@ -479,7 +489,7 @@ class DeviceSceneTable:
                 cd = torch.as_tensor(np.ascontiguousarray(m.phase_cdf), dtype=torch.float64, device=device)
                 ms = torch.as_tensor(phase_bin_mass(gm.phase_theta, gm.phase_cdf), dtype=torch.float64, device=device)
                 sc = dm = None
-                if adjoint and m.phase_g is not None:
+                if self.adjoint and m.phase_g is not None:
                     from .adjoint import phase_g_tables
 
                     score, dlm = phase_g_tables(m.phase_g)
@@ -489,17 +499,26 @@ class DeviceSceneTable:
             th, cd, ms, sc, dm = self.tables[h]
             structs[i] = abi.make_scene(m, _p(th), _p(cd), _p(ms), _p(sc), _p(dm))
             if m.layers:
-                if adjoint:
+                if self.adjoint:
                     raise NotImplementedError("the adjoint covers single constant-layer scenes only")
                 if id(m) not in self.layer_tables:
-                    self.layer_tables[id(m)] = _upload_layers(m, device)
+                    self.layer_tables[id(m)] = (m, _upload_layers(m, device))
                 structs[i].n_layers = len(m.layers)
-                structs[i].layers = _p(self.layer_tables[id(m)][0])
+                structs[i].layers = _p(self.layer_tables[id(m)][1][0])
         self.host = structs
-        raw = np.frombuffer(bytes(structs), dtype=np.uint8).copy()
-        self.dev = torch.as_tensor(raw, device=device)
-        self.n_bins = np.array([m.n_bins for m in self.models], dtype=np.int64)
-        self.device = device
+        self.n_bins = np.array([m.n_bins for m in models], dtype=np.int64)
+        return np.frombuffer(bytes(structs), dtype=np.uint8).copy()
+
+    def update(self, models, gather_phase=None):
+        """Rewrite the scene structs in place (same number of scenes), so that
+        launches captured in a HIP graph see the new scenes on replay."""
+        torch = _torch()
+        models = list(models)
+        if len(models) != self.n:
+            raise ValueError("update needs the same number of scenes")
+        raw = self._pack(models, gather_phase)
+        self.dev.copy_(torch.as_tensor(raw))
+        self.models = models
 
 
 def _dev_i64(values, device="cuda"):
@@ -557,6 +576,69 @@ def trace_batched(table: DeviceSceneTable, key, pass_: int, counts, tags: bool =
             return sinks, got
         caps = [max(got[t], 1) for t in range(4)]
     raise AlbError("alb_trace_batched: capacity overflow persisted after resizing")
+
+
+class BatchedTraceGraph:
+    """One pass of the persistent batched trace captured in a HIP graph.
+
+    The graph holds the reset of the work counter, the sink counts and the
+    event counters (fill kernels) and the alb_trace_batched launch over
+    table.dev, with sinks of fixed capacities.  replay() reruns it: after
+    table.update(models) the same launch traces the new scenes (same scene
+    count and photon counts).  It returns (sinks, counts) like trace_batched;
+    the sinks belong to the graph and are overwritten by the next replay.
+    counts above the capacities mean an overflow: replay() then raises
+    MemoryError and the caller re-plans (a new graph with larger caps).
+    """
+
+    def __init__(self, table: DeviceSceneTable, key, pass_: int, counts, caps, tags: bool = False,
+                 block: int = 256, refill: int = 4, stream_base: int = 0):
+        torch = _torch()
+        L = load_library()
+        counts = np.asarray(counts, dtype=np.int64)
+        if len(counts) != table.n:
+            raise ValueError("one photon count per scene")
+        self.table, self.pass_, self.caps = table, pass_, [int(c) for c in caps]
+        query = pass_ == abi.PASS_BACKWARD
+        self.rng = abi.make_rng(key, pass_ | (int(stream_base) << 1))
+        self.offsets = _dev_i64(np.concatenate([[0], np.cumsum(counts)]))
+        self.work = torch.zeros(1, dtype=torch.int64, device=table.device)
+        self.counters = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device=table.device)
+        self.sinks = [Sink.alloc(self.caps[t], query, tags, scenes=True) for t in range(abi.N_TYPES)]
+        self.arr = ((abi.Queries if query else abi.Records) * 4)()
+        for t in range(4):
+            self.arr[t] = self.sinks[t].struct
+        a = C.cast(self.arr, C.c_void_p)
+        self._args = (_p(table.dev), table.n, _p(self.offsets), C.byref(self.rng), pass_,
+                      None if query else a, a if query else None, _p(self.work), int(block), int(refill),
+                      _p(self.counters))
+        self._lib = L
+
+        def body():
+            self.work.zero_()
+            for sk in self.sinks:
+                sk.tensors["count"].zero_()
+            self.counters.zero_()
+            _check(L.alb_trace_batched(*self._args, C.c_void_p(torch.cuda.current_stream().cuda_stream)),
+                   "alb_trace_batched")
+
+        side = torch.cuda.Stream()  # default priority
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            body()  # one eager run before capture
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph, capture_error_mode="relaxed"):
+            body()
+
+    def replay(self):
+        torch = _torch()
+        self.graph.replay()
+        got = [int(x) for x in torch.cat([sk.tensors["count"] for sk in self.sinks]).cpu().tolist()]
+        if any(got[t] > self.caps[t] for t in range(4)):
+            raise MemoryError(f"alb_trace_batched: sink overflow in a captured launch ({got} > {self.caps})")
+        return self.sinks, got
 
 
 def build_map_batched(records: Sink, n: int, dims: int, n_scenes: int, stream=None) -> DeviceMap:

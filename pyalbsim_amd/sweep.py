@@ -34,11 +34,12 @@ RANGES = {
 }
 
 
-def shot_config(depth: float, a: float, b: float, albedo: float, roughness: float) -> SimulationConfig:
+def shot_config(depth: float, a: float, b: float, albedo: float, roughness: float,
+                sample_rate: int = int(1e10)) -> SimulationConfig:
     return SimulationConfig(
         scene=SceneConfig(flying_height=400.0),
         laser=LaserConfig(nadir_angle=0.0, divergence_angle=1e-3, wavelength=0.532),
-        sensor=SensorConfig(field_of_view=20e-3, sample_rate=int(1e10), aperture_radius=0.1),
+        sensor=SensorConfig(field_of_view=20e-3, sample_rate=int(sample_rate), aperture_radius=0.1),
         sea_surface=SeaSurfaceConfig(roughness=roughness),
         water=WaterConfig(layers=(TurbidityLayerConfig(height=depth, absorption_coefficient=a,
                                                        scattering_coefficient=b, refractive_index=1.34),)),

@@ -1,0 +1,47 @@
+"""Inversion driver on the GPU (Adam over the path-replay adjoint).
+
+* On clear shallow scenes (floor return well above the volume return) the
+  depth estimates started 4 m too deep move toward the truth within ten
+  iterations;
+* the plain form (one scene per launch) gives the same kind of result as the
+  batched form, and every quantity the driver reports is finite.
+"""
+import numpy as np
+import pytest
+
+from pyalbsim_amd.inversion import COLS, RANGES, inversion_scenes, invert
+
+pytestmark = pytest.mark.gpu
+
+
+def _clear_scenes():
+    th = inversion_scenes(4, seed=11)
+    th[:, COLS["depth"]] = [5.0, 6.0, 7.0, 8.0]
+    th[:, COLS["b"]] = 0.2
+    th[:, COLS["albedo"]] = 0.3
+    return th
+
+
+def _x0(th, depth_offset):
+    lo = np.array([RANGES[c][0] for c in ("depth", "b", "albedo")])
+    hi = np.array([RANGES[c][1] for c in ("depth", "b", "albedo")])
+    v = np.stack([th[:, COLS["depth"]] + depth_offset, th[:, COLS["b"]], th[:, COLS["albedo"]]], 1)
+    return (v - lo) / (hi - lo)
+
+
+def test_depth_moves_toward_truth_on_clear_scenes(gpu):
+    th = _clear_scenes()
+    res = invert(th, 40000, 10000, iterations=10, lr=0.02, x0=_x0(th, 4.0))
+    err0 = np.abs(4.0 * np.ones(4))
+    err = np.abs(res.theta[:, COLS["depth"]] - th[:, COLS["depth"]])
+    assert np.all(np.isfinite(res.loss)) and np.all(np.isfinite(res.theta))
+    assert err.mean() < 0.6 * err0.mean(), err
+    assert res.depth_rmse[-1] < res.depth_rmse[0]
+
+
+def test_plain_form_runs(gpu):
+    th = _clear_scenes()[:2]
+    res = invert(th, 20000, 5000, iterations=2, lr=0.02, x0=_x0(th, 4.0), form="plain")
+    assert len(res.seconds_per_iteration) == 2 and np.all(np.isfinite(res.loss))
+    lo, hi = RANGES["depth"]
+    assert np.all((res.theta[:, COLS["depth"]] >= lo) & (res.theta[:, COLS["depth"]] <= hi))
