@@ -17,12 +17,16 @@
 //      at <= 64 records, scanned; scanned records with d2 <= theta go to an
 //      LDS candidate buffer, four record ranges at a time so that their loads
 //      are in flight together;
-//   3. a full buffer is cut back to its k smallest (d2, index) entries by a
-//      wave-wide radix select (8-bit digits of the FP64 bit patterns, ties by
-//      index), which tightens theta.  The result is the brute-force neighbour
-//      set bit for bit, because squared distances are evaluated in FP64 in one
-//      fixed order and box distances are, by monotone rounding, never larger
-//      than those of the records inside.
+//   3. a full buffer (and the seed) is cut with a 64-bucket histogram of
+//      [0, theta]: entries up to the bucket holding the k-th smallest stay,
+//      and theta drops to the largest of them (an upper bound of the k-th);
+//      when that would not free 64 slots, the cut is exact: a wave-wide
+//      bitwise radix select of the FP64 bit patterns, ties by index.  The
+//      final k are taken by the exact select.  The result is the brute-force
+//      neighbour set bit for bit, because squared distances are evaluated in
+//      FP64 in one fixed order, box distances are, by monotone rounding,
+//      never larger than those of the records inside, and every cut keeps all
+//      entries at or below its bound.
 //  Queries are first sorted by the Morton key of their position on the map's
 //  grid, so neighbouring wavefronts walk the same nodes and buckets (L2 hits).
 //  contrib_kernel — one thread per (query, neighbour): energy multiplier,
@@ -309,6 +313,73 @@ __device__ double wave_keep_k(double* d, int32_t* idx, int n, int k, uint32_t* h
   return th;
 }
 
+// Cheap buffer cut: an upper bound of the k-th smallest d, not the k-th
+// itself.  The n <= kBuf entries (all <= hi) are counted into 64 buckets of
+// [0, hi] (LDS histogram); the bucket holding the k-th smallest bounds it.
+// Entries of the buckets up to that one are kept (the buckets are monotone
+// in d, so the kept set is exactly {d <= theta'} with theta' the largest kept
+// d) and theta' is returned.  Returns a negative value, changing nothing,
+// when the cut would not free 64 slots (the caller then cuts exactly).
+__device__ double wave_cut_hist(double* d, int32_t* idx, int n, int k, double hi, uint32_t* hist, uint32_t lane,
+                                int* n_out) {
+  if (!(hi > 0.0)) return -1.0;
+  constexpr int U = kBuf / 64;
+  hist[lane] = 0u;
+  wave_sync();
+  const double scale = 64.0 / hi;
+  int bk[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int t = (int)lane + 64 * u;
+    bk[u] = 64;
+    if (t < n) {
+      const double q = d[t] * scale;
+      bk[u] = q < 63.0 ? (int)q : 63;
+      atomicAdd(&hist[bk[u]], 1u);
+    }
+  }
+  wave_sync();
+  const uint32_t c = hist[lane];
+  const uint32_t cum = wave_incl_scan(c, lane);
+  const uint64_t reach = ballot(cum >= (uint32_t)k);
+  const int sel = __ffsll((long long)reach) - 1;  // n >= k, so some bucket reaches k
+  const int kept = (int)__builtin_amdgcn_readlane((int)cum, sel);
+  if (kept > kBuf - 64) return -1.0;
+  double mx = 0.0;
+  double vd[U];
+  int32_t vi[U];
+  bool keep[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int t = (int)lane + 64 * u;
+    keep[u] = bk[u] <= sel;
+    vd[u] = 0.0;
+    vi[u] = 0;
+    if (keep[u]) {
+      vd[u] = d[t];
+      vi[u] = idx[t];
+      mx = fmax(mx, vd[u]);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+  wave_sync();
+  int base = 0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const uint64_t m = ballot(keep[u]);
+    if (keep[u]) {
+      const int slot = base + (int)mask_rank(m);
+      d[slot] = vd[u];
+      idx[slot] = vi[u];
+    }
+    base += __popcll(m);
+  }
+  wave_sync();
+  *n_out = kept;
+  return mx;
+}
+
 // first index in [0, n) whose key is >= key (n if none); wave-cooperative
 __device__ int64_t wave_lower_bound(const uint64_t* __restrict__ keys, int64_t n, uint64_t key, uint32_t lane) {
   int64_t lo = 0, hi = n;
@@ -454,8 +525,19 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     }
     c_cand += (uint32_t)W;
     wave_sync();
-    double theta = wave_keep_k(bd, bi, W, k, hist, lane);
-    int buf_n = k;
+    // first bound: a histogram cut of the seed over [0, its largest distance]
+    double smax = 0.0;
+    for (int t = (int)lane; t < W; t += 64) smax = fmax(smax, bd[t]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) smax = fmax(smax, __shfl_xor(smax, off));
+    int buf_n = 0;
+    double theta = wave_cut_hist(bd, bi, W, k, smax, hist, lane, &buf_n);
+    bool theta_exact = false;  // theta is the k-th of the buffer (true after an exact cut)
+    if (theta < 0.0) {
+      theta = wave_keep_k(bd, bi, W, k, hist, lane);
+      buf_n = k;
+      theta_exact = true;
+    }
 
     // ---- 2. pruned traversal of the stacked subtrees
     while (sp > 0) {
@@ -571,8 +653,17 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
           bool take = tk[u];
           if (buf_n + __popcll(mt) > kBuf) {
             c_sorts++;
-            theta = wave_keep_k(bd, bi, buf_n, k, hist, lane);
-            buf_n = k;
+            int kept = 0;
+            const double th2 = wave_cut_hist(bd, bi, buf_n, k, theta, hist, lane, &kept);
+            if (th2 >= 0.0) {
+              theta = th2;
+              buf_n = kept;
+              theta_exact = false;
+            } else {
+              theta = wave_keep_k(bd, bi, buf_n, k, hist, lane);
+              buf_n = k;
+              theta_exact = true;
+            }
             take = take && d2v[u] <= theta;
           }
           const uint64_t mt2 = ballot(take);
@@ -589,7 +680,8 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
 
     // ---- 3. the k nearest: cut the buffer back to k; in (distance, index)
     // order when the caller asked for the neighbour sets
-    const double r2 = buf_n > k ? wave_keep_k(bd, bi, buf_n, k, hist, lane) : theta;
+    // exact k-th distance and set (histogram cuts only bound them)
+    const double r2 = (buf_n > k || !theta_exact) ? wave_keep_k(bd, bi, buf_n, k, hist, lane) : theta;
     if (a.direct) wave_sort(bd, bi, k, lane);
     for (int t = (int)lane; t < k; t += 64) a.nbr[qi * k + t] = bi[t];
     if (lane == 0) a.radius[qi] = sqrt(r2);
