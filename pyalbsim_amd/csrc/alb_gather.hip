@@ -412,7 +412,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   __shared__ uint32_t hist_s[kWaves][256];
 
   const uint32_t lane = lane_id();
-  const int w = threadIdx.x >> 6;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: scalar slot loop
   int32_t* sn = sn_s[w];
   double* sd = sd_s[w];
   double* bd = bd_s[w];

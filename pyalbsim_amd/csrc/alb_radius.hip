@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) void radius_kernel(RadiusArgs a) {
   __syncthreads();
 
   const uint32_t lane = lane_id();
-  const int w = threadIdx.x >> 6;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: scalar slot loop
   int32_t* sn = sn_s[w];
   int32_t* rlo = rlo_s[w];
   int32_t* rpre = rpre_s[w];
