@@ -35,9 +35,10 @@ per pass that recomputes every path from its Philox stream and reduces
 w * S per scene.  No path is stored.
 
 Depth moves the seafloor plane: the paths' support changes, which detached
-sampling does not cover.  Its derivative is taken by a central difference
-with common random numbers (two more batched forward passes), or left out
-(depth_grad="none").
+sampling does not cover.  Its derivative is taken by a difference with
+common random numbers: one-sided from the forward pass's own waveforms (one
+more batched forward pass, depth_grad="fd1", the default), central (two
+more, "fd"), or left out ("none").
 """
 from __future__ import annotations
 
@@ -226,6 +227,7 @@ class BatchedWaveformFn:
                 wl = spec.get("win_len", 1000)
                 out = torch.stack([res.gated(s, models[s].gate(wl)) for s in range(len(models))])
                 ctx.res = res
+                ctx.out = out.detach()  # the base of the one-sided depth difference
                 ctx.models = models
                 ctx.theta = th
                 ctx.spec = spec
@@ -246,23 +248,29 @@ class BatchedWaveformFn:
                 acc, direct = adjoint_sums(res, dloss)
                 derivs = [scene_derivatives(m, th[s]) for s, m in enumerate(models)]
                 grads = gradients_from_sums(acc, direct, derivs)
-                if spec.get("depth_grad", "fd") == "fd":
+                mode = spec.get("depth_grad", "fd1")
+                if mode == "fd":
                     grads[:, 3] = depth_fd(th, spec, go, spec.get("depth_rel", 1e-3))
+                elif mode == "fd1":
+                    grads[:, 3] = depth_fd(th, spec, go, spec.get("depth_rel", 1e-3), base=ctx.out)
+                ctx.out = None
                 ctx.res = None
                 return torch.as_tensor(grads, dtype=torch.float64, device=ctx.device), None
 
         return _Fn
 
 
-def depth_fd(th, spec, go, rel: float = 1e-3):
-    """d loss / d depth by central differences with common random numbers."""
+def depth_fd(th, spec, go, rel: float = 1e-3, base=None):
+    """d loss / d depth by differences with common random numbers: central
+    (two more forward passes), or one-sided from base, the gated waveforms
+    already computed at th with the same key (one more forward pass)."""
     import torch
 
     from .execution.batched import simulate_batched
 
     wl = spec.get("win_len", 1000)
     outs = []
-    for sign in (1.0, -1.0):
+    for sign in ((1.0,) if base is not None else (1.0, -1.0)):
         t2 = th.copy()
         t2[:, 3] = th[:, 3] * (1.0 + sign * rel)
         models = [model_from_theta(row, spec.get("sample_rate")) for row in t2]
@@ -271,6 +279,9 @@ def depth_fd(th, spec, go, rel: float = 1e-3):
                              block=spec.get("block", 256), refill=spec.get("refill", 4),
                              graphs=None if gr is None else gr.setdefault("depth_difference", {}))
         outs.append(torch.stack([r.gated(s, models[s].gate(wl)) for s in range(len(models))]))
+    if base is not None:
+        d = (outs[0] - base.to(outs[0].device)) * go
+        return (d.sum(dim=(1, 2)) / torch.as_tensor(rel * th[:, 3], device=d.device)).cpu().numpy()
     d = (outs[0] - outs[1]) * go
     return (d.sum(dim=(1, 2)) / torch.as_tensor(2.0 * rel * th[:, 3], device=d.device)).cpu().numpy()
 

@@ -173,6 +173,17 @@ def test_autograd_function_and_depth(gpu):
     assert np.allclose(g[:, [0, 1, 2, 4, 5]], ref[:, [0, 1, 2, 4, 5]], rtol=1e-9, atol=0)
     fd = _fd(THETAS[:2], 3, (21, 22), R, 1e-3)
     assert np.allclose(g[:, 3], fd, rtol=1e-9)
+    # default: one-sided difference from the forward pass's own waveforms
+    th1 = torch.tensor(THETAS[:2], dtype=torch.float64, requires_grad=True)
+    out1 = adjoint.batched_waveforms(th1, NF, NB, (21, 22), win_len=WL)
+    (out1 * torch.as_tensor(np.array(R), device=out1.device)).sum().backward()
+    t2 = THETAS[:2].copy()
+    t2[:, 3] *= 1 + 1e-3
+    ms = _models(t2)
+    r = simulate_batched(ms, NF, NB, (21, 22), win_len=WL, strict=False)
+    up = np.array([float(np.sum(r.gated(s, ms[s].gate(WL)).cpu().numpy() * R[s])) for s in range(2)])
+    base = np.array([float(np.sum(out1[s].detach().cpu().numpy() * R[s])) for s in range(2)])
+    assert np.allclose(th1.grad.cpu().numpy()[:, 3], (up - base) / (1e-3 * THETAS[:2, 3]), rtol=1e-9)
 
 
 def _replay_sums(res, weights_by_type, pass_):
