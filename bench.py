@@ -337,11 +337,16 @@ def bench_batched(n_scenes: int):
     nb = [x.n_backward for x in sc]
     paths = sum(nf) + sum(nb)
     key = (0xBA7C, 1)
+    from pyalbsim_amd.autotune import tune_batched
+
+    t0 = time.perf_counter()
+    block, refill, _ = tune_batched(models, nf, key)  # launch parameters for this sweep (cached)
+    t_tune = time.perf_counter() - t0
     simulate_batched(models[:8], nf[:8], nb[:8], key, strict=False)  # warm-up
     simulate(models[0], nf[0], nb[0], key, strict=False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = simulate_batched(models, nf, nb, key, strict=False)
+    res = simulate_batched(models, nf, nb, key, strict=False, block=block, refill=refill)
     torch.cuda.synchronize()
     tb = time.perf_counter() - t0
     counts = (res.map_counts, res.query_counts)
@@ -359,7 +364,7 @@ def bench_batched(n_scenes: int):
             "forward_photons": int(sum(nf)), "backward_photons": int(sum(nb)),
             "map_records": counts[0], "gather_queries": counts[1],
             "batched_paths_per_s": paths / tb, "plain_paths_per_s": paths / tp, "batched_s": tb, "plain_s": tp,
-            "speedup": tp / tb}
+            "speedup": tp / tb, "autotuned": {"block": block, "refill": refill, "tuning_s": t_tune}}
 
 
 def bench_adjoint(n_scenes: int):
