@@ -286,6 +286,21 @@ def depth_fd(th, spec, go, rel: float = 1e-3, base=None):
     return (d.sum(dim=(1, 2)) / torch.as_tensor(2.0 * rel * th[:, 3], device=d.device)).cpu().numpy()
 
 
+def gated_waveforms(theta, n_forward, n_backward, key, **spec):
+    """Gated waveforms [S, 4, win_len] of the scenes theta [S, 6] without
+    what the adjoint needs (no kept sinks or neighbour rows): for values only."""
+    import torch
+
+    from .execution.batched import simulate_batched
+
+    th = theta.detach().cpu().numpy().astype(np.float64) if hasattr(theta, "detach") else np.asarray(theta)
+    models = [model_from_theta(row, spec.get("sample_rate")) for row in th]
+    wl = spec.get("win_len", 1000)
+    res = simulate_batched(models, n_forward, n_backward, key, win_len=wl, strict=False,
+                           block=spec.get("block", 256), refill=spec.get("refill", 4))
+    return torch.stack([res.gated(s, models[s].gate(wl)) for s in range(len(models))])
+
+
 _FN = None
 
 

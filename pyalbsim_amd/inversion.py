@@ -94,7 +94,7 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
     the depth RMSE and loss of every iteration."""
     import torch
 
-    from .adjoint import batched_waveforms
+    from .adjoint import batched_waveforms, gated_waveforms
 
     S = theta_true.shape[0]
     dev = "cuda"
@@ -112,13 +112,11 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
         return torch.cat([batched_waveforms(theta_t[s:s + 1], n_forward, n_backward, k, **spec) for s in range(S)])
 
     with torch.no_grad():
-        tspec = dict(spec, graphs=None)
-        th_t = torch.as_tensor(theta_true, dtype=torch.float64, device=dev)
-        target = (batched_waveforms(th_t, n_forward, n_backward, target_key, **tspec) if form == "batched" else
-                  torch.cat([batched_waveforms(th_t[s:s + 1], n_forward, n_backward, target_key, **tspec)
+        target = (gated_waveforms(theta_true, n_forward, n_backward, target_key, **spec) if form == "batched" else
+                  torch.cat([gated_waveforms(theta_true[s:s + 1], n_forward, n_backward, target_key, **spec)
                              for s in range(S)]))
-    x = torch.tensor(np.full((S, len(UNKNOWN)), 0.5) if x0 is None else x0, dtype=torch.float64, device=dev,
-                     requires_grad=True)
+    x_init = np.full((S, len(UNKNOWN)), 0.5) if x0 is None else np.array(x0, dtype=np.float64)
+    x = torch.tensor(x_init, dtype=torch.float64, device=dev, requires_grad=True)
     opt = torch.optim.Adam([x], lr=lr)
     base = torch.as_tensor(theta_true, dtype=torch.float64, device=dev).clone()
     lo_t = torch.as_tensor(lo, device=dev)
