@@ -49,7 +49,7 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
              tags: bool = False, dscene: Optional[runtime.DeviceScene] = None, sync_timing: bool = False,
              strict: bool = True, events: Optional[dict] = None,
              kernel_events: Optional[dict] = None, scene_id: int = 0, heatmaps=None,
-             points: bool = False) -> SimulationResult:
+             points: bool = False, replay_forward=None, replay_backward=None) -> SimulationResult:
     """Forward trace -> map build -> backward trace -> gather, all on the GPU.
 
     events: optional dict; HIP events are recorded on the launch stream at the
@@ -65,6 +65,9 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
     (core/simulation.py:51-69, 197-214, 523-537) into res.heatmaps.
     points: keep the records' first water / seafloor interaction points in
     the maps (implied by heatmaps).
+    replay_forward / replay_backward: optional host-supplied uniforms, FP64
+    arrays [n_forward, L] / [n_backward, L] (photon slot i draws row i in
+    order) in place of the Philox streams of that pass.
     """
     import torch
 
@@ -91,14 +94,17 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
     mark("start")
     cf = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device="cuda")
     cb = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device="cuda")
+    rep_f = _device_replay(replay_forward, n_forward)
+    rep_b = _device_replay(replay_backward, n_backward)
     recs, rcounts = runtime.trace(ds, key, abi.PASS_FORWARD, n_forward, tags=tags, counters=cf, scene_id=scene_id,
-                                  points=points)
+                                  points=points, replay=rep_f)
     mark("forward")
     maps = [runtime.build_map(recs[ti], rcounts[ti], runtime.MAP_DIMS[ti]) for ti in range(abi.N_TYPES)]
     if not keep:
         del recs
     mark("build")
-    qs, qcounts = runtime.trace(ds, key, abi.PASS_BACKWARD, n_backward, tags=tags, counters=cb, scene_id=scene_id)
+    qs, qcounts = runtime.trace(ds, key, abi.PASS_BACKWARD, n_backward, tags=tags, counters=cb, scene_id=scene_id,
+                                replay=rep_b)
     mark("backward")
     wf = torch.zeros((abi.N_TYPES, model.n_bins), dtype=torch.float64, device="cuda")
     keep_alive = []
@@ -136,25 +142,52 @@ def photon_map_stats_table(counts) -> str:
     return tabulate(rows, headers=["Photon type", "Records", "Device MiB (32 B/record)"], tablefmt="github")
 
 
+def _device_replay(arr, n):
+    if arr is None:
+        return None
+    import torch
+
+    a = np.ascontiguousarray(arr, dtype=np.float64)
+    if a.ndim != 2 or a.shape[0] != n:
+        raise ValueError(f"replay uniforms must be [{n}, L]")
+    return torch.as_tensor(a, device="cuda")
+
+
 def run_linear(simulation_config: SimulationConfig, run_config: RunConfig,
                rng: Optional[np.random.Generator] = None, *, as_float32: bool = False,
-               window=None, verbose: bool = True) -> dict:
+               window=None, gate=None, rng_mode: str = "philox", replay_uniforms=None,
+               device_tensors: bool = False, verbose: bool = True) -> dict:
     """Full forward + backward simulation of one scene on the GPU.
 
     Same contract as the reference run_linear (execution/linear.py:77-117):
     returns {PhotonType: waveform}, consumes ``rng`` (a fresh secrets-seeded
-    generator when None).
+    generator when None).  Options beyond the reference's: gate (= window):
+    (first bin, length) of the LDS-privatised waveform window; rng_mode
+    "replay" with replay_uniforms = (forward [N_f, L], backward [N_b, L]) host
+    FP64 uniforms per photon slot instead of the Philox streams (rng is then
+    not consumed); device_tensors: return torch float64 device tensors.
     """
-    if rng is None:
-        rng = np.random.default_rng(secrets.randbits(128))
-    key = abi.key_from_generator(rng)
+    if rng_mode not in ("philox", "replay"):
+        raise ValueError("rng_mode must be 'philox' or 'replay'")
+    if rng_mode == "replay":
+        if replay_uniforms is None or len(replay_uniforms) != 2:
+            raise ValueError("rng_mode='replay' needs replay_uniforms = (forward, backward)")
+        key = (0, 0)
+    else:
+        if rng is None:
+            rng = np.random.default_rng(secrets.randbits(128))
+        key = abi.key_from_generator(rng)
+    window = gate if gate is not None else window
     model = build_scene(simulation_config)
     n_f = run_config.photons_per_batch_forward * run_config.batches_forward
     n_b = run_config.photons_per_batch_backward * run_config.batches_backward
     if verbose:
         print("Starting forwards pass")
     t0 = time.perf_counter()
-    res = simulate(model, n_f, n_b, key, window=window)
+    rf, rb = replay_uniforms if rng_mode == "replay" else (None, None)
+    res = simulate(model, n_f, n_b, key, window=window, replay_forward=rf, replay_backward=rb)
+    if device_tensors:
+        return {pt: res.waveforms[pt.abi_index] for pt in PhotonType}
     wf = res.waveforms.cpu().numpy()
     if verbose:
         print(f"Finished forwards and backwards pass: {n_f} + {n_b} photon paths in "

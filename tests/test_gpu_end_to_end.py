@@ -49,6 +49,37 @@ def test_run_linear_dropin_contract(gpu):
         assert np.allclose(wf2[t], wf[t].astype(np.float32), rtol=1e-5, atol=0)
 
 
+def test_run_linear_replay_mode_matches_oracle(gpu):
+    """rng_mode="replay": both passes draw host-supplied uniforms per photon
+    slot; the waveforms match the oracle fed the same rows (whole-path
+    tolerance), and device_tensors returns the same values on the GPU."""
+    from alb_sim.config.run import RunConfig
+    from alb_sim.execution.linear import run_linear
+
+    cfg = single_shot()
+    rc = RunConfig(photons_per_batch_forward=3000, batches_forward=1, photons_per_batch_backward=600,
+                   batches_backward=1)
+    rs = np.random.default_rng(99)
+    rf, rb = rs.random((3000, 256)), rs.random((600, 256))
+    wf = run_linear(cfg, rc, rng_mode="replay", replay_uniforms=(rf, rb), verbose=False)
+    ref = oracle.run(build_scene(cfg), 3000, 600, (0, 0), replay_fwd=rf, replay_bwd=rb)
+    for pt in PhotonType:
+        g, r = wf[pt], ref["waveforms"][pt.abi_index]
+        tot = float(np.sum(r))
+        if tot == 0:
+            assert float(np.sum(g)) == 0
+            continue
+        assert float(np.sum(g)) == pytest.approx(tot, rel=1e-6), pt
+        agree = np.abs(g - r) <= 1e-6 * np.maximum(np.abs(r), 1e-300)
+        assert np.sum(r[agree]) >= 0.99 * tot, pt
+    dev = run_linear(cfg, rc, rng_mode="replay", replay_uniforms=(rf, rb), device_tensors=True, verbose=False)
+    for pt in PhotonType:
+        assert dev[pt].is_cuda
+        assert np.allclose(dev[pt].cpu().numpy(), wf[pt], rtol=1e-9, atol=1e-12 * max(float(wf[pt].max()), 1e-300))
+    with pytest.raises(ValueError):
+        run_linear(cfg, rc, rng_mode="replay", verbose=False)
+
+
 @pytest.mark.parametrize("scene", ["single_shot", "turbid"])
 def test_whole_path_matches_oracle_same_streams(gpu, scene):
     model = build_scene(single_shot() if scene == "single_shot" else run_linear_script_scene())
