@@ -305,17 +305,19 @@ def bench_inversion(n_scenes: int, iters: int, n_forward: int = 20000):
     and seconds per iteration, batched form vs plain (one scene per launch)."""
     import numpy as np
 
-    from pyalbsim_amd.inversion import invert, inversion_scenes
+    from pyalbsim_amd.inversion import invert, inversion_scenes, inversion_tilts
 
     th = inversion_scenes(n_scenes, seed=3)
-    rb = invert(th, n_forward, n_forward // 4, iterations=iters, lr=0.02)
+    tilts = inversion_tilts(n_scenes, seed=3)
+    rb = invert(th, n_forward, n_forward // 4, iterations=iters, lr=0.02, tilts=tilts)
     print("inversion batched done", file=sys.stderr, flush=True)
-    rp = invert(th, n_forward, n_forward // 4, iterations=2, lr=0.02, form="plain")
+    rp = invert(th, n_forward, n_forward // 4, iterations=2, lr=0.02, form="plain", tilts=tilts)
     print("inversion plain done", file=sys.stderr, flush=True)
     sb = float(np.median(rb.seconds_per_iteration[1:] or rb.seconds_per_iteration))
     sp = float(np.median(rp.seconds_per_iteration))
     return {"workload": f"{n_scenes} scenes, depth U[2,40] m, b U[0.1,1.5] /m, albedo U[0.05,0.4] unknown "
-                        f"(a = 0.1 /m, g = 0.9 known; roughness from wind U[0,10] m/s; flat floor), "
+                        f"(a = 0.1 /m, g = 0.9 known; roughness from wind U[0,10] m/s; floor tilted U[0,15] deg, "
+                        f"azimuth U[0,360) deg, known), "
                         f"{n_forward} forward + {n_forward // 4} backward photons per scene per pass, "
                         f"Adam lr 0.02 on [0,1]-scaled unknowns, {iters} iterations",
             "depth_rmse_start_m": rb.depth_rmse[0], "depth_rmse_end_m": rb.depth_rmse[-1],

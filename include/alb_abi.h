@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define ALB_ABI_VERSION 7
+#define ALB_ABI_VERSION 8
 
 /* photon types, reference enum order (BOTTOM_REFLECTION = auto() = 1, ...) */
 enum alb_photon_type {
@@ -182,6 +182,17 @@ typedef struct alb_scene {
   unsigned int* heat_radius;     /* [bins * bins] counts                            */
   double* heat_water;            /* [bins * bins] energy                            */
   double* heat_floor;            /* [bins * bins] energy                            */
+  /* tilted floor (extension of the reference's flat floor y = y_sf): the
+   * plane floor_normal . p = floor_offset, water on the side where it is
+   * positive, with the frame (floor_t1, floor_normal, floor_t2) for the
+   * cosine-hemisphere reflection.  tilted = 1 needs the stepped water legs
+   * (n_layers >= 1); 0: the flat floor, these fields unused. */
+  int32_t tilted;
+  int32_t pad3;
+  double floor_normal[3];
+  double floor_offset;
+  double floor_t1[3];
+  double floor_t2[3];
 } alb_scene_t;
 
 typedef struct alb_rng {

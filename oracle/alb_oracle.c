@@ -375,9 +375,29 @@ static void orc_emit_query(const alb_queries_t* q, const double p[3], const doub
   if (q->tag) q->tag[i] = tag;
 }
 
+/* ------------------------------------------------------------ tilted floor
+ * Extension of the reference's flat floor y = y_sf: the plane
+ * floor_normal . p = floor_offset (water on the positive side); the floor
+ * test, the intersection fraction and the cosine hemisphere follow the flat
+ * forms with y replaced by the signed distance and +y by the normal. */
+static inline double orc_floor_dot(const alb_scene_t* sc, const double v[3]) {
+  return (sc->floor_normal[0] * v[0] + sc->floor_normal[1] * v[1]) + sc->floor_normal[2] * v[2];
+}
+
+static inline double orc_floor_dist(const alb_scene_t* sc, const double v[3]) {
+  return orc_floor_dot(sc, v) - sc->floor_offset;
+}
+
+/* BOTTOM multiplier (core/simulation.py:155-163): (albedo / pi) max(0, n . (-d_s)) */
+static inline double orc_bottom_mult(const alb_scene_t* sc, const double ds[3]) {
+  if (!sc->tilted) return (sc->floor_albedo / ORC_PI) * fmax(0.0, -ds[1]);
+  const double c = (sc->floor_normal[0] * -ds[0] + sc->floor_normal[1] * -ds[1]) + sc->floor_normal[2] * -ds[2];
+  return (sc->floor_albedo / ORC_PI) * fmax(0.0, c);
+}
+
 /* ------------------------------------------------------------ the stepper */
 
-enum { ST_AIR = 0, ST_ENTER = 1, ST_WATER = 2, ST_FLOOR = 3, ST_EXIT = 4 };
+enum { ST_AIR = 0, ST_ENTER = 1, ST_WATER = 2, ST_FLOOR = 3, ST_EXIT = 4, ST_DEAD = 5 };
 
 /* -log of the uniform rounded to float32, kept in float32
  * (src/alb_sim/core/simulation.py:370-371, 428-429) */
@@ -520,9 +540,19 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
     /* evaluate_state (core/simulation.py:220-244), later masks win */
     int state = ST_AIR;
     if (y > y_ws && yn <= y_ws) state = ST_ENTER;
-    if (y <= y_ws && yn > y_sf) state = ST_WATER;
-    if (y > y_sf && yn <= y_sf) state = ST_FLOOR;
+    if (!sc->tilted) {
+      if (y <= y_ws && yn > y_sf) state = ST_WATER;
+      if (y > y_sf && yn <= y_sf) state = ST_FLOOR;
+    } else {  /* tilted floor: the plane's signed distance, positive on the water side */
+      /* The plane bounds the water column only below the surface: a step
+       * in the air passes through it (where it rises above the surface), and
+       * a photon in the water on its far side that does not cross back is
+       * absorbed (it entered the water over dry ground). */
+      const double g0 = orc_floor_dist(sc, p), g1 = orc_floor_dist(sc, nx);
+      if (y <= y_ws) state = (g1 > 0.0) ? ST_WATER : (g0 > 0.0 ? ST_FLOOR : ST_DEAD);
+    }
     if (y <= y_ws && yn > y_ws) state = ST_EXIT;
+    if (state == ST_DEAD) break;
     const float t_evt = (float)s;
     const int64_t tag = (int64_t)(photon << 16) | (event_no & 0xFFFF);
 
@@ -559,6 +589,8 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
           const alb_layer_t* L = &sc->layers[(int)lk[0]];
           orc_scatter_direction_tab(L->phase_theta, L->phase_cdf, L->phase_n, d, u1, u2, nd);
           w0 = (float)lk[3];
+          /* phase-sampling score (one constant layer: its table is the scene's) */
+          if (sc->phase_score) g_path.gs += sc->phase_score[orc_phase_segment(u1, sc)];
         } else {
           orc_scatter_direction(sc, d, u1, u2, nd);
           if (sc->phase_score) g_path.gs += sc->phase_score[orc_phase_segment(u1, sc)];
@@ -646,7 +678,8 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
     } else { /* ST_FLOOR */
       /* handle_seafloor (core/simulation.py:504-570) */
       double sv[3] = {nx[0] - p[0], nx[1] - p[1], nx[2] - p[2]};
-      double f = (y_sf - p[1]) / sv[1];
+      double f = sc->tilted ? (sc->floor_offset - orc_floor_dot(sc, p)) / orc_floor_dot(sc, sv)
+                            : (y_sf - p[1]) / sv[1];
       if (f < 0.0) f = 0.0;
       if (f > 1.0) f = 1.0;
       double P[3];
@@ -664,6 +697,11 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
       double u1 = orc_uniform(&st);
       double u2 = orc_uniform(&st);
       orc_lambert(u1, u2, d);
+      if (sc->tilted) { /* the same hemisphere sample about the plane's normal */
+        const double l[3] = {d[0], d[1], d[2]};
+        for (int i = 0; i < 3; ++i)
+          d[i] = (sc->floor_t1[i] * l[0] + sc->floor_normal[i] * l[1]) + sc->floor_t2[i] * l[2];
+      }
       double len = orc_norm3(sv);
       double rem = (1.0 - f) * len;
       for (int i = 0; i < 3; ++i) p[i] = P[i] + d[i] * rem;
@@ -997,7 +1035,7 @@ int orc_gather(const alb_scene_t* sc, int32_t type, const float* x, const float*
     if (type == ALB_SCATTER) ksize = (4.0 / 3.0) * ORC_PI * pow(r, 3.0);
     else ksize = ORC_PI * (r * r);
     double knorm = 1.0 / ksize;
-    double m_bottom = (sc->floor_albedo / ORC_PI) * fmax(0.0, -ds[1]);
+    double m_bottom = orc_bottom_mult(sc, ds);
     if (nbr_out) memcpy(nbr_out + j * k, idx, sizeof(int32_t) * (size_t)k);
     if (radius_out) radius_out[j] = r;
     const orc_phase_tab tab = orc_gather_phase(sc, qy);
@@ -1027,7 +1065,7 @@ int orc_radius_gather(const alb_scene_t* sc, int32_t type, const float* x, const
   for (int64_t j = 0; j < nq; ++j) {
     const double qx = q->x[j], qy = q->y[j], qz = q->z[j];
     const double ds[3] = {q->dx[j], q->dy[j], q->dz[j]};
-    const double m_bottom = (sc->floor_albedo / ORC_PI) * fmax(0.0, -ds[1]);
+    const double m_bottom = orc_bottom_mult(sc, ds);
     const orc_phase_tab tab = orc_gather_phase(sc, qy);
     int64_t cnt = 0, isum = 0;
     for (int64_t i = 0; i < n; ++i) {

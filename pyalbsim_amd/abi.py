@@ -5,7 +5,7 @@ import ctypes as C
 
 import numpy as np
 
-ALB_ABI_VERSION = 7
+ALB_ABI_VERSION = 8
 N_TYPES = 4
 N_COUNTERS = 19
 
@@ -49,7 +49,9 @@ class Scene(C.Structure):
         ("heat_bins", C.c_int32), ("pad2", C.c_int32), ("heat_water_extent", C.c_double),
         ("heat_floor_extent", C.c_double), ("heat_water_center", C.c_double * 3),
         ("heat_floor_center", C.c_double * 3), ("heat_radius", C.c_void_p), ("heat_water", C.c_void_p),
-        ("heat_floor", C.c_void_p),
+        ("heat_floor", C.c_void_p), ("tilted", C.c_int32), ("pad3", C.c_int32),
+        ("floor_normal", C.c_double * 3), ("floor_offset", C.c_double), ("floor_t1", C.c_double * 3),
+        ("floor_t2", C.c_double * 3),
     ]
 
 
@@ -165,6 +167,18 @@ def make_scene(model, theta_ptr: int, cdf_ptr: int, mass_ptr: int = 0, score_ptr
     s.phase_mass = mass_ptr
     s.phase_score = score_ptr
     s.phase_dlog_mass = dlog_mass_ptr
+    s.tilted = 1 if getattr(model, "tilted", False) else 0
+    fn = getattr(model, "floor_normal", None)
+    fn = (0.0, 1.0, 0.0) if fn is None else fn
+    t1 = getattr(model, "floor_t1", None)
+    t1 = (1.0, 0.0, 0.0) if t1 is None else t1
+    t2 = getattr(model, "floor_t2", None)
+    t2 = (0.0, 0.0, 1.0) if t2 is None else t2
+    for i in range(3):
+        s.floor_normal[i] = float(fn[i])
+        s.floor_t1[i] = float(t1[i])
+        s.floor_t2[i] = float(t2[i])
+    s.floor_offset = float(getattr(model, "floor_offset", model.y_sf))
     return s
 
 

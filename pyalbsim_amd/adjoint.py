@@ -108,14 +108,22 @@ def scene_derivatives(model, theta) -> SceneDerivs:
     return SceneDerivs(model.alpha_l, a, albedo, rough, d)
 
 
-def model_from_theta(theta, sample_rate=None):
+def model_from_theta(theta, sample_rate=None, tilt=None):
     """SceneModel of the sweep geometry for theta = (a, b, g, depth, albedo,
-    roughness); sample_rate: receiver sample rate (default 1e10 /s)."""
+    roughness); sample_rate: receiver sample rate (default 1e10 /s); tilt:
+    optional (slope, azimuth) of the floor in degrees."""
     from .model import build_scene
 
     a, b, g, depth, albedo, rough = [float(v) for v in theta]
     sr = int(1e10) if sample_rate is None else int(sample_rate)
-    return build_scene(shot_config(depth, a, b, albedo, rough, sample_rate=sr), hg_g=g)
+    sl, az = (0.0, 0.0) if tilt is None else (float(tilt[0]), float(tilt[1]))
+    return build_scene(shot_config(depth, a, b, albedo, rough, sample_rate=sr, slope=sl, azimuth=az), hg_g=g)
+
+
+def _models(th, spec):
+    tilts = spec.get("tilts")
+    return [model_from_theta(row, spec.get("sample_rate"), None if tilts is None else tilts[i])
+            for i, row in enumerate(th)]
 
 
 def _replay_index(torch, sink, count, offsets, dev):
@@ -218,7 +226,7 @@ class BatchedWaveformFn:
                 from .execution.batched import simulate_batched
 
                 th = theta.detach().cpu().numpy().astype(np.float64)
-                models = [model_from_theta(row, spec.get("sample_rate")) for row in th]
+                models = _models(th, spec)
                 gr = spec.get("graphs")
                 res = simulate_batched(models, spec["n_forward"], spec["n_backward"], spec["key"],
                                        win_len=spec.get("win_len", 1000), strict=False, adjoint=True,
@@ -273,7 +281,7 @@ def depth_fd(th, spec, go, rel: float = 1e-3, base=None):
     for sign in ((1.0,) if base is not None else (1.0, -1.0)):
         t2 = th.copy()
         t2[:, 3] = th[:, 3] * (1.0 + sign * rel)
-        models = [model_from_theta(row, spec.get("sample_rate")) for row in t2]
+        models = _models(t2, spec)
         gr = spec.get("graphs")
         r = simulate_batched(models, spec["n_forward"], spec["n_backward"], spec["key"], win_len=wl, strict=False,
                              block=spec.get("block", 256), refill=spec.get("refill", 4),
@@ -294,7 +302,7 @@ def gated_waveforms(theta, n_forward, n_backward, key, **spec):
     from .execution.batched import simulate_batched
 
     th = theta.detach().cpu().numpy().astype(np.float64) if hasattr(theta, "detach") else np.asarray(theta)
-    models = [model_from_theta(row, spec.get("sample_rate")) for row in th]
+    models = _models(th, spec)
     wl = spec.get("win_len", 1000)
     res = simulate_batched(models, n_forward, n_backward, key, win_len=wl, strict=False,
                            block=spec.get("block", 256), refill=spec.get("refill", 4))

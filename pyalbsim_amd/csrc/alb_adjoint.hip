@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void adjoint_contrib_kernel(AdjointArgs a) {
     double mul, dl = 0.0;
     int ch = -1;
     if (a.type == ALB_BOTTOM_REFLECTION) {
-      mul = (S->floor_albedo / ALB_PI) * fmax(0.0, -sdy);
+      mul = bottom_mult(*S, sdx, sdy, sdz);
       ch = 2;
       dl = 1.0;  // the host divides by the albedo
     } else if (a.type == ALB_SCATTER) {

@@ -445,6 +445,14 @@ __device__ __forceinline__ double scatter_mult(float px, float py, float pz, dou
   return interp_uniform_grid(hi, theta, cdf, n) - interp_uniform_grid(lo, theta, cdf, n);
 }
 
+// BOTTOM multiplier (core/simulation.py:155-163): (albedo / pi) max(0, n . (-d_s))
+// with n = +y, or the tilted floor's normal
+__device__ __forceinline__ double bottom_mult(const alb_scene_t& sc, double sdx, double sdy, double sdz) {
+  if (!sc.tilted) return (sc.floor_albedo / ALB_PI) * fmax(0.0, -sdy);
+  const double c = (sc.floor_normal[0] * -sdx + sc.floor_normal[1] * -sdy) + sc.floor_normal[2] * -sdz;
+  return (sc.floor_albedo / ALB_PI) * fmax(0.0, c);
+}
+
 // ------------------------------------------------------------- heat maps
 // accumulate_to_heatmap (src/alb_sim/math/accumulate_heatmap.py:6-61) for one
 // point: bins of (x - centre + extent) / (2 extent / bins) along x and z,

@@ -745,7 +745,7 @@ __global__ __launch_bounds__(256) void contrib_kernel(ContribArgs a) {
     const double sdx = a.q.dx[qg], sdy = a.q.dy[qg], sdz = a.q.dz[qg];
     double mul;
     if (a.type == ALB_BOTTOM_REFLECTION) {
-      mul = (a.sc.floor_albedo / ALB_PI) * fmax(0.0, -sdy);
+      mul = bottom_mult(a.sc, sdx, sdy, sdz);
     } else if (a.type == ALB_SCATTER) {
       int pn;
       const double* pm = gather_phase_mass(a.sc, a.q.y[qg], &pn);
@@ -847,7 +847,7 @@ __global__ __launch_bounds__(256) void contrib_batched_kernel(ContribBatchedArgs
     const double sdx = a.q.dx[qg], sdy = a.q.dy[qg], sdz = a.q.dz[qg];
     double mul;
     if (a.type == ALB_BOTTOM_REFLECTION) {
-      mul = (S->floor_albedo / ALB_PI) * fmax(0.0, -sdy);
+      mul = bottom_mult(*S, sdx, sdy, sdz);
     } else if (a.type == ALB_SCATTER) {
       int pn;
       const double* pm = gather_phase_mass(*S, a.q.y[qg], &pn);

@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void radius_kernel(RadiusArgs a) {
     const double sdx = a.q.dx[qg], sdy = a.q.dy[qg], sdz = a.q.dz[qg];
     const float qe = a.q.energy[qg], qt = a.q.time[qg];
     // multiplier parts fixed by the query
-    const double m_bottom = (a.sc.floor_albedo / ALB_PI) * fmax(0.0, -sdy);
+    const double m_bottom = bottom_mult(a.sc, sdx, sdy, sdz);
     int pn = 0;
     const double* pm = TYPE == ALB_SCATTER ? gather_phase_mass(a.sc, qy, &pn) : nullptr;
     c_q++;

@@ -324,7 +324,17 @@ __device__ __forceinline__ void emit_photon(const alb_scene_t& sc, bool fwd, con
   }
 }
 
-// Layered or depth-profiled water column: sub-steps are taken one at a time,
+// tilted floor: n . p (terms summed x, y, z) and the signed distance n . p - c
+__device__ __forceinline__ double floor_dot(const alb_scene_t& sc, double x, double y, double z) {
+  return (sc.floor_normal[0] * x + sc.floor_normal[1] * y) + sc.floor_normal[2] * z;
+}
+
+__device__ __forceinline__ double floor_dist(const alb_scene_t& sc, double x, double y, double z) {
+  return floor_dot(sc, x, y, z) - sc.floor_offset;
+}
+
+// Layered or depth-profiled water column, or a tilted floor: sub-steps are
+// taken one at a time,
 // as Simulation.simulate_photon_step does (core/simulation.py:246-322): the
 // speed and the attenuation come from the layer of the pre-step position
 // (physics/models/simulation.py:214-265), the phase table and the
@@ -352,10 +362,18 @@ __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, c
     const double zl = dep - L.top;
     const double v = layer_velocity(L, zl);
     const double nx = ph.px + (ph.dx * v) * sc.dt, ny = ph.py + (ph.dy * v) * sc.dt, nz = ph.pz + (ph.dz * v) * sc.dt;
-    // evaluate_state (core/simulation.py:220-244) for a position at or below the surface
+    // evaluate_state (core/simulation.py:220-244) for a position at or below
+    // the surface; a tilted floor compares the plane's signed distance
+    // (positive on the water side) where the flat floor compares y with y_sf
     int state = 0;
-    if (ny > y_sf) state = 1;                // in water
-    if (y > y_sf && ny <= y_sf) state = 2;   // hitting the floor
+    if (!sc.tilted) {
+      if (ny > y_sf) state = 1;                // in water
+      if (y > y_sf && ny <= y_sf) state = 2;   // hitting the floor
+    } else {
+      const double g0 = floor_dist(sc, ph.px, y, ph.pz), g1 = floor_dist(sc, nx, ny, nz);
+      if (g1 > 0.0) state = 1;
+      if (g0 > 0.0 && g1 <= 0.0) state = 2;
+    }
     if (ny > y_ws) state = 3;                // leaving the water
     if (state == 0) {
       ph.alive = false;  // below the floor and not rising: no event can follow
@@ -388,7 +406,14 @@ __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, c
       const alb_layer_t& L2 = Ls[layer_of(Ls, nl, dep2)];
       const double zl2 = dep2 - L2.top;
       const double u = uniform(rng, st);
-      const double theta = interp(u, L2.phase_cdf, L2.phase_theta, L2.phase_n);
+      double theta;
+      if constexpr (STATS) {  // phase-sampling score of the adjoint (one constant layer: the scene's table)
+        int seg;
+        theta = interp_seg(u, L2.phase_cdf, L2.phase_theta, L2.phase_n, &seg);
+        if (sc.phase_score) ph.gs += sc.phase_score[seg];
+      } else {
+        theta = interp(u, L2.phase_cdf, L2.phase_theta, L2.phase_n);
+      }
       const double phi = two_pi * uniform(rng, st);
       double sth, cth, sph, cph;
       sincos(theta, &sth, &cth);
@@ -413,7 +438,8 @@ __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, c
     if (state == 2) {
       // handle_seafloor (core/simulation.py:504-570)
       c.fl++;
-      double f = (y_sf - ph.py) / svy;
+      double f = sc.tilted ? (sc.floor_offset - floor_dot(sc, ph.px, ph.py, ph.pz)) / floor_dot(sc, svx, svy, svz)
+                           : (y_sf - ph.py) / svy;
       f = clipd(f, 0.0, 1.0);
       const double Px = ph.px + f * svx, Py = ph.py + f * svy, Pz = ph.pz + f * svz;
       if constexpr (HEAT) {  // seafloor interaction, set before the record (core/simulation.py:521)
@@ -433,9 +459,16 @@ __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, c
       const double sr = sqrt(u1);
       double sp, cp;
       sincos(two_pi * u2, &sp, &cp);
-      ph.dx = sr * cp;
-      ph.dy = sqrt(1.0 - u1);
-      ph.dz = sr * sp;
+      if (!sc.tilted) {
+        ph.dx = sr * cp;
+        ph.dy = sqrt(1.0 - u1);
+        ph.dz = sr * sp;
+      } else {  // the same hemisphere sample about the plane's normal
+        const double lx = sr * cp, ly = sqrt(1.0 - u1), lz = sr * sp;
+        ph.dx = (sc.floor_t1[0] * lx + sc.floor_normal[0] * ly) + sc.floor_t2[0] * lz;
+        ph.dy = (sc.floor_t1[1] * lx + sc.floor_normal[1] * ly) + sc.floor_t2[1] * lz;
+        ph.dz = (sc.floor_t1[2] * lx + sc.floor_normal[2] * ly) + sc.floor_t2[2] * lz;
+      }
       const double rem = (1.0 - f) * slen;
       ph.px = Px + ph.dx * rem;
       ph.py = Py + ph.dy * rem;
@@ -549,7 +582,7 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
     ph.pz = Pz + ph.dz * rem;
     ph.tau_t = tau_target(uniform(rng, st));
     ph.s = ph.s + j + 1;
-  } else if (ph.py <= y_sf && ph.dy <= 0.0) {
+  } else if (!(LAYERED && sc.tilted) && ph.py <= y_sf && ph.dy <= 0.0) {
     ph.alive = false;  // below the floor and not rising: the reference finds no event
     return;
   } else if (LAYERED && sc.layers) {
@@ -943,7 +976,7 @@ __global__ __launch_bounds__(256) void replay_batched_kernel(const alb_scene_t* 
     if (!ballot(ph.alive || !done)) break;
     if (ph.alive) {
       Emit e{-1, 0, 0, 0, 0, 0, 0, 0.0f, 0.0f, 0u, 0.0, 0.0, 0, 0};
-      next_event<true, false>(scenes[sid], FWD, lrng, st, ph, e, c);
+      next_event<true, true>(scenes[sid], FWD, lrng, st, ph, e, c);  // stepped legs for tilted floors
       if (e.et >= 0) {
         const uint64_t key = ((uint64_t)g << 16) | (uint64_t)(e.eno & 0xFFFFu);
 #pragma unroll

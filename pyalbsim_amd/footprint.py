@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .model import SceneModel, eval_profile, LIGHT_SPEED_AIR
+from .model import LIGHT_SPEED_AIR, SceneModel, beam_floor_point, eval_profile
 
 
 def water_surface_intersection(model: SceneModel) -> np.ndarray:
@@ -26,16 +26,7 @@ def water_surface_intersection(model: SceneModel) -> np.ndarray:
 
 def seafloor_intersection(model: SceneModel) -> np.ndarray:
     """Point where the laser boresight, refracted at the surface, meets the floor."""
-    d = np.asarray(model.laser_dir, dtype=np.float64)
-    eta = 1.0 / model.n_water
-    normal = np.array([0.0, 1.0, 0.0])
-    cos_i = float(np.clip(np.dot(-d, normal), -1.0, 1.0))
-    sin2_t = eta ** 2 * (1.0 - cos_i ** 2)
-    cos_t = np.sqrt(max(0.0, 1.0 - sin2_t))
-    t = eta * d + (eta * cos_i - cos_t) * normal
-    t = t / np.linalg.norm(t)
-    surface = (model.y_ws / d[1]) * d
-    return surface + (-model.depth / t[1]) * t
+    return beam_floor_point(model.laser_dir, model.y_ws, model.depth, model.n_water)
 
 
 def _water_speed(model: SceneModel) -> float:

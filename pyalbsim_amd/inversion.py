@@ -22,10 +22,14 @@ not overlap yet), plus the squared difference of the log totals (the
 amplitude).  Depth is identifiable only where the floor return survives the
 water column; in deep turbid scenes the estimate drifts.
 
+The floor may be tilted (slope U[0, 15] degrees, azimuth U[0, 360), known
+per scene: inversion_tilts); tilted scenes are traced with the stepped water
+legs and the tilted-floor handlers, an extension of the reference's flat
+floor.
+
 form="batched" runs all scenes per launch; form="plain" runs one scene per
 launch with the same code path, for the plain-vs-batched time comparison.
-The reference has no inversion, and its seafloor is flat: the tilted floor
-of the extended workload is not modelled.
+The reference has no inversion.
 """
 from __future__ import annotations
 
@@ -61,6 +65,14 @@ def inversion_scenes(n: int, seed: int = 3) -> np.ndarray:
     return th
 
 
+def inversion_tilts(n: int, seed: int = 3) -> np.ndarray:
+    """Known floor tilts [n, 2] = (slope U[0, 15], azimuth U[0, 360)) in degrees."""
+    from scipy.stats import qmc
+
+    u = qmc.LatinHypercube(d=2, seed=seed + 1000).random(n)
+    return np.stack([15.0 * u[:, 0], 360.0 * u[:, 1]], 1)
+
+
 @dataclass
 class InversionResult:
     theta_true: np.ndarray
@@ -89,7 +101,8 @@ def waveform_loss(w, target, shape_weight: float = 1000.0):
 
 def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: int = 200, lr: float = 1e-2,
            key=(0x1A7E, 5), target_key=(0x7A26, 6), form: str = "batched", win_len: int = 1000,
-           depth_rel: float = 1e-2, x0=None, log=None, shape_weight: float = 1000.0) -> InversionResult:
+           depth_rel: float = 1e-2, x0=None, log=None, shape_weight: float = 1000.0,
+           tilts=None) -> InversionResult:
     """Adam on (depth, b, albedo) scaled to [0, 1]; returns the estimates and
     the depth RMSE and loss of every iteration."""
     import torch
@@ -101,19 +114,22 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
     # batched form: the scene table is updated in place and the trace passes
     # replay HIP graphs captured in the first iteration
     spec = dict(win_len=win_len, sample_rate=SAMPLE_RATE, depth_rel=depth_rel,
-                graphs={} if form == "batched" else None)
+                graphs={} if form == "batched" else None, tilts=None if tilts is None else np.asarray(tilts))
     lo = np.array([RANGES[c][0] for c in UNKNOWN])
     hi = np.array([RANGES[c][1] for c in UNKNOWN])
     idx = [COLS[c] for c in UNKNOWN]
 
+    def one(s):  # spec of scene s alone (plain form)
+        return dict(spec, tilts=None if tilts is None else spec["tilts"][s:s + 1])
+
     def run(theta_t, k):
         if form == "batched":
             return batched_waveforms(theta_t, n_forward, n_backward, k, **spec)
-        return torch.cat([batched_waveforms(theta_t[s:s + 1], n_forward, n_backward, k, **spec) for s in range(S)])
+        return torch.cat([batched_waveforms(theta_t[s:s + 1], n_forward, n_backward, k, **one(s)) for s in range(S)])
 
     with torch.no_grad():
         target = (gated_waveforms(theta_true, n_forward, n_backward, target_key, **spec) if form == "batched" else
-                  torch.cat([gated_waveforms(theta_true[s:s + 1], n_forward, n_backward, target_key, **spec)
+                  torch.cat([gated_waveforms(theta_true[s:s + 1], n_forward, n_backward, target_key, **one(s))
                              for s in range(S)]))
     x_init = np.full((S, len(UNKNOWN)), 0.5) if x0 is None else np.array(x0, dtype=np.float64)
     x = torch.tensor(x_init, dtype=torch.float64, device=dev, requires_grad=True)

@@ -244,6 +244,12 @@ class SceneModel:
     backscatter: float
     phase_g: float | None = None  # Henyey-Greenstein asymmetry when the table is HG
     layers: list = None           # LayerModel per layer when the column is layered or depth-profiled
+    # tilted floor (extension): plane floor_normal . p = floor_offset through the beam's floor point
+    tilted: bool = False
+    floor_normal: np.ndarray = None
+    floor_offset: float = None
+    floor_t1: np.ndarray = None
+    floor_t2: np.ndarray = None
 
     @property
     def n_substeps(self) -> int:
@@ -326,6 +332,35 @@ def layer_arrays(model: "SceneModel") -> list:
     return out
 
 
+def beam_floor_point(d_l: np.ndarray, y_ws: float, depth: float, n: float) -> np.ndarray:
+    """Point where the laser boresight, refracted at the surface with eta = 1/n
+    (physics/reflection/snell_refraction.py:7-45), meets the flat floor
+    (SimulationModel.sea_floor_intersection, physics/models/simulation.py:104-124)."""
+    d = np.asarray(d_l, dtype=np.float64)
+    eta = 1.0 / n
+    normal = np.array([0.0, 1.0, 0.0])
+    cos_i = float(np.clip(np.dot(-d, normal), -1.0, 1.0))
+    sin2_t = eta ** 2 * (1.0 - cos_i ** 2)
+    cos_t = np.sqrt(max(0.0, 1.0 - sin2_t))
+    t = eta * d + (eta * cos_i - cos_t) * normal
+    t = t / np.linalg.norm(t)
+    surface = (y_ws / d[1]) * d
+    return surface + (-depth / t[1]) * t
+
+
+def floor_plane(slope_deg: float, azimuth_deg: float, point: np.ndarray):
+    """(normal, offset, t1, t2) of a floor plane through point, tilted by
+    slope about the horizontal, its normal leaning toward azimuth in the x-z
+    plane; (t1, normal, t2) is orthonormal and equals the axes for slope 0,
+    azimuth 0."""
+    s, a = math.radians(slope_deg), math.radians(azimuth_deg)
+    n = np.array([math.sin(s) * math.cos(a), math.cos(s), math.sin(s) * math.sin(a)])
+    t1 = np.array([math.cos(s) * math.cos(a), -math.sin(s), math.cos(s) * math.sin(a)])
+    t2 = np.array([-math.sin(a), 0.0, math.cos(a)])
+    off = float(n[0] * point[0] + n[1] * point[1] + n[2] * point[2])
+    return n, off, t1, t2
+
+
 def build_scene(cfg: SimulationConfig, hg_g: float | None = None) -> SceneModel:
     """Scene constants of cfg.  hg_g: use a Henyey-Greenstein phase function
     with asymmetry hg_g instead of the layers' Fournier-Forand ones (its
@@ -340,7 +375,10 @@ def build_scene(cfg: SimulationConfig, hg_g: float | None = None) -> SceneModel:
     depth = float(cum[-1])
     tops32 = np.concatenate(([0.0], cum[:-1])).astype(np.float32)
     lmodels = [_layer_model(layer, tops32[i], cum[i], cfg, hg_g) for i, layer in enumerate(layers)]
-    layered = len(layers) > 1 or lmodels[0].profiled
+    slope = float(getattr(cfg.sea_floor, "slope", 0.0))
+    tilted = slope > 0.0
+    # a tilted floor is traced with the stepped water legs (per-sub-step floor test)
+    layered = len(layers) > 1 or lmodels[0].profiled or tilted
     layer = layers[0]
     l0 = lmodels[0]
     H = cfg.scene.flying_height
@@ -368,4 +406,13 @@ def build_scene(cfg: SimulationConfig, hg_g: float | None = None) -> SceneModel:
         steps=steps, mult=int(mult), k=int(cfg.photon_mapping_k), phase_theta=l0.phase_theta, phase_pf=l0.phase_pf,
         phase_cdf=l0.phase_cdf, backscatter=l0.backscatter, phase_g=None if hg_g is None else float(hg_g),
         layers=lmodels if layered else None,
+        **_floor_fields(tilted, slope, float(getattr(cfg.sea_floor, "azimuth", 0.0)), d_l, y_ws, y_sf, depth, n),
     )
+
+
+def _floor_fields(tilted, slope, azimuth, d_l, y_ws, y_sf, depth, n) -> dict:
+    if not tilted:
+        return dict(tilted=False, floor_normal=np.array([0.0, 1.0, 0.0]), floor_offset=float(y_sf),
+                    floor_t1=np.array([1.0, 0.0, 0.0]), floor_t2=np.array([0.0, 0.0, 1.0]))
+    nrm, off, t1, t2 = floor_plane(slope, azimuth, beam_floor_point(d_l, y_ws, depth, n))
+    return dict(tilted=True, floor_normal=nrm, floor_offset=off, floor_t1=t1, floor_t2=t2)

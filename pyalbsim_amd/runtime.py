@@ -499,7 +499,7 @@ class DeviceSceneTable:
             th, cd, ms, sc, dm = self.tables[h]
             structs[i] = abi.make_scene(m, _p(th), _p(cd), _p(ms), _p(sc), _p(dm))
             if m.layers:
-                if self.adjoint:
+                if self.adjoint and (len(m.layers) > 1 or m.layers[0].profiled):
                     raise NotImplementedError("the adjoint covers single constant-layer scenes only")
                 if id(m) not in self.layer_tables:
                     self.layer_tables[id(m)] = (m, _upload_layers(m, device))

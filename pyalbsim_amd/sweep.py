@@ -35,7 +35,7 @@ RANGES = {
 
 
 def shot_config(depth: float, a: float, b: float, albedo: float, roughness: float,
-                sample_rate: int = int(1e10)) -> SimulationConfig:
+                sample_rate: int = int(1e10), slope: float = 0.0, azimuth: float = 0.0) -> SimulationConfig:
     return SimulationConfig(
         scene=SceneConfig(flying_height=400.0),
         laser=LaserConfig(nadir_angle=0.0, divergence_angle=1e-3, wavelength=0.532),
@@ -43,7 +43,8 @@ def shot_config(depth: float, a: float, b: float, albedo: float, roughness: floa
         sea_surface=SeaSurfaceConfig(roughness=roughness),
         water=WaterConfig(layers=(TurbidityLayerConfig(height=depth, absorption_coefficient=a,
                                                        scattering_coefficient=b, refractive_index=1.34),)),
-        sea_floor=SeaFloorConfig(albedo=albedo), sample_multiplier=10, photon_mapping_k=100,
+        sea_floor=SeaFloorConfig(albedo=albedo, slope=slope, azimuth=azimuth), sample_multiplier=10,
+        photon_mapping_k=100,
         heatmap=HeatmapConfig(enabled=False))
 
 

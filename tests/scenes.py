@@ -66,3 +66,12 @@ def stacked_copies(cfg, n: int):
     (layer,) = cfg.water.layers
     part = dataclasses.replace(layer, height=layer.height / n)
     return dataclasses.replace(cfg, water=WaterConfig(layers=(part,) * n))
+
+
+def tilted_shot(slope=10.0, azimuth=40.0, **kw):
+    """single_shot with the floor tilted by slope degrees toward azimuth (extension)."""
+    import dataclasses
+
+    cfg = single_shot(**kw)
+    return dataclasses.replace(cfg, sea_floor=SeaFloorConfig(albedo=cfg.sea_floor.albedo, slope=slope,
+                                                             azimuth=azimuth))

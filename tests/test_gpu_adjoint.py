@@ -221,17 +221,19 @@ def _replay_sums(res, weights_by_type, pass_):
     return acc.view(tab.n, 5).cpu().numpy()
 
 
+@pytest.mark.parametrize("tilt", [None, (9.0, 120.0)])
 @pytest.mark.parametrize("pass_", [0, 1])
-def test_replay_statistics_match_time_stepped_oracle(gpu, pass_):
+def test_replay_statistics_match_time_stepped_oracle(gpu, pass_, tilt):
     """The replay trace's per-record path statistics (water length, scatter and
     floor counts, phase-sampling score), weighted by random per-record
     weights, equal the time-stepped CPU oracle's, record by record (matched by
-    (photon, event) tag; photons with a one-sub-step rounding flip excluded)."""
+    (photon, event) tag; photons with a one-sub-step rounding flip excluded),
+    with the flat floor (closed-form legs) and a tilted one (stepped legs)."""
     import oracle
     from pyalbsim_amd import abi
 
     thetas = THETAS[:2]
-    models = _models(thetas)
+    models = [adjoint.model_from_theta(t, None, tilt) for t in thetas]  # tilt: stepped legs, tilted floor
     nf, nb = [1500, 1200], [600, 500]
     key = (4321, 1234)
     res = simulate_batched(models, nf, nb, key, win_len=WL, strict=False, adjoint=True)
