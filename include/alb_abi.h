@@ -185,8 +185,9 @@ typedef struct alb_scene {
   /* tilted floor (extension of the reference's flat floor y = y_sf): the
    * plane floor_normal . p = floor_offset, water on the side where it is
    * positive, with the frame (floor_t1, floor_normal, floor_t2) for the
-   * cosine-hemisphere reflection.  tilted = 1 needs the stepped water legs
-   * (n_layers >= 1); 0: the flat floor, these fields unused. */
+   * cosine-hemisphere reflection.  1: a tilted floor (single-layer scenes keep
+   * closed-form water legs, layered ones the stepped legs); 0: the flat
+   * floor, these fields unused. */
   int32_t tilted;
   int32_t pad3;
   double floor_normal[3];

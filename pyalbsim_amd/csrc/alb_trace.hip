@@ -333,6 +333,25 @@ __device__ __forceinline__ double floor_dist(const alb_scene_t& sc, double x, do
   return floor_dot(sc, x, y, z) - sc.floor_offset;
 }
 
+// first j >= 0 whose sub-step ends on the floor side of a tilted plane,
+// floor_dist((p + j*D) + D) <= 0, with g0 = floor_dist(p) and gD = n . D < 0:
+// the signed distance falls linearly with j, so the estimate -g0/gD is
+// corrected on the positions the leg itself computes (cap if none)
+__device__ __forceinline__ int64_t first_step_through_plane(const alb_scene_t& sc, double px, double py, double pz,
+                                                            double Dx, double Dy, double Dz, double g0, double gD,
+                                                            int64_t cap) {
+  const double x = -g0 / gD;
+  if (!(x < (double)cap + 2.0)) return cap;
+  auto ends_below = [&](int64_t j) {
+    const double a = (double)j;
+    return floor_dist(sc, (px + a * Dx) + Dx, (py + a * Dy) + Dy, (pz + a * Dz) + Dz) <= 0.0;
+  };
+  int64_t j = x > 1.0 ? (int64_t)ceil(x) - 1 : 0;
+  while (j > 0 && ends_below(j - 1)) --j;
+  while (j < cap && !ends_below(j)) ++j;
+  return j;
+}
+
 // Layered or depth-profiled water column, or a tilted floor: sub-steps are
 // taken one at a time,
 // as Simulation.simulate_photon_step does (core/simulation.py:246-322): the
@@ -528,8 +547,9 @@ __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, c
 // event leaves a record (forward) or a query (backward).  STATS: also keep the
 // path statistics the adjoint needs (water length, scatter and floor counts,
 // phase-sampling score) and snapshot them into e; the physics is unchanged.
-// LAYERED: the scene may be a layered column (checked at run time); without
-// it the kernel carries only the closed-form water legs.
+// LAYERED: the scene may be a layered column or have a tilted floor (checked
+// at run time); without it the kernel carries only the flat floor's
+// closed-form water legs.
 template <bool STATS, bool LAYERED = true, bool HEAT = false>
 __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, const alb_rng_t& rng, Stream& st,
                                            Photon& ph, Emit& e, Tally& c) {
@@ -594,9 +614,30 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
                  Dz = (ph.dz * sc.v_water) * sc.dt;
     const int64_t avail = S - ph.s;
     int64_t jb;
-    if (Dy < 0.0) jb = first_step_below(ph.py, Dy, y_sf, avail);
-    else if (Dy > 0.0) jb = first_step_above(ph.py, Dy, y_ws, avail);
-    else jb = avail;
+    bool to_floor;
+    if (LAYERED && sc.tilted) {
+      // A single constant layer over a tilted plane: the plane's signed
+      // distance is linear in the sub-step index, so the leg keeps its closed
+      // form, with the per-sub-step classification of the stepped legs (in
+      // water while the step ends on the water side, the floor when it
+      // crosses to the far side, the surface when it rises above it, which
+      // wins a tie).
+      const double g0 = floor_dist(sc, ph.px, ph.py, ph.pz);
+      if (!(g0 > 0.0) && !(ph.py + Dy > y_ws) && !(floor_dist(sc, ph.px + Dx, ph.py + Dy, ph.pz + Dz) > 0.0)) {
+        ph.alive = false;  // on the far side of the plane (entered over dry ground) and not crossing back
+        return;
+      }
+      const double gD = floor_dot(sc, Dx, Dy, Dz);
+      const int64_t jf = gD < 0.0 ? first_step_through_plane(sc, ph.px, ph.py, ph.pz, Dx, Dy, Dz, g0, gD, avail) : avail;
+      const int64_t je = Dy > 0.0 ? first_step_above(ph.py, Dy, y_ws, avail) : avail;
+      to_floor = jf < je;
+      jb = to_floor ? jf : je;
+    } else {
+      if (Dy < 0.0) jb = first_step_below(ph.py, Dy, y_sf, avail);
+      else if (Dy > 0.0) jb = first_step_above(ph.py, Dy, y_ws, avail);
+      else jb = avail;
+      to_floor = Dy < 0.0;
+    }
     const int64_t jmax = jb < avail ? jb : avail;
     int64_t j = 0;
     bool scattered = false;
@@ -659,10 +700,11 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
       const double svx = nx - qx, svy = ny - qy, svz = nz - qz;
       const double slen = norm3(svx, svy, svz);
       if constexpr (STATS) ph.lam += (double)jb * len;
-      if (Dy < 0.0) {
+      if (to_floor) {
         // handle_seafloor (core/simulation.py:504-570)
         c.fl++;
-        double f = (y_sf - qy) / svy;
+        double f = (LAYERED && sc.tilted) ? (sc.floor_offset - floor_dot(sc, qx, qy, qz)) / floor_dot(sc, svx, svy, svz)
+                                          : (y_sf - qy) / svy;
         f = clipd(f, 0.0, 1.0);
         const double Px = qx + f * svx, Py = qy + f * svy, Pz = qz + f * svz;
         if constexpr (HEAT) {  // seafloor interaction, set before the record (core/simulation.py:521)
@@ -686,9 +728,16 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
         const double sr = sqrt(u1);
         double sp, cp;
         sincos(two_pi * u2, &sp, &cp);
-        ph.dx = sr * cp;
-        ph.dy = sqrt(1.0 - u1);
-        ph.dz = sr * sp;
+        if (!(LAYERED && sc.tilted)) {
+          ph.dx = sr * cp;
+          ph.dy = sqrt(1.0 - u1);
+          ph.dz = sr * sp;
+        } else {  // the same hemisphere sample about the plane's normal
+          const double lx = sr * cp, ly = sqrt(1.0 - u1), lz = sr * sp;
+          ph.dx = (sc.floor_t1[0] * lx + sc.floor_normal[0] * ly) + sc.floor_t2[0] * lz;
+          ph.dy = (sc.floor_t1[1] * lx + sc.floor_normal[1] * ly) + sc.floor_t2[1] * lz;
+          ph.dz = (sc.floor_t1[2] * lx + sc.floor_normal[2] * ly) + sc.floor_t2[2] * lz;
+        }
         const double rem = (1.0 - f) * slen;
         ph.px = Px + ph.dx * rem;
         ph.py = Py + ph.dy * rem;
@@ -976,7 +1025,7 @@ __global__ __launch_bounds__(256) void replay_batched_kernel(const alb_scene_t* 
     if (!ballot(ph.alive || !done)) break;
     if (ph.alive) {
       Emit e{-1, 0, 0, 0, 0, 0, 0, 0.0f, 0.0f, 0u, 0.0, 0.0, 0, 0};
-      next_event<true, true>(scenes[sid], FWD, lrng, st, ph, e, c);  // stepped legs for tilted floors
+      next_event<true, true>(scenes[sid], FWD, lrng, st, ph, e, c);  // tilted floors too
       if (e.et >= 0) {
         const uint64_t key = ((uint64_t)g << 16) | (uint64_t)(e.eno & 0xFFFFu);
 #pragma unroll
@@ -1050,7 +1099,7 @@ extern "C" int alb_trace(const alb_scene_t* scene, const alb_rng_t* rng, int32_t
   const int block = 256;
   const int64_t grid = (n + block - 1) / block;
   hipStream_t s = (hipStream_t)stream;
-  const bool layered = scene->layers != nullptr;
+  const bool layered = scene->layers != nullptr || scene->tilted;
   // interaction points: written into forward records whose sinks hold them;
   // floor hits fill the radius histogram when the scene has one
   const bool heat = pass == ALB_PASS_FORWARD && ((scene->heat_bins > 0 && scene->heat_radius) || out[0].fwi_x);

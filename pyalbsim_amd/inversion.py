@@ -23,9 +23,8 @@ amplitude).  Depth is identifiable only where the floor return survives the
 water column; in deep turbid scenes the estimate drifts.
 
 The floor may be tilted (slope U[0, 15] degrees, azimuth U[0, 360), known
-per scene: inversion_tilts); tilted scenes are traced with the stepped water
-legs and the tilted-floor handlers, an extension of the reference's flat
-floor.
+per scene: inversion_tilts); the tilted-floor handlers are an extension of
+the reference's flat floor.
 
 form="batched" runs all scenes per launch; form="plain" runs one scene per
 launch with the same code path, for the plain-vs-batched time comparison.

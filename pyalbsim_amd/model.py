@@ -377,8 +377,7 @@ def build_scene(cfg: SimulationConfig, hg_g: float | None = None) -> SceneModel:
     lmodels = [_layer_model(layer, tops32[i], cum[i], cfg, hg_g) for i, layer in enumerate(layers)]
     slope = float(getattr(cfg.sea_floor, "slope", 0.0))
     tilted = slope > 0.0
-    # a tilted floor is traced with the stepped water legs (per-sub-step floor test)
-    layered = len(layers) > 1 or lmodels[0].profiled or tilted
+    layered = len(layers) > 1 or lmodels[0].profiled
     layer = layers[0]
     l0 = lmodels[0]
     H = cfg.scene.flying_height

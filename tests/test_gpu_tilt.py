@@ -2,8 +2,9 @@
 the oracle: trajectory parity forward and backward (the tolerance of
 test_gpu_trace_parity), the gather of the BOTTOM map with the tilted
 multiplier, the whole path, and the batched form with tilted and flat scenes
-mixed (bit for bit against the one-scene traces).  The oracle's tilted floor
-is pinned in test_oracle_tilt.py."""
+mixed (bit for bit against the one-scene traces), and the closed-form water
+legs of a single tilted layer against the stepped legs of the same layer
+split in three.  The oracle's tilted floor is pinned in test_oracle_tilt.py."""
 import numpy as np
 import pytest
 
@@ -11,8 +12,8 @@ import oracle
 from pyalbsim_amd import abi, runtime
 from pyalbsim_amd.execution.linear import simulate
 from pyalbsim_amd.model import build_scene
-from tests.scenes import single_shot, tilted_shot
-from tests.test_gpu_trace_parity import _compare
+from tests.scenes import single_shot, stacked_copies, tilted_shot
+from tests.test_gpu_trace_parity import _compare, _download, _match
 
 pytestmark = pytest.mark.gpu
 
@@ -23,6 +24,22 @@ F = ("x", "y", "z", "dx", "dy", "dz", "energy", "time")
 def test_tilted_trace_matches_oracle(gpu, pass_):
     counts = _compare(build_scene(tilted_shot(slope=12.0, azimuth=70.0, b=0.5)), (61, 62), pass_, 400)
     assert counts[abi.BOTTOM_REFLECTION] > 50
+
+
+@pytest.mark.parametrize("pass_", [abi.PASS_FORWARD, abi.PASS_BACKWARD])
+def test_closed_form_legs_match_stepped_legs(gpu, pass_):
+    """A single constant layer over a tilted plane keeps closed-form water
+    legs; the same layer split in three takes the stepped legs."""
+    cfg = tilted_shot(slope=12.0, azimuth=70.0, b=0.6)
+    one = runtime.DeviceScene(build_scene(cfg))
+    many = runtime.DeviceScene(build_scene(stacked_copies(cfg, 3)))
+    assert one.model.tilted and not one.model.layers and many.model.layers
+    a, ca = runtime.trace(one, (8, 9), pass_, 2000, tags=True)
+    b, cb = runtime.trace(many, (8, 9), pass_, 2000, tags=True)
+    assert ca[abi.BOTTOM_REFLECTION] > 200
+    bad, allp, n = _match(_download(b, cb), _download(a, ca))
+    assert n > 2000
+    assert 1.0 - len(bad) / len(allp) >= 0.995
 
 
 def test_tilted_whole_path_matches_oracle(gpu):

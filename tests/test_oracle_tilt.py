@@ -36,7 +36,7 @@ def test_flat_scene_through_the_tilted_path_is_unchanged(pass_):
 
 def test_floor_records_lie_on_the_tilted_plane():
     m = build_scene(tilted_shot(slope=12.0, azimuth=70.0, b=0.1))
-    assert m.tilted and m.layers is not None
+    assert m.tilted and not m.layers
     p0 = beam_floor_point(m.laser_dir, m.y_ws, m.depth, m.n_water)
     assert abs(float(np.dot(m.floor_normal, p0)) - m.floor_offset) < 1e-9
     recs, _, _ = oracle.trace(m, (6, 7), abi.PASS_FORWARD, 0, 3000)
