@@ -560,6 +560,21 @@ __device__ __forceinline__ uint64_t morton_key(double px, double py, double pz, 
 
 // Phase mass of the scattering-angle bin towards the sensor with the per-bin
 // masses tabulated on the host (same values as scatter_mult computes).
+// the same with the outgoing direction o = normalised -s given
+__device__ __forceinline__ int scatter_bin_o(float px, float py, float pz, double ox, double oy, double oz, int n) {
+  normalize_f32(px, py, pz);
+  double c = (double)px * ox;
+  c = c + (double)py * oy;
+  c = c + (double)pz * oz;
+  c = clipd(c, -1.0, 1.0);
+  const double ang = acos(c);
+  const float delta_f = (3.14159265358979323846f - 1.1920928955078125e-07f) / (float)n;
+  long long i = (long long)((ang - ALB_FLT_EPS) / (double)delta_f);
+  if (i < 0) i = 0;
+  if (i > n - 1) i = n - 1;
+  return (int)i;
+}
+
 __device__ __forceinline__ int scatter_bin(float px, float py, float pz, double sx, double sy, double sz, int n) {
   normalize_f32(px, py, pz);
   double ox = -sx, oy = -sy, oz = -sz;

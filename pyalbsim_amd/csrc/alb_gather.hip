@@ -721,53 +721,110 @@ __global__ __launch_bounds__(256) void knorm_perm_kernel(const double* __restric
   knorm[r] = 1.0 / ksize;
 }
 
+// Rows staged per contribution block: what depends only on the query (its
+// direction, the outgoing direction -s normalised for the scatter map, the
+// BOTTOM multiplier, energy, time, kernel normalisation, phase table) is
+// computed once per row into LDS, and the block's threads then take the
+// tile's (row, neighbour) pairs.
+constexpr int kRowTile = 64;
+
+struct RowTile {
+  double s0[kRowTile], s1[kRowTile], s2[kRowTile];  // query direction, or -s normalised (SCATTER)
+  double kn[kRowTile];                              // kernel normalisation
+  double mul[kRowTile];                             // BOTTOM multiplier
+  const double* pm[kRowTile];                       // SCATTER phase masses of the query's layer
+  int32_t pn[kRowTile];
+  float e[kRowTile], t[kRowTile];                   // query energy and time
+  int32_t scene[kRowTile];
+};
+
+__device__ __forceinline__ void stage_row(RowTile& R, int r, const alb_scene_t& sc, const alb_queries_t& q,
+                                          int64_t qg, int32_t type, double knorm) {
+  double sx = q.dx[qg], sy = q.dy[qg], sz = q.dz[qg];
+  if (type == ALB_SCATTER) {
+    double ox = -sx, oy = -sy, oz = -sz;
+    normalize_f64_keep_zero(ox, oy, oz);
+    sx = ox;
+    sy = oy;
+    sz = oz;
+    int pn;
+    R.pm[r] = gather_phase_mass(sc, q.y[qg], &pn);
+    R.pn[r] = pn;
+  } else if (type == ALB_BOTTOM_REFLECTION) {
+    R.mul[r] = bottom_mult(sc, sx, sy, sz);
+  }
+  R.s0[r] = sx;
+  R.s1[r] = sy;
+  R.s2[r] = sz;
+  R.kn[r] = knorm;
+  R.e[r] = q.energy[qg];
+  R.t[r] = q.time[qg];
+}
+
+// row of item it of a tile (it < kRowTile * k < 2^24: the float quotient is
+// within one of the row and corrected)
+__device__ __forceinline__ int tile_row(int it, int k, float inv_k) {
+  int r = (int)(((float)it + 0.5f) * inv_k);
+  if (r * k > it) --r;
+  else if ((r + 1) * k <= it) ++r;
+  return r;
+}
+
+__device__ __forceinline__ double pair_mult(const RowTile& R, int r, const alb_scene_t& sc, const alb_map_t& m,
+                                            int32_t i, int32_t type) {
+  if (type == ALB_BOTTOM_REFLECTION) return R.mul[r];
+  if (type == ALB_SCATTER) return R.pm[r][scatter_bin_o(m.dx[i], m.dy[i], m.dz[i], R.s0[r], R.s1[r], R.s2[r], R.pn[r])];
+  if (type == ALB_SURFACE_REFLECTION)
+    return (double)brdf_reflect(m.dx[i], m.dy[i], m.dz[i], R.s0[r], R.s1[r], R.s2[r], sc.alpha_ggx, sc.F0);
+  return btdf_transmit(m.dx[i], m.dy[i], m.dz[i], R.s0[r], R.s1[r], R.s2[r], sc.alpha_ggx, sc.n_water, sc.F0,
+                       sc.surface_albedo);
+}
+
 // HEAT: also add every contribution into the scene's water and seafloor heat
 // maps at the record's interaction points (core/simulation.py:197-214)
 template <bool HEAT>
 __global__ __launch_bounds__(256) void contrib_kernel(ContribArgs a) {
   extern __shared__ double hist[];  // [win_len]
+  __shared__ RowTile R;
   for (int t = threadIdx.x; t < a.win_len; t += blockDim.x) hist[t] = 0.0;
-  __syncthreads();
   const int k = a.sc.k;
-  const int64_t total = a.q_n * (int64_t)k;
+  const float inv_k = 1.0f / (float)k;
   const float fmult = (float)a.sc.mult;
   const alb_map_t& m = a.map;
-  // XCD-aware item ranges (as in knn_kernel)
+  // XCD-aware tile ranges (as in knn_kernel)
+  const int64_t tiles = (a.q_n + kRowTile - 1) / kRowTile;
   const int xcd = (int)(blockIdx.x % kXcds);
   const int64_t xblocks = gridDim.x / kXcds;
-  const int64_t i_lo = total * xcd / kXcds, i_hi = total * (xcd + 1) / kXcds;
-  for (int64_t it = i_lo + (int64_t)(blockIdx.x / kXcds) * blockDim.x + threadIdx.x; it < i_hi;
-       it += xblocks * blockDim.x) {
-    const int64_t qi = it / k;  // row
-    const int64_t qg = a.q_begin + (a.direct ? qi : (int64_t)a.order[a.c0 + qi]);
-    const int32_t i = a.nbr[it];
-    const double knorm = a.knorm[qi];
-    const double sdx = a.q.dx[qg], sdy = a.q.dy[qg], sdz = a.q.dz[qg];
-    double mul;
-    if (a.type == ALB_BOTTOM_REFLECTION) {
-      mul = bottom_mult(a.sc, sdx, sdy, sdz);
-    } else if (a.type == ALB_SCATTER) {
-      int pn;
-      const double* pm = gather_phase_mass(a.sc, a.q.y[qg], &pn);
-      mul = scatter_mass(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, pm, pn);
-    } else if (a.type == ALB_SURFACE_REFLECTION) {
-      mul = (double)brdf_reflect(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, a.sc.alpha_ggx, a.sc.F0);
-    } else {
-      mul = btdf_transmit(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, a.sc.alpha_ggx, a.sc.n_water, a.sc.F0,
-                          a.sc.surface_albedo);
+  const int64_t t_lo = tiles * xcd / kXcds, t_hi = tiles * (xcd + 1) / kXcds;
+  for (int64_t tile = t_lo + (int64_t)(blockIdx.x / kXcds); tile < t_hi; tile += xblocks) {
+    const int64_t row0 = tile * kRowTile;
+    const int nrow = (int)(a.q_n - row0 < kRowTile ? a.q_n - row0 : kRowTile);
+    __syncthreads();  // the previous tile's rows are no longer read
+    if ((int)threadIdx.x < nrow) {
+      const int64_t qi = row0 + threadIdx.x;
+      const int64_t qg = a.q_begin + (a.direct ? qi : (int64_t)a.order[a.c0 + qi]);
+      stage_row(R, threadIdx.x, a.sc, a.q, qg, a.type, a.knorm[qi]);
     }
-    const float ee = a.q.energy[qg] * m.energy[i];
-    const double e = (double)ee * knorm * mul;
-    const float tt = a.q.time[qg] + m.time[i];
-    const int64_t bin = (int64_t)(tt / fmult);
-    if (bin >= a.win0 && bin < (int64_t)a.win0 + a.win_len) atomicAdd(&hist[bin - a.win0], e);
-    else if (bin >= 0 && bin < a.n_bins)
-      unsafeAtomicAdd(a.rep ? &a.rep[(int64_t)(blockIdx.x % a.n_rep) * a.n_bins + bin] : &a.wf[bin], e);
-    if constexpr (HEAT) {
-      heat_add(a.sc.heat_water, m.fwi_x[i], m.fwi_z[i], a.sc.heat_water_center, a.sc.heat_water_extent,
-               a.sc.heat_bins, e);
-      heat_add(a.sc.heat_floor, m.sfi_x[i], m.sfi_z[i], a.sc.heat_floor_center, a.sc.heat_floor_extent,
-               a.sc.heat_bins, e);
+    __syncthreads();
+    const int items = nrow * k;
+    const int32_t* __restrict__ nb = a.nbr + row0 * k;
+    for (int it = threadIdx.x; it < items; it += blockDim.x) {
+      const int r = tile_row(it, k, inv_k);
+      const int32_t i = nb[it];
+      const double mul = pair_mult(R, r, a.sc, m, i, a.type);
+      const float ee = R.e[r] * m.energy[i];
+      const double e = (double)ee * R.kn[r] * mul;
+      const float tt = R.t[r] + m.time[i];
+      const int64_t bin = (int64_t)(tt / fmult);
+      if (bin >= a.win0 && bin < (int64_t)a.win0 + a.win_len) atomicAdd(&hist[bin - a.win0], e);
+      else if (bin >= 0 && bin < a.n_bins)
+        unsafeAtomicAdd(a.rep ? &a.rep[(int64_t)(blockIdx.x % a.n_rep) * a.n_bins + bin] : &a.wf[bin], e);
+      if constexpr (HEAT) {
+        heat_add(a.sc.heat_water, m.fwi_x[i], m.fwi_z[i], a.sc.heat_water_center, a.sc.heat_water_extent,
+                 a.sc.heat_bins, e);
+        heat_add(a.sc.heat_floor, m.sfi_x[i], m.sfi_z[i], a.sc.heat_floor_center, a.sc.heat_floor_extent,
+                 a.sc.heat_bins, e);
+      }
     }
   }
   __syncthreads();
@@ -812,11 +869,11 @@ __device__ __forceinline__ void flush_window(double* hist, int win_len, double* 
 
 __global__ __launch_bounds__(256) void contrib_batched_kernel(ContribBatchedArgs a) {
   extern __shared__ double hist[];  // [win_len]
+  __shared__ RowTile R;
   for (int t = threadIdx.x; t < a.win_len; t += blockDim.x) hist[t] = 0.0;
-  __syncthreads();
   const int k = a.k;
-  const int64_t total = a.q_n * (int64_t)k;
-  const int64_t tiles = (total + 255) / 256;
+  const float inv_k = 1.0f / (float)k;
+  const int64_t tiles = (a.q_n + kRowTile - 1) / kRowTile;
   const int64_t per = (tiles + gridDim.x - 1) / gridDim.x;
   const int64_t t0 = (int64_t)blockIdx.x * per;
   const int64_t t1 = t0 + per < tiles ? t0 + per : tiles;
@@ -824,10 +881,11 @@ __global__ __launch_bounds__(256) void contrib_batched_kernel(ContribBatchedArgs
   int32_t cur = -1, cw0 = 0;
   double* cdst = nullptr;  // global bins of the LDS window
   for (int64_t tile = t0; tile < t1; ++tile) {
-    const int64_t it0 = tile * 256;
-    const int32_t ts = a.q.scene[a.order[a.c0 + it0 / k]];
+    const int64_t row0 = tile * kRowTile;
+    const int nrow = (int)(a.q_n - row0 < kRowTile ? a.q_n - row0 : kRowTile);
+    const int32_t ts = a.q.scene[a.order[a.c0 + row0]];
+    __syncthreads();  // the previous tile's rows and window adds are done
     if (ts != cur) {  // uniform: every thread read the same slot
-      __syncthreads();
       if (cur >= 0) flush_window(hist, a.win_len, cdst);
       __syncthreads();
       cur = ts;
@@ -835,41 +893,36 @@ __global__ __launch_bounds__(256) void contrib_batched_kernel(ContribBatchedArgs
       const int64_t nb = 2 * (int64_t)a.scenes[ts].steps;
       cdst = a.wf + a.wf_off[ts] + (int64_t)a.type * nb + cw0;
     }
-    const int64_t it = it0 + threadIdx.x;
-    if (it >= total) continue;
-    const int64_t qi = it / k;
-    const int64_t qg = a.order[a.c0 + qi];
-    const int32_t sc = a.q.scene[qg];
-    const alb_scene_t* S = a.scenes + sc;
-    const int32_t i = a.nbr[it];
-    const double knorm = a.knorm[qi];
-    if (i < 0) continue;  // scene with fewer records than neighbours (counted by the k-NN pass)
-    const double sdx = a.q.dx[qg], sdy = a.q.dy[qg], sdz = a.q.dz[qg];
-    double mul;
-    if (a.type == ALB_BOTTOM_REFLECTION) {
-      mul = bottom_mult(*S, sdx, sdy, sdz);
-    } else if (a.type == ALB_SCATTER) {
-      int pn;
-      const double* pm = gather_phase_mass(*S, a.q.y[qg], &pn);
-      mul = scatter_mass(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, pm, pn);
-    } else if (a.type == ALB_SURFACE_REFLECTION) {
-      mul = (double)brdf_reflect(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, S->alpha_ggx, S->F0);
-    } else {
-      mul = btdf_transmit(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, S->alpha_ggx, S->n_water, S->F0,
-                          S->surface_albedo);
+    if ((int)threadIdx.x < nrow) {
+      const int64_t qi = row0 + threadIdx.x;
+      const int64_t qg = a.order[a.c0 + qi];
+      const int32_t sc = a.q.scene[qg];
+      R.scene[threadIdx.x] = sc;
+      stage_row(R, threadIdx.x, a.scenes[sc], a.q, qg, a.type, a.knorm[qi]);
     }
-    const float ee = a.q.energy[qg] * m.energy[i];
-    const double e = (double)ee * knorm * mul;
-    const float tt = a.q.time[qg] + m.time[i];
-    const int64_t bin = (int64_t)(tt / (float)S->mult);
-    if (sc == cur && bin >= cw0 && bin < (int64_t)cw0 + a.win_len) {
-      atomicAdd(&hist[bin - cw0], e);
-    } else {
-      const int64_t nb = 2 * (int64_t)S->steps;
-      if (bin >= 0 && bin < nb)
-        unsafeAtomicAdd(a.rep ? &a.rep[((int64_t)(blockIdx.x % a.n_rep) * a.n_scenes + sc) * a.nb_max + bin]
-                              : &a.wf[a.wf_off[sc] + (int64_t)a.type * nb + bin],
-                        e);
+    __syncthreads();
+    const int items = nrow * k;
+    const int32_t* __restrict__ nb = a.nbr + row0 * k;
+    for (int it = threadIdx.x; it < items; it += blockDim.x) {
+      const int r = tile_row(it, k, inv_k);
+      const int32_t i = nb[it];
+      if (i < 0) continue;  // scene with fewer records than neighbours (counted by the k-NN pass)
+      const int32_t sc = R.scene[r];
+      const alb_scene_t& S = a.scenes[sc];
+      const double mul = pair_mult(R, r, S, m, i, a.type);
+      const float ee = R.e[r] * m.energy[i];
+      const double e = (double)ee * R.kn[r] * mul;
+      const float tt = R.t[r] + m.time[i];
+      const int64_t bin = (int64_t)(tt / (float)S.mult);
+      if (sc == cur && bin >= cw0 && bin < (int64_t)cw0 + a.win_len) {
+        atomicAdd(&hist[bin - cw0], e);
+      } else {
+        const int64_t nbins = 2 * (int64_t)S.steps;
+        if (bin >= 0 && bin < nbins)
+          unsafeAtomicAdd(a.rep ? &a.rep[((int64_t)(blockIdx.x % a.n_rep) * a.n_scenes + sc) * a.nb_max + bin]
+                                : &a.wf[a.wf_off[sc] + (int64_t)a.type * nbins + bin],
+                          e);
+      }
     }
   }
   __syncthreads();
@@ -1076,7 +1129,7 @@ extern "C" int alb_contrib(const alb_scene_t* scene, const alb_map_t* map, int32
                        dim3(256), 0, s, replicas, rn);
   ContribArgs ca{*scene, *map, *q, q_begin, c0, cn, order, rows_by_query ? 1 : 0, photon_type, nbr, knorm,
                  waveform, n_bins, win0, win_len, n_replicas > 0 ? replicas : nullptr, n_replicas > 0 ? n_replicas : 1};
-  int64_t cb = (cn * scene->k + 255) / 256;
+  int64_t cb = (cn + kRowTile - 1) / kRowTile;
   if (cb > (int64_t)cus * 8) cb = (int64_t)cus * 8;
   cb = (cb + kXcds - 1) / kXcds * kXcds;
   const bool heat = scene->heat_bins > 0 && scene->heat_water && scene->heat_floor && map->fwi_x && map->sfi_x;
@@ -1219,7 +1272,7 @@ extern "C" int alb_contrib_batched(const alb_scene_t* scenes, int32_t n_scenes, 
   ContribBatchedArgs ca{scenes, *map, *q, c0, cn, order, photon_type, k, nbr, knorm, waveforms, wf_offset, win0,
                         win_len, n_replicas > 0 ? replicas : nullptr, n_replicas > 0 ? n_replicas : 1, n_scenes,
                         nb_max};
-  const int64_t tiles = (cn * k + 255) / 256;
+  const int64_t tiles = (cn + kRowTile - 1) / kRowTile;
   int64_t cb = (int64_t)cus * 8;
   if (cb > tiles) cb = tiles;
   hipLaunchKernelGGL(contrib_batched_kernel, dim3((unsigned)cb), dim3(256), sizeof(double) * (size_t)win_len, s, ca);

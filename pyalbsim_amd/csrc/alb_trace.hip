@@ -334,9 +334,12 @@ __device__ __forceinline__ double floor_dist(const alb_scene_t& sc, double x, do
 }
 
 // first j >= 0 whose sub-step ends on the floor side of a tilted plane,
-// floor_dist((p + j*D) + D) <= 0, with g0 = floor_dist(p) and gD = n . D < 0:
-// the signed distance falls linearly with j, so the estimate -g0/gD is
-// corrected on the positions the leg itself computes (cap if none)
+// floor_dist((p + j*D) + D) <= 0, with g0 = floor_dist(p) and gD = n . D < 0
+// (cap if none).  The signed distance is affine in j, so ceil(-g0/gD) - 1 is
+// the index up to rounding; one correction on the positions the leg itself
+// computes settles it.  (A correction loop, as the flat floor's search has,
+// costs the batched kernels 35 VGPRs and half their occupancy; only a leg
+// grazing the plane for ~1e5 sub-steps could need a second step.)
 __device__ __forceinline__ int64_t first_step_through_plane(const alb_scene_t& sc, double px, double py, double pz,
                                                             double Dx, double Dy, double Dz, double g0, double gD,
                                                             int64_t cap) {
@@ -347,9 +350,9 @@ __device__ __forceinline__ int64_t first_step_through_plane(const alb_scene_t& s
     return floor_dist(sc, (px + a * Dx) + Dx, (py + a * Dy) + Dy, (pz + a * Dz) + Dz) <= 0.0;
   };
   int64_t j = x > 1.0 ? (int64_t)ceil(x) - 1 : 0;
-  while (j > 0 && ends_below(j - 1)) --j;
-  while (j < cap && !ends_below(j)) ++j;
-  return j;
+  if (j > 0 && ends_below(j - 1)) --j;
+  else if (!ends_below(j)) ++j;
+  return j < cap ? j : cap;
 }
 
 // Layered or depth-profiled water column, or a tilted floor: sub-steps are
