@@ -31,10 +31,13 @@ The two other forms of the metric, on bounded samples:
   * adjoint: --adjoint-scenes sweep scenes at 1e5 forward / 2.5e4 backward
     photons; wall time of forward + adjoint (path replay, five parameters)
     over the forward alone, and with the one-sided depth difference (one
-    more forward pass).
+    more forward pass);
+  * with --sweep-full N: the first N sweep scenes at their full photon
+    counts, batched in memory-bounded scene groups vs plain per-scene runs.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--forward N]
        [--backward N] [--no-cpu] [--no-batched] [--no-adjoint] [--no-radius]
+       [--no-inversion] [--sweep-full N]
 """
 from __future__ import annotations
 
@@ -67,6 +70,29 @@ def trace_flops(ctr, forward: bool) -> float:
     return float((F_FWD_EMIT if forward else F_BWD_EMIT) * ctr[abi.CTR_PATHS] + F_AIR * ctr[abi.CTR_AIR_LEGS]
                  + F_DEPTH_SOLVE * ctr[abi.CTR_DEPTH_SOLVES] + F_SCATTER * ctr[abi.CTR_SCATTER]
                  + F_ENTER * ctr[abi.CTR_ENTER] + F_EXIT * ctr[abi.CTR_EXIT] + F_FLOOR * ctr[abi.CTR_FLOOR])
+
+
+def host_cpu() -> dict:
+    """The host's CPU model, logical CPUs (os.cpu_count) and physical cores."""
+    model, phys = None, set()
+    try:
+        cur = {}
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if ":" not in line:
+                    if "physical id" in cur and "core id" in cur:
+                        phys.add((cur["physical id"], cur["core id"]))
+                    cur = {}
+                    continue
+                k_, v = (x.strip() for x in line.split(":", 1))
+                cur[k_] = v
+                if k_ == "model name" and model is None:
+                    model = v
+        if "physical id" in cur and "core id" in cur:
+            phys.add((cur["physical id"], cur["core id"]))
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "physical_cores": len(phys) or None}
 
 
 def cpu_baseline(model, key, n_f: int, n_b: int, threads: int):
@@ -120,6 +146,8 @@ def main(argv=None):
     ap.add_argument("--inversion-iters", type=int, default=10)
     ap.add_argument("--radius-grid", type=int, default=256)
     ap.add_argument("--sweep-scenes", type=int, default=256)
+    ap.add_argument("--sweep-full", type=int, default=0,
+                    help="also run this many sweep scenes at full photon counts (grouped batched vs plain)")
     ap.add_argument("--adjoint-scenes", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 2 GiB device copy at the end (reference reading for HBM byte counters)")
@@ -213,6 +241,8 @@ def main(argv=None):
                    "forward_photons": n_f, "backward_photons": n_b, "k": k,
                    "map_records": res.map_counts, "gather_queries": qc},
         "phases_ms": phases,
+        "forward_paths_per_s": n_f / (phases["forward"] / 1e3) if phases["forward"] > 0 else None,
+        "backward_paths_per_s": n_b / (phases["backward"] / 1e3) if phases["backward"] > 0 else None,
         "kernel_ms_per_step": {"knn": sum(knn_ms) / args.steps, "contrib": sum(contrib_ms) / args.steps},
         "roofline": roof,
         "trace_tflops": (flops_fwd + flops_bwd) / t_trace / 1e12,
@@ -230,6 +260,7 @@ def main(argv=None):
         threads = min(os.cpu_count() or 1, 16)
         rate, cwall = cpu_baseline(model, key, args.cpu_forward, args.cpu_backward, threads)
         out["cpu_baseline"] = {"value": rate, "unit": "photon paths/s", "cores": threads, "kind": "port",
+                               "host": host_cpu(),
                                "sample": f"time-stepped C oracle, same scene, {args.cpu_forward} forward + "
                                          f"{args.cpu_backward} backward photons, brute-force k-NN, "
                                          f"{cwall:.1f} s wall"}
@@ -238,6 +269,8 @@ def main(argv=None):
         out["radius_mode"] = bench_radius(model, ds, key, n_f, args.radius_grid)
     if not args.no_batched:
         out["batched"] = bench_batched(args.sweep_scenes)
+    if args.sweep_full > 0:
+        out["batched_full"] = bench_batched_full(args.sweep_full)
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(args.adjoint_scenes)
     if not args.no_inversion:
@@ -367,6 +400,42 @@ def bench_batched(n_scenes: int):
             "map_records": counts[0], "gather_queries": counts[1],
             "batched_paths_per_s": paths / tb, "plain_paths_per_s": paths / tp, "batched_s": tb, "plain_s": tp,
             "speedup": tp / tb, "autotuned": {"block": block, "refill": refill, "tuning_s": t_tune}}
+
+
+def bench_batched_full(n_scenes: int):
+    """The first n_scenes sweep scenes at their full photon counts (10^U[5,7]):
+    the grouped batched form (memory-bounded scene groups) vs plain per-scene
+    runs."""
+    import torch
+
+    from pyalbsim_amd.autotune import tune_batched
+    from pyalbsim_amd.execution.linear import simulate
+    from pyalbsim_amd.sweep import hypercube, run_sweep
+
+    sc = hypercube(1024, 1, (5.0, 7.0))[:n_scenes]
+    nf = [x.n_forward for x in sc]
+    nb = [x.n_backward for x in sc]
+    paths = sum(nf) + sum(nb)
+    key = (0xBA7C, 1)
+    block, refill, _ = tune_batched([x.model for x in sc], nf, key)
+    torch.cuda.synchronize()
+    res = run_sweep(sc, key, block=block, refill=refill, strict=False,
+                    log=lambda m: print(m, file=sys.stderr, flush=True))
+    tb = res.seconds
+    print(f"batched full-size {n_scenes} scenes done ({len(res.groups)} groups)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for s in range(n_scenes):
+        simulate(sc[s].model, nf[s], nb[s], key, strict=False, scene_id=s)
+        if s % 8 == 7:
+            print(f"plain full-size scene {s + 1}/{n_scenes}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    print(f"plain full-size {n_scenes} scenes done", file=sys.stderr, flush=True)
+    return {"workload": f"first {n_scenes} of 1024 Latin-hypercube sweep scenes (seed 1) at full photon counts "
+                        f"10^U[5,7], backward = forward / 4; batched in memory-bounded scene groups",
+            "forward_photons": int(sum(nf)), "backward_photons": int(sum(nb)), "groups": len(res.groups),
+            "batched_paths_per_s": paths / tb, "plain_paths_per_s": paths / tp, "batched_s": tb, "plain_s": tp,
+            "speedup": tp / tb, "autotuned": {"block": block, "refill": refill}}
 
 
 def bench_adjoint(n_scenes: int):

@@ -73,3 +73,90 @@ def hypercube(n: int = 1024, seed: int = 1, log10_photons=(5.0, 7.0), backward_r
         model = build_scene(cfg, hg_g=p["g"])
         out.append(SweepScene(p, model, int(nf[i]), int(max(1, round(backward_ratio * nf[i])))))
     return out
+
+
+@dataclass
+class SweepResult:
+    groups: list          # (first, end) scene ranges run as one batched group each
+    waveforms: list       # per scene: float64 numpy [4, win_len] over model.gate(win_len)
+    windows: list         # per scene: (first bin, length) of its waveform
+    seconds: float
+
+
+def estimate_group_bytes(models, n_forward, n_backward, key, pilot_fraction: float = 1 / 100,
+                         bytes_per_record: float = 200.0, bytes_per_query: float = 80.0) -> np.ndarray:
+    """Device bytes each scene needs inside a batched group: a pilot forward
+    trace of pilot_fraction of every scene's photons gives its records per
+    photon by type; records cost sinks + map + radix tree + build workspace
+    (bytes_per_record) and backward queries, estimated as records x
+    n_backward / n_forward, bytes_per_query."""
+    import torch
+
+    from . import abi, runtime
+
+    nf = np.asarray(n_forward, dtype=np.int64)
+    nb = np.asarray(n_backward, dtype=np.int64)
+    pc = np.maximum(np.ceil(nf * pilot_fraction).astype(np.int64), np.minimum(nf, 256))
+    tab = runtime.DeviceSceneTable(list(models))
+    sinks, got = runtime.trace_batched(tab, key, abi.PASS_FORWARD, pc)
+    per = np.zeros(len(nf))
+    for t in range(abi.N_TYPES):
+        if got[t]:
+            per += torch.bincount(sinks[t].tensors["scene"][:got[t]].long(), minlength=len(nf)).cpu().numpy()
+    rec = per / pc * nf
+    return rec * bytes_per_record + rec * (nb / np.maximum(nf, 1)) * bytes_per_query
+
+
+def plan_groups(scene_bytes, budget: float) -> list:
+    """Contiguous scene ranges whose summed bytes stay within budget (a scene
+    larger than the budget forms a group of its own)."""
+    groups, start, acc = [], 0, 0.0
+    for s, b in enumerate(scene_bytes):
+        if s > start and acc + b > budget:
+            groups.append((start, s))
+            start, acc = s, 0.0
+        acc += float(b)
+    if len(scene_bytes):
+        groups.append((start, len(scene_bytes)))
+    return groups
+
+
+def run_sweep(scenes, key, budget_bytes: float = None, win_len: int = 1000, margin: float = 1.3, log=None,
+              **batched_kw) -> SweepResult:
+    """All scenes of a sweep in the batched form, in contiguous groups whose
+    photon maps fit the device: a full-size sweep of 1024 scenes at 10^U[5,7]
+    photons holds ~1e11 map records, far beyond one GPU's memory, so the
+    scenes are packed into groups by a pilot estimate (estimate_group_bytes,
+    times margin) against budget_bytes (default: half the free device
+    memory).  Scene s keeps Philox stream id s in every grouping, so its
+    waveform does not depend on the grouping.  Returns each scene's waveform
+    over its gate window (model.gate(win_len)); log, if given, receives a
+    line per finished group."""
+    import time as _time
+
+    import torch
+
+    from .execution.batched import simulate_batched
+
+    models = [x.model for x in scenes]
+    nf = np.array([x.n_forward for x in scenes], dtype=np.int64)
+    nb = np.array([x.n_backward for x in scenes], dtype=np.int64)
+    t0 = _time.perf_counter()
+    if budget_bytes is None:
+        free, _ = torch.cuda.mem_get_info()
+        budget_bytes = 0.5 * free
+    est = estimate_group_bytes(models, nf, nb, key) * margin
+    groups = plan_groups(est, budget_bytes)
+    wfs, wins = [], []
+    for a, b in groups:
+        res = simulate_batched(models[a:b], nf[a:b], nb[a:b], key, scene_ids=range(a, b), win_len=win_len,
+                               **batched_kw)
+        for s in range(b - a):
+            w = models[a + s].gate(win_len)
+            wins.append(w)
+            wfs.append(res.gated(s, w).cpu().numpy())
+        del res
+        if log is not None:
+            log(f"sweep group {a}-{b} of {len(models)} scenes done")
+    torch.cuda.synchronize()
+    return SweepResult(groups, wfs, wins, _time.perf_counter() - t0)

@@ -222,3 +222,27 @@ def test_autotuned_launch_gives_same_waveforms(gpu):
     for s in range(3):
         x, y = a.waveform(s).cpu().numpy(), c.waveform(s).cpu().numpy()
         assert np.allclose(x, y, rtol=1e-9, atol=1e-12 * float(np.abs(y).max()))
+
+
+def test_grouped_sweep_equals_one_group(gpu):
+    """run_sweep packs scenes into contiguous memory-bounded groups; scene s
+    keeps stream id s, so any grouping gives the one-group waveforms up to
+    FP64 summation order."""
+    from pyalbsim_amd.sweep import estimate_group_bytes, run_sweep
+
+    sc = hypercube(24, seed=1, log10_photons=(2.5, 3.5))
+    key = (9, 2)
+    models = [x.model for x in sc]
+    nf, nb = [x.n_forward for x in sc], [x.n_backward for x in sc]
+    est = estimate_group_bytes(models, nf, nb, key)
+    assert np.all(est > 0)
+    budget = float(est.sum()) / 5.0
+    res = run_sweep(sc, key, budget_bytes=budget, margin=1.0, strict=False)
+    assert len(res.groups) >= 4 and res.groups[0][0] == 0 and res.groups[-1][1] == len(sc)
+    assert all(a < b and b == c for (a, b), (c, _) in zip(res.groups, res.groups[1:]))
+    one = simulate_batched(models, nf, nb, key, win_len=1000, strict=False)
+    for s in range(len(sc)):
+        ref = one.gated(s, res.windows[s]).cpu().numpy()
+        got = res.waveforms[s]
+        assert got.shape == ref.shape
+        assert np.allclose(got, ref, rtol=1e-9, atol=1e-12 * max(float(np.abs(ref).max()), 1e-300)), s
