@@ -738,10 +738,11 @@ struct RowTile {
   int32_t scene[kRowTile];
 };
 
+template <int TYPE>
 __device__ __forceinline__ void stage_row(RowTile& R, int r, const alb_scene_t& sc, const alb_queries_t& q,
-                                          int64_t qg, int32_t type, double knorm) {
+                                          int64_t qg, double knorm) {
   double sx = q.dx[qg], sy = q.dy[qg], sz = q.dz[qg];
-  if (type == ALB_SCATTER) {
+  if constexpr (TYPE == ALB_SCATTER) {
     double ox = -sx, oy = -sy, oz = -sz;
     normalize_f64_keep_zero(ox, oy, oz);
     sx = ox;
@@ -750,7 +751,7 @@ __device__ __forceinline__ void stage_row(RowTile& R, int r, const alb_scene_t& 
     int pn;
     R.pm[r] = gather_phase_mass(sc, q.y[qg], &pn);
     R.pn[r] = pn;
-  } else if (type == ALB_BOTTOM_REFLECTION) {
+  } else if constexpr (TYPE == ALB_BOTTOM_REFLECTION) {
     R.mul[r] = bottom_mult(sc, sx, sy, sz);
   }
   R.s0[r] = sx;
@@ -761,34 +762,54 @@ __device__ __forceinline__ void stage_row(RowTile& R, int r, const alb_scene_t& 
   R.t[r] = q.time[qg];
 }
 
-// row of item it of a tile (it < kRowTile * k < 2^24: the float quotient is
-// within one of the row and corrected)
-__device__ __forceinline__ int tile_row(int it, int k, float inv_k) {
-  int r = (int)(((float)it + 0.5f) * inv_k);
-  if (r * k > it) --r;
-  else if ((r + 1) * k <= it) ++r;
-  return r;
+// it / d for it < kRowTile * kMaxK < 2^24 (the float quotient is within one
+// of it and corrected)
+__device__ __forceinline__ int small_div(int it, int d, float inv_d) {
+  int q = (int)(((float)it + 0.5f) * inv_d);
+  if (q * d > it) --q;
+  else if ((q + 1) * d <= it) ++q;
+  return q;
 }
 
-__device__ __forceinline__ double pair_mult(const RowTile& R, int r, const alb_scene_t& sc, const alb_map_t& m,
-                                            int32_t i, int32_t type) {
-  if (type == ALB_BOTTOM_REFLECTION) return R.mul[r];
-  if (type == ALB_SCATTER) return R.pm[r][scatter_bin_o(m.dx[i], m.dy[i], m.dz[i], R.s0[r], R.s1[r], R.s2[r], R.pn[r])];
-  if (type == ALB_SURFACE_REFLECTION)
-    return (double)brdf_reflect(m.dx[i], m.dy[i], m.dz[i], R.s0[r], R.s1[r], R.s2[r], sc.alpha_ggx, sc.F0);
-  return btdf_transmit(m.dx[i], m.dy[i], m.dz[i], R.s0[r], R.s1[r], R.s2[r], sc.alpha_ggx, sc.n_water, sc.F0,
-                       sc.surface_albedo);
+// the kernels are instantiated per photon type: each carries only its own
+// multiplier's code and registers (the BRDF and BTDF paths would otherwise set
+// the register budget of the scatter kernel)
+// a neighbour's direction and energy (the BOTTOM multiplier needs the energy only)
+struct NbrRec {
+  float dx, dy, dz, e;
+};
+
+template <int TYPE>
+__device__ __forceinline__ NbrRec nbr_rec(const alb_map_t& m, int32_t i) {
+  if constexpr (TYPE == ALB_BOTTOM_REFLECTION) {
+    return {0.0f, 0.0f, 0.0f, m.energy[i]};
+  } else {
+    return {m.dx[i], m.dy[i], m.dz[i], m.energy[i]};
+  }
+}
+
+template <int TYPE>
+__device__ __forceinline__ double pair_mult(const RowTile& R, int r, const alb_scene_t& sc, const NbrRec& p) {
+  if constexpr (TYPE == ALB_BOTTOM_REFLECTION) {
+    return R.mul[r];
+  } else if constexpr (TYPE == ALB_SCATTER) {
+    return R.pm[r][scatter_bin_o(p.dx, p.dy, p.dz, R.s0[r], R.s1[r], R.s2[r], R.pn[r])];
+  } else if constexpr (TYPE == ALB_SURFACE_REFLECTION) {
+    return (double)brdf_reflect(p.dx, p.dy, p.dz, R.s0[r], R.s1[r], R.s2[r], sc.alpha_ggx, sc.F0);
+  } else {
+    return btdf_transmit(p.dx, p.dy, p.dz, R.s0[r], R.s1[r], R.s2[r], sc.alpha_ggx, sc.n_water, sc.F0,
+                         sc.surface_albedo);
+  }
 }
 
 // HEAT: also add every contribution into the scene's water and seafloor heat
 // maps at the record's interaction points (core/simulation.py:197-214)
-template <bool HEAT>
+template <int TYPE, bool HEAT>
 __global__ __launch_bounds__(256) void contrib_kernel(ContribArgs a) {
   extern __shared__ double hist[];  // [win_len]
   __shared__ RowTile R;
   for (int t = threadIdx.x; t < a.win_len; t += blockDim.x) hist[t] = 0.0;
   const int k = a.sc.k;
-  const float inv_k = 1.0f / (float)k;
   const float fmult = (float)a.sc.mult;
   const alb_map_t& m = a.map;
   // XCD-aware tile ranges (as in knn_kernel)
@@ -803,16 +824,21 @@ __global__ __launch_bounds__(256) void contrib_kernel(ContribArgs a) {
     if ((int)threadIdx.x < nrow) {
       const int64_t qi = row0 + threadIdx.x;
       const int64_t qg = a.q_begin + (a.direct ? qi : (int64_t)a.order[a.c0 + qi]);
-      stage_row(R, threadIdx.x, a.sc, a.q, qg, a.type, a.knorm[qi]);
+      stage_row<TYPE>(R, threadIdx.x, a.sc, a.q, qg, a.knorm[qi]);
     }
     __syncthreads();
+    // item it -> (row it / k, neighbour it % k): consecutive lanes take a
+    // row's consecutive neighbours (coalesced neighbour loads, records close
+    // in the map); lanes taking different rows were slower
     const int items = nrow * k;
+    const float inv_k = 1.0f / (float)k;
     const int32_t* __restrict__ nb = a.nbr + row0 * k;
     for (int it = threadIdx.x; it < items; it += blockDim.x) {
-      const int r = tile_row(it, k, inv_k);
+      const int r = small_div(it, k, inv_k);
       const int32_t i = nb[it];
-      const double mul = pair_mult(R, r, a.sc, m, i, a.type);
-      const float ee = R.e[r] * m.energy[i];
+      const NbrRec p = nbr_rec<TYPE>(m, i);
+      const double mul = pair_mult<TYPE>(R, r, a.sc, p);
+      const float ee = R.e[r] * p.e;
       const double e = (double)ee * R.kn[r] * mul;
       const float tt = R.t[r] + m.time[i];
       const int64_t bin = (int64_t)(tt / fmult);
@@ -867,12 +893,12 @@ __device__ __forceinline__ void flush_window(double* hist, int win_len, double* 
   }
 }
 
+template <int TYPE>
 __global__ __launch_bounds__(256) void contrib_batched_kernel(ContribBatchedArgs a) {
   extern __shared__ double hist[];  // [win_len]
   __shared__ RowTile R;
   for (int t = threadIdx.x; t < a.win_len; t += blockDim.x) hist[t] = 0.0;
   const int k = a.k;
-  const float inv_k = 1.0f / (float)k;
   const int64_t tiles = (a.q_n + kRowTile - 1) / kRowTile;
   const int64_t per = (tiles + gridDim.x - 1) / gridDim.x;
   const int64_t t0 = (int64_t)blockIdx.x * per;
@@ -898,19 +924,21 @@ __global__ __launch_bounds__(256) void contrib_batched_kernel(ContribBatchedArgs
       const int64_t qg = a.order[a.c0 + qi];
       const int32_t sc = a.q.scene[qg];
       R.scene[threadIdx.x] = sc;
-      stage_row(R, threadIdx.x, a.scenes[sc], a.q, qg, a.type, a.knorm[qi]);
+      stage_row<TYPE>(R, threadIdx.x, a.scenes[sc], a.q, qg, a.knorm[qi]);
     }
     __syncthreads();
-    const int items = nrow * k;
+    const int items = nrow * k;  // (row, neighbour) as in contrib_kernel
+    const float inv_k = 1.0f / (float)k;
     const int32_t* __restrict__ nb = a.nbr + row0 * k;
     for (int it = threadIdx.x; it < items; it += blockDim.x) {
-      const int r = tile_row(it, k, inv_k);
+      const int r = small_div(it, k, inv_k);
       const int32_t i = nb[it];
       if (i < 0) continue;  // scene with fewer records than neighbours (counted by the k-NN pass)
       const int32_t sc = R.scene[r];
       const alb_scene_t& S = a.scenes[sc];
-      const double mul = pair_mult(R, r, S, m, i, a.type);
-      const float ee = R.e[r] * m.energy[i];
+      const NbrRec p = nbr_rec<TYPE>(m, i);
+      const double mul = pair_mult<TYPE>(R, r, S, p);
+      const float ee = R.e[r] * p.e;
       const double e = (double)ee * R.kn[r] * mul;
       const float tt = R.t[r] + m.time[i];
       const int64_t bin = (int64_t)(tt / (float)S.mult);
@@ -1133,7 +1161,13 @@ extern "C" int alb_contrib(const alb_scene_t* scene, const alb_map_t* map, int32
   if (cb > (int64_t)cus * 8) cb = (int64_t)cus * 8;
   cb = (cb + kXcds - 1) / kXcds * kXcds;
   const bool heat = scene->heat_bins > 0 && scene->heat_water && scene->heat_floor && map->fwi_x && map->sfi_x;
-  hipLaunchKernelGGL(heat ? contrib_kernel<true> : contrib_kernel<false>, dim3((unsigned)cb), dim3(256),
+  using CK = void (*)(ContribArgs);
+  static const CK kern[2][ALB_N_TYPES] = {
+      {contrib_kernel<ALB_BOTTOM_REFLECTION, false>, contrib_kernel<ALB_SCATTER, false>,
+       contrib_kernel<ALB_SURFACE_REFLECTION, false>, contrib_kernel<ALB_SURFACE_TRANSMISSION_UP, false>},
+      {contrib_kernel<ALB_BOTTOM_REFLECTION, true>, contrib_kernel<ALB_SCATTER, true>,
+       contrib_kernel<ALB_SURFACE_REFLECTION, true>, contrib_kernel<ALB_SURFACE_TRANSMISSION_UP, true>}};
+  hipLaunchKernelGGL(kern[heat ? 1 : 0][photon_type], dim3((unsigned)cb), dim3(256),
                      sizeof(double) * (size_t)win_len, s, ca);
   if (n_replicas > 0)
     hipLaunchKernelGGL(replica_reduce_kernel, dim3((unsigned)((n_bins + 255) / 256)), dim3(256), 0, s, replicas,
@@ -1275,7 +1309,11 @@ extern "C" int alb_contrib_batched(const alb_scene_t* scenes, int32_t n_scenes, 
   const int64_t tiles = (cn + kRowTile - 1) / kRowTile;
   int64_t cb = (int64_t)cus * 8;
   if (cb > tiles) cb = tiles;
-  hipLaunchKernelGGL(contrib_batched_kernel, dim3((unsigned)cb), dim3(256), sizeof(double) * (size_t)win_len, s, ca);
+  using CBK = void (*)(ContribBatchedArgs);
+  static const CBK bkern[ALB_N_TYPES] = {contrib_batched_kernel<ALB_BOTTOM_REFLECTION>, contrib_batched_kernel<ALB_SCATTER>,
+                                         contrib_batched_kernel<ALB_SURFACE_REFLECTION>,
+                                         contrib_batched_kernel<ALB_SURFACE_TRANSMISSION_UP>};
+  hipLaunchKernelGGL(bkern[photon_type], dim3((unsigned)cb), dim3(256), sizeof(double) * (size_t)win_len, s, ca);
   if (n_replicas > 0) {
     const int64_t tot = (int64_t)n_scenes * nb_max;
     hipLaunchKernelGGL(replica_reduce_batched_kernel, dim3((unsigned)((tot + 255) / 256 < 8192 ? (tot + 255) / 256 : 8192)),
