@@ -149,6 +149,7 @@ def main(argv=None):
     ap.add_argument("--sweep-full", type=int, default=0,
                     help="also run this many sweep scenes at full photon counts (grouped batched vs plain)")
     ap.add_argument("--adjoint-scenes", type=int, default=16)
+    ap.add_argument("--adjoint-photons", type=float, default=1e5)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 2 GiB device copy at the end (reference reading for HBM byte counters)")
     args = ap.parse_args(argv)
@@ -272,7 +273,7 @@ def main(argv=None):
     if args.sweep_full > 0:
         out["batched_full"] = bench_batched_full(args.sweep_full)
     if not args.no_adjoint:
-        out["adjoint"] = bench_adjoint(args.adjoint_scenes)
+        out["adjoint"] = bench_adjoint(args.adjoint_scenes, int(args.adjoint_photons))
     if not args.no_inversion:
         out["inversion"] = bench_inversion(args.inversion_scenes, args.inversion_iters)
     if args.copy_calibration:
@@ -438,8 +439,9 @@ def bench_batched_full(n_scenes: int):
             "speedup": tp / tb, "autotuned": {"block": block, "refill": refill}}
 
 
-def bench_adjoint(n_scenes: int):
-    """Forward vs forward + path-replay adjoint on sweep scenes."""
+def bench_adjoint(n_scenes: int, photons: int = 100000):
+    """Forward vs forward + path-replay adjoint on sweep scenes (photons
+    forward, photons / 4 backward per scene)."""
     import numpy as np
     import torch
 
@@ -450,7 +452,7 @@ def bench_adjoint(n_scenes: int):
     sc = hypercube(256, 1)[:n_scenes]
     th = np.array([[x.params[k] for k in adjoint.PARAMS] for x in sc])
     models = [adjoint.model_from_theta(t) for t in th]
-    nf, nb = 100000, 25000
+    nf, nb = int(photons), int(photons) // 4
     key = (0xAD70, 2)
     simulate_batched(models[:2], 1000, 250, key, strict=False, adjoint=True)
     torch.cuda.synchronize()
