@@ -849,20 +849,37 @@ __device__ __forceinline__ void flush_tally(const Tally& c, unsigned long long* 
   if (lane == 0 && ts) atomicAdd(&ctr[ALB_CTR_DEPTH_SOLVES], ts);
 }
 
-// One scene, one lane per photon; LAYERED selects the stepped water legs of
+// One scene in a persistent launch; LAYERED selects the stepped water legs of
 // a layered column, HEAT the interaction points and radius histogram of the
-// heat maps.
+// heat maps.  Lane g takes photons g, g + G, g + 2G, ... (G = lanes in the
+// grid) and starts its next photon in the iteration after the current one
+// ends, so a wave's lanes stay busy while path lengths differ (a lane per
+// photon, with the wave waiting for its longest path, took 2.2x as long).
+// The photon index alone selects the random stream, so the records do not
+// depend on the grid.
 template <bool FWD, bool LAYERED, bool HEAT>
-__global__ __launch_bounds__(256) void trace_kernel(alb_scene_t sc, alb_rng_t rng, int64_t begin,
+__global__ __launch_bounds__(256, 2) void trace_kernel(alb_scene_t sc, alb_rng_t rng, int64_t begin,
                                                     int64_t n, Sinks sk, unsigned long long* ctr) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t G = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - G;  // advanced before its first use
   const uint32_t lane = lane_id();
-  Stream st{(uint64_t)(begin + i), i, 0u, false, 0.0};
+  Stream st{0, 0, 0u, false, 0.0};
   Photon ph;
   ph.alive = false;
+  ph.event_no = 0;
+  bool done = false;
   Tally c{};
-  if (i < n) emit_photon(sc, FWD, rng, st, ph, c);
-  while (ballot(ph.alive)) {
+  for (;;) {
+    if (!ph.alive && !done) {
+      i += G;
+      if (i < n) {
+        st = Stream{(uint64_t)(begin + i), i, 0u, false, 0.0};
+        emit_photon(sc, FWD, rng, st, ph, c);
+      } else {
+        done = true;
+      }
+    }
+    if (!ballot(ph.alive || !done)) break;
     Emit e{-1, 0, 0, 0, 0, 0, 0, 0.0f, 0.0f, 0u};
     if (ph.alive) next_event<false, LAYERED, HEAT>(sc, FWD, rng, st, ph, e, c);
     const int64_t tag = (int64_t)(((uint64_t)(begin + i) << 16) | (uint64_t)(e.eno & 0xFFFFu));
@@ -1100,7 +1117,6 @@ extern "C" int alb_trace(const alb_scene_t* scene, const alb_rng_t* rng, int32_t
   if (rc != ALB_OK) return rc;
   if (n == 0) return ALB_OK;
   const int block = 256;
-  const int64_t grid = (n + block - 1) / block;
   hipStream_t s = (hipStream_t)stream;
   const bool layered = scene->layers != nullptr || scene->tilted;
   // interaction points: written into forward records whose sinks hold them;
@@ -1110,6 +1126,16 @@ extern "C" int alb_trace(const alb_scene_t* scene, const alb_rng_t* rng, int32_t
                   ? (heat ? (layered ? trace_kernel<true, true, true> : trace_kernel<true, false, true>)
                           : (layered ? trace_kernel<true, true, false> : trace_kernel<true, false, false>))
                   : (layered ? trace_kernel<false, true, false> : trace_kernel<false, false, false>);
+  // persistent grid: the resident blocks of every CU, fewer when n is small
+  int dev = 0, cus = 256, per_cu = 1;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return set_error(ALB_ERR_HIP, "alb_trace: no HIP device");
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kern, block, 0) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  int64_t grid = (int64_t)cus * per_cu;
+  const int64_t need = (n + block - 1) / block;
+  if (grid > need) grid = need;
   hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), 0, s, *scene, *rng, photon_begin, n, sk, counters);
   return check_launch("alb_trace");
 }
