@@ -48,7 +48,7 @@ constexpr int kStack = 256;       // per-wave node stack (LDS)
 constexpr int kSeed = 192;        // records of the descent seed (<= kBuf)
 constexpr int kBuf = 256;         // per-wave candidate buffer (LDS), power of two
 constexpr int kBucket = 64;       // nodes with <= kBucket records are scanned
-constexpr int kMaxWindow = 2048;  // LDS-privatised waveform bins
+constexpr int kMaxWindow = 7680;  // LDS-privatised waveform bins (60 KB of FP64: with the row tile, 64 KB per block)
 constexpr int kMaxK = kSeed;      // the seed must hold k records
 constexpr int64_t kChunk = 1 << 22;  // queries per k-NN / contribution round
 constexpr int kXcds = 8;          // MI355X XCDs; workgroups are dealt to them round-robin

@@ -69,7 +69,8 @@ def simulate_batched(models: Sequence[SceneModel], n_forward, n_backward, key, *
     scene_ids: global scene numbers selecting the Philox streams (default
     0..n-1).  Only contiguous ids are supported by the trace kernel, so
     scene_ids must be range(base, base + n).
-    win_len: LDS-privatised bins per scene, starting at model.gate(win_len).
+    win_len: LDS-privatised bins per scene, starting at model.gate(win_len)
+    (widened to runtime.LDS_WINDOW_BATCHED bins; the window affects speed only).
     block / refill: launch parameters of the batched trace; block="auto" picks
     both with pyalbsim_amd.autotune on a pilot sample (cached per workload).
     strict: raise KeyError / IndexError, as the reference does per scene
@@ -158,7 +159,7 @@ def simulate_batched(models: Sequence[SceneModel], n_forward, n_backward, key, *
     offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
     wf = torch.zeros(int(sizes.sum()), dtype=torch.float64, device="cuda")
     wf_off = torch.as_tensor(offsets, device="cuda")
-    gates = [m.gate(win_len) for m in models]
+    gates = [runtime.lds_window(m.n_bins, *m.gate(win_len), runtime.LDS_WINDOW_BATCHED) for m in models]
     wl = min(g[1] for g in gates)
     win0 = torch.as_tensor(np.array([g[0] for g in gates], dtype=np.int32), device="cuda")
     # records per scene and type, from the scene ranges of the maps
