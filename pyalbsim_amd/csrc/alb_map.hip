@@ -92,14 +92,26 @@ __global__ __launch_bounds__(256) void key_kernel(const float* __restrict__ x, c
   vals[i] = (int32_t)i;
 }
 
-__global__ __launch_bounds__(256) void permute_kernel(alb_records_t in, alb_map_t m) {
+// The sort permutation scatters records across the input: gathering the
+// eight SoA arrays through it directly costs eight random cache lines per
+// record.  The records are first packed into 32-byte rows (sequential reads
+// and writes), so the gather touches one row per record.
+__global__ __launch_bounds__(256) void pack_records_kernel(alb_records_t in, int64_t n, float4* __restrict__ rows) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  rows[2 * j] = make_float4(in.x[j], in.y[j], in.z[j], in.dx[j]);
+  rows[2 * j + 1] = make_float4(in.dy[j], in.dz[j], in.energy[j], in.time[j]);
+}
+
+__global__ __launch_bounds__(256) void permute_kernel(const float4* __restrict__ rows, alb_map_t m) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.n) return;
-  const int32_t j = m.perm[i];
-  m.x[i] = in.x[j]; m.y[i] = in.y[j]; m.z[i] = in.z[j];
-  m.dx[i] = in.dx[j]; m.dy[i] = in.dy[j]; m.dz[i] = in.dz[j];
-  m.energy[i] = in.energy[j];
-  m.time[i] = in.time[j];
+  const int64_t j = m.perm[i];
+  const float4 a = rows[2 * j], b = rows[2 * j + 1];
+  m.x[i] = a.x; m.y[i] = a.y; m.z[i] = a.z;
+  m.dx[i] = a.w; m.dy[i] = b.x; m.dz[i] = b.y;
+  m.energy[i] = b.z;
+  m.time[i] = b.w;
 }
 
 // interaction points (heat maps) follow the records' permutation
@@ -285,7 +297,7 @@ __global__ __launch_bounds__(256) void scene_root_kernel(const Node* __restrict_
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct WsLayout {
-  size_t keys_in, vals_in, level, order, small, temp, total, temp_bytes;
+  size_t keys_in, vals_in, level, order, small, temp, rows, total, temp_bytes;
 };
 
 static int layout(int64_t n, WsLayout* L) {
@@ -302,6 +314,7 @@ static int layout(int64_t n, WsLayout* L) {
   L->order = off; off = align256(off + 4 * (size_t)nn);
   L->small = off; off = align256(off + 4 * (3 * kLevels + 8) + 64);
   L->temp = off; off = align256(off + temp_bytes);
+  L->rows = off; off = align256(off + 32 * (size_t)nn);  // packed records for the permutation
   L->total = off;
   L->temp_bytes = temp_bytes;
   return ALB_OK;
@@ -379,7 +392,9 @@ static int map_build(const alb_records_t* in, int64_t n, int32_t dims, alb_map_t
                                                     map->perm, (int)n, 0, key_bits, s);
   if (e != hipSuccess) return set_error(ALB_ERR_HIP, "%s: sort failed: %s", who, hipGetErrorString(e));
   alb_map_t m = *map;
-  hipLaunchKernelGGL(permute_kernel, dim3(grid), dim3(B), 0, s, *in, m);
+  float4* rows = (float4*)(ws + L.rows);
+  hipLaunchKernelGGL(pack_records_kernel, dim3(grid), dim3(B), 0, s, *in, n, rows);
+  hipLaunchKernelGGL(permute_kernel, dim3(grid), dim3(B), 0, s, (const float4*)rows, m);
   if (in->fwi_x && m.fwi_x) hipLaunchKernelGGL(permute_points_kernel, dim3(grid), dim3(B), 0, s, *in, m);
   if (n > 1) {
     const unsigned g2 = (unsigned)((n - 1 + B - 1) / B);

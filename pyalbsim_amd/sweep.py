@@ -84,7 +84,7 @@ class SweepResult:
 
 
 def estimate_group_bytes(models, n_forward, n_backward, key, pilot_fraction: float = 1 / 100,
-                         bytes_per_record: float = 200.0, bytes_per_query: float = 80.0) -> np.ndarray:
+                         bytes_per_record: float = 240.0, bytes_per_query: float = 80.0) -> np.ndarray:
     """Device bytes each scene needs inside a batched group: a pilot forward
     trace of pilot_fraction of every scene's photons gives its records per
     photon by type; records cost sinks + map + radix tree + build workspace
