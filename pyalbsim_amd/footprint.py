@@ -1,6 +1,7 @@
 """Footprint-grid queries for the fixed-radius gather.
 
-A grid x grid set of query points on the seafloor plane, centred on the
+A grid x grid set of query points on the seafloor plane (the tilted plane
+n . p = c where the floor is tilted, over the same (x, z) grid), centred on the
 laser's seafloor intersection (SimulationModel.sea_floor_intersection,
 src/alb_sim/physics/models/simulation.py:104-124: the boresight refracted at
 the surface with eta = 1 / n of layer 0, physics/reflection/snell_refraction.py:7-45),
@@ -48,10 +49,14 @@ def footprint_queries(model: SceneModel, grid: int = 256, pitch: float = 0.01) -
     gx, gz = np.meshgrid(c[0] + off, c[2] + off, indexing="ij")
     x = gx.ravel()
     z = gz.ravel()
-    y = np.full_like(x, model.y_sf)
+    if getattr(model, "tilted", False):
+        nrm = np.asarray(model.floor_normal, dtype=np.float64)
+        y = (model.floor_offset - nrm[0] * x - nrm[2] * z) / nrm[1]
+    else:
+        y = np.full_like(x, model.y_sf)
     length = np.sqrt(x * x + y * y + z * z)
     dx, dy, dz = x / length, y / length, z / length
-    air = length * (model.y_ws / y)
+    air = length * (model.y_ws / y)  # the straight sensor -> point path crosses the surface at y_ws
     water = length - air
     steps = (air / model.v_air + water / _water_speed(model)) / model.dt
     return {"x": x, "y": y, "z": z, "dx": dx, "dy": dy, "dz": dz,

@@ -19,7 +19,7 @@ import oracle
 from pyalbsim_amd import abi, runtime
 from pyalbsim_amd.footprint import footprint_queries, upload_queries
 from pyalbsim_amd.model import build_scene
-from tests.scenes import layered_scene, run_linear_script_scene, single_shot
+from tests.scenes import layered_scene, run_linear_script_scene, single_shot, tilted_shot
 
 pytestmark = pytest.mark.gpu
 
@@ -56,9 +56,10 @@ def _check(model, ds, dmap, n, t, sink, qhost, nq, r, sort=True):
     return int(rcnt.sum())
 
 
-@pytest.mark.parametrize("scene", ["single_shot", "turbid", "layered"])
+@pytest.mark.parametrize("scene", ["single_shot", "turbid", "layered", "tilted"])
 def test_footprint_grid_matches_oracle(gpu, scene):
-    cfg = {"single_shot": single_shot, "turbid": run_linear_script_scene, "layered": layered_scene}[scene]()
+    cfg = {"single_shot": single_shot, "turbid": run_linear_script_scene, "layered": layered_scene,
+           "tilted": lambda: tilted_shot(slope=8.0, azimuth=150.0)}[scene]()
     model = build_scene(cfg)
     ds, recs, rc, maps, qs, qc = _setup(model, 30000, 10, (21, 22))
     fq = footprint_queries(model, grid=24, pitch=0.05)
