@@ -144,6 +144,7 @@ def main(argv=None):
     ap.add_argument("--no-inversion", action="store_true")
     ap.add_argument("--inversion-scenes", type=int, default=32)
     ap.add_argument("--inversion-iters", type=int, default=10)
+    ap.add_argument("--inversion-lr", type=float, default=0.02)
     ap.add_argument("--radius-grid", type=int, default=256)
     ap.add_argument("--sweep-scenes", type=int, default=256)
     ap.add_argument("--sweep-full", type=int, default=0,
@@ -275,7 +276,7 @@ def main(argv=None):
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(args.adjoint_scenes, int(args.adjoint_photons))
     if not args.no_inversion:
-        out["inversion"] = bench_inversion(args.inversion_scenes, args.inversion_iters)
+        out["inversion"] = bench_inversion(args.inversion_scenes, args.inversion_iters, lr=args.inversion_lr)
     if args.copy_calibration:
         x = torch.empty(1 << 28, dtype=torch.float64, device="cuda")
         y = torch.empty_like(x)
@@ -334,7 +335,7 @@ def bench_radius(model, ds, key, n_f: int, grid: int, radius: float = 0.05, pitc
     return out
 
 
-def bench_inversion(n_scenes: int, iters: int, n_forward: int = 20000):
+def bench_inversion(n_scenes: int, iters: int, n_forward: int = 20000, lr: float = 0.02):
     """Adam over the batched adjoint on synthetic inversion scenes: depth RMSE
     and seconds per iteration, batched form vs plain (one scene per launch)."""
     import numpy as np
@@ -343,9 +344,10 @@ def bench_inversion(n_scenes: int, iters: int, n_forward: int = 20000):
 
     th = inversion_scenes(n_scenes, seed=3)
     tilts = inversion_tilts(n_scenes, seed=3)
-    rb = invert(th, n_forward, n_forward // 4, iterations=iters, lr=0.02, tilts=tilts)
+    rb = invert(th, n_forward, n_forward // 4, iterations=iters, lr=lr, tilts=tilts,
+                log=(lambda m: print(m, file=sys.stderr, flush=True)) if iters > 20 else None)
     print("inversion batched done", file=sys.stderr, flush=True)
-    rp = invert(th, n_forward, n_forward // 4, iterations=2, lr=0.02, form="plain", tilts=tilts)
+    rp = invert(th, n_forward, n_forward // 4, iterations=2, lr=lr, form="plain", tilts=tilts)
     print("inversion plain done", file=sys.stderr, flush=True)
     sb = float(np.median(rb.seconds_per_iteration[1:] or rb.seconds_per_iteration))
     sp = float(np.median(rp.seconds_per_iteration))
@@ -353,8 +355,9 @@ def bench_inversion(n_scenes: int, iters: int, n_forward: int = 20000):
                         f"(a = 0.1 /m, g = 0.9 known; roughness from wind U[0,10] m/s; floor tilted U[0,15] deg, "
                         f"azimuth U[0,360) deg, known), "
                         f"{n_forward} forward + {n_forward // 4} backward photons per scene per pass, "
-                        f"Adam lr 0.02 on [0,1]-scaled unknowns, {iters} iterations",
+                        f"Adam lr {lr:g} on [0,1]-scaled unknowns, {iters} iterations (plain form timed over 2)",
             "depth_rmse_start_m": rb.depth_rmse[0], "depth_rmse_end_m": rb.depth_rmse[-1],
+            "depth_rmse_every_10": rb.depth_rmse[::10],
             "loss_start": rb.loss[0], "loss_end": rb.loss[-1],
             "batched_s_per_iteration": sb, "plain_s_per_iteration": sp, "speedup": sp / sb}
 

