@@ -49,6 +49,7 @@ constexpr int kSeed = 192;        // records of the descent seed (<= kBuf)
 constexpr int kBuf = 256;         // per-wave candidate buffer (LDS), power of two
 constexpr int kBucket = 64;       // nodes with <= kBucket records are scanned
 constexpr int kMaxWindow = 7680;  // LDS-privatised waveform bins (60 KB of FP64: with the row tile, 64 KB per block)
+constexpr int kMaxWindowLarge = 16384;  // batched contributions with one 1024-thread block per CU (132 KB)
 constexpr int kMaxK = kSeed;      // the seed must hold k records
 constexpr int64_t kChunk = 1 << 22;  // queries per k-NN / contribution round
 constexpr int kXcds = 8;          // MI355X XCDs; workgroups are dealt to them round-robin
@@ -893,8 +894,8 @@ __device__ __forceinline__ void flush_window(double* hist, int win_len, double* 
   }
 }
 
-template <int TYPE>
-__global__ __launch_bounds__(256) void contrib_batched_kernel(ContribBatchedArgs a) {
+template <int TYPE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void contrib_batched_kernel(ContribBatchedArgs a) {
   extern __shared__ double hist[];  // [win_len]
   __shared__ RowTile R;
   for (int t = threadIdx.x; t < a.win_len; t += blockDim.x) hist[t] = 0.0;
@@ -1291,8 +1292,8 @@ extern "C" int alb_contrib_batched(const alb_scene_t* scenes, int32_t n_scenes, 
   if (!q->scene) return set_error(ALB_ERR_ARG, "alb_contrib_batched: queries carry no scene array");
   if (photon_type < 0 || photon_type >= ALB_N_TYPES) return set_error(ALB_ERR_ARG, "alb_contrib_batched: bad photon type");
   if (k <= 0 || k > kMaxK) return set_error(ALB_ERR_UNSUPPORTED, "alb_contrib_batched: k = %d outside [1, %d]", k, kMaxK);
-  if (win_len < 0 || win_len > kMaxWindow)
-    return set_error(ALB_ERR_ARG, "alb_contrib_batched: window length %d outside [0, %d]", win_len, kMaxWindow);
+  if (win_len < 0 || win_len > kMaxWindowLarge)
+    return set_error(ALB_ERR_ARG, "alb_contrib_batched: window length %d outside [0, %d]", win_len, kMaxWindowLarge);
   if (n_replicas < 0 || (n_replicas > 0 && (!replicas || nb_max <= 0)))
     return set_error(ALB_ERR_ARG, "alb_contrib_batched: replicas missing");
   if (cn <= 0) return ALB_OK;
@@ -1307,13 +1308,32 @@ extern "C" int alb_contrib_batched(const alb_scene_t* scenes, int32_t n_scenes, 
                         win_len, n_replicas > 0 ? replicas : nullptr, n_replicas > 0 ? n_replicas : 1, n_scenes,
                         nb_max};
   const int64_t tiles = (cn + kRowTile - 1) / kRowTile;
-  int64_t cb = (int64_t)cus * 8;
-  if (cb > tiles) cb = tiles;
   using CBK = void (*)(ContribBatchedArgs);
-  static const CBK bkern[ALB_N_TYPES] = {contrib_batched_kernel<ALB_BOTTOM_REFLECTION>, contrib_batched_kernel<ALB_SCATTER>,
-                                         contrib_batched_kernel<ALB_SURFACE_REFLECTION>,
-                                         contrib_batched_kernel<ALB_SURFACE_TRANSMISSION_UP>};
-  hipLaunchKernelGGL(bkern[photon_type], dim3((unsigned)cb), dim3(256), sizeof(double) * (size_t)win_len, s, ca);
+  static const CBK bkern[2][ALB_N_TYPES] = {
+      {contrib_batched_kernel<ALB_BOTTOM_REFLECTION, 256>, contrib_batched_kernel<ALB_SCATTER, 256>,
+       contrib_batched_kernel<ALB_SURFACE_REFLECTION, 256>, contrib_batched_kernel<ALB_SURFACE_TRANSMISSION_UP, 256>},
+      {contrib_batched_kernel<ALB_BOTTOM_REFLECTION, 1024>, contrib_batched_kernel<ALB_SCATTER, 1024>,
+       contrib_batched_kernel<ALB_SURFACE_REFLECTION, 1024>, contrib_batched_kernel<ALB_SURFACE_TRANSMISSION_UP, 1024>}};
+  const size_t lds = sizeof(double) * (size_t)win_len;
+  if (win_len <= kMaxWindow) {  // 256-thread blocks, several per CU
+    int64_t cb = (int64_t)cus * 8;
+    if (cb > tiles) cb = tiles;
+    hipLaunchKernelGGL(bkern[0][photon_type], dim3((unsigned)cb), dim3(256), lds, s, ca);
+  } else {  // a window beyond one block's default 64 KB of LDS: one 1024-thread block per CU
+    int dev = 0, maxsh = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      return set_error(ALB_ERR_HIP, "alb_contrib_batched: no HIP device");
+    if (lds + sizeof(RowTile) > (size_t)maxsh)
+      return set_error(ALB_ERR_UNSUPPORTED, "alb_contrib_batched: window of %d bins needs %zu B of LDS per block, %d available",
+                       win_len, lds + sizeof(RowTile), maxsh);
+    if (hipFuncSetAttribute((const void*)bkern[1][photon_type], hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess)
+      return set_error(ALB_ERR_HIP, "alb_contrib_batched: LDS size %zu refused", lds);
+    int64_t cb = (int64_t)cus;
+    if (cb > tiles) cb = tiles;
+    hipLaunchKernelGGL(bkern[1][photon_type], dim3((unsigned)cb), dim3(1024), lds, s, ca);
+  }
   if (n_replicas > 0) {
     const int64_t tot = (int64_t)n_scenes * nb_max;
     hipLaunchKernelGGL(replica_reduce_batched_kernel, dim3((unsigned)((tot + 255) / 256 < 8192 ? (tot + 255) / 256 : 8192)),

@@ -349,11 +349,11 @@ CHUNK = 1 << 22  # queries per k-NN / contribution round
 # start takes ~99 % of their FP64 adds into LDS; a 1000-bin window took
 # 5-12 % and left the rest to global atomics.  The window changes the order
 # of the FP64 additions only.  The batched sweep's deeper, more turbid scenes
-# have later tails: their window is the largest that fits one block's 64 KB
-# (contribution 251 -> 177 ms on 256 sweep scenes; the single shot gains
-# nothing beyond 4096).
+# have later tails: their window is 16384 bins (128 KB of LDS, one 1024-thread
+# block per CU; contribution 251 -> 177 ms at 7680 bins, 108 ms at 16384 on
+# 256 sweep scenes); the single shot gains nothing beyond 4096.
 LDS_WINDOW = 4096
-LDS_WINDOW_BATCHED = 7680
+LDS_WINDOW_BATCHED = 16384
 
 
 def lds_window(n_bins: int, win0: int, win_len: int, width: int = None) -> tuple:
