@@ -82,6 +82,7 @@ def inversion_tilts(n: int, seed: int = 3) -> np.ndarray:
 class InversionResult:
     theta_true: np.ndarray
     theta: np.ndarray                      # final estimates [S, 6]
+    scene_loss: np.ndarray = None          # per-scene loss of the last iteration [S]
     depth_rmse: list = field(default_factory=list)   # per iteration (before the step)
     loss: list = field(default_factory=list)
     seconds_per_iteration: list = field(default_factory=list)
@@ -166,11 +167,12 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
         theta[:, idx] = lo_t + x * span_t
         w = run(theta, key)
         if loss == "cumulative":
-            loss_v = waveform_loss(w, target, shape_weight).sum()
+            per = waveform_loss(w, target, shape_weight)
         elif loss == "log":
-            loss_v = log_waveform_loss(w, target).sum()
+            per = log_waveform_loss(w, target)
         else:  # "hybrid": both terms
-            loss_v = (waveform_loss(w, target, shape_weight) + log_waveform_loss(w, target)).sum()
+            per = waveform_loss(w, target, shape_weight) + log_waveform_loss(w, target)
+        loss_v = per.sum()
         loss_v.backward()
         opt.step()
         with torch.no_grad():
@@ -181,6 +183,7 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
         rmse = float(np.sqrt(np.mean((th_np[:, COLS["depth"]] - theta_true[:, COLS["depth"]]) ** 2)))
         out.depth_rmse.append(rmse)
         out.loss.append(float(loss_v.item()))
+        out.scene_loss = per.detach().cpu().numpy()
         out.seconds_per_iteration.append(dt)
         if log is not None:
             log(f"iteration {it + 1}/{iterations}: loss {out.loss[-1]:.4g}, depth RMSE {rmse:.3f} m, {dt:.2f} s")
