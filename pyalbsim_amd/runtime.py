@@ -437,7 +437,7 @@ def radius_gather(dscene: DeviceScene, dmap: DeviceMap, ptype: int, queries: Sin
     keep), keep holding the buffers the launch uses."""
     torch = _torch()
     L = load_library()
-    win0, win_len = window
+    win0, win_len = window  # its pairs fall in the gate; a wider window only costs occupancy
     s = stream if stream is not None else _stream_ptr()
     dev = waveform.device
     order = ws = None
