@@ -854,7 +854,7 @@ __device__ __forceinline__ void flush_tally(const Tally& c, unsigned long long* 
 // heat maps.  Lane g takes photons g, g + G, g + 2G, ... (G = lanes in the
 // grid) and starts its next photon in the iteration after the current one
 // ends, so a wave's lanes stay busy while path lengths differ (a lane per
-// photon, with the wave waiting for its longest path, took 2.2x as long).
+// photon, with the wave waiting for its longest path, took twice as long).
 // The photon index alone selects the random stream, so the records do not
 // depend on the grid.
 template <bool FWD, bool LAYERED, bool HEAT>
