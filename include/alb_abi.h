@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define ALB_ABI_VERSION 8
+#define ALB_ABI_VERSION 9
 
 /* photon types, reference enum order (BOTTOM_REFLECTION = auto() = 1, ...) */
 enum alb_photon_type {
@@ -385,6 +385,10 @@ int alb_gather(const alb_scene_t* scene, const alb_map_t* map,
  *   knorm[r] the kernel normalisation 1/(pi r^2), or 1/(4/3 pi r^3) for
  *   photon_type ALB_SCATTER.  Neighbours are in (distance, index) order when
  *   rows_by_query != 0, else in no particular order.
+ *   work: device unsigned long long[ALB_KNN_WORK_WORDS] of scratch (the
+ *   counters from which the wavefronts take queries); the call clears it on
+ *   the stream, so calls in sequence on one stream may share it and calls
+ *   that can overlap need their own.
  * alb_contrib: adds the energies of those rows to the waveform.  Bins outside
  *   the LDS window go through n_replicas copies of the waveform (replicas:
  *   device double[n_replicas * n_bins], contents need not be zero), so that
@@ -393,9 +397,11 @@ int alb_gather(const alb_scene_t* scene, const alb_map_t* map,
  *   stream).  n_replicas = 0 adds straight into waveform. */
 int alb_gather_order(const alb_map_t* map, const alb_queries_t* q, int64_t q_begin, int64_t q_n,
                      int32_t* order, void* workspace, size_t workspace_bytes, void* stream);
+#define ALB_KNN_WORK_WORDS 256
 int alb_knn(const alb_map_t* map, int32_t photon_type, int32_t k, const alb_queries_t* q,
             int64_t q_begin, const int32_t* order, int64_t c0, int64_t cn, int32_t rows_by_query,
-            int32_t* nbr, double* radius, double* knorm, unsigned long long* counters, void* stream);
+            int32_t* nbr, double* radius, double* knorm, unsigned long long* counters,
+            unsigned long long* work, void* stream);
 int alb_contrib(const alb_scene_t* scene, const alb_map_t* map, int32_t photon_type,
                 const alb_queries_t* q, int64_t q_begin, const int32_t* order, int64_t c0, int64_t cn,
                 int32_t rows_by_query, const int32_t* nbr, const double* knorm, double* waveform,
@@ -425,7 +431,8 @@ int alb_radius_gather(const alb_scene_t* scene, const alb_map_t* map, int32_t ph
  * s gets exactly the neighbour sets and waveform a one-scene gather of its
  * records would give.  alb_gather_order_batched sorts queries by (scene,
  * Morton key); alb_knn_batched writes rows in slot order (rows of scenes with
- * fewer than k records are -1 / NaN and counted in ALB_CTR_SMALL_SCENE);
+ * fewer than k records are -1 / NaN and counted in ALB_CTR_SMALL_SCENE; work
+ * as for alb_knn);
  * alb_contrib_batched reads the scene table scenes[n_scenes] (device) and adds
  * scene s's type-t waveform into waveforms[wf_offset[s] + t * 2 * steps_s ...],
  * LDS-privatising bins [win0[s], win0[s] + win_len) (device arrays); other
@@ -440,7 +447,7 @@ int alb_gather_order_batched(const alb_map_t* map, const alb_scene_index_t* inde
 int alb_knn_batched(const alb_map_t* map, const alb_scene_index_t* index, int32_t photon_type,
                     int32_t k, const alb_queries_t* q, const int32_t* order, int64_t c0,
                     int64_t cn, int32_t* nbr, double* radius, double* knorm,
-                    unsigned long long* counters, void* stream);
+                    unsigned long long* counters, unsigned long long* work, void* stream);
 int alb_contrib_batched(const alb_scene_t* scenes, int32_t n_scenes, int32_t k,
                         const alb_map_t* map, int32_t photon_type, const alb_queries_t* q,
                         const int32_t* order, int64_t c0, int64_t cn, const int32_t* nbr,

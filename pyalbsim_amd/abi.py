@@ -5,7 +5,8 @@ import ctypes as C
 
 import numpy as np
 
-ALB_ABI_VERSION = 8
+ALB_ABI_VERSION = 9
+KNN_WORK_WORDS = 256  # ALB_KNN_WORK_WORDS: scratch words of alb_knn / alb_knn_batched
 N_TYPES = 4
 N_COUNTERS = 19
 

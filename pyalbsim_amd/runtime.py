@@ -58,7 +58,7 @@ def load_library(path: str = LIB_PATH):
                                     C.c_int32, C.c_int32, P, P, P, P]
     L.alb_radius_gather.restype = C.c_int
     L.alb_gather_order.restype = C.c_int
-    L.alb_knn.argtypes = [P, C.c_int32, C.c_int32, P, C.c_int64, P, C.c_int64, C.c_int64, C.c_int32, P, P, P, P, P]
+    L.alb_knn.argtypes = [P, C.c_int32, C.c_int32, P, C.c_int64, P, C.c_int64, C.c_int64, C.c_int32, P, P, P, P, P, P]
     L.alb_knn.restype = C.c_int
     L.alb_contrib.argtypes = [P, P, C.c_int32, P, C.c_int64, P, C.c_int64, C.c_int64, C.c_int32, P, P, P, C.c_int64,
                               C.c_int32, C.c_int32, P, C.c_int32, P]
@@ -69,7 +69,7 @@ def load_library(path: str = LIB_PATH):
     L.alb_map_build_batched.restype = C.c_int
     L.alb_gather_order_batched.argtypes = [P, P, P, C.c_int64, P, P, C.c_size_t, P]
     L.alb_gather_order_batched.restype = C.c_int
-    L.alb_knn_batched.argtypes = [P, P, C.c_int32, C.c_int32, P, P, C.c_int64, C.c_int64, P, P, P, P, P]
+    L.alb_knn_batched.argtypes = [P, P, C.c_int32, C.c_int32, P, P, C.c_int64, C.c_int64, P, P, P, P, P, P]
     L.alb_knn_batched.restype = C.c_int
     L.alb_contrib_batched.argtypes = [P, C.c_int32, C.c_int32, P, C.c_int32, P, P, C.c_int64, C.c_int64, P, P, P,
                                       P, P, C.c_int32, P, C.c_int32, C.c_int64, P]
@@ -404,11 +404,13 @@ def gather(dscene: DeviceScene, dmap: DeviceMap, ptype: int, queries: Sink, q_n:
         e.record(torch.cuda.current_stream())
         return e
 
+    work = torch.empty(abi.KNN_WORK_WORDS, dtype=torch.int64, device=dev)  # cleared by each alb_knn call
     for c0 in range(0, q_n, chunk):
         cn = min(chunk, q_n - c0)
         e0 = ev() if events is not None else None
         _check(L.alb_knn(C.byref(dmap.struct), ptype, k, C.byref(queries.struct), q_begin, _p(order), c0, cn,
-                         1 if direct else 0, _p(nbr), _p(rad), _p(knorm), _p(counters), C.c_void_p(s)), "alb_knn")
+                         1 if direct else 0, _p(nbr), _p(rad), _p(knorm), _p(counters), _p(work), C.c_void_p(s)),
+               "alb_knn")
         e1 = ev() if events is not None else None
         _check(L.alb_contrib(C.byref(dscene.s), C.byref(dmap.struct), ptype, C.byref(queries.struct), q_begin,
                              _p(order), c0, cn, 1 if direct else 0, _p(nbr), _p(knorm), _p(waveform),
@@ -732,12 +734,13 @@ def gather_batched(table: DeviceSceneTable, dmap: DeviceMap, ptype: int, queries
         e.record(torch.cuda.current_stream())
         return e
 
+    work = torch.empty(abi.KNN_WORK_WORDS, dtype=torch.int64, device=dev)  # cleared by each alb_knn_batched call
     for c0 in range(0, q_n, chunk):
         cn = min(chunk, q_n - c0)
         e0 = ev() if events is not None else None
         _check(L.alb_knn_batched(C.byref(dmap.struct), C.byref(dmap.index), ptype, k, C.byref(queries.struct),
-                                 _p(order), c0, cn, _p(nbr), _p(rad), _p(knorm), _p(counters), C.c_void_p(s)),
-               "alb_knn_batched")
+                                 _p(order), c0, cn, _p(nbr), _p(rad), _p(knorm), _p(counters), _p(work),
+                                 C.c_void_p(s)), "alb_knn_batched")
         e1 = ev() if events is not None else None
         _check(L.alb_contrib_batched(_p(table.dev), table.n, k, C.byref(dmap.struct), ptype, C.byref(queries.struct),
                                      _p(order), c0, cn, _p(nbr), _p(knorm), _p(waveforms), _p(wf_offset), _p(win0),

@@ -48,7 +48,7 @@ def test_last_error_is_set_by_argument_checks():
     assert rc == abi.ALB_ERR_ARG and b"null" in lib.alb_last_error()
     rc = lib.alb_map_build_batched(None, 0, 2, None, None, None, 0, None)
     assert rc == abi.ALB_ERR_ARG
-    rc = lib.alb_knn_batched(None, None, 1, 100, None, None, 0, 1, None, None, None, None, None)
+    rc = lib.alb_knn_batched(None, None, 1, 100, None, None, 0, 1, None, None, None, None, None, None)
     assert rc == abi.ALB_ERR_ARG
     rc = lib.alb_contrib_batched(None, 1, 100, None, 1, None, None, 0, 1, None, None, None, None, None, 0, None, 0, 0,
                                  None)
