@@ -436,8 +436,8 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   const float* __restrict__ mz = a.z;
   const int64_t n = a.n;
   const int k = a.k;
-  // per-wave event counts of one launch (at most 2^22 queries per launch spread
-  // over thousands of waves, so 32 bits hold them)
+  // per-wave event counts of one launch (diagnostics only; thousands of waves
+  // share a launch's queries, so 32 bits hold them)
   uint32_t c_q = 0, c_cand = 0, c_nodes = 0, c_ovf = 0, c_steps = 0, c_sorts = 0, c_desc = 0, c_small = 0;
 
   // XCD-aware slot ranges: the grid is a multiple of kXcds blocks and block b
