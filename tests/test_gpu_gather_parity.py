@@ -126,3 +126,32 @@ def test_gather_errors_like_reference(gpu):
     model = build_scene(single_shot(k=100))
     with pytest.raises((IndexError, KeyError)):
         simulate(model, 20, 200, (1, 1))
+
+
+def test_gather_in_many_rounds_equals_one_round(gpu, monkeypatch):
+    """Queries split into rounds of runtime.CHUNK (slot ranges c0 > 0, rows
+    reused) give the one-round waveforms up to FP64 summation order, plain
+    and batched."""
+    torch = gpu
+    from pyalbsim_amd.execution.batched import simulate_batched
+
+    model = build_scene(run_linear_script_scene())
+    ds, recs, rc, maps, qs, qc = _setup(model, 20000, 1500, (9, 10))
+    t = abi.SCATTER
+    assert qc[t] > 5000
+    one = torch.zeros(model.n_bins, dtype=torch.float64, device="cuda")
+    runtime.gather(ds, maps[t], t, qs[t], qc[t], one, model.gate())
+    monkeypatch.setattr(runtime, "CHUNK", 1000)
+    many = torch.zeros(model.n_bins, dtype=torch.float64, device="cuda")
+    runtime.gather(ds, maps[t], t, qs[t], qc[t], many, model.gate())
+    torch.cuda.synchronize()
+    a, b = many.cpu().numpy(), one.cpu().numpy()
+    assert np.allclose(a, b, rtol=1e-10, atol=1e-14 * float(np.abs(b).max()))
+    models = [build_scene(single_shot(b=0.6)), model]
+    r_many = simulate_batched(models, [4000, 3000], [800, 600], (3, 5), strict=False)
+    monkeypatch.setattr(runtime, "CHUNK", 1 << 24)
+    r_one = simulate_batched(models, [4000, 3000], [800, 600], (3, 5), strict=False)
+    assert r_one.query_counts[abi.SCATTER] > 1000
+    for s in range(2):
+        x, y = r_many.waveform(s).cpu().numpy(), r_one.waveform(s).cpu().numpy()
+        assert np.allclose(x, y, rtol=1e-10, atol=1e-14 * float(np.abs(y).max()))
