@@ -851,17 +851,24 @@ __device__ __forceinline__ void flush_tally(const Tally& c, unsigned long long* 
 
 // One scene in a persistent launch; LAYERED selects the stepped water legs of
 // a layered column, HEAT the interaction points and radius histogram of the
-// heat maps.  Lane g takes photons g, g + G, g + 2G, ... (G = lanes in the
-// grid) and starts its next photon in the iteration after the current one
-// ends, so a wave's lanes stay busy while path lengths differ (a lane per
-// photon, with the wave waiting for its longest path, took twice as long).
-// The photon index alone selects the random stream, so the records do not
-// depend on the grid.
+// heat maps.  Each wavefront owns a contiguous range of photons; a lane whose
+// photon has ended takes the range's next index in the following iteration
+// (a wave-uniform counter held in a register), so the wave's lanes stay busy
+// while path lengths differ and the wave ends with its range.  (A lane per
+// photon, with the wave waiting for its longest path, took twice as long;
+// fixed per-lane lists of photons left each wave waiting for its slowest
+// lane.)  The photon index alone selects the random stream, so the records
+// do not depend on the grid.
 template <bool FWD, bool LAYERED, bool HEAT>
 __global__ __launch_bounds__(256, 2) void trace_kernel(alb_scene_t sc, alb_rng_t rng, int64_t begin,
                                                     int64_t n, Sinks sk, unsigned long long* ctr) {
-  const int64_t G = (int64_t)gridDim.x * blockDim.x;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - G;  // advanced before its first use
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const int64_t wid = (int64_t)blockIdx.x * (blockDim.x >> 6) + (int64_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t per = (n + waves - 1) / waves;
+  const int64_t w_lo = wid * per < n ? wid * per : n;
+  const int64_t w_hi = w_lo + per < n ? w_lo + per : n;
+  int64_t w_next = w_lo;  // wave-uniform
+  int64_t i = 0;
   const uint32_t lane = lane_id();
   Stream st{0, 0, 0u, false, 0.0};
   Photon ph;
@@ -870,13 +877,19 @@ __global__ __launch_bounds__(256, 2) void trace_kernel(alb_scene_t sc, alb_rng_t
   bool done = false;
   Tally c{};
   for (;;) {
-    if (!ph.alive && !done) {
-      i += G;
-      if (i < n) {
-        st = Stream{(uint64_t)(begin + i), i, 0u, false, 0.0};
-        emit_photon(sc, FWD, rng, st, ph, c);
-      } else {
-        done = true;
+    const bool need = !ph.alive && !done;
+    const uint64_t m = ballot(need);
+    if (m) {
+      const int64_t idx = w_next + (int64_t)mask_rank(m);
+      w_next += (int64_t)__popcll(m);
+      if (need) {
+        if (idx < w_hi) {
+          i = idx;
+          st = Stream{(uint64_t)(begin + i), i, 0u, false, 0.0};
+          emit_photon(sc, FWD, rng, st, ph, c);
+        } else {
+          done = true;
+        }
       }
     }
     if (!ballot(ph.alive || !done)) break;
