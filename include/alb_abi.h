@@ -314,8 +314,10 @@ const char* alb_last_error(void);      /* thread-local message of last error  */
  * writes map records into out[4]; backward pass writes gather queries into
  * q[4] (enter vertices go to both surface types).  counters: device
  * uint64[ALB_N_COUNTERS] or NULL.
- * Replaces core/simulation.py:71-107 (simulate_batch) with one lane per photon,
- * closed-form air legs and exact sub-step quantisation in water. */
+ * Replaces core/simulation.py:71-107 (simulate_batch) with a persistent grid
+ * (each wavefront refills its lanes from its own range of photons; a photon's
+ * random stream depends on its index only), closed-form air legs and exact
+ * sub-step quantisation in water. */
 int alb_trace(const alb_scene_t* scene, const alb_rng_t* rng, int32_t pass,
               int64_t photon_begin, int64_t n, const alb_records_t* out,
               const alb_queries_t* q, unsigned long long* counters,
