@@ -355,7 +355,9 @@ def bench_inversion(n_scenes: int, iters: int, n_forward: int = 20000, lr: float
                         f"(a = 0.1 /m, g = 0.9 known; roughness from wind U[0,10] m/s; floor tilted U[0,15] deg, "
                         f"azimuth U[0,360) deg, known), "
                         f"{n_forward} forward + {n_forward // 4} backward photons per scene per pass, "
-                        f"Adam lr {lr:g} on [0,1]-scaled unknowns, {iters} iterations (plain form timed over 2)",
+                        f"Adam lr {lr:g} on [0,1]-scaled unknowns, depth started from the target's "
+                        f"floor-return leading edge (mid-range where none shows), {iters} iterations "
+                        f"(plain form timed over 2)",
             "depth_rmse_start_m": rb.depth_rmse[0], "depth_rmse_end_m": rb.depth_rmse[-1],
             "depth_rmse_every_10": rb.depth_rmse[::10],
             "loss_start": rb.loss[0], "loss_end": rb.loss[-1],

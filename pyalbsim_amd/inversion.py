@@ -28,6 +28,13 @@ surface's, out of sight.  Together they gave the lowest depth RMSE (32
 scenes, 100 iterations: 8.7 m against 11.1 m and 13.8 m alone).  Depth is
 identifiable only where the floor return survives the water column.
 
+Start (start="timing", the default): the depth of each scene whose target
+shows a floor return is read from that return's leading edge
+(floor_timing_depth), the rest start at mid-range.  The leading edge is
+measured once from the target before the first iteration; no iteration
+depends on it.  32 scenes, 10 iterations: depth RMSE 8.35 m from mid-range,
+4.53 m from the timing start (128 other scenes: 9.23 m and 5.99 m).
+
 The floor may be tilted (slope U[0, 15] degrees, azimuth U[0, 360), known
 per scene: inversion_tilts); the tilted-floor handlers are an extension of
 the reference's flat floor.
@@ -121,16 +128,60 @@ def log_waveform_loss(w, target, floor_rel: float = 1e-4, pool: int = 4):
     return (d * d).mean(dim=1)
 
 
+def floor_timing_depth(total: np.ndarray, bins_per_m, surface_bin, pool: int = 4, smooth: int = 3,
+                       floor_rel: float = 1e-6, min_rise: float = 0.4, skip_m: float = 1.5) -> np.ndarray:
+    """Per-scene depth [S] from the leading edge of the floor return.
+
+    total: [S, L] gated waveforms summed over photon types; bins_per_m: gate
+    bins per metre of (nadir, two-way) water depth, 2 / (v_water x bin
+    width); surface_bin: the surface return's bin within the gate.  The
+    volume return only decays after the surface, so the steepest rise of the
+    log waveform (pooled over `pool` bins, then a running mean over `smooth`
+    pooled bins) later than skip_m metres below the surface marks the floor.
+    NaN where no rise of at least min_rise (natural-log units per pooled bin)
+    between pooled bins above floor_rel x the peak exists: deep or turbid
+    columns whose floor return does not survive."""
+    total = np.asarray(total, dtype=np.float64)
+    S, L = total.shape
+    P = L // pool
+    p = total[:, :P * pool].reshape(S, P, pool).sum(axis=2)
+    bpm = np.broadcast_to(np.asarray(bins_per_m, dtype=np.float64), (S,))
+    sb = np.broadcast_to(np.asarray(surface_bin, dtype=np.float64), (S,))
+    out = np.full(S, np.nan)
+    half = smooth // 2
+    for i in range(S):
+        mx = p[i].max()
+        if not mx > 0.0:
+            continue
+        lg = np.log(p[i] + 1e-12 * mx)
+        if smooth > 1:  # running mean, edge values repeated (no zero padding)
+            lg = np.convolve(np.pad(lg, half, mode="edge"), np.ones(smooth) / smooth, mode="valid")[:P]
+        rise = np.diff(lg)
+        live = (p[i][1:] > floor_rel * mx) & (p[i][:-1] > floor_rel * mx)
+        live[:max(0, int((sb[i] + skip_m * bpm[i]) // pool))] = False
+        rise = np.where(live, rise, -np.inf)
+        j = int(np.argmax(rise))
+        if rise[j] >= min_rise:
+            out[i] = ((j + 1) * pool - sb[i]) / bpm[i]
+    return out
+
+
 def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: int = 200, lr: float = 1e-2,
            key=(0x1A7E, 5), target_key=(0x7A26, 6), form: str = "batched", win_len: int = 1000,
            depth_rel: float = 1e-2, x0=None, log=None, shape_weight: float = 1000.0,
-           tilts=None, loss: str = "hybrid") -> InversionResult:
+           tilts=None, loss: str = "hybrid", start: str = "timing") -> InversionResult:
     """Adam on (depth, b, albedo) scaled to [0, 1]; returns the estimates and
-    the depth RMSE and loss of every iteration."""
+    the depth RMSE and loss of every iteration.
+
+    Start (unless x0 is given): every unknown at mid-range ("mid"), or depth
+    from the target's floor-return timing (floor_timing_depth) where the
+    floor return shows and mid-range elsewhere ("timing")."""
     import torch
 
     from .adjoint import batched_waveforms, gated_waveforms
 
+    if start not in ("mid", "timing"):
+        raise ValueError(f"start must be 'mid' or 'timing', not {start!r}")
     S = theta_true.shape[0]
     dev = "cuda"
     # batched form: the scene table is updated in place and the trace passes
@@ -154,6 +205,19 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
                   torch.cat([gated_waveforms(theta_true[s:s + 1], n_forward, n_backward, target_key, **one(s))
                              for s in range(S)]))
     x_init = np.full((S, len(UNKNOWN)), 0.5) if x0 is None else np.array(x0, dtype=np.float64)
+    if x0 is None and start == "timing":
+        from .adjoint import _models
+
+        # the gate and bin widths of the start parameters: nothing of the unknowns enters
+        th0 = np.array(theta_true, dtype=np.float64)
+        th0[:, idx] = lo + 0.5 * (hi - lo)
+        models = _models(th0, spec)
+        bpm = [2.0 / (m.v_water * m.dt * m.mult) for m in models]
+        sb = [m.surface_return_bin() - m.gate(win_len)[0] for m in models]
+        d = floor_timing_depth(target.sum(dim=1).cpu().numpy(), bpm, sb)
+        seen = ~np.isnan(d)
+        x_init[seen, 0] = np.clip((d[seen] - lo[0]) / (hi[0] - lo[0]), 0.0, 1.0)
+
     x = torch.tensor(x_init, dtype=torch.float64, device=dev, requires_grad=True)
     opt = torch.optim.Adam([x], lr=lr)
     base = torch.as_tensor(theta_true, dtype=torch.float64, device=dev).clone()

@@ -46,3 +46,14 @@ def test_plain_form_runs(gpu):
     assert len(res.seconds_per_iteration) == 2 and np.all(np.isfinite(res.loss))
     lo, hi = RANGES["depth"]
     assert np.all((res.theta[:, COLS["depth"]] >= lo) & (res.theta[:, COLS["depth"]] <= hi))
+
+
+def test_timing_start_on_clear_scenes(gpu):
+    """The floor-return leading edge of the target puts the start depth of
+    clear shallow scenes within 1 m; from mid-range it is 12-16 m off."""
+    th = _clear_scenes()
+    mid = invert(th, 40000, 10000, iterations=1, lr=0.02, start="mid")
+    tim = invert(th, 40000, 10000, iterations=1, lr=0.02, start="timing")
+    assert mid.depth_rmse[0] > 10.0
+    assert tim.depth_rmse[0] < 1.0, tim.depth_rmse
+    assert np.all(np.isfinite(tim.theta))
