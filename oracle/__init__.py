@@ -27,11 +27,21 @@ _lock = threading.Lock()
 
 
 def build(force: bool = False) -> str:
-    """Compile alb_oracle.c with gcc (oracle/Makefile)."""
+    """Compile alb_oracle.c with gcc (oracle/Makefile).  A file lock keeps
+    concurrent processes (parallel test workers) from building at once."""
+    import fcntl
+
     srcs = (os.path.join(HERE, "alb_oracle.c"), os.path.join(HERE, "..", "include", "alb_abi.h"))
-    if force or not os.path.exists(LIB_PATH) or \
-            os.path.getmtime(LIB_PATH) < max(os.path.getmtime(p) for p in srcs):
-        subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
+
+    def stale():
+        return not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < max(os.path.getmtime(p) for p in srcs)
+
+    if force or stale():
+        os.makedirs(os.path.dirname(LIB_PATH), exist_ok=True)
+        with open(LIB_PATH + ".lock", "w") as lk:
+            fcntl.flock(lk, fcntl.LOCK_EX)
+            if force or stale():
+                subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
     return LIB_PATH
 
 
