@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define ALB_ABI_VERSION 9
+#define ALB_ABI_VERSION 10
 
 /* photon types, reference enum order (BOTTOM_REFLECTION = auto() = 1, ...) */
 enum alb_photon_type {
@@ -376,6 +376,25 @@ int alb_gather(const alb_scene_t* scene, const alb_map_t* map,
                int32_t win_len, int32_t* nbr_out, double* radius_out,
                void* workspace, size_t workspace_bytes,
                unsigned long long* counters, void* stream);
+
+/* alb_gather without a workspace argument and without any device
+ * allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, results and
+ * errors as alb_gather; the query sort buffers, neighbour rows (in rounds of
+ * 2^14 queries when nbr_out is NULL), kernel normalisations and k-NN
+ * counters live in a static device array of ALB_GATHER_STATIC_SCRATCH_BYTES
+ * that each GPU the library runs on holds until the process ends.
+ * q_n <= ALB_GATHER_STATIC_MAX_QUERIES and k <= 192; larger calls fail with
+ * ALB_ERR_ARG.  Every call on one GPU shares its array: calls that can
+ * overlap on one GPU (other streams, other host threads) race and must be
+ * serialised by the caller; calls in sequence on one stream are safe. */
+#define ALB_GATHER_STATIC_MAX_QUERIES (1 << 20)
+#define ALB_GATHER_STATIC_SCRATCH_BYTES (64u << 20)
+int alb_gather_static(const alb_scene_t* scene, const alb_map_t* map, int32_t photon_type,
+                      const alb_queries_t* q, int64_t q_begin, int64_t q_n, double* waveform,
+                      int64_t n_bins, int32_t win0, int32_t win_len, int32_t* nbr_out,
+                      double* radius_out, unsigned long long* counters, void* stream);
 
 /* The three stages of alb_gather, for callers that time or overlap them.
  * alb_gather_order: order[q_n] (int32) = query offsets sorted by the Morton

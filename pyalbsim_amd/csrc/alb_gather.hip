@@ -36,6 +36,8 @@
 //   traversal at low register pressure (higher occupancy).
 #include <limits.h>
 
+#include <mutex>
+
 #include <hipcub/hipcub.hpp>
 
 #include "alb_device.h"
@@ -1204,36 +1206,26 @@ extern "C" int alb_contrib(const alb_scene_t* scene, const alb_map_t* map, int32
   return check_launch("alb_contrib");
 }
 
-extern "C" int alb_gather(const alb_scene_t* scene, const alb_map_t* map, int32_t photon_type,
-                          const alb_queries_t* q, int64_t q_begin, int64_t q_n, double* waveform,
-                          int64_t n_bins, int32_t win0, int32_t win_len, int32_t* nbr_out, double* radius_out,
-                          void* workspace, size_t workspace_bytes, unsigned long long* counters, void* stream) {
+// The gather sequence of alb_gather over a laid-out workspace: query order,
+// then k-NN + contributions per chunk of L.chunk rows (or all rows at once
+// into the caller's neighbour arrays).
+static int gather_run(const alb_scene_t* scene, const alb_map_t* map, int32_t photon_type, const alb_queries_t* q,
+                      int64_t q_begin, int64_t q_n, double* waveform, int64_t n_bins, int32_t win0, int32_t win_len,
+                      int32_t* nbr_out, double* radius_out, char* ws, size_t ws_bytes, const alb::GatherWs& L,
+                      unsigned long long* counters, void* stream, const char* who) {
   using namespace alb;
-  if (!scene || !map || !q || !waveform) return set_error(ALB_ERR_ARG, "alb_gather: null argument");
-  if (q_n < 0 || q_begin < 0 || q_n > 0x7FFFFFF0) return set_error(ALB_ERR_ARG, "alb_gather: bad query range");
-  if (q_n == 0) return ALB_OK;
   const int k = scene->k;
-  int rc = check_map(map, k, "alb_gather");
-  if (rc != ALB_OK) return rc;
-  if ((nbr_out == nullptr) != (radius_out == nullptr))
-    return set_error(ALB_ERR_ARG, "alb_gather: nbr_out and radius_out go together");
   const bool direct = nbr_out != nullptr;
-  GatherWs L;
-  rc = gather_layout(q_n, k, q_n < kChunk ? q_n : kChunk, &L);
-  if (rc != ALB_OK) return rc;
-  if (!workspace || workspace_bytes < L.total)
-    return set_error(ALB_ERR_WORKSPACE, "alb_gather: workspace too small (%zu < %zu)", workspace_bytes, L.total);
-  char* ws = (char*)workspace;
   int32_t* order = (int32_t*)(ws + L.order);
-  rc = alb_gather_order(map, q, q_begin, q_n, order, workspace, workspace_bytes, stream);
+  int rc = alb_gather_order(map, q, q_begin, q_n, order, ws, ws_bytes, stream);
   if (rc != ALB_OK) return rc;
+  if (direct && L.knorm_rows < q_n) return set_error(ALB_ERR_WORKSPACE, "%s: workspace too small", who);
   const int64_t chunk = direct ? q_n : L.chunk;
   for (int64_t c0 = 0; c0 < q_n; c0 += chunk) {
     const int64_t cn = (q_n - c0) < chunk ? (q_n - c0) : chunk;
     int32_t* nbr = direct ? nbr_out : (int32_t*)(ws + L.nbr);
     double* rad = direct ? radius_out : (double*)(ws + L.rad);
-    double* kn = (double*)(ws + L.knorm) + (direct ? 0 : 0);
-    if (direct && L.knorm_rows < q_n) return set_error(ALB_ERR_WORKSPACE, "alb_gather: workspace too small");
+    double* kn = (double*)(ws + L.knorm);
     rc = alb_knn(map, photon_type, k, q, q_begin, order, c0, cn, direct ? 1 : 0, nbr, rad, kn, counters,
                  (unsigned long long*)(ws + L.work), stream);
     if (rc != ALB_OK) return rc;
@@ -1242,6 +1234,90 @@ extern "C" int alb_gather(const alb_scene_t* scene, const alb_map_t* map, int32_
     if (rc != ALB_OK) return rc;
   }
   return ALB_OK;
+}
+
+static int gather_args(const alb_scene_t* scene, const alb_map_t* map, const alb_queries_t* q, int64_t q_begin,
+                       int64_t q_n, double* waveform, int32_t* nbr_out, double* radius_out, const char* who) {
+  using namespace alb;
+  if (!scene || !map || !q || !waveform) return set_error(ALB_ERR_ARG, "%s: null argument", who);
+  if (q_n < 0 || q_begin < 0 || q_n > 0x7FFFFFF0) return set_error(ALB_ERR_ARG, "%s: bad query range", who);
+  if (q_n == 0) return ALB_OK;
+  int rc = check_map(map, scene->k, who);
+  if (rc != ALB_OK) return rc;
+  if ((nbr_out == nullptr) != (radius_out == nullptr))
+    return set_error(ALB_ERR_ARG, "%s: nbr_out and radius_out go together", who);
+  return ALB_OK;
+}
+
+extern "C" int alb_gather(const alb_scene_t* scene, const alb_map_t* map, int32_t photon_type,
+                          const alb_queries_t* q, int64_t q_begin, int64_t q_n, double* waveform,
+                          int64_t n_bins, int32_t win0, int32_t win_len, int32_t* nbr_out, double* radius_out,
+                          void* workspace, size_t workspace_bytes, unsigned long long* counters, void* stream) {
+  using namespace alb;
+  int rc = gather_args(scene, map, q, q_begin, q_n, waveform, nbr_out, radius_out, "alb_gather");
+  if (rc != ALB_OK || q_n == 0) return rc;
+  GatherWs L;
+  rc = gather_layout(q_n, scene->k, q_n < kChunk ? q_n : kChunk, &L);
+  if (rc != ALB_OK) return rc;
+  if (!workspace || workspace_bytes < L.total)
+    return set_error(ALB_ERR_WORKSPACE, "alb_gather: workspace too small (%zu < %zu)", workspace_bytes, L.total);
+  return gather_run(scene, map, photon_type, q, q_begin, q_n, waveform, n_bins, win0, win_len, nbr_out, radius_out,
+                    (char*)workspace, workspace_bytes, L, counters, stream, "alb_gather");
+}
+
+// ------------------------------------------------ gather without allocation
+// alb_gather_static keeps its workspace in `scratch`, a file-scope device
+// array each GPU holds for the life of the process.  Host code never passes
+// the array's name to a kernel or a copy: on the host it is only a stand-in
+// for the symbol, and its device address on the current GPU comes from
+// hipGetSymbolAddress, taken once per device (static_scratch) and kept.
+namespace alb {
+constexpr size_t kStaticBytes = (size_t)ALB_GATHER_STATIC_SCRATCH_BYTES;
+constexpr int64_t kStaticChunk = 1 << 14;  // neighbour rows per k-NN round (12.6 MB at k = 192)
+__device__ __align__(256) unsigned char scratch[kStaticBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+}  // namespace alb
+
+static int static_scratch(void** out) {
+  using namespace alb;
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_error(ALB_ERR_HIP, "alb_gather_static: no HIP device: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(ALB_ERR_UNSUPPORTED, "alb_gather_static: device %d", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(ALB_ERR_HIP, "alb_gather_static: scratch address on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return ALB_OK;
+}
+
+extern "C" int alb_gather_static(const alb_scene_t* scene, const alb_map_t* map, int32_t photon_type,
+                                 const alb_queries_t* q, int64_t q_begin, int64_t q_n, double* waveform,
+                                 int64_t n_bins, int32_t win0, int32_t win_len, int32_t* nbr_out, double* radius_out,
+                                 unsigned long long* counters, void* stream) {
+  using namespace alb;
+  int rc = gather_args(scene, map, q, q_begin, q_n, waveform, nbr_out, radius_out, "alb_gather_static");
+  if (rc != ALB_OK || q_n == 0) return rc;
+  if (q_n > ALB_GATHER_STATIC_MAX_QUERIES)
+    return set_error(ALB_ERR_ARG, "alb_gather_static: %lld queries exceed the maximum of %d per call",
+                     (long long)q_n, ALB_GATHER_STATIC_MAX_QUERIES);
+  GatherWs L;
+  rc = gather_layout(q_n, scene->k, q_n < kStaticChunk ? q_n : kStaticChunk, &L);
+  if (rc != ALB_OK) return rc;
+  if (L.total > kStaticBytes)
+    return set_error(ALB_ERR_WORKSPACE, "alb_gather_static: %lld queries at k = %d need %zu B of scratch, %zu held",
+                     (long long)q_n, scene->k, L.total, kStaticBytes);
+  void* ws = nullptr;
+  if ((rc = static_scratch(&ws)) != ALB_OK) return rc;
+  return gather_run(scene, map, photon_type, q, q_begin, q_n, waveform, n_bins, win0, win_len, nbr_out, radius_out,
+                    (char*)ws, kStaticBytes, L, counters, stream, "alb_gather_static");
 }
 
 extern "C" int alb_gather_order_batched(const alb_map_t* map, const alb_scene_index_t* index,

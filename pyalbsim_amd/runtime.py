@@ -23,7 +23,7 @@ EXPORTS = ["alb_abi_version", "alb_scene_size", "alb_last_error", "alb_trace", "
            "alb_map_build", "alb_gather_workspace_size", "alb_gather", "alb_gather_order", "alb_knn",
            "alb_contrib", "alb_trace_batched", "alb_map_build_batched", "alb_gather_order_batched",
            "alb_knn_batched", "alb_contrib_batched", "alb_adjoint_contrib", "alb_replay_index_workspace_size",
-           "alb_replay_index_build", "alb_replay_batched", "alb_radius_gather"]
+           "alb_replay_index_build", "alb_replay_batched", "alb_radius_gather", "alb_gather_static"]
 
 
 class AlbError(RuntimeError):
@@ -53,6 +53,9 @@ def load_library(path: str = LIB_PATH):
     L.alb_gather.argtypes = [P, P, C.c_int32, P, C.c_int64, C.c_int64, P, C.c_int64, C.c_int32, C.c_int32,
                              P, P, P, C.c_size_t, P, P]
     L.alb_gather.restype = C.c_int
+    L.alb_gather_static.argtypes = [P, P, C.c_int32, P, C.c_int64, C.c_int64, P, C.c_int64, C.c_int32, C.c_int32,
+                                    P, P, P, P]
+    L.alb_gather_static.restype = C.c_int
     L.alb_gather_order.argtypes = [P, P, C.c_int64, C.c_int64, P, P, C.c_size_t, P]
     L.alb_radius_gather.argtypes = [P, P, C.c_int32, P, C.c_int64, C.c_int64, P, C.c_double, P, C.c_int64,
                                     C.c_int32, C.c_int32, P, P, P, P]
