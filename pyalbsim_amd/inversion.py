@@ -129,7 +129,8 @@ def log_waveform_loss(w, target, floor_rel: float = 1e-4, pool: int = 4):
 
 
 def floor_timing_depth(total: np.ndarray, bins_per_m, surface_bin, pool: int = 4, smooth: int = 3,
-                       floor_rel: float = 1e-6, min_rise: float = 0.4, skip_m: float = 1.5) -> np.ndarray:
+                       floor_rel: float = 1e-6, min_rise: float = 0.4, skip_m: float = 1.5, z: float = 4.0,
+                       hold: int = 2) -> np.ndarray:
     """Per-scene depth [S] from the leading edge of the floor return.
 
     total: [S, L] gated waveforms summed over photon types; bins_per_m: gate
@@ -138,31 +139,68 @@ def floor_timing_depth(total: np.ndarray, bins_per_m, surface_bin, pool: int = 4
     volume return only decays after the surface, so the steepest rise of the
     log waveform (pooled over `pool` bins, then a running mean over `smooth`
     pooled bins) later than skip_m metres below the surface marks the floor.
-    NaN where no rise of at least min_rise (natural-log units per pooled bin)
-    between pooled bins above floor_rel x the peak exists: deep or turbid
-    columns whose floor return does not survive."""
+
+    A Monte Carlo tail that runs out of photons also rises now and then, so
+    a rise counts only where
+      * the pooled bins of the smoothing window before it hold signal, every
+        raw bin of the pooled bins after it (smoothing window and `hold`
+        bins) holds signal, and the pooled bin after the rise is above
+        floor_rel x the peak;
+      * it is at least min_rise (natural-log units per pooled bin) and at
+        least z times its own noise: each pooled bin's relative standard
+        error is estimated from the spread of its raw bins (sd / (mean
+        sqrt(pool))), as the median over 5 pooled bins (the bin that holds
+        the rise itself has a large spread) and averaged over the smoothing
+        window;
+      * the smoothed level `hold` pooled bins later is still at least
+        min_rise above the level before the rise (a floor return persists;
+        a count fluctuation falls back).
+    NaN where no rise qualifies: deep or turbid columns whose floor return
+    does not survive, and tails too sparse to show one."""
     total = np.asarray(total, dtype=np.float64)
     S, L = total.shape
     P = L // pool
-    p = total[:, :P * pool].reshape(S, P, pool).sum(axis=2)
+    raw = total[:, :P * pool].reshape(S, P, pool)
+    p = raw.sum(axis=2)
     bpm = np.broadcast_to(np.asarray(bins_per_m, dtype=np.float64), (S,))
     sb = np.broadcast_to(np.asarray(surface_bin, dtype=np.float64), (S,))
     out = np.full(S, np.nan)
     half = smooth // 2
+    win = np.ones(smooth) / smooth
+
+    def running_mean(v):  # edge values repeated (no zero padding)
+        return np.convolve(np.pad(v, half, mode="edge"), win, mode="valid")[:P] if smooth > 1 else v
+
     for i in range(S):
         mx = p[i].max()
         if not mx > 0.0:
             continue
-        lg = np.log(p[i] + 1e-12 * mx)
-        if smooth > 1:  # running mean, edge values repeated (no zero padding)
-            lg = np.convolve(np.pad(lg, half, mode="edge"), np.ones(smooth) / smooth, mode="valid")[:P]
-        rise = np.diff(lg)
-        live = (p[i][1:] > floor_rel * mx) & (p[i][:-1] > floor_rel * mx)
+        r = raw[i]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            rse = r.std(axis=1, ddof=1) / (r.mean(axis=1) * np.sqrt(pool))
+        full = np.all(r > 0.0, axis=1)  # every raw bin holds signal
+        some = (p[i] > 0.0) & np.isfinite(rse)
+        rse = np.where(some, rse, np.inf)
+        rse = np.median(np.lib.stride_tricks.sliding_window_view(np.pad(rse, 2, mode="edge"), 5), axis=1)
+        lg = running_mean(np.log(p[i] + 1e-12 * mx))
+        noise = np.sqrt(running_mean(rse * rse) / smooth)  # SE of a smoothed log level
+        rise = np.diff(lg)  # rise[j]: pooled bin j -> j + 1
+        j = np.arange(P - 1)
+        gap = np.concatenate([np.zeros(1), np.cumsum(~some)])
+        part = np.concatenate([np.zeros(1), np.cumsum(~full)])
+        b0 = np.clip(j - half, 0, P)
+        b1 = np.clip(j + 1, 0, P)
+        b2 = np.clip(j + 1 + half + hold, 0, P - 1) + 1
+        supported = ((gap[b1] - gap[b0]) == 0) & ((part[b2] - part[b1]) == 0)
+        jh = np.clip(j + hold, 0, P - 1)
+        persists = lg[jh] - lg[j] >= min_rise
+        sig = rise >= z * np.sqrt(noise[j] ** 2 + noise[j + 1] ** 2)
+        live = supported & persists & sig & (j + hold <= P - 1) & (p[i][np.minimum(j + 1, P - 1)] > floor_rel * mx)
         live[:max(0, int((sb[i] + skip_m * bpm[i]) // pool))] = False
         rise = np.where(live, rise, -np.inf)
-        j = int(np.argmax(rise))
-        if rise[j] >= min_rise:
-            out[i] = ((j + 1) * pool - sb[i]) / bpm[i]
+        jj = int(np.argmax(rise))
+        if rise[jj] >= min_rise:
+            out[i] = ((jj + 1) * pool - sb[i]) / bpm[i]
     return out
 
 

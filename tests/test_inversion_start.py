@@ -67,3 +67,48 @@ def test_unknown_start_is_refused_before_any_device_work():
 
     with pytest.raises(ValueError, match="start"):
         invert(inversion_scenes(2), 100, 25, iterations=1, start="grid")
+
+
+def _counted(L, surface_bin, bins_per_m, n0, decay_per_bin, sigma_w, rs, depth=None, floor_amp=0.0):
+    """Monte Carlo-like waveform: each raw bin holds Poisson(lambda) photons of
+    lognormal(0, sigma_w) weight; lambda is the expected volume decay (plus an
+    optional floor pulse), n0 photons per bin at the surface."""
+    t = np.arange(L, dtype=np.float64)
+    lam = np.where(t >= surface_bin, n0 * np.exp(-decay_per_bin * (t - surface_bin)), 0.0)
+    lam[int(surface_bin):int(surface_bin) + 3] += 50.0 * n0
+    if depth is not None:
+        t_f = surface_bin + depth * bins_per_m
+        lam += floor_amp * n0 * np.exp(-decay_per_bin * depth * bins_per_m) * np.where(
+            t >= t_f, np.exp(-(t - t_f) / 6.0), 0.0)
+    c = rs.poisson(lam)
+    w = np.zeros(L)
+    # exact sums of few weights; the normal limit of the sum for many
+    m1 = np.exp(sigma_w ** 2 / 2)
+    sd1 = np.sqrt((np.exp(sigma_w ** 2) - 1) * np.exp(sigma_w ** 2))
+    few = (c > 0) & (c <= 64)
+    for b in np.nonzero(few)[0]:
+        w[b] = rs.lognormal(0.0, sigma_w, c[b]).sum()
+    many = c > 64
+    w[many] = c[many] * m1 + np.sqrt(c[many]) * sd1 * rs.standard_normal(np.count_nonzero(many))
+    return w
+
+
+@pytest.mark.parametrize("n0,decay,sigma_w", [(1e4, 0.3, 0.5), (1e5, 0.3, 0.3), (1e4, 0.15, 0.5), (1e3, 0.3, 0.5)])
+def test_sparse_volume_tail_without_floor_gives_nan(n0, decay, sigma_w):
+    """A volume return that runs out of photons (Poisson counts, lognormal
+    weights, no floor) must not be read as a floor return."""
+    bpm, sb, L = 17.879, 64.9, 1000
+    rs = np.random.default_rng(int(n0) + int(100 * decay))
+    w = np.stack([_counted(L, sb, bpm, n0, decay / bpm, sigma_w, rs) for _ in range(200)])
+    d = floor_timing_depth(w, bpm, sb)
+    assert np.count_nonzero(np.isfinite(d)) <= 2, np.sort(d[np.isfinite(d)])
+
+
+@pytest.mark.parametrize("depth", [6.0, 15.0, 25.0])
+def test_counted_floor_return_is_recovered(depth):
+    """The same counting noise with a floor pulse that carries enough photons."""
+    bpm, sb, L = 17.879, 64.9, 1000
+    rs = np.random.default_rng(int(depth))
+    w = np.stack([_counted(L, sb, bpm, 1e4, 0.3 / bpm, 0.5, rs, depth, floor_amp=30.0) for _ in range(20)])
+    d = floor_timing_depth(w, bpm, sb)
+    assert np.all(np.abs(d - depth) <= 8.0 / bpm + 1e-9), d
