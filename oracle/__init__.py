@@ -65,6 +65,9 @@ def lib():
             L.orc_gather.argtypes = [P, C.c_int32, P, P, P, P, P, P, P, P, C.c_int64, P,
                                      C.c_int64, P, C.c_int64, P, P]
             L.orc_gather.restype = C.c_int
+            L.orc_gather_ex.argtypes = [P, C.c_int32, P, P, P, P, P, P, P, P, C.c_int64, P,
+                                        C.c_int64, P, C.c_int64, P, P, P, P, P, P, P]
+            L.orc_gather_ex.restype = C.c_int
             L.orc_radius_gather.argtypes = [P, C.c_int32, P, P, P, P, P, P, P, P, C.c_int64, P,
                                             C.c_int64, C.c_double, P, C.c_int64, P, P]
             L.orc_radius_gather.restype = C.c_int
@@ -81,7 +84,6 @@ def lib():
             L.orc_scatter_mult.restype = C.c_double
             L.orc_emission_dir.argtypes = [P, P, C.c_int32, C.c_uint64, P]
             L.orc_layer_lookup.argtypes = [P, C.c_double, P]
-            L.orc_set_map_points.argtypes = [P, P, P, P]
             L.orc_lambert.argtypes = [C.c_double, C.c_double, P]
             L.orc_scatter_direction.argtypes = [P, P, C.c_double, C.c_double, P]
             for name in ("orc_ggx_d_x", "orc_g1_x", "orc_schlick_x"):
@@ -361,16 +363,20 @@ def knn(x, y, z, queries: np.ndarray, k: int):
 
 
 def gather(model, ptype: int, mp: dict, qs: dict, n_bins: int | None = None,
-           want_nbrs: bool = False, heat: dict | None = None):
+           want_nbrs: bool = False, heat: dict | None = None, nbrs: np.ndarray | None = None):
     """FP64 waveform of one photon type from map arrays mp and queries qs.
     heat: optional {"bins", "water_extent", "seafloor_extent"} (mp must hold
     the interaction points); receives this gather's "water" and "floor" heat
-    maps ([bins, bins], FP64)."""
+    maps ([bins, bins], FP64).  nbrs: optional [nq, k] neighbour indices from
+    an exact k-NN (e.g. a k-d tree), used instead of the brute-force search."""
     hs = HostScene(model, None, heat)
-    pts = None
+    pts = [None] * 4
     if heat is not None:
         pts = [np.ascontiguousarray(mp[f], dtype=np.float32) for f in ("fwi_x", "fwi_z", "sfi_x", "sfi_z")]
-        lib().orc_set_map_points(*[_ptr(a) for a in pts])
+    nb_in = None
+    if nbrs is not None:
+        nb_in = np.ascontiguousarray(nbrs, dtype=np.int32)
+        assert nb_in.shape == (len(qs["x"]), model.k)
     n_bins = model.n_bins if n_bins is None else n_bins
     m = {f: np.ascontiguousarray(mp[f], dtype=np.float32) for f in _FIELDS_F32}
     nq = len(qs["x"])
@@ -383,13 +389,13 @@ def gather(model, ptype: int, mp: dict, qs: dict, n_bins: int | None = None,
     wf = np.zeros(n_bins, dtype=np.float64)
     nbr = np.zeros((nq, model.k), dtype=np.int32) if want_nbrs else None
     rad = np.zeros(nq, dtype=np.float64) if want_nbrs else None
-    try:
-        rc = lib().orc_gather(C.byref(hs.s), ptype, *[_ptr(m[f]) for f in _FIELDS_F32], len(m["x"]),
-                              C.byref(qst), nq, _ptr(wf), n_bins,
-                              _ptr(nbr) if want_nbrs else None, _ptr(rad) if want_nbrs else None)
-    finally:
-        if heat is not None:
-            lib().orc_set_map_points(None, None, None, None)
+    rc = lib().orc_gather_ex(C.byref(hs.s), ptype, *[_ptr(m[f]) for f in _FIELDS_F32], len(m["x"]),
+                             C.byref(qst), nq, _ptr(wf), n_bins,
+                             _ptr(nbr) if want_nbrs else None, _ptr(rad) if want_nbrs else None,
+                             _ptr(nb_in) if nb_in is not None else None,
+                             *[_ptr(a) if a is not None else None for a in pts])
+    if rc == -2:
+        raise IndexError("a given neighbour index lies outside the map")
     if rc != 0:
         if len(m["x"]) == 0:
             raise KeyError(ptype)

@@ -317,23 +317,28 @@ typedef struct orc_stats_sink {
   int32_t* nfl;
 } orc_stats_sink;
 
-static const orc_stats_sink* g_stats = NULL;  /* [4] while orc_trace_stats runs */
-static orc_path g_path;
+/* Per-call state of one trace: the optional statistics sinks and the state
+ * of the photon being traced (its path statistics and its first water and
+ * latest seafloor interaction, float32, NaN before them;
+ * core/simulation.py:85-86, 393, 521).  Each call keeps its own, so calls
+ * from several threads do not share anything mutable. */
+typedef struct orc_ctx {
+  const orc_stats_sink* stats;  /* [4] or NULL */
+  orc_path path;
+  float fwi[3], sfi[3];
+} orc_ctx;
 
-static void orc_emit_stats(int type, unsigned long long i, int64_t capacity) {
-  if (!g_stats || !g_stats[type].lam || (int64_t)i >= capacity) return;
-  g_stats[type].lam[i] = g_path.lam;
-  g_stats[type].gs[i] = g_path.gs;
-  g_stats[type].nsc[i] = g_path.nsc;
-  g_stats[type].nfl[i] = g_path.nfl;
+static void orc_emit_stats(const orc_ctx* cx, int type, unsigned long long i, int64_t capacity) {
+  const orc_stats_sink* st = cx->stats;
+  if (!st || !st[type].lam || (int64_t)i >= capacity) return;
+  st[type].lam[i] = cx->path.lam;
+  st[type].gs[i] = cx->path.gs;
+  st[type].nsc[i] = cx->path.nsc;
+  st[type].nfl[i] = cx->path.nfl;
 }
 
-/* the traced photon's first water interaction and latest seafloor
- * interaction, float32, NaN before them (core/simulation.py:85-86, 393, 521);
- * stored with every forward record whose sink holds them */
-static float g_fwi[3], g_sfi[3];
-
-static void orc_emit_record(const alb_records_t* r, const double p[3], const double d[3],
+/* stored with every forward record whose sink holds the interaction points */
+static void orc_emit_record(const orc_ctx* cx, const alb_records_t* r, const double p[3], const double d[3],
                             float E, float t, int64_t tag) {
   unsigned long long i = (*r->count)++;
   if ((int64_t)i >= r->capacity) return;
@@ -343,8 +348,8 @@ static void orc_emit_record(const alb_records_t* r, const double p[3], const dou
   r->time[i] = t;
   if (r->tag) r->tag[i] = tag;
   if (r->fwi_x) {
-    r->fwi_x[i] = g_fwi[0]; r->fwi_y[i] = g_fwi[1]; r->fwi_z[i] = g_fwi[2];
-    r->sfi_x[i] = g_sfi[0]; r->sfi_y[i] = g_sfi[1]; r->sfi_z[i] = g_sfi[2];
+    r->fwi_x[i] = cx->fwi[0]; r->fwi_y[i] = cx->fwi[1]; r->fwi_z[i] = cx->fwi[2];
+    r->sfi_x[i] = cx->sfi[0]; r->sfi_y[i] = cx->sfi[1]; r->sfi_z[i] = cx->sfi[2];
   }
 }
 
@@ -352,10 +357,10 @@ static void orc_emit_record(const alb_records_t* r, const double p[3], const dou
  * (math/accumulate_heatmap.py:64-124): surface radius from the float32 first
  * water interaction, floor radius from the FP64 hit point, each over its
  * extent times bins, truncated; photons without water entry are skipped */
-static void orc_count_radius(const alb_scene_t* sc, const double P[3]) {
-  if (sc->heat_bins <= 0 || !sc->heat_radius || isnan(g_fwi[0])) return;
-  const double sx = (double)g_fwi[0] - sc->heat_water_center[0];
-  const double sz = (double)g_fwi[2] - sc->heat_water_center[2];
+static void orc_count_radius(const orc_ctx* cx, const alb_scene_t* sc, const double P[3]) {
+  if (sc->heat_bins <= 0 || !sc->heat_radius || isnan(cx->fwi[0])) return;
+  const double sx = (double)cx->fwi[0] - sc->heat_water_center[0];
+  const double sz = (double)cx->fwi[2] - sc->heat_water_center[2];
   const double fx = P[0] - sc->heat_floor_center[0], fz = P[2] - sc->heat_floor_center[2];
   const double rs = sqrt(sx * sx + sz * sz), rf = sqrt(fx * fx + fz * fz);
   const int64_t bins = sc->heat_bins;
@@ -498,7 +503,7 @@ void orc_scatter_direction(const alb_scene_t* sc, const double din[3], double u1
 
 /* Trace one photon through all S sub-steps.  Returns 0, or 1 if its replay
  * stream ran out (the photon is then abandoned). */
-static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pass,
+static int orc_trace_photon(orc_ctx* cx, const alb_scene_t* sc, const alb_rng_t* rng, int pass,
                             uint64_t photon, int64_t slot, const alb_records_t* out,
                             const alb_queries_t* q, unsigned long long* ctr) {
   orc_stream st = {rng, photon, slot, 0, 0};
@@ -513,11 +518,11 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
 
   float E = 1.0f, tau = 0.0f, tau_t = INFINITY;
   int64_t event_no = 0;
-  for (int i = 0; i < 3; ++i) g_fwi[i] = g_sfi[i] = NAN;
-  g_path.lam = 0.0;
-  g_path.gs = 0.0;
-  g_path.nsc = 0;
-  g_path.nfl = 0;
+  for (int i = 0; i < 3; ++i) cx->fwi[i] = cx->sfi[i] = NAN;
+  cx->path.lam = 0.0;
+  cx->path.gs = 0.0;
+  cx->path.nsc = 0;
+  cx->path.nfl = 0;
   const double y_ws = sc->y_ws, y_sf = sc->y_sf;
   const double up[3] = {0.0, 1.0, 0.0}, down[3] = {0.0, -1.0, 0.0};
   const double alpha = (double)sc->alpha_l;
@@ -565,14 +570,14 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
       /* attenuation at the pre-step position (the lookup above) */
       tau = (float)((double)tau + (layered ? look[2] : alpha) * len);
       int scatter = tau >= tau_t;
-      if (!scatter) g_path.lam += len;
+      if (!scatter) cx->path.lam += len;
       p[0] = nx[0]; p[1] = nx[1]; p[2] = nx[2];
       if (ctr) ctr[ALB_CTR_WATER_STEPS]++;
       if (scatter) {
-        g_path.lam += len;
-        if (fwd) orc_emit_record(&out[ALB_SCATTER], p, d, E, t_evt, tag);
+        cx->path.lam += len;
+        if (fwd) orc_emit_record(cx, &out[ALB_SCATTER], p, d, E, t_evt, tag);
         else orc_emit_query(&q[ALB_SCATTER], p, d, E, t_evt, tag);
-        orc_emit_stats(ALB_SCATTER, (fwd ? *out[ALB_SCATTER].count : *q[ALB_SCATTER].count) - 1,
+        orc_emit_stats(cx, ALB_SCATTER, (fwd ? *out[ALB_SCATTER].count : *q[ALB_SCATTER].count) - 1,
                        fwd ? out[ALB_SCATTER].capacity : q[ALB_SCATTER].capacity);
         event_no++;
         if (ctr) ctr[ALB_CTR_SCATTER]++;
@@ -590,12 +595,12 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
           orc_scatter_direction_tab(L->phase_theta, L->phase_cdf, L->phase_n, d, u1, u2, nd);
           w0 = (float)lk[3];
           /* phase-sampling score (one constant layer: its table is the scene's) */
-          if (sc->phase_score) g_path.gs += sc->phase_score[orc_phase_segment(u1, sc)];
+          if (sc->phase_score) cx->path.gs += sc->phase_score[orc_phase_segment(u1, sc)];
         } else {
           orc_scatter_direction(sc, d, u1, u2, nd);
-          if (sc->phase_score) g_path.gs += sc->phase_score[orc_phase_segment(u1, sc)];
+          if (sc->phase_score) cx->path.gs += sc->phase_score[orc_phase_segment(u1, sc)];
         }
-        g_path.nsc++;
+        cx->path.nsc++;
         d[0] = nd[0]; d[1] = nd[1]; d[2] = nd[2];
         E = E * w0;
         tau = tau - tau_t;
@@ -612,17 +617,17 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
       double ci = -d[1];
       double Rf = orc_schlick(ci, sc->F0);
       double T = 1.0 - Rf;
-      for (int i = 0; i < 3; ++i) g_fwi[i] = (float)P[i];  /* core/simulation.py:393 */
+      for (int i = 0; i < 3; ++i) cx->fwi[i] = (float)P[i];  /* core/simulation.py:393 */
       if (fwd) {
-        orc_emit_record(&out[ALB_SURFACE_REFLECTION], P, d, E, t_evt, tag);
-        orc_emit_stats(ALB_SURFACE_REFLECTION, *out[ALB_SURFACE_REFLECTION].count - 1,
+        orc_emit_record(cx, &out[ALB_SURFACE_REFLECTION], P, d, E, t_evt, tag);
+        orc_emit_stats(cx, ALB_SURFACE_REFLECTION, *out[ALB_SURFACE_REFLECTION].count - 1,
                        out[ALB_SURFACE_REFLECTION].capacity);
       } else {
         orc_emit_query(&q[ALB_SURFACE_REFLECTION], P, d, E, t_evt, tag);
         orc_emit_query(&q[ALB_SURFACE_TRANSMISSION_UP], P, d, E, t_evt, tag);
-        orc_emit_stats(ALB_SURFACE_REFLECTION, *q[ALB_SURFACE_REFLECTION].count - 1,
+        orc_emit_stats(cx, ALB_SURFACE_REFLECTION, *q[ALB_SURFACE_REFLECTION].count - 1,
                        q[ALB_SURFACE_REFLECTION].capacity);
-        orc_emit_stats(ALB_SURFACE_TRANSMISSION_UP, *q[ALB_SURFACE_TRANSMISSION_UP].count - 1,
+        orc_emit_stats(cx, ALB_SURFACE_TRANSMISSION_UP, *q[ALB_SURFACE_TRANSMISSION_UP].count - 1,
                        q[ALB_SURFACE_TRANSMISSION_UP].capacity);
       }
       event_no++;
@@ -647,8 +652,8 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
       double Rf = orc_schlick(ci, sc->F0);
       int reflect = orc_uniform(&st) < Rf;
       if (!reflect && fwd) {
-        orc_emit_record(&out[ALB_SURFACE_TRANSMISSION_UP], P, d, E, t_evt, tag);
-        orc_emit_stats(ALB_SURFACE_TRANSMISSION_UP, *out[ALB_SURFACE_TRANSMISSION_UP].count - 1,
+        orc_emit_record(cx, &out[ALB_SURFACE_TRANSMISSION_UP], P, d, E, t_evt, tag);
+        orc_emit_stats(cx, ALB_SURFACE_TRANSMISSION_UP, *out[ALB_SURFACE_TRANSMISSION_UP].count - 1,
                        out[ALB_SURFACE_TRANSMISSION_UP].capacity);
       }
       event_no++;
@@ -673,7 +678,7 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
           a_post = lk[2];
         }
         tau = (float)((double)tau + a_post * len);
-        g_path.lam += len;
+        cx->path.lam += len;
       }
     } else { /* ST_FLOOR */
       /* handle_seafloor (core/simulation.py:504-570) */
@@ -684,11 +689,11 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
       if (f > 1.0) f = 1.0;
       double P[3];
       for (int i = 0; i < 3; ++i) P[i] = p[i] + f * sv[i];
-      for (int i = 0; i < 3; ++i) g_sfi[i] = (float)P[i];  /* core/simulation.py:521 */
-      if (fwd) orc_count_radius(sc, P);                   /* core/simulation.py:523-537 */
-      if (fwd) orc_emit_record(&out[ALB_BOTTOM_REFLECTION], P, d, E, t_evt, tag);
+      for (int i = 0; i < 3; ++i) cx->sfi[i] = (float)P[i];  /* core/simulation.py:521 */
+      if (fwd) orc_count_radius(cx, sc, P);                   /* core/simulation.py:523-537 */
+      if (fwd) orc_emit_record(cx, &out[ALB_BOTTOM_REFLECTION], P, d, E, t_evt, tag);
       else orc_emit_query(&q[ALB_BOTTOM_REFLECTION], P, d, E, t_evt, tag);
-      orc_emit_stats(ALB_BOTTOM_REFLECTION,
+      orc_emit_stats(cx, ALB_BOTTOM_REFLECTION,
                      (fwd ? *out[ALB_BOTTOM_REFLECTION].count : *q[ALB_BOTTOM_REFLECTION].count) - 1,
                      fwd ? out[ALB_BOTTOM_REFLECTION].capacity : q[ALB_BOTTOM_REFLECTION].capacity);
       event_no++;
@@ -713,8 +718,8 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
         a_post = lk[2];
       }
       tau = (float)((double)tau + a_post * len);
-      g_path.lam += len;
-      g_path.nfl++;
+      cx->path.lam += len;
+      cx->path.nfl++;
       E = E * (float)sc->floor_albedo;
     }
     if (st.exhausted) return 1;
@@ -725,12 +730,15 @@ static int orc_trace_photon(const alb_scene_t* sc, const alb_rng_t* rng, int pas
 /* Trace photons [photon_begin, photon_begin+n) of one pass.  Same sink
  * semantics as alb_trace (host pointers).  Returns the number of photons whose
  * replay stream ran out. */
-int64_t orc_trace(const alb_scene_t* sc, const alb_rng_t* rng, int32_t pass,
-                  int64_t photon_begin, int64_t n, const alb_records_t* out,
-                  const alb_queries_t* q, unsigned long long* counters) {
+static int64_t orc_trace_ctx(const alb_scene_t* sc, const alb_rng_t* rng, int32_t pass, int64_t photon_begin,
+                             int64_t n, const alb_records_t* out, const alb_queries_t* q,
+                             unsigned long long* counters, const orc_stats_sink* stats) {
+  orc_ctx cx;
+  memset(&cx, 0, sizeof cx);
+  cx.stats = stats;
   int64_t exhausted = 0;
   for (int64_t i = 0; i < n; ++i) {
-    int ex = orc_trace_photon(sc, rng, pass, (uint64_t)(photon_begin + i), i, out, q, counters);
+    int ex = orc_trace_photon(&cx, sc, rng, pass, (uint64_t)(photon_begin + i), i, out, q, counters);
     if (ex) {
       exhausted++;
       if (counters) counters[ALB_CTR_REPLAY_EXHAUSTED]++;
@@ -739,16 +747,19 @@ int64_t orc_trace(const alb_scene_t* sc, const alb_rng_t* rng, int32_t pass,
   return exhausted;
 }
 
+int64_t orc_trace(const alb_scene_t* sc, const alb_rng_t* rng, int32_t pass,
+                  int64_t photon_begin, int64_t n, const alb_records_t* out,
+                  const alb_queries_t* q, unsigned long long* counters) {
+  return orc_trace_ctx(sc, rng, pass, photon_begin, n, out, q, counters, NULL);
+}
+
 /* orc_trace with the adjoint path statistics of every record / query
  * written into stats[type] at the record's slot (test of the replay trace). */
 int64_t orc_trace_stats(const alb_scene_t* sc, const alb_rng_t* rng, int32_t pass,
                         int64_t photon_begin, int64_t n, const alb_records_t* out,
                         const alb_queries_t* q, unsigned long long* counters,
                         const orc_stats_sink* stats) {
-  g_stats = stats;
-  int64_t ex = orc_trace(sc, rng, pass, photon_begin, n, out, q, counters);
-  g_stats = NULL;
-  return ex;
+  return orc_trace_ctx(sc, rng, pass, photon_begin, n, out, q, counters, stats);
 }
 
 /* --------------------------------------------------------------- k-NN */
@@ -963,21 +974,14 @@ static orc_phase_tab orc_gather_phase(const alb_scene_t* sc, double qy) {
   return t;
 }
 
-/* interaction points of the map records being gathered (heat maps), set by
- * orc_set_map_points; NULL: no heat maps */
-static struct {
+/* interaction points of the map records being gathered (heat maps); all
+ * NULL: no heat maps */
+typedef struct orc_map_pts {
   const float* fwi_x;
   const float* fwi_z;
   const float* sfi_x;
   const float* sfi_z;
-} g_map_pts;
-
-void orc_set_map_points(const float* fwi_x, const float* fwi_z, const float* sfi_x, const float* sfi_z) {
-  g_map_pts.fwi_x = fwi_x;
-  g_map_pts.fwi_z = fwi_z;
-  g_map_pts.sfi_x = sfi_x;
-  g_map_pts.sfi_z = sfi_z;
-}
+} orc_map_pts;
 
 /* accumulate_to_heatmap (math/accumulate_heatmap.py:6-61) for one point:
  * bins of (x - centre + extent) / (2 extent / bins) along x and z, truncated
@@ -996,7 +1000,7 @@ static void orc_heat_add(double* h, float px, float pz, const double c[3], doubl
 static void orc_contribute(const alb_scene_t* sc, int32_t type, orc_phase_tab tab, double m_bottom,
                            const double ds[3], float q_energy, float q_time, int32_t i, const float* dx,
                            const float* dy, const float* dz, const float* energy, const float* time,
-                           double knorm, double* waveform, int64_t n_bins) {
+                           double knorm, double* waveform, int64_t n_bins, const orc_map_pts* pts) {
   float dp[3] = {dx[i], dy[i], dz[i]};
   double m;
   if (type == ALB_BOTTOM_REFLECTION) m = m_bottom;
@@ -1008,29 +1012,53 @@ static void orc_contribute(const alb_scene_t* sc, int32_t type, orc_phase_tab ta
   float tt = q_time + time[i];
   int64_t bin = (int64_t)(tt / (float)sc->mult);
   if (bin >= 0 && bin < n_bins) waveform[bin] += e;
-  if (sc->heat_bins > 0 && sc->heat_water && g_map_pts.fwi_x) {
-    orc_heat_add(sc->heat_water, g_map_pts.fwi_x[i], g_map_pts.fwi_z[i], sc->heat_water_center,
+  if (sc->heat_bins > 0 && sc->heat_water && pts && pts->fwi_x) {
+    orc_heat_add(sc->heat_water, pts->fwi_x[i], pts->fwi_z[i], sc->heat_water_center,
                  sc->heat_water_extent, sc->heat_bins, e);
-    orc_heat_add(sc->heat_floor, g_map_pts.sfi_x[i], g_map_pts.sfi_z[i], sc->heat_floor_center,
+    orc_heat_add(sc->heat_floor, pts->sfi_x[i], pts->sfi_z[i], sc->heat_floor_center,
                  sc->heat_floor_extent, sc->heat_bins, e);
   }
 }
 
-int orc_gather(const alb_scene_t* sc, int32_t type, const float* x, const float* y,
-               const float* z, const float* dx, const float* dy, const float* dz,
-               const float* energy, const float* time, int64_t n, const alb_queries_t* q,
-               int64_t nq, double* waveform, int64_t n_bins, int32_t* nbr_out,
-               double* radius_out) {
+/* sample_photons as above; nbr_in (optional, [nq*k] indices into the map):
+ * the neighbour sets are given (from any exact k-NN) instead of found by
+ * brute force, and the kernel radius is the largest distance within each
+ * set.  fwi_x / fwi_z / sfi_x / sfi_z (optional): the records' interaction
+ * points for the heat maps. */
+int orc_gather_ex(const alb_scene_t* sc, int32_t type, const float* x, const float* y,
+                  const float* z, const float* dx, const float* dy, const float* dz,
+                  const float* energy, const float* time, int64_t n, const alb_queries_t* q,
+                  int64_t nq, double* waveform, int64_t n_bins, int32_t* nbr_out,
+                  double* radius_out, const int32_t* nbr_in, const float* fwi_x, const float* fwi_z,
+                  const float* sfi_x, const float* sfi_z) {
   const int32_t k = sc->k;
   if (nq == 0) return 0;
   if (n <= 0 || k > n) return -1;
+  const orc_map_pts pts = {fwi_x, fwi_z, sfi_x, sfi_z};
   int32_t* idx = (int32_t*)malloc(sizeof(int32_t) * (size_t)k);
   double* d2 = (double*)malloc(sizeof(double) * (size_t)k);
   for (int64_t j = 0; j < nq; ++j) {
     double qx = q->x[j], qy = q->y[j], qz = q->z[j];
     double ds[3] = {q->dx[j], q->dy[j], q->dz[j]};
-    orc_knn_one(x, y, z, n, qx, qy, qz, k, idx, d2);
-    double r = sqrt(d2[k - 1]);
+    double r;
+    if (nbr_in) {
+      double mx = 0.0;
+      for (int32_t t = 0; t < k; ++t) {
+        const int32_t i = nbr_in[j * k + t];
+        if (i < 0 || i >= n) {
+          free(idx);
+          free(d2);
+          return -2;
+        }
+        idx[t] = i;
+        const double dd = orc_d2(qx, qy, qz, x[i], y[i], z[i]);
+        if (dd > mx) mx = dd;
+      }
+      r = sqrt(mx);
+    } else {
+      orc_knn_one(x, y, z, n, qx, qy, qz, k, idx, d2);
+      r = sqrt(d2[k - 1]);
+    }
     double ksize;
     if (type == ALB_SCATTER) ksize = (4.0 / 3.0) * ORC_PI * pow(r, 3.0);
     else ksize = ORC_PI * (r * r);
@@ -1041,11 +1069,20 @@ int orc_gather(const alb_scene_t* sc, int32_t type, const float* x, const float*
     const orc_phase_tab tab = orc_gather_phase(sc, qy);
     for (int32_t t = 0; t < k; ++t)
       orc_contribute(sc, type, tab, m_bottom, ds, q->energy[j], q->time[j], idx[t], dx, dy, dz, energy, time,
-                     knorm, waveform, n_bins);
+                     knorm, waveform, n_bins, &pts);
   }
   free(idx);
   free(d2);
   return 0;
+}
+
+int orc_gather(const alb_scene_t* sc, int32_t type, const float* x, const float* y,
+               const float* z, const float* dx, const float* dy, const float* dz,
+               const float* energy, const float* time, int64_t n, const alb_queries_t* q,
+               int64_t nq, double* waveform, int64_t n_bins, int32_t* nbr_out,
+               double* radius_out) {
+  return orc_gather_ex(sc, type, x, y, z, dx, dy, dz, energy, time, n, q, nq, waveform, n_bins, nbr_out,
+                       radius_out, NULL, NULL, NULL, NULL, NULL);
 }
 
 /* Fixed-radius gather: every record within distance r of a query (squared
@@ -1073,7 +1110,7 @@ int orc_radius_gather(const alb_scene_t* sc, int32_t type, const float* x, const
       cnt++;
       isum += i;
       orc_contribute(sc, type, tab, m_bottom, ds, q->energy[j], q->time[j], (int32_t)i, dx, dy, dz, energy,
-                     time, knorm, waveform, n_bins);
+                     time, knorm, waveform, n_bins, NULL);
     }
     if (count_out) count_out[j] = cnt;
     if (isum_out) isum_out[j] = isum;
