@@ -50,6 +50,7 @@ constexpr int kStack = 256;       // per-wave node stack (LDS)
 constexpr int kSeed = 192;        // records of the descent seed (<= kBuf)
 constexpr int kBuf = 256;         // per-wave candidate buffer (LDS), power of two
 constexpr int kBucket = 64;       // nodes with <= kBucket records are scanned
+constexpr int kPath = 128;        // descent levels kept for the next query (LDS)
 constexpr int kMaxWindow = 7680;  // LDS-privatised waveform bins (60 KB of FP64: with the row tile, 64 KB per block)
 constexpr int kMaxWindowLarge = 16384;  // batched contributions with one 1024-thread block per CU (132 KB)
 constexpr int kMaxK = kSeed;      // the seed must hold k records
@@ -109,6 +110,18 @@ static KnnArgs knn_args(const alb_map_t& m, const alb_queries_t& q, int64_t q_be
   a.slast = idx ? idx->last : nullptr;
   a.sroot = idx ? idx->root : nullptr;
   return a;
+}
+
+// map record coordinate i: through a 32-bit byte offset from the array base
+// when the map holds fewer than 2^30 records (one shift per index, shared by
+// the three arrays), else through 64-bit addresses
+template <bool OFF32>
+__device__ __forceinline__ float rec_ld(const float* __restrict__ p, int32_t i) {
+  if constexpr (OFF32) {
+    return *(const float*)((const char*)p + ((uint32_t)i << 2));
+  } else {
+    return p[(uint32_t)i];
+  }
 }
 
 struct ContribArgs {
@@ -414,6 +427,7 @@ __global__ __launch_bounds__(256) void knn_work_clear_kernel(unsigned long long*
   for (int t = threadIdx.x; t < ALB_KNN_WORK_WORDS; t += blockDim.x) w[t] = 0ull;
 }
 
+template <bool OFF32>
 __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   __shared__ int32_t sn_s[kWaves][kStack];
   __shared__ double sd_s[kWaves][kStack];
@@ -422,6 +436,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   __shared__ int32_t rlo_s[kWaves][128];
   __shared__ int32_t rhi_s[kWaves][128];
   __shared__ uint32_t hist_s[kWaves][256];
+  __shared__ int32_t path_s[kWaves][kPath];
 
   const uint32_t lane = lane_id();
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: scalar slot loop
@@ -432,6 +447,9 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   int32_t* rlo = rlo_s[w];
   int32_t* rhi = rhi_s[w];
   uint32_t* hist = hist_s[w];
+  int32_t* path = path_s[w];
+  int plen = 0;            // levels of the previous query's descent held in path
+  int32_t prev_root = -1;  // the subtree root it started from
   const Node* __restrict__ nodes = a.nodes;
   const float* __restrict__ mx = a.x;
   const float* __restrict__ my = a.y;
@@ -487,8 +505,51 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
       }
     }
     if (s1 - s0 + 1 > kSeed) {
+      // Descent toward the nearer child box.  Queries are Morton-sorted, so
+      // a wavefront's consecutive queries mostly take the same path: the
+      // previous query's path (path[0..plen), root first) is tested for this
+      // query at all levels at once, one level per lane.  Along the longest
+      // prefix where this query takes the same child it is this query's own
+      // descent, so the far children of that prefix are stacked with the
+      // distances the lanes computed, and the descent continues one level at
+      // a time only from the first level where it turns elsewhere.  Seed and
+      // stack are exactly those of a descent from the root.
       int32_t node = root;
+      int lvl = 0;
+      if (root == prev_root && plen > 0) {
+        const int lp = plen < 64 ? plen : 64;
+        bool agree = false;
+        int32_t far_code = 0;
+        double far_d = 0.0;
+        if ((int)lane < lp) {
+          const Node nd = nodes[path[lane]];
+          const double dl = box_dist2(qx, qy, qz, nd.lb);
+          const double dr = box_dist2(qx, qy, qz, nd.rb);
+          const bool gl = dl <= dr;
+          const int32_t child = gl ? nd.split : nd.split + 1;
+          agree = (int)lane + 1 < plen && child == path[lane + 1];
+          const int32_t far_id = gl ? nd.split + 1 : nd.split;
+          const bool far_leaf = gl ? (nd.leaf_mask & 2) : (nd.leaf_mask & 1);
+          far_code = far_leaf ? ~far_id : far_id;
+          far_d = gl ? dr : dl;
+        }
+        const uint64_t dis = ballot((int)lane < lp && !agree);
+        const int m = dis ? __ffsll((long long)dis) - 1 : lp;  // levels shared with the previous path
+        wave_sync();
+        if ((int)lane < m) {
+          sn[sp + (int)lane] = far_code;
+          sd[sp + (int)lane] = far_d;
+        }
+        sp += m;
+        c_nodes += (uint32_t)m;
+        c_desc += (uint32_t)m;
+        lvl = m;
+        node = path[m];
+        wave_sync();
+      }
       for (;;) {
+        if (lvl < kPath && lane == 0) path[lvl] = node;
+        ++lvl;
         const Node nd = nodes[node];
         c_nodes++;
         c_desc++;
@@ -526,6 +587,9 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
         if (stop) break;
         node = go_left ? nd.split : nd.split + 1;
       }
+      plen = lvl < kPath ? lvl : kPath;
+      prev_root = root;
+      wave_sync();
     }
     const int W = s1 - s0 + 1;
     {
@@ -536,9 +600,9 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
       for (int u = 0; u < SU; ++u) {
         const int t = (int)lane + 64 * u;
         const int32_t r = s0 + (t < W ? t : 0);
-        xs[u] = mx[r];
-        ys[u] = my[r];
-        zs[u] = mz[r];
+        xs[u] = rec_ld<OFF32>(mx, r);
+        ys[u] = rec_ld<OFF32>(my, r);
+        zs[u] = rec_ld<OFF32>(mz, r);
       }
 #pragma unroll
       for (int u = 0; u < SU; ++u) {
@@ -663,9 +727,9 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
           c_cand += (uint32_t)(rh - rl + 1);
           tk[u] = idx <= rh && !(idx >= s0 && idx <= s1);
           const int32_t li = idx <= rh ? idx : rl;
-          xs[u] = mx[li];
-          ys[u] = my[li];
-          zs[u] = mz[li];
+          xs[u] = rec_ld<OFF32>(mx, li);
+          ys[u] = rec_ld<OFF32>(my, li);
+          zs[u] = rec_ld<OFF32>(mz, li);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -1148,7 +1212,10 @@ extern "C" int alb_knn(const alb_map_t* map, int32_t photon_type, int32_t k, con
   if (kb > (int64_t)cus * 16) kb = (int64_t)cus * 16;
   kb = (kb + kXcds - 1) / kXcds * kXcds;
   hipLaunchKernelGGL(knn_work_clear_kernel, dim3(1), dim3(256), 0, (hipStream_t)stream, work);
-  hipLaunchKernelGGL(knn_kernel, dim3((unsigned)kb), dim3(64 * kWaves), 0, (hipStream_t)stream, ka);
+  if (map->n < (1ll << 30))
+    hipLaunchKernelGGL(knn_kernel<true>, dim3((unsigned)kb), dim3(64 * kWaves), 0, (hipStream_t)stream, ka);
+  else
+    hipLaunchKernelGGL(knn_kernel<false>, dim3((unsigned)kb), dim3(64 * kWaves), 0, (hipStream_t)stream, ka);
   // rows written: [0, cn) in slot order, or the query offsets of the slots when rows follow queries
   if (rows_by_query) {
     // rows are query offsets: normalise every row of this chunk's queries (all rows when one chunk)
@@ -1381,7 +1448,10 @@ extern "C" int alb_knn_batched(const alb_map_t* map, const alb_scene_index_t* in
   if (kb > (int64_t)cus * 16) kb = (int64_t)cus * 16;
   kb = (kb + kXcds - 1) / kXcds * kXcds;
   hipLaunchKernelGGL(knn_work_clear_kernel, dim3(1), dim3(256), 0, (hipStream_t)stream, work);
-  hipLaunchKernelGGL(knn_kernel, dim3((unsigned)kb), dim3(64 * kWaves), 0, (hipStream_t)stream, ka);
+  if (map->n < (1ll << 30))
+    hipLaunchKernelGGL(knn_kernel<true>, dim3((unsigned)kb), dim3(64 * kWaves), 0, (hipStream_t)stream, ka);
+  else
+    hipLaunchKernelGGL(knn_kernel<false>, dim3((unsigned)kb), dim3(64 * kWaves), 0, (hipStream_t)stream, ka);
   hipLaunchKernelGGL(knorm_kernel, dim3((unsigned)((cn + 255) / 256)), dim3(256), 0, (hipStream_t)stream, radius,
                      knorm, cn, photon_type == ALB_SCATTER ? 1 : 0);
   return check_launch("alb_knn_batched");
