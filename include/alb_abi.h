@@ -87,7 +87,8 @@ enum alb_counter {
   ALB_CTR_SMALL_SCENE = 16,  /* batched queries whose scene map holds fewer
                                 than k records (rows left at -1 / NaN)      */
   ALB_CTR_REPLAY_MISSES = 17,/* replayed events whose record was not found   */
-  ALB_CTR_NEIGHBOURS = 18,   /* records taken by the fixed-radius gather     */
+  ALB_CTR_NEIGHBOURS = 18,   /* records taken by the fixed-radius gather;
+                                k-NN: candidates entering a candidate buffer */
   ALB_N_COUNTERS = 19
 };
 

@@ -450,6 +450,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   int32_t* path = path_s[w];
   int plen = 0;            // levels of the previous query's descent held in path
   int32_t prev_root = -1;  // the subtree root it started from
+  int32_t nn_root = -1;    // subtree root of the previous query whose neighbours bi[0, k) holds
   const Node* __restrict__ nodes = a.nodes;
   const float* __restrict__ mx = a.x;
   const float* __restrict__ my = a.y;
@@ -458,7 +459,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   const int k = a.k;
   // per-wave event counts of one launch (diagnostics only; thousands of waves
   // share a launch's queries, so 32 bits hold them)
-  uint32_t c_q = 0, c_cand = 0, c_nodes = 0, c_ovf = 0, c_steps = 0, c_sorts = 0, c_desc = 0, c_small = 0;
+  uint32_t c_q = 0, c_cand = 0, c_nodes = 0, c_ovf = 0, c_steps = 0, c_sorts = 0, c_desc = 0, c_small = 0, c_ins = 0;
 
   // XCD-aware slot ranges: the grid is a multiple of kXcds blocks and block b
   // runs on XCD b % kXcds, so XCD x walks its own contiguous 1/kXcds of the
@@ -591,6 +592,30 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
       prev_root = root;
       wave_sync();
     }
+    // A bound from the previous query's neighbours (still in bi[0, k)): k
+    // distinct records of this query's map lie within their largest distance
+    // from this query, so its k-th distance is no larger.
+    double theta_prev = __builtin_inf();
+    if (nn_root == root) {
+      constexpr int KU = (kMaxK + 63) / 64;
+      float xs[KU], ys[KU], zs[KU];
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+        const int t = (int)lane + 64 * u;
+        const int32_t r = bi[t < k ? t : 0];
+        xs[u] = rec_ld<OFF32>(mx, r);
+        ys[u] = rec_ld<OFF32>(my, r);
+        zs[u] = rec_ld<OFF32>(mz, r);
+      }
+      double m = 0.0;
+#pragma unroll
+      for (int u = 0; u < KU; ++u)
+        if ((int)lane + 64 * u < k) m = fmax(m, dist2(qx, qy, qz, xs[u], ys[u], zs[u]));
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+      theta_prev = m;
+      wave_sync();
+    }
     const int W = s1 - s0 + 1;
     {
       // W <= kSeed: the loads of all slots issue before one wait
@@ -621,13 +646,59 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) smax = fmax(smax, __shfl_xor(smax, off));
     int buf_n = 0;
-    double theta = wave_cut_hist(bd, bi, W, k, smax, hist, lane, &buf_n);
+    double theta;
     bool theta_exact = false;  // theta is the k-th of the buffer (true after an exact cut)
-    if (theta < 0.0) {
-      theta = wave_keep_k(bd, bi, W, k, hist, lane);
-      buf_n = k;
-      theta_exact = true;
+    if (theta_prev < smax) {
+      // the previous query's bound is tighter: keep the seed entries within
+      // it (possibly fewer than k; the traversal adds the rest)
+      constexpr int SU = (kSeed + 63) / 64;
+      double vd[SU];
+      int32_t vi[SU];
+      bool keep[SU];
+#pragma unroll
+      for (int u = 0; u < SU; ++u) {
+        const int t = (int)lane + 64 * u;
+        keep[u] = false;
+        vd[u] = 0.0;
+        vi[u] = 0;
+        if (t < W) {
+          vd[u] = bd[t];
+          vi[u] = bi[t];
+          keep[u] = vd[u] <= theta_prev;
+        }
+      }
+      wave_sync();
+      int base = 0;
+#pragma unroll
+      for (int u = 0; u < SU; ++u) {
+        const uint64_t mk = ballot(keep[u]);
+        if (keep[u]) {
+          const int slot = base + (int)mask_rank(mk);
+          bd[slot] = vd[u];
+          bi[slot] = vi[u];
+        }
+        base += __popcll(mk);
+      }
+      wave_sync();
+      buf_n = base;
+      theta = theta_prev;
+      if (buf_n >= k) {
+        int kept = 0;
+        const double th2 = wave_cut_hist(bd, bi, buf_n, k, theta, hist, lane, &kept);
+        if (th2 >= 0.0) {
+          theta = th2;
+          buf_n = kept;
+        }
+      }
+    } else {
+      theta = wave_cut_hist(bd, bi, W, k, smax, hist, lane, &buf_n);
+      if (theta < 0.0) {
+        theta = wave_keep_k(bd, bi, W, k, hist, lane);
+        buf_n = k;
+        theta_exact = true;
+      }
     }
+    c_ins += (uint32_t)buf_n;
 
     // ---- 2. pruned traversal of the stacked subtrees
     while (sp > 0) {
@@ -763,6 +834,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
             bi[slot] = ix[u];
           }
           buf_n += __popcll(mt2);
+          c_ins += (uint32_t)__popcll(mt2);
           wave_sync();
         }
       }
@@ -775,6 +847,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     if (a.direct) wave_sort(bd, bi, k, lane);
     for (int t = (int)lane; t < k; t += 64) a.nbr[qi * k + t] = bi[t];
     if (lane == 0) a.radius[qi] = sqrt(r2);
+    nn_root = root;
     wave_sync();
   }
   }
@@ -787,6 +860,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     if (c_sorts) atomicAdd(&a.ctr[ALB_CTR_KNN_SORTS], (unsigned long long)c_sorts);
     if (c_desc) atomicAdd(&a.ctr[ALB_CTR_KNN_DESCENT], (unsigned long long)c_desc);
     if (c_small) atomicAdd(&a.ctr[ALB_CTR_SMALL_SCENE], (unsigned long long)c_small);
+    if (c_ins) atomicAdd(&a.ctr[ALB_CTR_NEIGHBOURS], (unsigned long long)c_ins);
   }
 }
 
