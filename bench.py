@@ -7,37 +7,39 @@ N_b sensor paths, k-NN gather into FP64 waveforms.  Default workload: the
 (wall seconds per step), wall time bracketed by device synchronisation.
 
 Also reported: per-phase times (HIP events on the launch stream), the
-roofline position of the dominant kernel (algorithmic flops or bytes from the
-kernels' own event counters / its time), its HBM traffic per launch when a
-measured summary is present (profiles/knn_kernel_traffic.json, from rocprofv3
-FETCH_SIZE / WRITE_SIZE runs scaled by a copy kernel's reading), and a CPU
-baseline: the time-stepped C oracle on a bounded sample of the same scene, on
-this host's cores.
+roofline position of the dominant kernel (k-NN: algorithmic bytes and flops
+from the kernel's own event counters over its launch time; the binding unit
+from its hardware counters in profiles/knn_kernel_counters.json; HBM traffic
+per launch from profiles/knn_kernel_traffic.json, rocprofv3 FETCH_SIZE /
+WRITE_SIZE runs scaled by a copy kernel's reading), and the CPU baseline: the
+same scene at 1e5 + 2.5e4 photons in full (time-stepped C oracle trace on
+this host's cores, scipy cKDTree k-NN, oracle gather), with the GPU's rate on
+that same small workload beside it.
 
-Radius mode (cfg of the 1e8 shot): the fixed-radius gather of a 256 x 256
-footprint grid on the seafloor plane (1 cm pitch, r = 0.05 m) over the BOTTOM
-and SCATTER maps of one more 1e8-photon forward pass; per map the gather
-time (HIP events), records taken and examined per query, and algorithmic
-GB/s.
+Radius mode (the 1e8 shot): the fixed-radius gather of a 256 x 256 footprint
+grid on the seafloor plane (1 cm pitch, r = 0.05 m) over the BOTTOM and
+SCATTER maps of one more 1e8-photon forward pass; per map the gather time
+(HIP events), records taken and examined per query, and algorithmic GB/s.
 
-Inversion: Adam over the batched adjoint on --inversion-scenes synthetic
-scenes (unknown depth, b, albedo) for --inversion-iters iterations; depth
-RMSE and seconds per iteration, batched form vs plain.
-
-The two other forms of the metric, on bounded samples:
-  * batched: the first --sweep-scenes scenes of the Latin-hypercube sweep
-    (seed 1) at 1/100 of their photon counts, once as one batched run and
-    once as plain per-scene runs; paths/s of each and the ratio;
-  * adjoint: --adjoint-scenes sweep scenes at 1e5 forward / 2.5e4 backward
-    photons; wall time of forward + adjoint (path replay, five parameters)
-    over the forward alone, and with the one-sided depth difference (one
-    more forward pass);
-  * with --sweep-full N: the first N sweep scenes at their full photon
-    counts, batched in memory-bounded scene groups vs plain per-scene runs.
+The other forms of the metric:
+  * batched: the first --sweep-full scenes of the 1024-scene Latin-hypercube
+    sweep (seed 1) at their full photon counts 10^U[5,7], batched in
+    memory-bounded scene groups vs plain per-scene runs; batched_small: the
+    first --sweep-scenes scenes at 1/100 of those counts (the regime where
+    per-scene launch and synchronisation costs dominate);
+  * adjoint: --adjoint-scenes sweep scenes at --adjoint-photons forward (1/4
+    backward) photons, MSE against targets from perturbed parameters: wall
+    time of forward + path-replay adjoint over the forward alone, in
+    memory-bounded scene groups;
+  * inversion: Adam over the batched adjoint on --inversion-scenes synthetic
+    scenes (unknown depth, b, albedo), --inversion-iters iterations; depth
+    RMSE and seconds per iteration, batched vs plain; with
+    --inversion-full-iteration also one iteration over all 4096 scenes.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--forward N]
        [--backward N] [--no-cpu] [--no-batched] [--no-adjoint] [--no-radius]
-       [--no-inversion] [--sweep-full N]
+       [--no-inversion] [--sweep-full N] [--adjoint-scenes N]
+       [--inversion-scenes N] [--inversion-full-iteration]
 """
 from __future__ import annotations
 
@@ -53,6 +55,7 @@ if ROOT not in sys.path:
 
 METRIC = "photon paths/s on 1 GPU, plain vs batched; adjoint/forward time ratio"
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 # algorithmic work per event (flops: add/sub/mul/div/cmp = 1, FMA = 2,
@@ -96,36 +99,75 @@ def host_cpu() -> dict:
 
 
 def cpu_baseline(model, key, n_f: int, n_b: int, threads: int):
-    """Time the C oracle (time-stepped, brute-force k-NN) on n_f + n_b paths."""
+    """The C oracle on n_f + n_b paths of the scene, on `threads` host threads:
+    the time-stepped trace of both passes (each thread a range of photons),
+    the maps per type, the k-NN with scipy's cKDTree (the k-d tree query the
+    reference's gather uses, photon_map_index.py:17, core/simulation.py:
+    139-142) on all threads, and the oracle's gather over those neighbour
+    sets.  Returns (paths/s, wall seconds, seconds per stage)."""
     from concurrent.futures import ThreadPoolExecutor
 
     import numpy as np
+    from scipy.spatial import cKDTree
 
     import oracle
     from pyalbsim_amd import abi
 
     oracle.lib()
+    stages = {}
     t0 = time.perf_counter()
 
-    def chunks(n):
-        step = max(1, (n + threads - 1) // threads)
+    def chunks(n, parts):
+        step = max(1, (n + parts - 1) // parts)
         return [(b, min(step, n - b)) for b in range(0, n, step)]
 
     with ThreadPoolExecutor(threads) as ex:
-        fw = list(ex.map(lambda bc: oracle.trace(model, key, abi.PASS_FORWARD, bc[0], bc[1])[0], chunks(n_f)))
-        bw = list(ex.map(lambda bc: oracle.trace(model, key, abi.PASS_BACKWARD, bc[0], bc[1])[0], chunks(n_b)))
+        # photon ranges several times the thread count, so slow paths even out
+        fw = list(ex.map(lambda bc: oracle.trace(model, key, abi.PASS_FORWARD, bc[0], bc[1])[0],
+                         chunks(n_f, 8 * threads)))
+        t1 = time.perf_counter()
+        bw = list(ex.map(lambda bc: oracle.trace(model, key, abi.PASS_BACKWARD, bc[0], bc[1])[0],
+                         chunks(n_b, 8 * threads)))
+        t2 = time.perf_counter()
         maps = [{f: np.concatenate([c[t][f] for c in fw]) for f in fw[0][t]} for t in range(4)]
         qs = [{f: np.concatenate([c[t][f] for c in bw]) for f in bw[0][t]} for t in range(4)]
+        t_knn = 0.0
         jobs = []
         for t in range(4):
             nq = len(qs[t]["x"])
             if nq == 0 or len(maps[t]["x"]) < model.k:
                 continue
-            for b, c in chunks(nq):
-                jobs.append((t, {f: v[b:b + c] for f, v in qs[t].items()}))
-        list(ex.map(lambda j: oracle.gather(model, j[0], maps[j[0]], j[1]), jobs))
+            ta = time.perf_counter()
+            pts = np.stack([maps[t]["x"], maps[t]["y"], maps[t]["z"]], 1).astype(np.float64)
+            qp = np.stack([qs[t]["x"], qs[t]["y"], qs[t]["z"]], 1)
+            _, nb = cKDTree(pts).query(qp, k=model.k, workers=threads)
+            t_knn += time.perf_counter() - ta
+            for b, c in chunks(nq, 8 * threads):
+                jobs.append((t, {f: v[b:b + c] for f, v in qs[t].items()}, nb[b:b + c]))
+        t3 = time.perf_counter()
+        list(ex.map(lambda j: oracle.gather(model, j[0], maps[j[0]], j[1], nbrs=j[2]), jobs))
     wall = time.perf_counter() - t0
-    return (n_f + n_b) / wall, wall
+    stages = {"forward_trace_s": t1 - t0, "backward_trace_s": t2 - t1, "knn_s": t_knn, "gather_s": time.perf_counter() - t3}
+    return (n_f + n_b) / wall, wall, stages
+
+
+def gpu_small_shot(model, key, n_f: int, n_b: int, repeats: int = 5):
+    """GPU paths/s of the plain form on a small shot (n_f + n_b photons), the
+    median of `repeats` timed runs after one warm-up."""
+    import numpy as np
+    import torch
+
+    from pyalbsim_amd.execution.linear import simulate
+
+    simulate(model, n_f, n_b, key)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        simulate(model, n_f, n_b, key)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return (n_f + n_b) / float(np.median(ts)), float(np.median(ts))
 
 
 def main(argv=None):
@@ -135,22 +177,24 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--forward", type=float, default=1e8)
     ap.add_argument("--backward", type=float, default=2.5e7)
-    ap.add_argument("--cpu-forward", type=int, default=16000)
-    ap.add_argument("--cpu-backward", type=int, default=4000)
+    ap.add_argument("--cpu-forward", type=int, default=100000)
+    ap.add_argument("--cpu-backward", type=int, default=25000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-radius", action="store_true")
     ap.add_argument("--no-inversion", action="store_true")
-    ap.add_argument("--inversion-scenes", type=int, default=32)
+    ap.add_argument("--inversion-scenes", type=int, default=512)
     ap.add_argument("--inversion-iters", type=int, default=10)
     ap.add_argument("--inversion-lr", type=float, default=0.02)
     ap.add_argument("--radius-grid", type=int, default=256)
     ap.add_argument("--sweep-scenes", type=int, default=256)
-    ap.add_argument("--sweep-full", type=int, default=0,
-                    help="also run this many sweep scenes at full photon counts (grouped batched vs plain)")
-    ap.add_argument("--adjoint-scenes", type=int, default=16)
-    ap.add_argument("--adjoint-photons", type=float, default=1e5)
+    ap.add_argument("--sweep-full", type=int, default=32,
+                    help="sweep scenes run at their full photon counts (grouped batched vs plain); 0: none")
+    ap.add_argument("--adjoint-scenes", type=int, default=256)
+    ap.add_argument("--adjoint-photons", type=float, default=1e6)
+    ap.add_argument("--inversion-full-iteration", action="store_true",
+                    help="also time one inversion iteration at 4096 scenes x 2e5 photons")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 2 GiB device copy at the end (reference reading for HBM byte counters)")
     args = ap.parse_args(argv)
@@ -205,33 +249,52 @@ def main(argv=None):
     k = model.k
     qc = res.query_counts
     g_flops = sum(qc[t] * (4 + (7 + M_TYPE[t]) * k) for t in range(4)) + 15 * cand
-    # k-NN kernel: candidate coordinates (12 B), neighbour indices written (4 B x k), node lines (64 B)
-    knn_bytes = 12 * cand + 4 * k * nq + 64 * nodes
+    # k-NN algorithmic work per query (the gather's counting rules): bytes
+    # 12 per candidate's coordinates + 20 per neighbour (index, direction,
+    # energy, time) + 8 per tree node visited; flops 4 + 15 per candidate +
+    # 7 per neighbour
+    knn_bytes = 12 * cand + 20 * k * nq + 8 * nodes
+    knn_flops = 4 * nq + 15 * cand + 7 * k * nq
+    # the same with this tree's node size and the neighbour payload left to
+    # the contribution kernel: 12 per candidate + 4 per neighbour index
+    # written + 64 per node line
+    knn_bytes_lines = 12 * cand + 4 * k * nq + 64 * nodes
     knn_ms = [s_.elapsed_time(e_) for s_, e_ in kev["knn"]]
     contrib_ms = [s_.elapsed_time(e_) for s_, e_ in kev["contrib"]]
     knn_launches_per_step = len(knn_ms) / args.steps
     knn_avg_ms = sum(knn_ms) / len(knn_ms)
-    t_trace = (phases["forward"] + phases["backward"]) / 1e3
-    t_knn = sum(knn_ms) / args.steps / 1e3
-    if t_knn >= t_trace:
-        # bytes per launch: the step's algorithmic bytes spread over its launches
-        roof = {"kernel": "knn_kernel", "bound": "hbm",
-                "achieved": knn_bytes / knn_launches_per_step / (knn_avg_ms / 1e3) / 1e9,
-                "peak": PEAK_HBM_GBS, "unit": "GB/s", "avg_launch_ms": knn_avg_ms,
-                "launches_per_step": knn_launches_per_step}
-    else:
-        roof = {"kernel": "trace_kernel", "bound": "valu", "achieved": (flops_fwd + flops_bwd) / t_trace / 1e12,
-                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s"}
+    t_launch = knn_avg_ms / 1e3
+    per_launch = 1.0 / knn_launches_per_step
+    roof = {"kernel": "knn_kernel", "bound": "valu",
+            "achieved": knn_bytes * per_launch / t_launch / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "byte_model": "12 B per candidate + 20 B per neighbour + 8 B per tree node, per query",
+            "avg_launch_ms": knn_avg_ms, "launches_per_step": knn_launches_per_step,
+            "achieved_node_lines": knn_bytes_lines * per_launch / t_launch / 1e9,
+            "flops": {"achieved": knn_flops * per_launch / t_launch / 1e12, "peak": PEAK_FP64_TFLOPS,
+                      "unit": "TFLOP/s"}}
     roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["frac_node_lines"] = roof["achieved_node_lines"] / roof["peak"]
+    roof["flops"]["frac"] = roof["flops"]["achieved"] / roof["flops"]["peak"]
     roof["traffic"] = None
     tpath = os.path.join(ROOT, "profiles", "knn_kernel_traffic.json")
-    if roof["kernel"] == "knn_kernel" and os.path.exists(tpath):
+    if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
         roof["traffic"] = tr.get("bytes_per_launch")
         roof["traffic_source"] = tr.get("source")
-    g_bytes = knn_bytes + (20 + 48) * k * nq
+    cpath = os.path.join(ROOT, "profiles", "knn_kernel_counters.json")
+    if os.path.exists(cpath):
+        # the binding unit from the kernel's counters: VALU issue against DRAM traffic
+        with open(cpath) as f:
+            cn = json.load(f)
+        roof["counters"] = {x: cn.get(x) for x in ("valu_issue_frac", "salu_issue_frac", "lds_bank_conflict_frac",
+                                                   "dram_frac", "source")}
+        roof["bound"] = "valu" if (cn.get("valu_issue_frac") or 0) >= (cn.get("dram_frac") or 0) else "hbm"
+    # whole gather (k-NN + contributions): node lines, candidates, neighbour
+    # indices, and per (query, neighbour) pair the record payload and row data
+    g_bytes = knn_bytes_lines + (20 + 48) * k * nq
     t_gather = phases["gather"] / 1e3
+    t_trace = (phases["forward"] + phases["backward"]) / 1e3
 
     out = {
         "metric": METRIC, "value": value, "unit": "photon paths/s", "n_gpus": 1, "steps": args.steps,
@@ -258,25 +321,31 @@ def main(argv=None):
         "depth_solves_per_path": float(cf[abi.CTR_DEPTH_SOLVES]) / max(float(cf[abi.CTR_PATHS]), 1.0),
         "cpu_baseline": None,
     }
+    del res
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
-        rate, cwall = cpu_baseline(model, key, args.cpu_forward, args.cpu_backward, threads)
+        rate, cwall, stages = cpu_baseline(model, key, args.cpu_forward, args.cpu_backward, threads)
+        g_rate, g_s = gpu_small_shot(model, key, args.cpu_forward, args.cpu_backward)
         out["cpu_baseline"] = {"value": rate, "unit": "photon paths/s", "cores": threads, "kind": "port",
-                               "host": host_cpu(),
-                               "sample": f"time-stepped C oracle, same scene, {args.cpu_forward} forward + "
-                                         f"{args.cpu_backward} backward photons, brute-force k-NN, "
-                                         f"{cwall:.1f} s wall"}
-    del res
+                               "host": host_cpu(), "stages": stages,
+                               "sample": f"the same scene at {args.cpu_forward} forward + {args.cpu_backward} "
+                                         f"backward photons, in full: time-stepped C oracle trace on {threads} "
+                                         f"threads, scipy cKDTree k-NN, oracle gather; {cwall:.1f} s wall",
+                               "gpu_same_workload": {"value": g_rate, "unit": "photon paths/s", "seconds": g_s},
+                               "gpu_over_cpu": g_rate / rate}
+        print("cpu baseline done", file=sys.stderr, flush=True)
     if not args.no_radius:
         out["radius_mode"] = bench_radius(model, ds, key, n_f, args.radius_grid)
     if not args.no_batched:
-        out["batched"] = bench_batched(args.sweep_scenes)
-    if args.sweep_full > 0:
-        out["batched_full"] = bench_batched_full(args.sweep_full)
+        if args.sweep_full > 0:
+            out["batched"] = bench_batched_full(args.sweep_full)
+        out["batched_small"] = bench_batched(args.sweep_scenes)
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(args.adjoint_scenes, int(args.adjoint_photons))
     if not args.no_inversion:
         out["inversion"] = bench_inversion(args.inversion_scenes, args.inversion_iters, lr=args.inversion_lr)
+        if args.inversion_full_iteration:
+            out["inversion_full_iteration"] = bench_inversion_full_iteration()
     if args.copy_calibration:
         x = torch.empty(1 << 28, dtype=torch.float64, device="cuda")
         y = torch.empty_like(x)
@@ -335,37 +404,62 @@ def bench_radius(model, ds, key, n_f: int, grid: int, radius: float = 0.05, pitc
     return out
 
 
-def bench_inversion(n_scenes: int, iters: int, n_forward: int = 20000, lr: float = 0.02):
+def bench_inversion(n_scenes: int, iters: int, n_forward: int = 20000, lr: float = 0.02, plain_scenes: int = 32):
     """Adam over the batched adjoint on synthetic inversion scenes: depth RMSE
-    and seconds per iteration, batched form vs plain (one scene per launch)."""
+    and seconds per iteration, batched form vs plain (one scene per launch;
+    timed on the first plain_scenes scenes and scaled to n_scenes)."""
     import numpy as np
 
     from pyalbsim_amd.inversion import invert, inversion_scenes, inversion_tilts
 
     th = inversion_scenes(n_scenes, seed=3)
     tilts = inversion_tilts(n_scenes, seed=3)
-    rb = invert(th, n_forward, n_forward // 4, iterations=iters, lr=lr, tilts=tilts,
+    rb = invert(th, n_forward, n_forward // 4, iterations=iters, lr=lr, tilts=tilts, group_scenes=256,
                 log=(lambda m: print(m, file=sys.stderr, flush=True)) if iters > 20 else None)
     print("inversion batched done", file=sys.stderr, flush=True)
-    rp = invert(th, n_forward, n_forward // 4, iterations=2, lr=lr, form="plain", tilts=tilts)
+    ps = min(plain_scenes, n_scenes)
+    rp = invert(th[:ps], n_forward, n_forward // 4, iterations=2, lr=lr, form="plain", tilts=tilts[:ps])
     print("inversion plain done", file=sys.stderr, flush=True)
     sb = float(np.median(rb.seconds_per_iteration[1:] or rb.seconds_per_iteration))
-    sp = float(np.median(rp.seconds_per_iteration))
+    sp = float(np.median(rp.seconds_per_iteration)) * n_scenes / ps
     return {"workload": f"{n_scenes} scenes, depth U[2,40] m, b U[0.1,1.5] /m, albedo U[0.05,0.4] unknown "
                         f"(a = 0.1 /m, g = 0.9 known; roughness from wind U[0,10] m/s; floor tilted U[0,15] deg, "
                         f"azimuth U[0,360) deg, known), "
                         f"{n_forward} forward + {n_forward // 4} backward photons per scene per pass, "
                         f"Adam lr {lr:g} on [0,1]-scaled unknowns, depth started from the target's "
-                        f"floor-return leading edge (mid-range where none shows), {iters} iterations "
-                        f"(plain form timed over 2)",
+                        f"floor-return leading edge (mid-range where none shows), {iters} iterations; "
+                        f"batched in groups of 256 scenes; plain form timed over 2 iterations of the first {ps} "
+                        f"scenes, scaled to {n_scenes}",
             "depth_rmse_start_m": rb.depth_rmse[0], "depth_rmse_end_m": rb.depth_rmse[-1],
             "depth_rmse_every_10": rb.depth_rmse[::10],
             "loss_start": rb.loss[0], "loss_end": rb.loss[-1],
             "batched_s_per_iteration": sb, "plain_s_per_iteration": sp, "speedup": sp / sb}
 
 
+def bench_inversion_full_iteration(n_scenes: int = 4096, n_forward: int = 20000, group: int = 512):
+    """One Adam iteration over every inversion scene (batched, in groups of
+    `group` scenes); seconds per iteration at n_forward photons and the
+    estimate for 2e5 photons (the work is linear in photons) and 200
+    iterations."""
+    from pyalbsim_amd.inversion import invert, inversion_scenes, inversion_tilts
+
+    th = inversion_scenes(n_scenes, seed=3)
+    tilts = inversion_tilts(n_scenes, seed=3)
+    r = invert(th, n_forward, n_forward // 4, iterations=2, lr=1e-2, tilts=tilts, group_scenes=group,
+               log=lambda m: print(m, file=sys.stderr, flush=True))
+    s_it = float(r.seconds_per_iteration[-1])
+    est = s_it * (2e5 / n_forward)
+    print("inversion full iteration done", file=sys.stderr, flush=True)
+    return {"workload": f"{n_scenes} inversion scenes, {n_forward} forward + {n_forward // 4} backward photons per "
+                        f"scene, 2 Adam iterations (the second timed) in groups of {group} scenes",
+            "s_per_iteration": s_it, "estimated_s_per_iteration_at_2e5_photons": est,
+            "estimated_s_for_200_iterations_at_2e5_photons": 200 * est,
+            "depth_rmse": r.depth_rmse}
+
+
 def bench_batched(n_scenes: int):
-    """Plain per-scene runs vs one batched run over the same sweep scenes."""
+    """Plain per-scene runs vs one batched run over the same sweep scenes at
+    1/100 of their photon counts."""
     import torch
 
     from pyalbsim_amd.execution.batched import simulate_batched
@@ -401,7 +495,7 @@ def bench_batched(n_scenes: int):
     print(f"plain {n_scenes} scenes done", file=sys.stderr, flush=True)
     return {"workload": f"first {n_scenes} of 1024 Latin-hypercube sweep scenes (seed 1; depth U[1,30] m, "
                         f"a U[0.02,0.5], b U[0.05,2], HG g U[0.8,0.95], albedo U[0.05,0.4], wind U[0,10] m/s), "
-                        f"photons 10^U[3,5] per scene (1/100 of 10^U[5,7]), backward = forward / 4",
+                        f"photons 10^U[3,5] per scene (1/100 of the sweep's 10^U[5,7]), backward = forward / 4",
             "forward_photons": int(sum(nf)), "backward_photons": int(sum(nb)),
             "map_records": counts[0], "gather_queries": counts[1],
             "batched_paths_per_s": paths / tb, "plain_paths_per_s": paths / tp, "batched_s": tb, "plain_s": tp,
@@ -444,48 +538,75 @@ def bench_batched_full(n_scenes: int):
             "speedup": tp / tb, "autotuned": {"block": block, "refill": refill}}
 
 
-def bench_adjoint(n_scenes: int, photons: int = 100000):
+def bench_adjoint(n_scenes: int, photons: int = 1000000):
     """Forward vs forward + path-replay adjoint on sweep scenes (photons
-    forward, photons / 4 backward per scene)."""
+    forward, photons / 4 backward per scene), MSE loss against targets from
+    parameters perturbed by U[-10 %, 10 %] (seed 2).  The scenes run in
+    memory-bounded groups (the adjoint keeps the sinks, maps and neighbour
+    rows of its group); times are summed over the groups."""
     import numpy as np
     import torch
 
     from pyalbsim_amd import adjoint
     from pyalbsim_amd.execution.batched import simulate_batched
-    from pyalbsim_amd.sweep import hypercube
+    from pyalbsim_amd.sweep import estimate_group_bytes, free_device_bytes, hypercube, plan_groups
 
     sc = hypercube(256, 1)[:n_scenes]
     th = np.array([[x.params[k] for k in adjoint.PARAMS] for x in sc])
+    pert = th * (1 + np.random.default_rng(2).uniform(-0.1, 0.1, th.shape))
+    pert[:, 2] = np.clip(pert[:, 2], 0.5, 0.97)
     models = [adjoint.model_from_theta(t) for t in th]
+    tmodels = [adjoint.model_from_theta(t) for t in pert]
     nf, nb = int(photons), int(photons) // 4
     key = (0xAD70, 2)
-    simulate_batched(models[:2], 1000, 250, key, strict=False, adjoint=True)
+    simulate_batched(models[:2], 1000, 250, key, strict=False, adjoint=True)  # warm-up
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    simulate_batched(models, nf, nb, key, win_len=1000, strict=False)
-    torch.cuda.synchronize()
-    tf = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    res = simulate_batched(models, nf, nb, key, win_len=1000, strict=False, adjoint=True)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    d = torch.zeros_like(res.waveforms)
-    for s, m in enumerate(models):
-        g0, gl = m.gate(1000)
-        o = int(res.offsets[s])
-        d[o:o + 4 * m.n_bins].view(4, m.n_bins)[:, g0:g0 + gl] = res.waveform(s)[:, g0:g0 + gl]
-    acc, direct = adjoint.adjoint_sums(res, d)
-    derivs = [adjoint.scene_derivatives(m, th[s]) for s, m in enumerate(models)]
-    adjoint.gradients_from_sums(acc, direct, derivs)
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    print(f"adjoint {n_scenes} scenes done", file=sys.stderr, flush=True)
+    est = estimate_group_bytes(models, np.full(n_scenes, nf), np.full(n_scenes, nb), key,
+                               bytes_per_record=320.0, bytes_per_query=600.0) * 1.3
+    groups = plan_groups(est, 0.5 * free_device_bytes())
+    tf = tk = ta = 0.0
+    for a, b in groups:
+        ids = range(a, b)
+        # targets at a tenth of the photons, outside the timed region
+        rt = simulate_batched(tmodels[a:b], nf // 10, nb // 10, (0x7A26, 7), scene_ids=ids, win_len=1000,
+                              strict=False)
+        T = [rt.gated(s, tmodels[a + s].gate(1000)).clone() for s in range(b - a)]
+        del rt
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = simulate_batched(models[a:b], nf, nb, key, scene_ids=ids, win_len=1000, strict=False)
+        torch.cuda.synchronize()
+        tf += time.perf_counter() - t0
+        del r
+        t0 = time.perf_counter()
+        res = simulate_batched(models[a:b], nf, nb, key, scene_ids=ids, win_len=1000, strict=False, adjoint=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        d = torch.zeros_like(res.waveforms)
+        for s in range(b - a):
+            m = models[a + s]
+            g0, gl = m.gate(1000)
+            o = int(res.offsets[s])
+            d[o:o + 4 * m.n_bins].view(4, m.n_bins)[:, g0:g0 + gl] = 2 * (res.waveform(s)[:, g0:g0 + gl] - T[s])
+        acc, direct = adjoint.adjoint_sums(res, d)
+        derivs = [adjoint.scene_derivatives(m, th[a + s]) for s, m in enumerate(models[a:b])]
+        g = adjoint.gradients_from_sums(acc, direct, derivs)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        tk += t1 - t0
+        ta += t2 - t1
+        del res, d
+        if not np.all(np.isfinite(g)):
+            raise RuntimeError("non-finite adjoint gradient")
+        print(f"adjoint scenes {a}-{b} of {n_scenes} done", file=sys.stderr, flush=True)
     return {"workload": f"first {n_scenes} Latin-hypercube sweep scenes (seed 1), {nf} forward + {nb} backward "
-                        f"photons each; loss 0.5 |W|^2 over the 1000-bin gate",
-            "forward_s": tf, "forward_kept_s": t1 - t0, "adjoint_s": t2 - t1,
-            "ratio": (t2 - t0) / tf, "ratio_with_depth_difference": (t2 - t0 + tf) / tf,
+                        f"photons each, MSE over the 1000-bin gate against targets from parameters perturbed by "
+                        f"U[-10%,10%] (seed 2), in {len(groups)} memory-bounded groups",
+            "forward_s": tf, "forward_kept_s": tk, "adjoint_s": ta,
+            "ratio": (tk + ta) / tf, "ratio_with_depth_difference": (tk + ta + tf) / tf,
             "depth_difference": "one-sided, common random numbers: one more forward pass",
-            "parameters": list(adjoint.PARAMS)}
+            "parameters": list(adjoint.PARAMS), "groups": len(groups),
+            "forward_paths_per_s": n_scenes * (nf + nb) / tf}
 
 
 if __name__ == "__main__":

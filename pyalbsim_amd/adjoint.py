@@ -120,6 +120,14 @@ def model_from_theta(theta, sample_rate=None, tilt=None):
     return build_scene(shot_config(depth, a, b, albedo, rough, sample_rate=sr, slope=sl, azimuth=az), hg_g=g)
 
 
+def _scene_ids(spec, n):
+    """Philox scene streams of a batch: scene_base + 0 .. n - 1 (spec
+    "scene_base", default 0), so that groups of one scene set keep distinct
+    streams."""
+    b = int(spec.get("scene_base", 0))
+    return range(b, b + n)
+
+
 def _models(th, spec):
     tilts = spec.get("tilts")
     return [model_from_theta(row, spec.get("sample_rate"), None if tilts is None else tilts[i])
@@ -229,6 +237,7 @@ class BatchedWaveformFn:
                 models = _models(th, spec)
                 gr = spec.get("graphs")
                 res = simulate_batched(models, spec["n_forward"], spec["n_backward"], spec["key"],
+                                       scene_ids=_scene_ids(spec, len(models)),
                                        win_len=spec.get("win_len", 1000), strict=False, adjoint=True,
                                        block=spec.get("block", 256), refill=spec.get("refill", 4),
                                        graphs=None if gr is None else gr.setdefault("forward", {}))
@@ -254,6 +263,10 @@ class BatchedWaveformFn:
                     nb = m.n_bins
                     dloss[o:o + 4 * nb].view(4, nb)[:, g0:g0 + gl] = go[s]
                 acc, direct = adjoint_sums(res, dloss)
+                # the kept forward state is no longer needed: free it before
+                # the depth difference runs its own forward pass
+                ctx.res = None
+                del res, dloss
                 derivs = [scene_derivatives(m, th[s]) for s, m in enumerate(models)]
                 grads = gradients_from_sums(acc, direct, derivs)
                 mode = spec.get("depth_grad", "fd1")
@@ -284,6 +297,7 @@ def depth_fd(th, spec, go, rel: float = 1e-3, base=None):
         models = _models(t2, spec)
         gr = spec.get("graphs")
         r = simulate_batched(models, spec["n_forward"], spec["n_backward"], spec["key"], win_len=wl, strict=False,
+                             scene_ids=_scene_ids(spec, len(models)),
                              block=spec.get("block", 256), refill=spec.get("refill", 4),
                              graphs=None if gr is None else gr.setdefault("depth_difference", {}))
         outs.append(torch.stack([r.gated(s, models[s].gate(wl)) for s in range(len(models))]))
@@ -305,6 +319,7 @@ def gated_waveforms(theta, n_forward, n_backward, key, **spec):
     models = _models(th, spec)
     wl = spec.get("win_len", 1000)
     res = simulate_batched(models, n_forward, n_backward, key, win_len=wl, strict=False,
+                           scene_ids=_scene_ids(spec, len(models)),
                            block=spec.get("block", 256), refill=spec.get("refill", 4))
     return torch.stack([res.gated(s, models[s].gate(wl)) for s in range(len(models))])
 

@@ -49,7 +49,8 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
              tags: bool = False, dscene: Optional[runtime.DeviceScene] = None, sync_timing: bool = False,
              strict: bool = True, events: Optional[dict] = None,
              kernel_events: Optional[dict] = None, scene_id: int = 0, heatmaps=None,
-             points: bool = False, replay_forward=None, replay_backward=None) -> SimulationResult:
+             points: bool = False, replay_forward=None, replay_backward=None,
+             copies=(1, 1)) -> SimulationResult:
     """Forward trace -> map build -> backward trace -> gather, all on the GPU.
 
     events: optional dict; HIP events are recorded on the launch stream at the
@@ -68,6 +69,11 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
     replay_forward / replay_backward: optional host-supplied uniforms, FP64
     arrays [n_forward, L] / [n_backward, L] (photon slot i draws row i in
     order) in place of the Philox streams of that pass.
+    copies: (forward, backward) repetitions of the traced photons: the maps
+    hold every forward record `forward` times, and the waveforms and heat
+    maps count every backward path `backward` times, as identical batches
+    would (the reference's run_parallel with a seed gives every batch the
+    same seed, execution/parallel.py:66-68, 110-112).
     """
     import torch
 
@@ -96,8 +102,15 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
     cb = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device="cuda")
     rep_f = _device_replay(replay_forward, n_forward)
     rep_b = _device_replay(replay_backward, n_backward)
+    cp_f, cp_b = (int(copies[0]), int(copies[1]))
+    if cp_f < 1 or cp_b < 1:
+        raise ValueError("copies must be positive")
     recs, rcounts = runtime.trace(ds, key, abi.PASS_FORWARD, n_forward, tags=tags, counters=cf, scene_id=scene_id,
                                   points=points, replay=rep_f)
+    if cp_f > 1:
+        recs, rcounts = _repeat_records(recs, rcounts, cp_f)
+        if ds.heat is not None:  # every copy of a batch counts its floor hits again
+            ds.heat["radius"].mul_(cp_f)
     mark("forward")
     maps = [runtime.build_map(recs[ti], rcounts[ti], runtime.MAP_DIMS[ti]) for ti in range(abi.N_TYPES)]
     if not keep:
@@ -121,6 +134,11 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
         keep_alive.append(runtime.gather(ds, maps[ti], ti, qs[ti], qcounts[ti], wf[ti], win, counters=cb,
                                          events=kernel_events))
     mark("gather")
+    if cp_b > 1:
+        wf.mul_(cp_b)
+        if ds.heat is not None:
+            ds.heat["water"].mul_(cp_b)
+            ds.heat["floor"].mul_(cp_b)
     res = SimulationResult(model, wf, rcounts, qcounts, None, None, timings=t)
     ctr_f, ctr_b = cf.cpu().numpy().view(np.uint64), cb.cpu().numpy().view(np.uint64)
     if ctr_b[abi.CTR_OVERFLOW]:
@@ -133,6 +151,23 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
         res.maps = maps
         res.queries = qs
     return res
+
+
+def _repeat_records(sinks, counts, c: int):
+    """Record sinks holding every record of sinks c times (copy j after copy j - 1)."""
+    out, nc = [], []
+    for sk, n in zip(sinks, counts):
+        t = sk.tensors
+        r = runtime.Sink.alloc(n * c, False, t.get("tag") is not None, device=t["x"].device,
+                               points=t.get("fwi_x") is not None)
+        for f, v in t.items():
+            if f == "count" or v is None:
+                continue
+            r.tensors[f][:n * c].copy_(v[:n].repeat(c))
+        r.tensors["count"].fill_(n * c)
+        out.append(r)
+        nc.append(n * c)
+    return out, nc
 
 
 def photon_map_stats_table(counts) -> str:

@@ -18,7 +18,15 @@ exactly as run_linear, and ``processes`` is ignored.  Returned:
     simulation_config.heatmap.enabled, else None.
 
 ``seed`` seeds numpy.random.default_rng, whose two draws make the Philox key
-(None: a fresh secrets-seeded generator, as the reference).
+(None: a fresh secrets-seeded generator, as the reference).  The photons of
+all batches are distinct Philox streams.  The reference instead gives every
+worker the same seed when one is passed (execution/parallel.py:66-68,
+110-112, 208-210, 240-242), so its maps hold batches_forward identical
+copies of one forward batch and its waveform sums batches_backward identical
+backward batches; same_seed_batches=True reproduces that (with a seed): one
+batch of each pass is traced, its records are repeated batches_forward
+times before the map build, and the backward paths count batches_backward
+times.
 """
 from __future__ import annotations
 
@@ -59,22 +67,27 @@ def photon_maps_from_result(res) -> dict:
 
 
 def run_parallel(simulation_config: SimulationConfig, run_config: RunConfig,
-                 seed: Union[int, None] = None, *, verbose: bool = True):
+                 seed: Union[int, None] = None, *, verbose: bool = True, same_seed_batches: bool = False):
     """(waveform, photon_maps_data, sampled heat maps or None) for one scene, computed on the GPU."""
     rng = np.random.default_rng(secrets.randbits(128) if seed is None else seed)
     key = abi.key_from_generator(rng)
     model = build_scene(simulation_config)
     n_f = run_config.photons_per_batch_forward * run_config.batches_forward
     n_b = run_config.photons_per_batch_backward * run_config.batches_backward
+    copies = (1, 1)
+    if same_seed_batches and seed is not None:
+        n_f, n_b = run_config.photons_per_batch_forward, run_config.photons_per_batch_backward
+        copies = (run_config.batches_forward, run_config.batches_backward)
     if verbose:
         print("Starting forwards pass")
     t0 = time.perf_counter()
     hc = simulation_config.heatmap
-    res = simulate(model, n_f, n_b, key, keep=True, points=True, heatmaps=hc if hc.enabled else None)
+    res = simulate(model, n_f, n_b, key, keep=True, points=True, heatmaps=hc if hc.enabled else None,
+                   copies=copies)
     maps = photon_maps_from_result(res)
     if verbose:
         print("\n" + photon_map_stats_table(res.map_counts) + "\n")
-        print(f"Finished {n_f} + {n_b} photon paths in {time.perf_counter() - t0:.2f} s")
+        print(f"Finished {n_f * copies[0]} + {n_b * copies[1]} photon paths in {time.perf_counter() - t0:.2f} s")
     wf = res.waveforms.cpu().numpy()
     waveform = {pt: wf[pt.abi_index].astype(np.float32) for pt in PhotonType}
     heat = None

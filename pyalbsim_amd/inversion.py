@@ -207,13 +207,19 @@ def floor_timing_depth(total: np.ndarray, bins_per_m, surface_bin, pool: int = 4
 def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: int = 200, lr: float = 1e-2,
            key=(0x1A7E, 5), target_key=(0x7A26, 6), form: str = "batched", win_len: int = 1000,
            depth_rel: float = 1e-2, x0=None, log=None, shape_weight: float = 1000.0,
-           tilts=None, loss: str = "hybrid", start: str = "timing") -> InversionResult:
+           tilts=None, loss: str = "hybrid", start: str = "timing",
+           group_scenes: int = None) -> InversionResult:
     """Adam on (depth, b, albedo) scaled to [0, 1]; returns the estimates and
     the depth RMSE and loss of every iteration.
 
     Start (unless x0 is given): every unknown at mid-range ("mid"), or depth
     from the target's floor-return timing (floor_timing_depth) where the
-    floor return shows and mid-range elsewhere ("timing")."""
+    floor return shows and mid-range elsewhere ("timing").
+
+    group_scenes: batched form in groups of at most this many scenes (the
+    kept forward state of all scenes at once may not fit the device); each
+    iteration runs forward and backward group by group (the loss is a sum
+    over scenes), and scene s keeps Philox scene stream s in any grouping."""
     import torch
 
     from .adjoint import batched_waveforms, gated_waveforms
@@ -222,26 +228,24 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
         raise ValueError(f"start must be 'mid' or 'timing', not {start!r}")
     S = theta_true.shape[0]
     dev = "cuda"
-    # batched form: the scene table is updated in place and the trace passes
-    # replay HIP graphs captured in the first iteration
-    spec = dict(win_len=win_len, sample_rate=SAMPLE_RATE, depth_rel=depth_rel,
-                graphs={} if form == "batched" else None, tilts=None if tilts is None else np.asarray(tilts))
+    tl = None if tilts is None else np.asarray(tilts)
+    # batched form: each group's scene table is updated in place and its
+    # trace passes replay HIP graphs captured in the first iteration
+    spec = dict(win_len=win_len, sample_rate=SAMPLE_RATE, depth_rel=depth_rel, graphs=None, tilts=tl)
     lo = np.array([RANGES[c][0] for c in UNKNOWN])
     hi = np.array([RANGES[c][1] for c in UNKNOWN])
     idx = [COLS[c] for c in UNKNOWN]
-
-    def one(s):  # spec of scene s alone (plain form)
-        return dict(spec, tilts=None if tilts is None else spec["tilts"][s:s + 1])
-
-    def run(theta_t, k):
-        if form == "batched":
-            return batched_waveforms(theta_t, n_forward, n_backward, k, **spec)
-        return torch.cat([batched_waveforms(theta_t[s:s + 1], n_forward, n_backward, k, **one(s)) for s in range(S)])
+    if form == "batched":
+        g = S if group_scenes is None else max(1, int(group_scenes))
+        groups = [(a, min(a + g, S)) for a in range(0, S, g)]
+    else:  # plain form: one scene per launch
+        groups = [(s, s + 1) for s in range(S)]
+    specs = [dict(spec, tilts=None if tl is None else tl[a:b], scene_base=a,
+                  graphs={} if form == "batched" else None) for a, b in groups]
 
     with torch.no_grad():
-        target = (gated_waveforms(theta_true, n_forward, n_backward, target_key, **spec) if form == "batched" else
-                  torch.cat([gated_waveforms(theta_true[s:s + 1], n_forward, n_backward, target_key, **one(s))
-                             for s in range(S)]))
+        target = torch.cat([gated_waveforms(theta_true[a:b], n_forward, n_backward, target_key, **specs[i])
+                            for i, (a, b) in enumerate(groups)])
     x_init = np.full((S, len(UNKNOWN)), 0.5) if x0 is None else np.array(x0, dtype=np.float64)
     if x0 is None and start == "timing":
         from .adjoint import _models
@@ -262,20 +266,26 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
     lo_t = torch.as_tensor(lo, device=dev)
     span_t = torch.as_tensor(hi - lo, device=dev)
     out = InversionResult(theta_true, theta_true.copy())
+    def scene_loss(w, tgt):
+        if loss == "cumulative":
+            return waveform_loss(w, tgt, shape_weight)
+        if loss == "log":
+            return log_waveform_loss(w, tgt)
+        return waveform_loss(w, tgt, shape_weight) + log_waveform_loss(w, tgt)  # "hybrid": both terms
+
+    per = torch.zeros(S, dtype=torch.float64, device=dev)
     for it in range(iterations):
         t0 = time.perf_counter()
         opt.zero_grad()
         theta = base.clone()
         theta[:, idx] = lo_t + x * span_t
-        w = run(theta, key)
-        if loss == "cumulative":
-            per = waveform_loss(w, target, shape_weight)
-        elif loss == "log":
-            per = log_waveform_loss(w, target)
-        else:  # "hybrid": both terms
-            per = waveform_loss(w, target, shape_weight) + log_waveform_loss(w, target)
+        for i, (a, b) in enumerate(groups):  # forward and backward of one group at a time
+            th_g = base[a:b].clone()
+            th_g[:, idx] = lo_t + x[a:b] * span_t
+            pg = scene_loss(batched_waveforms(th_g, n_forward, n_backward, key, **specs[i]), target[a:b])
+            pg.sum().backward()
+            per[a:b] = pg.detach()
         loss_v = per.sum()
-        loss_v.backward()
         opt.step()
         with torch.no_grad():
             x.clamp_(0.0, 1.0)

@@ -1,7 +1,9 @@
 """Dense float32 SoA of one photon map (photon_mapping/photon_map_data.py:6-15).
 
-``first_water_interaction`` and ``seafloor_interaction`` feed only the heat
-maps, which ``run_linear`` discards; maps built on the GPU leave them NaN.
+``first_water_interaction`` and ``seafloor_interaction`` are each record's
+photon's first water interaction (its surface entry point) and latest
+seafloor interaction, NaN before the interaction; they feed only the heat
+maps.  ``run_parallel`` fills them from the GPU trace.
 """
 from dataclasses import dataclass
 

@@ -107,6 +107,16 @@ def estimate_group_bytes(models, n_forward, n_backward, key, pilot_fraction: flo
     return rec * bytes_per_record + rec * (nb / np.maximum(nf, 1)) * bytes_per_query
 
 
+def free_device_bytes() -> int:
+    """Device memory free for new allocations, including the blocks torch's
+    caching allocator holds but no tensor uses (released first)."""
+    import torch
+
+    torch.cuda.empty_cache()
+    free, _ = torch.cuda.mem_get_info()
+    return int(free)
+
+
 def plan_groups(scene_bytes, budget: float) -> list:
     """Contiguous scene ranges whose summed bytes stay within budget (a scene
     larger than the budget forms a group of its own)."""
@@ -143,8 +153,7 @@ def run_sweep(scenes, key, budget_bytes: float = None, win_len: int = 1000, marg
     nb = np.array([x.n_backward for x in scenes], dtype=np.int64)
     t0 = _time.perf_counter()
     if budget_bytes is None:
-        free, _ = torch.cuda.mem_get_info()
-        budget_bytes = 0.5 * free
+        budget_bytes = 0.5 * free_device_bytes()
     est = estimate_group_bytes(models, nf, nb, key) * margin
     groups = plan_groups(est, budget_bytes)
     wfs, wins = [], []

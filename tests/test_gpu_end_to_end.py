@@ -166,12 +166,49 @@ def test_full_size_invariants(gpu):
     ys = recs[abi.SCATTER].view(rc[1])["y"]
     assert bool(((ys <= f32(model.y_ws)) & (ys >= f32(model.y_sf))).all())
     del recs
-    res = simulate(model, n_f, n_b, (5, 5))
+    res = simulate(model, n_f, n_b, (5, 5), keep=True, dscene=ds)
     assert res.query_counts[abi.SURFACE_REFLECTION] == n_b == res.query_counts[abi.SURFACE_TRANSMISSION_UP]
     wf = res.waveforms
     assert bool(torch.isfinite(wf).all()) and bool((wf >= 0).all())
     assert float(wf.sum()) > 0
     assert int(res.counters_fwd[abi.CTR_PATHS]) == n_f and int(res.counters_bwd[abi.CTR_PATHS]) == n_b
+    _full_size_neighbour_sets(torch, ds, res)
+
+
+def _full_size_neighbour_sets(torch, ds, res, n_q=32):
+    """On the full-size maps (up to 5e8 records), the neighbour sets and k-th
+    distances of n_q random queries per type equal brute force over every
+    record, bit for bit (the index and chunk arithmetic of the big launches)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    k = ds.model.k
+    rs = np.random.default_rng(8)
+    F = ("x", "y", "z", "dx", "dy", "dz", "energy", "time")
+    for t in range(4):
+        nq, nm = res.query_counts[t], res.map_counts[t]
+        pick = torch.as_tensor(np.sort(rs.choice(nq, size=n_q, replace=False)), device="cuda")
+        sub = runtime.Sink.alloc(n_q, True, False)
+        for f in F:
+            sub.tensors[f][:n_q].copy_(res.queries[t].tensors[f][pick])
+        nbr = torch.zeros((n_q, k), dtype=torch.int32, device="cuda")
+        rad = torch.zeros(n_q, dtype=torch.float64, device="cuda")
+        w = torch.zeros(ds.model.n_bins, dtype=torch.float64, device="cuda")
+        keep = runtime.gather(ds, res.maps[t], t, sub, n_q, w, ds.model.gate(), nbr_out=nbr, radius_out=rad)
+        torch.cuda.synchronize()
+        del keep
+        mx, my, mz = (res.maps[t].tensors[f][:nm].cpu().numpy() for f in ("x", "y", "z"))
+        q = np.stack([sub.tensors[f][:n_q].cpu().numpy() for f in ("x", "y", "z")], 1)
+
+        def brute(j):
+            return oracle.knn(mx, my, mz, q[j:j + 1], k)
+
+        with ThreadPoolExecutor(16) as ex:
+            out = list(ex.map(brute, range(n_q)))
+        idx = np.concatenate([o[0] for o in out])
+        d2 = np.concatenate([o[1] for o in out])
+        print(f"full-size neighbour sets: type {t}, {nm} records, {n_q} queries", flush=True)
+        assert np.array_equal(nbr.cpu().numpy(), idx), t
+        assert np.array_equal(rad.cpu().numpy(), np.sqrt(d2[:, k - 1])), t
 
 
 def test_run_parallel_contract(gpu):
