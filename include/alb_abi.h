@@ -105,8 +105,9 @@ typedef struct alb_layer {
   float alpha;              /* constant: float32 lidar attenuation (1/m)           */
   float omega0;             /* constant: float32 single-scattering albedo          */
   double v;                 /* constant: float32(c / n)                            */
-  int32_t a_kind;           /* absorption: 0 constant, 1 linear, 2 exponential     */
-  int32_t n_kind;           /* refractive index: same codes                        */
+  int32_t a_kind;           /* absorption: 0 constant, 1 linear, 2 exponential,
+                               3 tabulated (a_tab)                                  */
+  int32_t n_kind;           /* refractive index: same codes (3: n_tab)             */
   double a0, a1;            /* absorption at the top / bottom (profile start, end) */
   double n0, n1;            /* refractive index at the top / bottom                */
   const float* z_lut;       /* profiled: [lut_n] float32 local depths              */
@@ -116,6 +117,13 @@ typedef struct alb_layer {
   const double* phase_mass;
   int32_t phase_n;
   int32_t pad0;
+  /* a_kind / n_kind 3: any other depth profile, sampled at the fractional
+   * depths t = i / (tab_n - 1), i = 0 .. tab_n - 1, and interpolated linearly
+   * in t in FP64 (NULL / 0 when neither property uses one) */
+  const double* a_tab;
+  const double* n_tab;
+  int32_t tab_n;
+  int32_t pad1;
 } alb_layer_t;
 
 /* Scene constants, computed on the host in FP64 from SimulationConfig
@@ -386,8 +394,9 @@ int alb_gather(const alb_scene_t* scene, const alb_map_t* map,
  * 2^14 queries when nbr_out is NULL), kernel normalisations and k-NN
  * counters live in a static device array of ALB_GATHER_STATIC_SCRATCH_BYTES
  * that each GPU the library runs on holds until the process ends.
- * q_n <= ALB_GATHER_STATIC_MAX_QUERIES and k <= 192; larger calls fail with
- * ALB_ERR_ARG.  Every call on one GPU shares its array: calls that can
+ * q_n <= ALB_GATHER_STATIC_MAX_QUERIES and k <= 448, within the array
+ * (at k = 448 about 2^19 queries); larger calls fail with ALB_ERR_ARG,
+ * ALB_ERR_WORKSPACE or ALB_ERR_UNSUPPORTED.  Every call on one GPU shares its array: calls that can
  * overlap on one GPU (other streams, other host threads) race and must be
  * serialised by the caller; calls in sequence on one stream are safe. */
 #define ALB_GATHER_STATIC_MAX_QUERIES (1 << 20)

@@ -86,6 +86,7 @@ def lib():
             L.orc_layer_lookup.argtypes = [P, C.c_double, P]
             L.orc_lambert.argtypes = [C.c_double, C.c_double, P]
             L.orc_scatter_direction.argtypes = [P, P, C.c_double, C.c_double, P]
+            L.orc_layer_layout.argtypes = [P]
             for name in ("orc_ggx_d_x", "orc_g1_x", "orc_schlick_x"):
                 getattr(L, name).argtypes = [C.c_double, C.c_double]
                 getattr(L, name).restype = C.c_double
@@ -229,6 +230,14 @@ class HostScene:
             self.s.heat_radius = _ptr(self.heat["radius"])
             self.s.heat_water = _ptr(self.heat["water"])
             self.s.heat_floor = _ptr(self.heat["floor"])
+
+
+def layer_layout() -> tuple:
+    """(sizeof, offset of a_tab, offset of tab_n, offset of phase_n) of
+    alb_layer_t as the C compiler lays it out."""
+    out = np.zeros(4, dtype=np.int64)
+    lib().orc_layer_layout(_ptr(out))
+    return tuple(int(v) for v in out)
 
 
 def layer_lookup(model, depths) -> np.ndarray:

@@ -30,6 +30,7 @@
  * fused multiply-add, so every product and sum rounds as NumPy's do.
  */
 #include <math.h>
+#include <stddef.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -212,7 +213,7 @@ static int orc_layer_index(const alb_scene_t* sc, double d) {
  * LinearParameter casts t to float32 and computes
  * float32(start) + t * float32(end - start) in float32; ExponentialParameter
  * exp(log s + t (log e - log s)) in FP64 */
-static double orc_profile(int kind, double s, double e, double z, double h) {
+static double orc_profile(int kind, double s, double e, double z, double h, const double* tab, int tab_n) {
   if (kind == 0) return s;
   double t = z / h;
   if (kind == 1) {
@@ -221,6 +222,17 @@ static double orc_profile(int kind, double s, double e, double z, double h) {
     float prod = tf * span;
     float r = (float)s + prod;
     return (double)r;
+  }
+  if (kind == 3) {
+    /* any other profile: linear interpolation of its samples at
+     * t = i / (tab_n - 1), FP64 */
+    if (!(t > 0.0)) return tab[0];
+    if (t >= 1.0) return tab[tab_n - 1];
+    double u = t * (double)(tab_n - 1);
+    int i = (int)u;
+    if (i > tab_n - 2) i = tab_n - 2;
+    double f = u - (double)i;
+    return tab[i] + f * (tab[i + 1] - tab[i]);
   }
   double ls = log(s), le = log(e);
   return exp(ls + t * (le - ls));
@@ -260,15 +272,15 @@ void orc_layer_lookup(const alb_scene_t* sc, double d, double out[4]) {
     w32 = (double)L->omega0;
   } else {
     if (L->n_kind == 1) {
-      float n32 = (float)orc_profile(1, L->n0, L->n1, z, L->height);
+      float n32 = (float)orc_profile(1, L->n0, L->n1, z, L->height, L->n_tab, L->tab_n);
       float c32 = (float)2.998e8;
       v = (double)(c32 / n32);
     } else {
-      v = (double)(float)(2.998e8 / orc_profile(L->n_kind, L->n0, L->n1, z, L->height));
+      v = (double)(float)(2.998e8 / orc_profile(L->n_kind, L->n0, L->n1, z, L->height, L->n_tab, L->tab_n));
     }
     const double al = orc_layer_alpha64(L, z);
     a32 = (double)(float)al;
-    const double a = orc_profile(L->a_kind, L->a0, L->a1, z, L->height);
+    const double a = orc_profile(L->a_kind, L->a0, L->a1, z, L->height, L->a_tab, L->tab_n);
     w32 = (double)(float)((al - a) / al);
   }
   out[0] = (double)li;
@@ -1116,4 +1128,13 @@ int orc_radius_gather(const alb_scene_t* sc, int32_t type, const float* x, const
     if (isum_out) isum_out[j] = isum;
   }
   return 0;
+}
+
+/* layout of alb_layer_t as this compiler sees the header (checked against
+ * the ctypes mirror) */
+void orc_layer_layout(int64_t out[4]) {
+  out[0] = (int64_t)sizeof(alb_layer_t);
+  out[1] = (int64_t)offsetof(alb_layer_t, a_tab);
+  out[2] = (int64_t)offsetof(alb_layer_t, tab_n);
+  out[3] = (int64_t)offsetof(alb_layer_t, phase_n);
 }

@@ -65,11 +65,13 @@ class Layer(C.Structure):
         ("a_kind", C.c_int32), ("n_kind", C.c_int32), ("a0", C.c_double), ("a1", C.c_double),
         ("n0", C.c_double), ("n1", C.c_double), ("z_lut", C.c_void_p), ("alpha_lut", C.c_void_p),
         ("phase_theta", C.c_void_p), ("phase_cdf", C.c_void_p), ("phase_mass", C.c_void_p),
-        ("phase_n", C.c_int32), ("pad0", C.c_int32),
+        ("phase_n", C.c_int32), ("pad0", C.c_int32), ("a_tab", C.c_void_p), ("n_tab", C.c_void_p),
+        ("tab_n", C.c_int32), ("pad1", C.c_int32),
     ]
 
 
-def make_layer(lm, z_lut_ptr: int, alpha_lut_ptr: int, theta_ptr: int, cdf_ptr: int, mass_ptr: int) -> Layer:
+def make_layer(lm, z_lut_ptr: int, alpha_lut_ptr: int, theta_ptr: int, cdf_ptr: int, mass_ptr: int,
+               a_tab_ptr: int = 0, n_tab_ptr: int = 0) -> Layer:
     """Pack a model.LayerModel; the table pointers are device or host addresses."""
     L = Layer()
     L.top, L.bottom, L.height = lm.top, lm.bottom, lm.height
@@ -83,10 +85,13 @@ def make_layer(lm, z_lut_ptr: int, alpha_lut_ptr: int, theta_ptr: int, cdf_ptr: 
     L.z_lut, L.alpha_lut = z_lut_ptr, alpha_lut_ptr
     L.phase_theta, L.phase_cdf, L.phase_mass = theta_ptr, cdf_ptr, mass_ptr
     L.phase_n = len(lm.phase_theta)
+    L.a_tab = a_tab_ptr if len(lm.a_tab) else 0
+    L.n_tab = n_tab_ptr if len(lm.n_tab) else 0
+    L.tab_n = max(len(lm.a_tab), len(lm.n_tab))
     return L
 
 
-LAYER_TABLES = ("z_lut", "alpha_lut", "theta", "cdf", "mass")
+LAYER_TABLES = ("z_lut", "alpha_lut", "theta", "cdf", "mass", "a_tab", "n_tab")
 
 
 def make_layers(model, ptrs) -> C.Array:
@@ -95,7 +100,7 @@ def make_layers(model, ptrs) -> C.Array:
     arr = (Layer * len(model.layers))()
     for i, lm in enumerate(model.layers):
         p = ptrs[i]
-        arr[i] = make_layer(lm, p["z_lut"], p["alpha_lut"], p["theta"], p["cdf"], p["mass"])
+        arr[i] = make_layer(lm, p["z_lut"], p["alpha_lut"], p["theta"], p["cdf"], p["mass"], p["a_tab"], p["n_tab"])
     return arr
 
 

@@ -188,13 +188,26 @@ __device__ __forceinline__ int layer_of(const alb_layer_t* __restrict__ L, int n
 
 // a layer property at local depth z (TurbidityLayerModel._eval with NumPy's
 // dtype rules): a linear profile evaluates in float32 with t = float32(z / h),
-// an exponential one in FP64 (parameter_profile.py:47-99)
-__device__ __forceinline__ double layer_profile(int kind, double s, double e, double z, double h) {
+// an exponential one in FP64 (parameter_profile.py:47-99); any other profile
+// (kind 3) interpolates its samples tab[0, tab_n) at t = i / (tab_n - 1)
+// linearly in t, in FP64
+__device__ __forceinline__ double layer_profile(int kind, double s, double e, double z, double h,
+                                                const double* __restrict__ tab, int tab_n) {
   if (kind == 0) return s;
   if (kind == 1) {
     const float t = (float)(z / h);
     const float r = (float)s + t * (float)(e - s);
     return (double)r;
+  }
+  if (kind == 3) {
+    const double t = z / h;
+    if (!(t > 0.0)) return tab[0];
+    if (t >= 1.0) return tab[tab_n - 1];
+    const double u = t * (double)(tab_n - 1);
+    int i = (int)u;
+    if (i > tab_n - 2) i = tab_n - 2;
+    const double f = u - (double)i;
+    return tab[i] + f * (tab[i + 1] - tab[i]);
   }
   const double ls = log(s), le = log(e);
   return exp(ls + (z / h) * (le - ls));
@@ -205,10 +218,10 @@ __device__ __forceinline__ double layer_profile(int kind, double s, double e, do
 __device__ __forceinline__ double layer_velocity(const alb_layer_t& L, double z) {
   if (!L.profiled) return L.v;
   if (L.n_kind == 1) {
-    const float n32 = (float)layer_profile(1, L.n0, L.n1, z, L.height);
+    const float n32 = (float)layer_profile(1, L.n0, L.n1, z, L.height, L.n_tab, L.tab_n);
     return (double)(2.998e8f / n32);
   }
-  return (double)(float)(2.998e8 / layer_profile(L.n_kind, L.n0, L.n1, z, L.height));
+  return (double)(float)(2.998e8 / layer_profile(L.n_kind, L.n0, L.n1, z, L.height, L.n_tab, L.tab_n));
 }
 
 // lidar attenuation at local depth z: FP64 table value (np.interp at float32(z))
@@ -225,7 +238,7 @@ __device__ __forceinline__ float layer_alpha32(const alb_layer_t& L, double z) {
 __device__ __forceinline__ float layer_omega0(const alb_layer_t& L, double z) {
   if (!L.profiled) return L.omega0;
   const double al = layer_alpha64(L, z);
-  const double a = layer_profile(L.a_kind, L.a0, L.a1, z, L.height);
+  const double a = layer_profile(L.a_kind, L.a0, L.a1, z, L.height, L.a_tab, L.tab_n);
   return (float)((al - a) / al);
 }
 

@@ -49,6 +49,10 @@ constexpr int kWaves = 4;         // wavefronts per k-NN block
 constexpr int kStack = 256;       // per-wave node stack (LDS)
 constexpr int kSeed = 192;        // records of the descent seed (<= kBuf)
 constexpr int kBuf = 256;         // per-wave candidate buffer (LDS), power of two
+// k above kMaxK runs a variant with a buffer of kBufLarge entries (and a seed
+// of kBufLarge - 64); a buffer cut must free 64 slots, so k <= buffer - 64
+constexpr int kBufLarge = 512;
+constexpr int kMaxKLarge = kBufLarge - 64;
 constexpr int kBucket = 64;       // nodes with <= kBucket records are scanned
 constexpr int kPath = 128;        // descent levels kept for the next query (LDS)
 constexpr int kMaxWindow = 7680;  // LDS-privatised waveform bins (60 KB of FP64: with the row tile, 64 KB per block)
@@ -241,8 +245,9 @@ __device__ uint64_t wave_select(const double* d, const int32_t* idx, int n, int 
 // side holding rank kk stays active.  It stops when one key (or a run of equal
 // keys) is left.  *lt_out / *eq_out receive the counts of entries below and
 // equal to the result.
+template <int BUF>
 __device__ uint64_t wave_kth_key(const double* d, int n, int kk, uint32_t lane, int* lt_out, int* eq_out) {
-  constexpr int U = kBuf / 64;
+  constexpr int U = BUF / 64;
   uint64_t key[U];
   bool act[U];
   uint32_t olo = 0u, ohi = 0u, alo = ~0u, ahi = ~0u;
@@ -300,16 +305,17 @@ __device__ uint64_t wave_kth_key(const double* d, int n, int kk, uint32_t lane, 
 
 // Keep the k smallest entries by (d, idx) at the front of the buffer (any
 // order) and return the k-th smallest d.
+template <int BUF>
 __device__ double wave_keep_k(double* d, int32_t* idx, int n, int k, uint32_t* hist, uint32_t lane) {
   int lt = 0, eq = 0;
-  const double th = __longlong_as_double((long long)wave_kth_key(d, n, k, lane, &lt, &eq));
+  const double th = __longlong_as_double((long long)wave_kth_key<BUF>(d, n, k, lane, &lt, &eq));
   int32_t icut = INT_MAX;  // entries equal to th are kept up to this index
   if (lt + eq > k) icut = (int32_t)wave_select(d, idx, n, k - lt, true, th, hist, lane);
-  double vd[kBuf / 64];
-  int32_t vi[kBuf / 64];
-  bool keep[kBuf / 64];
+  double vd[BUF / 64];
+  int32_t vi[BUF / 64];
+  bool keep[BUF / 64];
 #pragma unroll
-  for (int u = 0; u < kBuf / 64; ++u) {
+  for (int u = 0; u < BUF / 64; ++u) {
     const int t = (int)lane + 64 * u;
     keep[u] = false;
     if (t < n) {
@@ -321,7 +327,7 @@ __device__ double wave_keep_k(double* d, int32_t* idx, int n, int k, uint32_t* h
   wave_sync();
   int base = 0;
 #pragma unroll
-  for (int u = 0; u < kBuf / 64; ++u) {
+  for (int u = 0; u < BUF / 64; ++u) {
     const uint64_t m = ballot(keep[u]);
     if (keep[u]) {
       const int slot = base + (int)mask_rank(m);
@@ -335,16 +341,17 @@ __device__ double wave_keep_k(double* d, int32_t* idx, int n, int k, uint32_t* h
 }
 
 // Cheap buffer cut: an upper bound of the k-th smallest d, not the k-th
-// itself.  The n <= kBuf entries (all <= hi) are counted into 64 buckets of
+// itself.  The n <= BUF entries (all <= hi) are counted into 64 buckets of
 // [0, hi] (LDS histogram); the bucket holding the k-th smallest bounds it.
 // Entries of the buckets up to that one are kept (the buckets are monotone
 // in d, so the kept set is exactly {d <= theta'} with theta' the largest kept
 // d) and theta' is returned.  Returns a negative value, changing nothing,
 // when the cut would not free 64 slots (the caller then cuts exactly).
+template <int BUF>
 __device__ double wave_cut_hist(double* d, int32_t* idx, int n, int k, double hi, uint32_t* hist, uint32_t lane,
                                 int* n_out) {
   if (!(hi > 0.0)) return -1.0;
-  constexpr int U = kBuf / 64;
+  constexpr int U = BUF / 64;
   hist[lane] = 0u;
   wave_sync();
   const double scale = 64.0 / hi;
@@ -365,7 +372,7 @@ __device__ double wave_cut_hist(double* d, int32_t* idx, int n, int k, double hi
   const uint64_t reach = ballot(cum >= (uint32_t)k);
   const int sel = __ffsll((long long)reach) - 1;  // n >= k, so some bucket reaches k
   const int kept = (int)__builtin_amdgcn_readlane((int)cum, sel);
-  if (kept > kBuf - 64) return -1.0;
+  if (kept > BUF - 64) return -1.0;
   double mx = 0.0;
   double vd[U];
   int32_t vi[U];
@@ -427,12 +434,15 @@ __global__ __launch_bounds__(256) void knn_work_clear_kernel(unsigned long long*
   for (int t = threadIdx.x; t < ALB_KNN_WORK_WORDS; t += blockDim.x) w[t] = 0ull;
 }
 
-template <bool OFF32>
+template <bool OFF32, int BUF>
 __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
+  constexpr int kBufK = BUF;          // candidate buffer entries per wave
+  constexpr int kSeedK = BUF - 64;    // seed records (at least k)
+  constexpr int kMaxKK = BUF - 64;    // largest k of this variant
   __shared__ int32_t sn_s[kWaves][kStack];
   __shared__ double sd_s[kWaves][kStack];
-  __shared__ double bd_s[kWaves][kBuf];
-  __shared__ int32_t bi_s[kWaves][kBuf];
+  __shared__ double bd_s[kWaves][kBufK];
+  __shared__ int32_t bi_s[kWaves][kBufK];
   __shared__ int32_t rlo_s[kWaves][128];
   __shared__ int32_t rhi_s[kWaves][128];
   __shared__ uint32_t hist_s[kWaves][256];
@@ -505,7 +515,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
         continue;
       }
     }
-    if (s1 - s0 + 1 > kSeed) {
+    if (s1 - s0 + 1 > kSeedK) {
       // Descent toward the nearer child box.  Queries are Morton-sorted, so
       // a wavefront's consecutive queries mostly take the same path: the
       // previous query's path (path[0..plen), root first) is tested for this
@@ -562,22 +572,22 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
         const bool far_leaf = go_left ? (nd.leaf_mask & 2) : (nd.leaf_mask & 1);
         const int32_t far_id = go_left ? nd.split + 1 : nd.split;
         bool stop = true;
-        if (cnt_near >= k && !near_leaf && cnt_near > kSeed) {
+        if (cnt_near >= k && !near_leaf && cnt_near > kSeedK) {
           stop = false;
         } else if (cnt_near >= k && !near_leaf) {  // the near child fits the seed buffer
           s0 = go_left ? nd.first : nd.split + 1;
           s1 = go_left ? nd.split : nd.last;
-        } else if (nd.last - nd.first + 1 <= kSeed) {
+        } else if (nd.last - nd.first + 1 <= kSeedK) {
           s0 = nd.first;
           s1 = nd.last;
         } else if (go_left) {  // near child (< k records) plus the first records of the far one
           s0 = nd.first;
-          s1 = nd.first + kSeed - 1;
+          s1 = nd.first + kSeedK - 1;
         } else {
-          s0 = nd.last - kSeed + 1;
+          s0 = nd.last - kSeedK + 1;
           s1 = nd.last;
         }
-        const bool far_in_seed = stop && (nd.last - nd.first + 1 <= kSeed);
+        const bool far_in_seed = stop && (nd.last - nd.first + 1 <= kSeedK);
         if (!far_in_seed) {
           if (lane == 0) {
             sn[sp] = far_leaf ? ~far_id : far_id;
@@ -597,7 +607,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     // from this query, so its k-th distance is no larger.
     double theta_prev = __builtin_inf();
     if (nn_root == root) {
-      constexpr int KU = (kMaxK + 63) / 64;
+      constexpr int KU = (kMaxKK + 63) / 64;
       float xs[KU], ys[KU], zs[KU];
 #pragma unroll
       for (int u = 0; u < KU; ++u) {
@@ -618,8 +628,8 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     }
     const int W = s1 - s0 + 1;
     {
-      // W <= kSeed: the loads of all slots issue before one wait
-      constexpr int SU = (kSeed + 63) / 64;
+      // W <= kSeedK: the loads of all slots issue before one wait
+      constexpr int SU = (kSeedK + 63) / 64;
       float xs[SU], ys[SU], zs[SU];
 #pragma unroll
       for (int u = 0; u < SU; ++u) {
@@ -651,7 +661,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     if (theta_prev < smax) {
       // the previous query's bound is tighter: keep the seed entries within
       // it (possibly fewer than k; the traversal adds the rest)
-      constexpr int SU = (kSeed + 63) / 64;
+      constexpr int SU = (kSeedK + 63) / 64;
       double vd[SU];
       int32_t vi[SU];
       bool keep[SU];
@@ -684,16 +694,16 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
       theta = theta_prev;
       if (buf_n >= k) {
         int kept = 0;
-        const double th2 = wave_cut_hist(bd, bi, buf_n, k, theta, hist, lane, &kept);
+        const double th2 = wave_cut_hist<BUF>(bd, bi, buf_n, k, theta, hist, lane, &kept);
         if (th2 >= 0.0) {
           theta = th2;
           buf_n = kept;
         }
       }
     } else {
-      theta = wave_cut_hist(bd, bi, W, k, smax, hist, lane, &buf_n);
+      theta = wave_cut_hist<BUF>(bd, bi, W, k, smax, hist, lane, &buf_n);
       if (theta < 0.0) {
-        theta = wave_keep_k(bd, bi, W, k, hist, lane);
+        theta = wave_keep_k<BUF>(bd, bi, W, k, hist, lane);
         buf_n = k;
         theta_exact = true;
       }
@@ -812,16 +822,16 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
           const uint64_t mt = ballot(tk[u]);
           if (mt == 0) continue;
           bool take = tk[u];
-          if (buf_n + __popcll(mt) > kBuf) {
+          if (buf_n + __popcll(mt) > kBufK) {
             c_sorts++;
             int kept = 0;
-            const double th2 = wave_cut_hist(bd, bi, buf_n, k, theta, hist, lane, &kept);
+            const double th2 = wave_cut_hist<BUF>(bd, bi, buf_n, k, theta, hist, lane, &kept);
             if (th2 >= 0.0) {
               theta = th2;
               buf_n = kept;
               theta_exact = false;
             } else {
-              theta = wave_keep_k(bd, bi, buf_n, k, hist, lane);
+              theta = wave_keep_k<BUF>(bd, bi, buf_n, k, hist, lane);
               buf_n = k;
               theta_exact = true;
             }
@@ -843,7 +853,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     // ---- 3. the k nearest: cut the buffer back to k; in (distance, index)
     // order when the caller asked for the neighbour sets
     // exact k-th distance and set (histogram cuts only bound them)
-    const double r2 = (buf_n > k || !theta_exact) ? wave_keep_k(bd, bi, buf_n, k, hist, lane) : theta;
+    const double r2 = (buf_n > k || !theta_exact) ? wave_keep_k<BUF>(bd, bi, buf_n, k, hist, lane) : theta;
     if (a.direct) wave_sort(bd, bi, k, lane);
     for (int t = (int)lane; t < k; t += 64) a.nbr[qi * k + t] = bi[t];
     if (lane == 0) a.radius[qi] = sqrt(r2);
@@ -927,7 +937,7 @@ __device__ __forceinline__ void stage_row(RowTile& R, int r, const alb_scene_t& 
   R.t[r] = q.time[qg];
 }
 
-// it / d for it < kRowTile * kMaxK < 2^24 (the float quotient is within one
+// it / d for it < kRowTile * kMaxKLarge < 2^24 (the float quotient is within one
 // of it and corrected)
 __device__ __forceinline__ int small_div(int it, int d, float inv_d) {
   int q = (int)(((float)it + 0.5f) * inv_d);
@@ -1229,13 +1239,28 @@ static int device_cus(int* cus) {
   return ALB_OK;
 }
 
+// the k-NN variant for the map size (32-bit record offsets below 2^30
+// records) and k (the larger candidate buffer above 192)
+static void launch_knn(const alb_map_t& map, int k, int64_t blocks, const alb::KnnArgs& ka, hipStream_t s) {
+  using namespace alb;
+  const dim3 g((unsigned)blocks), b(64 * kWaves);
+  const bool off32 = map.n < (1ll << 30);
+  if (k <= kMaxK) {
+    if (off32) hipLaunchKernelGGL((knn_kernel<true, kBuf>), g, b, 0, s, ka);
+    else hipLaunchKernelGGL((knn_kernel<false, kBuf>), g, b, 0, s, ka);
+  } else {
+    if (off32) hipLaunchKernelGGL((knn_kernel<true, kBufLarge>), g, b, 0, s, ka);
+    else hipLaunchKernelGGL((knn_kernel<false, kBufLarge>), g, b, 0, s, ka);
+  }
+}
+
 static int check_map(const alb_map_t* map, int k, const char* who) {
   using namespace alb;
   if (!map) return set_error(ALB_ERR_ARG, "%s: null map", who);
   if (map->n <= 0) return set_error(ALB_ERR_ARG, "%s: empty photon map", who);
   if (k <= 0 || k > map->n)
     return set_error(ALB_ERR_ARG, "%s: k = %d exceeds the map size %lld", who, k, (long long)map->n);
-  if (k > kMaxK) return set_error(ALB_ERR_UNSUPPORTED, "%s: k = %d above %d", who, k, kMaxK);
+  if (k > kMaxKLarge) return set_error(ALB_ERR_UNSUPPORTED, "%s: k = %d above %d", who, k, kMaxKLarge);
   if (!map->keys || !map->meta || !map->x || (map->n > 1 && !map->nodes))
     return set_error(ALB_ERR_ARG, "%s: map not built", who);
   return ALB_OK;
@@ -1286,10 +1311,7 @@ extern "C" int alb_knn(const alb_map_t* map, int32_t photon_type, int32_t k, con
   if (kb > (int64_t)cus * 16) kb = (int64_t)cus * 16;
   kb = (kb + kXcds - 1) / kXcds * kXcds;
   hipLaunchKernelGGL(knn_work_clear_kernel, dim3(1), dim3(256), 0, (hipStream_t)stream, work);
-  if (map->n < (1ll << 30))
-    hipLaunchKernelGGL(knn_kernel<true>, dim3((unsigned)kb), dim3(64 * kWaves), 0, (hipStream_t)stream, ka);
-  else
-    hipLaunchKernelGGL(knn_kernel<false>, dim3((unsigned)kb), dim3(64 * kWaves), 0, (hipStream_t)stream, ka);
+  launch_knn(*map, k, kb, ka, (hipStream_t)stream);
   // rows written: [0, cn) in slot order, or the query offsets of the slots when rows follow queries
   if (rows_by_query) {
     // rows are query offsets: normalise every row of this chunk's queries (all rows when one chunk)
@@ -1507,7 +1529,8 @@ extern "C" int alb_knn_batched(const alb_map_t* map, const alb_scene_index_t* in
   if (!map || !index || !q || !order || !nbr || !radius || !knorm || !work)
     return set_error(ALB_ERR_ARG, "alb_knn_batched: null argument");
   if (map->n <= 0) return set_error(ALB_ERR_ARG, "alb_knn_batched: empty photon map");
-  if (k <= 0 || k > kMaxK) return set_error(ALB_ERR_UNSUPPORTED, "alb_knn_batched: k = %d outside [1, %d]", k, kMaxK);
+  if (k <= 0 || k > kMaxKLarge)
+    return set_error(ALB_ERR_UNSUPPORTED, "alb_knn_batched: k = %d outside [1, %d]", k, kMaxKLarge);
   if (!map->keys || !map->meta || !map->x || (map->n > 1 && !map->nodes))
     return set_error(ALB_ERR_ARG, "alb_knn_batched: map not built");
   if (!index->first || !index->last || !index->root || !q->scene)
@@ -1522,10 +1545,7 @@ extern "C" int alb_knn_batched(const alb_map_t* map, const alb_scene_index_t* in
   if (kb > (int64_t)cus * 16) kb = (int64_t)cus * 16;
   kb = (kb + kXcds - 1) / kXcds * kXcds;
   hipLaunchKernelGGL(knn_work_clear_kernel, dim3(1), dim3(256), 0, (hipStream_t)stream, work);
-  if (map->n < (1ll << 30))
-    hipLaunchKernelGGL(knn_kernel<true>, dim3((unsigned)kb), dim3(64 * kWaves), 0, (hipStream_t)stream, ka);
-  else
-    hipLaunchKernelGGL(knn_kernel<false>, dim3((unsigned)kb), dim3(64 * kWaves), 0, (hipStream_t)stream, ka);
+  launch_knn(*map, k, kb, ka, (hipStream_t)stream);
   hipLaunchKernelGGL(knorm_kernel, dim3((unsigned)((cn + 255) / 256)), dim3(256), 0, (hipStream_t)stream, radius,
                      knorm, cn, photon_type == ALB_SCATTER ? 1 : 0);
   return check_launch("alb_knn_batched");
@@ -1542,7 +1562,8 @@ extern "C" int alb_contrib_batched(const alb_scene_t* scenes, int32_t n_scenes, 
   if (n_scenes < 1) return set_error(ALB_ERR_ARG, "alb_contrib_batched: bad scene count");
   if (!q->scene) return set_error(ALB_ERR_ARG, "alb_contrib_batched: queries carry no scene array");
   if (photon_type < 0 || photon_type >= ALB_N_TYPES) return set_error(ALB_ERR_ARG, "alb_contrib_batched: bad photon type");
-  if (k <= 0 || k > kMaxK) return set_error(ALB_ERR_UNSUPPORTED, "alb_contrib_batched: k = %d outside [1, %d]", k, kMaxK);
+  if (k <= 0 || k > kMaxKLarge)
+    return set_error(ALB_ERR_UNSUPPORTED, "alb_contrib_batched: k = %d outside [1, %d]", k, kMaxKLarge);
   if (win_len < 0 || win_len > kMaxWindowLarge)
     return set_error(ALB_ERR_ARG, "alb_contrib_batched: window length %d outside [0, %d]", win_len, kMaxWindowLarge);
   if (n_replicas < 0 || (n_replicas > 0 && (!replicas || nb_max <= 0)))

@@ -12,8 +12,9 @@ that is computed once per scene rather than per photon:
 * +z rotation to a vector  math/rotation_matrix.py:39-102
 
 The traced path (HIP) and the oracle both consume the resulting ``alb_scene_t``.
-Layer stacks and linear / exponential depth profiles are supported
-(LayerModel); other profile types raise ``NotImplementedError``.
+Layer stacks and depth profiles are supported (LayerModel): linear and
+exponential profiles in their own arithmetic, any other profile with at(t)
+through a sampled table (profile_table).
 """
 from __future__ import annotations
 
@@ -150,12 +151,16 @@ def base_reflectance(n: float) -> float:
     return ((1 - n) / (1 + n)) ** 2
 
 
-PROFILE_CONST, PROFILE_LINEAR, PROFILE_EXPONENTIAL = 0, 1, 2
+PROFILE_CONST, PROFILE_LINEAR, PROFILE_EXPONENTIAL, PROFILE_TABLE = 0, 1, 2, 3
+PROFILE_TABLE_N = 65537  # samples of a tabulated profile over the layer's fractional depth [0, 1]
 
 
 def profile_spec(value) -> tuple:
-    """(kind, start, end) of a layer property: a float or a Linear/Exponential
-    parameter (utils/parameter_profile.py:47-99)."""
+    """(kind, start, end) of a layer property: a float, a Linear/Exponential
+    parameter (utils/parameter_profile.py:47-99, evaluated on the device in
+    their own arithmetic), or any other ScalarParameter (an object with
+    at(t), utils/parameter_profile.py:9-19), which the device interpolates
+    from profile_table."""
     from .utils.parameter_profile import ExponentialParameter, LinearParameter
 
     if isinstance(value, float):
@@ -164,7 +169,26 @@ def profile_spec(value) -> tuple:
         return PROFILE_LINEAR, float(value.start), float(value.end)
     if isinstance(value, ExponentialParameter):
         return PROFILE_EXPONENTIAL, float(value.start), float(value.end)
-    raise NotImplementedError(f"depth profile {type(value).__name__} is not supported on the traced path")
+    if callable(getattr(value, "at", None)):
+        tab = profile_table(value)
+        return PROFILE_TABLE, float(tab[0]), float(tab[-1])
+    raise TypeError(f"layer property {value!r} is neither a number nor a profile with at(t)")
+
+
+def profile_table(value) -> np.ndarray:
+    """FP64 samples of value.at(t) at t = i / (PROFILE_TABLE_N - 1).  Linear
+    interpolation of these differs from at(t) by at most max|at''| / 8 /
+    (PROFILE_TABLE_N - 1)^2 (2.9e-11 max|at''|)."""
+    t = np.linspace(0.0, 1.0, PROFILE_TABLE_N)
+    try:
+        v = np.asarray(value.at(t), dtype=np.float64)
+    except (TypeError, ValueError):
+        v = None
+    if v is None or v.shape != t.shape:
+        v = np.array([float(value.at(float(x))) for x in t])
+    if not np.all(np.isfinite(v)):
+        raise ValueError(f"profile {value!r} is not finite on [0, 1]")
+    return v
 
 
 def eval_profile(value, z_local, height: float):
@@ -202,6 +226,8 @@ class LayerModel:
     v: float            # constant layers: float32(c / n)
     a_spec: tuple       # (kind, start, end) of the absorption coefficient
     n_spec: tuple       # (kind, start, end) of the refractive index
+    a_tab: np.ndarray   # tabulated absorption profile (kind 3), else empty
+    n_tab: np.ndarray   # tabulated refractive-index profile (kind 3), else empty
     z_lut: np.ndarray   # profiled layers: float32 z grid (num_sublayers)
     alpha_lut: np.ndarray  # profiled layers: attenuation at z_lut (as computed, float32 or float64)
     phase_theta: np.ndarray
@@ -291,7 +317,7 @@ def _layer_model(layer, top32: float, bottom: float, cfg, hg_g):
     names = ("scattering_coefficient", "absorption_coefficient", "salinity", "refractive_index")
     profiled = not all(isinstance(getattr(layer, nm), float) for nm in names)
     for nm in names:
-        profile_spec(getattr(layer, nm))  # rejects unsupported profile types
+        profile_spec(getattr(layer, nm))  # rejects values that are neither numbers nor profiles
     H = cfg.scene.flying_height
     lam, div = cfg.laser.wavelength, cfg.laser.divergence_angle
     a_spec, n_spec = profile_spec(layer.absorption_coefficient), profile_spec(layer.refractive_index)
@@ -313,6 +339,9 @@ def _layer_model(layer, top32: float, bottom: float, cfg, hg_g):
         v = float(np.float32(LIGHT_SPEED_AIR / layer.refractive_index))
     return LayerModel(top=float(top32), bottom=float(bottom), height=float(layer.height), profiled=profiled,
                       alpha64=float(alpha64), omega064=float(omega064), v=v, a_spec=a_spec, n_spec=n_spec,
+                      a_tab=profile_table(layer.absorption_coefficient) if a_spec[0] == PROFILE_TABLE
+                      else np.zeros(0),
+                      n_tab=profile_table(layer.refractive_index) if n_spec[0] == PROFILE_TABLE else np.zeros(0),
                       z_lut=z_lut, alpha_lut=lut, phase_theta=theta, phase_pf=pf, phase_cdf=cdf, backscatter=bpf)
 
 
@@ -328,6 +357,9 @@ def layer_arrays(model: "SceneModel") -> list:
             "theta": np.ascontiguousarray(lm.phase_theta, dtype=np.float64),
             "cdf": np.ascontiguousarray(lm.phase_cdf, dtype=np.float64),
             "mass": np.ascontiguousarray(phase_bin_mass(lm.phase_theta, lm.phase_cdf), dtype=np.float64),
+            # at least one entry each, so that every table has an address
+            "a_tab": np.ascontiguousarray(lm.a_tab if len(lm.a_tab) else np.zeros(1), dtype=np.float64),
+            "n_tab": np.ascontiguousarray(lm.n_tab if len(lm.n_tab) else np.zeros(1), dtype=np.float64),
         })
     return out
 

@@ -345,7 +345,13 @@ def build_map(records: Sink, n: int, dims: int, stream=None) -> DeviceMap:
     return m
 
 
-CHUNK = 1 << 24  # queries per k-NN / contribution round (2^22: SCATTER k-NN 3 % slower; rows take 4 k B per query)
+CHUNK = 1 << 24  # queries per k-NN / contribution round at k = 100 (2^22: SCATTER k-NN 3 % slower; rows take 4 k B per query)
+
+
+def gather_chunk(q_n: int, k: int) -> int:
+    """Queries per k-NN / contribution round: CHUNK at k = 100, fewer for
+    larger k so that the neighbour rows stay near 6.7 GB."""
+    return min(q_n, CHUNK if k <= 100 else max(1, CHUNK * 100 // k))
 # LDS-privatised bins of the contribution kernels.  Most (query, neighbour)
 # pairs fall in the first ~4000 bins after the surface return (backward paths
 # scatter on long after it), so a window of this many bins from the gate's
@@ -395,7 +401,7 @@ def gather(dscene: DeviceScene, dmap: DeviceMap, ptype: int, queries: Sink, q_n:
     order = torch.empty(q_n, dtype=torch.int32, device=dev)
     _check(L.alb_gather_order(C.byref(dmap.struct), C.byref(queries.struct), q_begin, q_n, _p(order), _p(ws),
                               b.value, C.c_void_p(s)), "alb_gather_order")
-    chunk = q_n if direct else min(q_n, CHUNK)
+    chunk = q_n if direct else gather_chunk(q_n, k)
     knorm = torch.empty(q_n if direct else chunk, dtype=torch.float64, device=dev)
     n_rep = REPLICAS
     rep = torch.empty(n_rep * int(waveform.numel()), dtype=torch.float64, device=dev) if n_rep else None
@@ -724,7 +730,7 @@ def gather_batched(table: DeviceSceneTable, dmap: DeviceMap, ptype: int, queries
     _check(L.alb_gather_order_batched(C.byref(dmap.struct), C.byref(dmap.index), C.byref(queries.struct), q_n,
                                       _p(order), _p(ws), b.value, C.c_void_p(s)), "alb_gather_order_batched")
     direct = nbr_out is not None
-    chunk = q_n if direct else min(q_n, CHUNK)
+    chunk = q_n if direct else gather_chunk(q_n, k)
     nbr = nbr_out if direct else torch.empty((chunk, k), dtype=torch.int32, device=dev)
     rad = radius_out if direct else torch.empty(chunk, dtype=torch.float64, device=dev)
     knorm = torch.empty(chunk, dtype=torch.float64, device=dev)

@@ -34,7 +34,12 @@ def test_layout_and_version():
     assert C.sizeof(abi.Rng) == 32
     assert C.sizeof(abi.Records) == 144 and C.sizeof(abi.Queries) == 144 and C.sizeof(abi.Map) == 160
     assert C.sizeof(abi.SceneIndex) == 32
-    assert C.sizeof(abi.Layer) == 136 and abi.Layer.z_lut.offset == 88 and abi.Layer.phase_n.offset == 128
+    assert C.sizeof(abi.Layer) == 160 and abi.Layer.z_lut.offset == 88 and abi.Layer.phase_n.offset == 128
+    assert abi.Layer.a_tab.offset == 136 and abi.Layer.tab_n.offset == 152
+    import oracle
+
+    assert oracle.layer_layout() == (C.sizeof(abi.Layer), abi.Layer.a_tab.offset, abi.Layer.tab_n.offset,
+                                     abi.Layer.phase_n.offset)
 
 
 def test_last_error_is_set_by_argument_checks():

@@ -155,3 +155,42 @@ def test_gather_in_many_rounds_equals_one_round(gpu, monkeypatch):
     for s in range(2):
         x, y = r_many.waveform(s).cpu().numpy(), r_one.waveform(s).cpu().numpy()
         assert np.allclose(x, y, rtol=1e-10, atol=1e-14 * float(np.abs(y).max()))
+
+
+@pytest.mark.parametrize("k", [193, 256, 448])
+def test_large_k_sets_bit_exact_and_waveform_matches(gpu, k):
+    """k above 192 runs the k-NN variant with the 512-entry candidate buffer:
+    neighbour sets and k-th distances equal brute force bit for bit, and the
+    waveforms the oracle's re-sum."""
+    torch = gpu
+    model = build_scene(single_shot(k=k))
+    ds, recs, rc, maps, qs, qc = _setup(model, 20000, 600, (13, 14))
+    for t in range(4):
+        if qc[t] == 0 or rc[t] < k:
+            continue
+        nq = min(qc[t], 300)
+        wf = torch.zeros(model.n_bins, dtype=torch.float64, device="cuda")
+        nbr = torch.zeros((nq, k), dtype=torch.int32, device="cuda")
+        rad = torch.zeros(nq, dtype=torch.float64, device="cuda")
+        runtime.gather(ds, maps[t], t, qs[t], nq, wf, model.gate(), nbr_out=nbr, radius_out=rad)
+        torch.cuda.synchronize()
+        mp = {f: maps[t].tensors[f][: rc[t]].cpu().numpy() for f in F}
+        qv = {f: qs[t].tensors[f][:nq].cpu().numpy() for f in F}
+        idx, d2 = oracle.knn(mp["x"], mp["y"], mp["z"], np.stack([qv["x"], qv["y"], qv["z"]], 1), k)
+        assert np.array_equal(nbr.cpu().numpy(), idx), (k, t)
+        assert np.array_equal(rad.cpu().numpy(), np.sqrt(d2[:, -1])), (k, t)
+        ref = oracle.gather(model, t, mp, qv)
+        assert np.max(np.abs(wf.cpu().numpy() - ref)) <= 1e-9 * np.max(np.abs(ref)), (k, t)
+
+
+def test_large_k_batched_equals_one_scene(gpu):
+    from pyalbsim_amd.execution.batched import simulate_batched
+    from pyalbsim_amd.execution.linear import simulate
+
+    k = 300
+    models = [build_scene(single_shot(k=k)), build_scene(single_shot(k=k, depth=8.0, b=0.5))]
+    r = simulate_batched(models, [20000, 15000], [2000, 1500], (3, 9), strict=False)
+    for s, m in enumerate(models):
+        one = simulate(m, [20000, 15000][s], [2000, 1500][s], (3, 9), scene_id=s, strict=False)
+        a, b = r.waveform(s).cpu().numpy(), one.waveforms.cpu().numpy()
+        assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(b)), s

@@ -7,7 +7,8 @@ NumPy in test_oracle_layers.py).  Checks:
 
 * trace parity photon by photon (the tolerance of test_gpu_trace_parity:
   positions 1e-5 relative, 99.5 % of photons), forward and backward, with a
-  linear or an exponential refractive-index profile;
+  linear or an exponential refractive-index profile, and with absorption,
+  scattering and refractive-index profiles of another shape (tabulated);
 * a constant layer split into identical layers traces like the closed-form
   single layer (the two GPU paths agree);
 * gather parity: neighbour sets bit for bit and waveforms to 1e-9 of the
@@ -35,10 +36,16 @@ F = ("x", "y", "z", "dx", "dy", "dz", "energy", "time")
 
 
 def _layered(kind):
+    if kind == "smooth":  # profiles of a, b and n that are neither linear nor exponential (tabulated)
+        from tests.scenes import SmoothStepParameter
+
+        return build_scene(layered_scene(n_profile=SmoothStepParameter(1.34, 1.352),
+                                         a_profile=SmoothStepParameter(0.04, 0.25),
+                                         b_profile=SmoothStepParameter(1.2, 0.3)))
     return build_scene(layered_scene(n_profile=ExponentialParameter(1.34, 1.35) if kind == "exp" else None))
 
 
-@pytest.mark.parametrize("kind", ["linear", "exp"])
+@pytest.mark.parametrize("kind", ["linear", "exp", "smooth"])
 @pytest.mark.parametrize("pass_", [abi.PASS_FORWARD, abi.PASS_BACKWARD])
 def test_layered_trace_matches_oracle(gpu, kind, pass_):
     counts = _compare(_layered(kind), (91, 92), pass_, 400)

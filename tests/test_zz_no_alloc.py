@@ -80,7 +80,7 @@ def test_static_gather_equals_workspace_gather_on_two_calls(gpu, direct):
 
 def test_static_gather_rejects_calls_beyond_its_scratch(gpu):
     """Checked on the host before any launch: more queries than the maximum,
-    or k beyond the supported 192."""
+    or k beyond the supported 448."""
     torch = gpu
     L = runtime.load_library()
     model, ds, maps, qs, qc = _setup(20000, 100, (5, 6))
@@ -91,7 +91,7 @@ def test_static_gather_rejects_calls_beyond_its_scratch(gpu):
                              runtime._p(wf), model.n_bins, 0, 0, None, None, None, C.c_void_p(runtime._stream_ptr()))
     assert rc == abi.ALB_ERR_ARG and b"maximum" in L.alb_last_error()
     s2 = abi.Scene.from_buffer_copy(ds.s)
-    s2.k = 193
+    s2.k = 449
     rc = L.alb_gather_static(C.byref(s2), C.byref(maps[t].struct), t, C.byref(qs[t].struct), 0, 10,
                              runtime._p(wf), model.n_bins, 0, 0, None, None, None, C.c_void_p(runtime._stream_ptr()))
     assert rc in (abi.ALB_ERR_UNSUPPORTED, abi.ALB_ERR_ARG)
