@@ -194,7 +194,8 @@ def main(argv=None):
     ap.add_argument("--adjoint-scenes", type=int, default=256)
     ap.add_argument("--adjoint-photons", type=float, default=1e6)
     ap.add_argument("--inversion-full-iteration", action="store_true",
-                    help="also time one inversion iteration at 4096 scenes x 2e5 photons")
+                    help="also time one inversion iteration over 4096 scenes at 2e4 photons per scene "
+                         "(and scale it to 2e5 photons x 200 iterations)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 2 GiB device copy at the end (reference reading for HBM byte counters)")
     args = ap.parse_args(argv)
@@ -334,18 +335,23 @@ def main(argv=None):
                                "gpu_same_workload": {"value": g_rate, "unit": "photon paths/s", "seconds": g_s},
                                "gpu_over_cpu": g_rate / rate}
         print("cpu baseline done", file=sys.stderr, flush=True)
+    def leg(name, fn, *a, **kw):
+        # each finished leg also goes to stderr, so a later failure keeps it
+        out[name] = fn(*a, **kw)
+        print(json.dumps({name: out[name]}), file=sys.stderr, flush=True)
+
     if not args.no_radius:
-        out["radius_mode"] = bench_radius(model, ds, key, n_f, args.radius_grid)
+        leg("radius_mode", bench_radius, model, ds, key, n_f, args.radius_grid)
     if not args.no_batched:
         if args.sweep_full > 0:
-            out["batched"] = bench_batched_full(args.sweep_full)
-        out["batched_small"] = bench_batched(args.sweep_scenes)
+            leg("batched", bench_batched_full, args.sweep_full)
+        leg("batched_small", bench_batched, args.sweep_scenes)
     if not args.no_adjoint:
-        out["adjoint"] = bench_adjoint(args.adjoint_scenes, int(args.adjoint_photons))
+        leg("adjoint", bench_adjoint, args.adjoint_scenes, int(args.adjoint_photons))
     if not args.no_inversion:
-        out["inversion"] = bench_inversion(args.inversion_scenes, args.inversion_iters, lr=args.inversion_lr)
+        leg("inversion", bench_inversion, args.inversion_scenes, args.inversion_iters, lr=args.inversion_lr)
         if args.inversion_full_iteration:
-            out["inversion_full_iteration"] = bench_inversion_full_iteration()
+            leg("inversion_full_iteration", bench_inversion_full_iteration)
     if args.copy_calibration:
         x = torch.empty(1 << 28, dtype=torch.float64, device="cuda")
         y = torch.empty_like(x)
@@ -436,7 +442,7 @@ def bench_inversion(n_scenes: int, iters: int, n_forward: int = 20000, lr: float
             "batched_s_per_iteration": sb, "plain_s_per_iteration": sp, "speedup": sp / sb}
 
 
-def bench_inversion_full_iteration(n_scenes: int = 4096, n_forward: int = 20000, group: int = 512):
+def bench_inversion_full_iteration(n_scenes: int = 4096, n_forward: int = 20000, group: int = 256):
     """One Adam iteration over every inversion scene (batched, in groups of
     `group` scenes); seconds per iteration at n_forward photons and the
     estimate for 2e5 photons (the work is linear in photons) and 200

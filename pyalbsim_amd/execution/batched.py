@@ -90,6 +90,7 @@ def simulate_batched(models: Sequence[SceneModel], n_forward, n_backward, key, *
     import torch
 
     models = list(models)
+    runtime.check_k(models)
     n = len(models)
     nf = np.broadcast_to(np.asarray(n_forward, dtype=np.int64), (n,)).copy()
     nb = np.broadcast_to(np.asarray(n_backward, dtype=np.int64), (n,)).copy()

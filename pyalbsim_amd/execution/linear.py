@@ -77,6 +77,7 @@ def simulate(model: SceneModel, n_forward: int, n_backward: int, key, *, window=
     """
     import torch
 
+    runtime.check_k([model])
     ds = dscene or runtime.DeviceScene(model)
     if heatmaps is not None and heatmaps.enabled:
         if ds.heat is None or ds.heat["bins"] != int(heatmaps.bins):

@@ -204,6 +204,15 @@ def floor_timing_depth(total: np.ndarray, bins_per_m, surface_bin, pool: int = 4
     return out
 
 
+def _free_and_total():
+    """(free, total) device bytes, after torch's cached unused blocks are released."""
+    import torch
+
+    from .sweep import free_device_bytes
+
+    return free_device_bytes(), torch.cuda.mem_get_info()[1]
+
+
 def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: int = 200, lr: float = 1e-2,
            key=(0x1A7E, 5), target_key=(0x7A26, 6), form: str = "batched", win_len: int = 1000,
            depth_rel: float = 1e-2, x0=None, log=None, shape_weight: float = 1000.0,
@@ -219,7 +228,12 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
     group_scenes: batched form in groups of at most this many scenes (the
     kept forward state of all scenes at once may not fit the device); each
     iteration runs forward and backward group by group (the loss is a sum
-    over scenes), and scene s keeps Philox scene stream s in any grouping."""
+    over scenes), and scene s keeps Philox scene stream s in any grouping.
+    Each group's trace passes are captured in HIP graphs in the first
+    iteration and replayed after it; a graph keeps its record and query
+    sinks, so with several groups a group whose graphs would leave less than
+    half of the device memory free runs its passes eagerly instead (the same
+    results: a replay equals the eager trace bit for bit)."""
     import torch
 
     from .adjoint import batched_waveforms, gated_waveforms
@@ -243,8 +257,9 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
     specs = [dict(spec, tilts=None if tl is None else tl[a:b], scene_base=a,
                   graphs={} if form == "batched" else None) for a, b in groups]
 
-    with torch.no_grad():
-        target = torch.cat([gated_waveforms(theta_true[a:b], n_forward, n_backward, target_key, **specs[i])
+    with torch.no_grad():  # one pass per group: nothing to replay
+        target = torch.cat([gated_waveforms(theta_true[a:b], n_forward, n_backward, target_key,
+                                            **dict(specs[i], graphs=None))
                             for i, (a, b) in enumerate(groups)])
     x_init = np.full((S, len(UNKNOWN)), 0.5) if x0 is None else np.array(x0, dtype=np.float64)
     if x0 is None and start == "timing":
@@ -285,6 +300,11 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
             pg = scene_loss(batched_waveforms(th_g, n_forward, n_backward, key, **specs[i]), target[a:b])
             pg.sum().backward()
             per[a:b] = pg.detach()
+            if it == 0 and len(groups) > 1 and specs[i]["graphs"] is not None:
+                del pg
+                free, total = _free_and_total()
+                if free < 0.5 * total:
+                    specs[i]["graphs"] = None  # releases this group's graphs and their sinks
         loss_v = per.sum()
         opt.step()
         with torch.no_grad():

@@ -30,6 +30,13 @@ class AlbError(RuntimeError):
     pass
 
 
+def check_k(models) -> None:
+    """Raise AlbError before any device work when a scene's k exceeds what the k-NN kernels take."""
+    for m in models:
+        if not 1 <= int(m.k) <= abi.MAX_K:
+            raise AlbError(f"photon_mapping_k = {m.k} outside [1, {abi.MAX_K}], the k-NN gather's range")
+
+
 def load_library(path: str = LIB_PATH):
     """Load libalbsim_hip.so and declare its C signatures (no GPU needed)."""
     global _lib

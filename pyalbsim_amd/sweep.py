@@ -85,26 +85,34 @@ class SweepResult:
 
 def estimate_group_bytes(models, n_forward, n_backward, key, pilot_fraction: float = 1 / 100,
                          bytes_per_record: float = 240.0, bytes_per_query: float = 80.0) -> np.ndarray:
-    """Device bytes each scene needs inside a batched group: a pilot forward
-    trace of pilot_fraction of every scene's photons gives its records per
-    photon by type; records cost sinks + map + radix tree + build workspace
-    (bytes_per_record) and backward queries, estimated as records x
-    n_backward / n_forward, bytes_per_query."""
+    """Device bytes each scene needs inside a batched group: pilot forward and
+    backward traces of pilot_fraction of every scene's photons give its map
+    records per forward photon and its gather queries per backward photon;
+    records cost sinks + map + radix tree + build workspace
+    (bytes_per_record), queries their sinks, order and neighbour rows
+    (bytes_per_query).  Backward paths keep scattering until their time
+    budget ends, so a scene's queries per photon differ from its records per
+    photon and are counted, not inferred."""
     import torch
 
     from . import abi, runtime
 
     nf = np.asarray(n_forward, dtype=np.int64)
     nb = np.asarray(n_backward, dtype=np.int64)
-    pc = np.maximum(np.ceil(nf * pilot_fraction).astype(np.int64), np.minimum(nf, 256))
     tab = runtime.DeviceSceneTable(list(models))
-    sinks, got = runtime.trace_batched(tab, key, abi.PASS_FORWARD, pc)
-    per = np.zeros(len(nf))
-    for t in range(abi.N_TYPES):
-        if got[t]:
-            per += torch.bincount(sinks[t].tensors["scene"][:got[t]].long(), minlength=len(nf)).cpu().numpy()
-    rec = per / pc * nf
-    return rec * bytes_per_record + rec * (nb / np.maximum(nf, 1)) * bytes_per_query
+
+    def per_photon(n, pass_):
+        pc = np.maximum(np.ceil(n * pilot_fraction).astype(np.int64), np.minimum(n, 256))
+        sinks, got = runtime.trace_batched(tab, key, pass_, pc)
+        cnt = np.zeros(len(n))
+        for t in range(abi.N_TYPES):
+            if got[t]:
+                cnt += torch.bincount(sinks[t].tensors["scene"][:got[t]].long(), minlength=len(n)).cpu().numpy()
+        return cnt / np.maximum(pc, 1)
+
+    rec = per_photon(nf, abi.PASS_FORWARD) * nf
+    qry = per_photon(nb, abi.PASS_BACKWARD) * nb
+    return rec * bytes_per_record + qry * bytes_per_query
 
 
 def free_device_bytes() -> int:

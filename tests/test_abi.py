@@ -72,3 +72,20 @@ def test_product_path_fails_loudly_without_gpu():
 
     with pytest.raises(runtime.AlbError):
         runtime.DeviceScene(build_scene(single_shot()))
+
+
+def test_k_limit_matches_the_header_and_is_checked_before_device_work():
+    m = re.search(r"#define ALB_MAX_K (\d+)", open(HEADER).read())
+    assert m and int(m.group(1)) == abi.MAX_K
+    from pyalbsim_amd.config.simulation import SimulationConfig
+    from pyalbsim_amd.execution.batched import simulate_batched
+    from pyalbsim_amd.execution.linear import simulate
+    from pyalbsim_amd.model import build_scene
+
+    for k in (abi.MAX_K + 1, 1000):
+        model = build_scene(SimulationConfig(photon_mapping_k=k))
+        with pytest.raises(runtime.AlbError, match=r"outside \[1, 448\]"):
+            simulate(model, 1000, 250, (1, 2))
+        with pytest.raises(runtime.AlbError, match=r"outside \[1, 448\]"):
+            simulate_batched([model], 1000, 250, (1, 2))
+    runtime.check_k([build_scene(SimulationConfig(photon_mapping_k=abi.MAX_K))])

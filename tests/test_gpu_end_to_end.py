@@ -176,28 +176,36 @@ def test_full_size_invariants(gpu):
 
 
 def _full_size_neighbour_sets(torch, ds, res, n_q=32):
-    """On the full-size maps (up to 5e8 records), the neighbour sets and k-th
-    distances of n_q random queries per type equal brute force over every
-    record, bit for bit (the index and chunk arithmetic of the big launches)."""
+    """On the full-size maps (up to 5e8 records), every type's queries are
+    gathered again in one launch that writes each query's neighbour row (up
+    to 1.25e8 rows, 5e10 bytes of rows).  n_q random rows per type, with
+    their k-th distances, equal brute force over every record bit for bit
+    (the index arithmetic of the big launch), and the one-launch waveform
+    equals the chunked gather's (launches of 2^24 queries) up to FP64
+    summation order, so no chunk drops, repeats or misplaces a query."""
     from concurrent.futures import ThreadPoolExecutor
 
     k = ds.model.k
     rs = np.random.default_rng(8)
-    F = ("x", "y", "z", "dx", "dy", "dz", "energy", "time")
     for t in range(4):
         nq, nm = res.query_counts[t], res.map_counts[t]
-        pick = torch.as_tensor(np.sort(rs.choice(nq, size=n_q, replace=False)), device="cuda")
-        sub = runtime.Sink.alloc(n_q, True, False)
-        for f in F:
-            sub.tensors[f][:n_q].copy_(res.queries[t].tensors[f][pick])
-        nbr = torch.zeros((n_q, k), dtype=torch.int32, device="cuda")
-        rad = torch.zeros(n_q, dtype=torch.float64, device="cuda")
+        print(f"full-size neighbour rows: type {t}, {nm} records, {nq} queries", flush=True)
+        nbr = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+        rad = torch.empty(nq, dtype=torch.float64, device="cuda")
         w = torch.zeros(ds.model.n_bins, dtype=torch.float64, device="cuda")
-        keep = runtime.gather(ds, res.maps[t], t, sub, n_q, w, ds.model.gate(), nbr_out=nbr, radius_out=rad)
+        keep = runtime.gather(ds, res.maps[t], t, res.queries[t], nq, w, ds.model.gate(), nbr_out=nbr,
+                              radius_out=rad)
         torch.cuda.synchronize()
         del keep
+        ref_w = res.waveforms[t]
+        assert float(torch.max(torch.abs(w - ref_w))) <= 1e-9 * float(torch.max(torch.abs(ref_w))), t
+        pick = np.sort(rs.choice(nq, size=n_q, replace=False))
+        pk = torch.as_tensor(pick, device="cuda")
+        got_i = nbr[pk].cpu().numpy()
+        got_r = rad[pk].cpu().numpy()
+        del nbr, rad
         mx, my, mz = (res.maps[t].tensors[f][:nm].cpu().numpy() for f in ("x", "y", "z"))
-        q = np.stack([sub.tensors[f][:n_q].cpu().numpy() for f in ("x", "y", "z")], 1)
+        q = np.stack([res.queries[t].tensors[f][pk].cpu().numpy() for f in ("x", "y", "z")], 1)
 
         def brute(j):
             return oracle.knn(mx, my, mz, q[j:j + 1], k)
@@ -206,9 +214,8 @@ def _full_size_neighbour_sets(torch, ds, res, n_q=32):
             out = list(ex.map(brute, range(n_q)))
         idx = np.concatenate([o[0] for o in out])
         d2 = np.concatenate([o[1] for o in out])
-        print(f"full-size neighbour sets: type {t}, {nm} records, {n_q} queries", flush=True)
-        assert np.array_equal(nbr.cpu().numpy(), idx), t
-        assert np.array_equal(rad.cpu().numpy(), np.sqrt(d2[:, k - 1])), t
+        assert np.array_equal(got_i, idx), t
+        assert np.array_equal(got_r, np.sqrt(d2[:, k - 1])), t
 
 
 def test_run_parallel_contract(gpu):
