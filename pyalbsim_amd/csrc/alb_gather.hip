@@ -237,25 +237,84 @@ __device__ uint64_t wave_select(const double* d, const int32_t* idx, int n, int 
   return prefix;
 }
 
-// k-th smallest (1-based) of the first n buffer distances, bit by bit: the
-// keys (FP64 bit patterns, ordered like the non-negative values) stay in
-// registers, kBuf / 64 per lane.  The bits on which all keys agree are found
-// with one wave OR / AND reduction and skipped; for every other bit, from the
-// top, one ballot per slot counts the still-active keys with a 0 there, and the
-// side holding rank kk stays active.  It stops when one key (or a run of equal
-// keys) is left.  *lt_out / *eq_out receive the counts of entries below and
-// equal to the result.
+// k-th smallest (1-based) of the first n buffer distances.  The distances
+// stay in registers, kBuf / 64 per lane.  Up to three histogram levels
+// narrow the active set to the bucket holding rank kk; then bit by bit over
+// the keys (FP64 bit patterns, ordered like the non-negative values): the
+// bits on which all active keys agree are found with one wave OR / AND
+// reduction and skipped; for every other bit, from the top, one ballot per
+// slot counts the still-active keys with a 0 there, and the side holding rank
+// kk stays active.  It stops when one key (or a run of equal keys) is left.
+// *lt_out / *eq_out receive the counts of entries below and equal to the
+// result.
 template <int BUF>
-__device__ uint64_t wave_kth_key(const double* d, int n, int kk, uint32_t lane, int* lt_out, int* eq_out) {
+__device__ uint64_t wave_kth_key(const double* d, int n, int kk, uint32_t lane, uint32_t* hist, int* lt_out,
+                                 int* eq_out) {
   constexpr int U = BUF / 64;
-  uint64_t key[U];
+  double dv[U];
   bool act[U];
-  uint32_t olo = 0u, ohi = 0u, alo = ~0u, ahi = ~0u;
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int t = (int)lane + 64 * u;
     act[u] = t < n;
-    key[u] = act[u] ? (uint64_t)__double_as_longlong(d[t]) : 0ull;
+    dv[u] = act[u] ? d[t] : 0.0;
+  }
+  int m = n, below = 0;
+  // Histogram levels first: the active entries are counted into 64 buckets
+  // of [their smallest, their largest] (monotone in d, so equal values share
+  // a bucket and the entries of lower buckets are strictly smaller), and the
+  // bucket holding rank kk becomes the active set.  Each level takes at
+  // least the largest entry out, and on spread values about 63/64 of them;
+  // after a few levels (or once the active values are all equal) the bit
+  // loop below finishes on the few left.
+  for (int level = 0; level < 3 && m > 1; ++level) {
+    double lo = __builtin_inf(), hi = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (act[u]) {
+        lo = fmin(lo, dv[u]);
+        hi = fmax(hi, dv[u]);
+      }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      lo = fmin(lo, __shfl_xor(lo, off));
+      hi = fmax(hi, __shfl_xor(hi, off));
+    }
+    if (!(hi > lo)) break;  // all active values equal: the bit loop ends at once
+    hist[lane] = 0u;
+    wave_sync();
+    const double scale = 64.0 / (hi - lo);
+    int bk[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      bk[u] = 64;
+      if (act[u]) {
+        const double q = (dv[u] - lo) * scale;
+        bk[u] = q < 63.0 ? (int)q : 63;
+        atomicAdd(&hist[bk[u]], 1u);
+      }
+    }
+    wave_sync();
+    const uint32_t c = hist[lane];
+    const uint32_t cum = wave_incl_scan(c, lane);
+    const int sel = __ffsll((long long)ballot(cum >= (uint32_t)kk)) - 1;  // m >= kk: some bucket reaches it
+    const int cnt = (int)__builtin_amdgcn_readlane((int)c, sel);
+    const int lower = (int)__builtin_amdgcn_readlane((int)cum, sel) - cnt;
+    below += lower;
+    kk -= lower;
+    m = cnt;
+#pragma unroll
+    for (int u = 0; u < U; ++u) act[u] = act[u] && bk[u] == sel;
+    wave_sync();
+  }
+  // Bitwise radix select over the FP64 bit patterns of the active entries
+  // (non-negative doubles order like their bits): bits shared by all of
+  // them are skipped, each other bit costs one ballot per slot.
+  uint64_t key[U];
+  uint32_t olo = 0u, ohi = 0u, alo = ~0u, ahi = ~0u;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    key[u] = act[u] ? (uint64_t)__double_as_longlong(dv[u]) : 0ull;
     if (act[u]) {
       olo |= (uint32_t)key[u];
       ohi |= (uint32_t)(key[u] >> 32);
@@ -271,9 +330,8 @@ __device__ uint64_t wave_kth_key(const double* d, int n, int kk, uint32_t lane, 
     ahi &= (uint32_t)__shfl_xor((int)ahi, off);
   }
   const uint64_t diff = ((((uint64_t)ohi) << 32) | olo) ^ ((((uint64_t)ahi) << 32) | alo);
-  int m = n, below = 0;
   for (int b = diff ? 63 - __clzll((long long)diff) : -1; b >= 0 && m > 1; --b) {
-    if (!((diff >> b) & 1ull)) continue;  // all keys share this bit
+    if (!((diff >> b) & 1ull)) continue;  // all active keys share this bit
     int c0 = 0;
 #pragma unroll
     for (int u = 0; u < U; ++u) c0 += __popcll(ballot(act[u] && !((key[u] >> b) & 1ull)));
@@ -308,7 +366,7 @@ __device__ uint64_t wave_kth_key(const double* d, int n, int kk, uint32_t lane, 
 template <int BUF>
 __device__ double wave_keep_k(double* d, int32_t* idx, int n, int k, uint32_t* hist, uint32_t lane) {
   int lt = 0, eq = 0;
-  const double th = __longlong_as_double((long long)wave_kth_key<BUF>(d, n, k, lane, &lt, &eq));
+  const double th = __longlong_as_double((long long)wave_kth_key<BUF>(d, n, k, lane, hist, &lt, &eq));
   int32_t icut = INT_MAX;  // entries equal to th are kept up to this index
   if (lt + eq > k) icut = (int32_t)wave_select(d, idx, n, k - lt, true, th, hist, lane);
   double vd[BUF / 64];
@@ -342,7 +400,8 @@ __device__ double wave_keep_k(double* d, int32_t* idx, int n, int k, uint32_t* h
 
 // Cheap buffer cut: an upper bound of the k-th smallest d, not the k-th
 // itself.  The n <= BUF entries (all <= hi) are counted into 64 buckets of
-// [0, hi] (LDS histogram); the bucket holding the k-th smallest bounds it.
+// [lo, hi], lo the smallest entry (LDS histogram); the bucket holding the
+// k-th smallest bounds it.
 // Entries of the buckets up to that one are kept (the buckets are monotone
 // in d, so the kept set is exactly {d <= theta'} with theta' the largest kept
 // d) and theta' is returned.  Returns a negative value, changing nothing,
@@ -350,18 +409,30 @@ __device__ double wave_keep_k(double* d, int32_t* idx, int n, int k, uint32_t* h
 template <int BUF>
 __device__ double wave_cut_hist(double* d, int32_t* idx, int n, int k, double hi, uint32_t* hist, uint32_t lane,
                                 int* n_out) {
-  if (!(hi > 0.0)) return -1.0;
   constexpr int U = BUF / 64;
+  double dv[U];
+  double lo = __builtin_inf();
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int t = (int)lane + 64 * u;
+    dv[u] = t < n ? d[t] : 0.0;
+    if (t < n) lo = fmin(lo, dv[u]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) lo = fmin(lo, __shfl_xor(lo, off));
+  // buckets span [smallest entry, hi]: entries far from the query crowd the
+  // top of [0, hi], where a histogram of [0, hi] cannot separate them
+  if (!(hi > lo)) return -1.0;
   hist[lane] = 0u;
   wave_sync();
-  const double scale = 64.0 / hi;
+  const double scale = 64.0 / (hi - lo);
   int bk[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int t = (int)lane + 64 * u;
     bk[u] = 64;
     if (t < n) {
-      const double q = d[t] * scale;
+      const double q = (dv[u] - lo) * scale;
       bk[u] = q < 63.0 ? (int)q : 63;
       atomicAdd(&hist[bk[u]], 1u);
     }
@@ -384,7 +455,7 @@ __device__ double wave_cut_hist(double* d, int32_t* idx, int n, int k, double hi
     vd[u] = 0.0;
     vi[u] = 0;
     if (keep[u]) {
-      vd[u] = d[t];
+      vd[u] = dv[u];
       vi[u] = idx[t];
       mx = fmax(mx, vd[u]);
     }
