@@ -510,25 +510,31 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   constexpr int kBufK = BUF;          // candidate buffer entries per wave
   constexpr int kSeedK = BUF - 64;    // seed records (at least k)
   constexpr int kMaxKK = BUF - 64;    // largest k of this variant
-  __shared__ int32_t sn_s[kWaves][kStack];
-  __shared__ double sd_s[kWaves][kStack];
-  __shared__ double bd_s[kWaves][kBufK];
-  __shared__ int32_t bi_s[kWaves][kBufK];
-  __shared__ int32_t rlo_s[kWaves][128];
-  __shared__ int32_t rhi_s[kWaves][128];
-  __shared__ uint32_t hist_s[kWaves][256];
-  __shared__ int32_t path_s[kWaves][kPath];
+  // one wavefront's LDS in one struct: its arrays are one base address plus
+  // constant offsets (one scalar register instead of one per array)
+  struct WaveLds {
+    double sd[kStack];
+    double bd[kBufK];
+    int32_t sn[kStack];
+    int32_t bi[kBufK];
+    int32_t rlo[128];
+    int32_t rhi[128];
+    uint32_t hist[256];
+    int32_t path[kPath];
+  };
+  __shared__ WaveLds lds_s[kWaves];
 
   const uint32_t lane = lane_id();
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: scalar slot loop
-  int32_t* sn = sn_s[w];
-  double* sd = sd_s[w];
-  double* bd = bd_s[w];
-  int32_t* bi = bi_s[w];
-  int32_t* rlo = rlo_s[w];
-  int32_t* rhi = rhi_s[w];
-  uint32_t* hist = hist_s[w];
-  int32_t* path = path_s[w];
+  WaveLds& L = lds_s[w];
+  int32_t* sn = L.sn;
+  double* sd = L.sd;
+  double* bd = L.bd;
+  int32_t* bi = L.bi;
+  int32_t* rlo = L.rlo;
+  int32_t* rhi = L.rhi;
+  uint32_t* hist = L.hist;
+  int32_t* path = L.path;
   int plen = 0;            // levels of the previous query's descent held in path
   int32_t prev_root = -1;  // the subtree root it started from
   int32_t nn_root = -1;    // subtree root of the previous query whose neighbours bi[0, k) holds
@@ -793,39 +799,29 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
       }
       sp -= p;
       wave_sync();
-      // per child (0 = farther, 1 = nearer): 1 = push, 2 = scan
-      int kind0 = 0, kind1 = 0;
-      int32_t lo0 = 0, hi0 = -1, lo1 = 0, hi1 = -1, id0 = 0, id1 = 0;
-      double md0 = 0.0, md1 = 0.0;
-      if (have && node < 0) {  // a single record
-        kind0 = 2;
-        lo0 = hi0 = ~node;
-      } else if (have) {
-        const Node nd = nodes[node];
-        if (nd.last - nd.first + 1 <= kBucket) {  // small subtree from the descent: scan it
-          kind0 = 2;
-          lo0 = nd.first;
-          hi0 = nd.last;
-        } else {
-          const double dl = box_dist2(qx, qy, qz, nd.lb);
-          const double dr = box_dist2(qx, qy, qz, nd.rb);
-          const bool left_near = dl <= dr;
-          const int32_t fl = left_near ? nd.split + 1 : nd.first, fh = left_near ? nd.last : nd.split;
-          const int32_t nl = left_near ? nd.first : nd.split + 1, nh = left_near ? nd.split : nd.last;
-          const double fd = left_near ? dr : dl, ndd = left_near ? dl : dr;
-          const bool fleaf = left_near ? (nd.leaf_mask & 2) : (nd.leaf_mask & 1);
-          const bool nleaf = left_near ? (nd.leaf_mask & 1) : (nd.leaf_mask & 2);
-          const int32_t fid = left_near ? nd.split + 1 : nd.split, nid = left_near ? nd.split : nd.split + 1;
-          if (fd <= theta && !(fl >= s0 && fh <= s1)) {
-            lo0 = fl; hi0 = fh; id0 = fid; md0 = fd;
-            kind0 = (fleaf || fh - fl + 1 <= kBucket) ? 2 : 1;
-          }
-          if (ndd <= theta && !(nl >= s0 && nh <= s1)) {
-            lo1 = nl; hi1 = nh; id1 = nid; md1 = ndd;
-            kind1 = (nleaf || nh - nl + 1 <= kBucket) ? 2 : 1;
-          }
-        }
-      }
+      // per child (0 = farther, 1 = nearer): 1 = push, 2 = scan.  Branch
+      // free: every lane loads a node (lanes without one load node 0, which
+      // exists whenever the stack is used) and computes both children, and
+      // selects keep what applies.  A node of <= kBucket records yields its
+      // children as ranges, each only when its own box is within theta.
+      const bool rec = have && node < 0;  // a single record
+      const bool inner = have && node >= 0;
+      const Node nd = nodes[inner ? node : 0];
+      const double dl = box_dist2(qx, qy, qz, nd.lb);
+      const double dr = box_dist2(qx, qy, qz, nd.rb);
+      const bool left_near = dl <= dr;
+      const int32_t fl = left_near ? nd.split + 1 : nd.first, fh = left_near ? nd.last : nd.split;
+      const int32_t nl = left_near ? nd.first : nd.split + 1, nh = left_near ? nd.split : nd.last;
+      const double md0 = left_near ? dr : dl, md1 = left_near ? dl : dr;
+      const bool fleaf = (nd.leaf_mask & (left_near ? 2 : 1)) != 0;
+      const bool nleaf = (nd.leaf_mask & (left_near ? 1 : 2)) != 0;
+      const int32_t id0 = left_near ? nd.split + 1 : nd.split, id1 = left_near ? nd.split : nd.split + 1;
+      const bool far_ok = inner && md0 <= theta && !(fl >= s0 && fh <= s1);
+      const bool near_ok = inner && md1 <= theta && !(nl >= s0 && nh <= s1);
+      const int kind0 = rec ? 2 : (far_ok ? ((fleaf || fh - fl + 1 <= kBucket) ? 2 : 1) : 0);
+      const int kind1 = near_ok ? ((nleaf || nh - nl + 1 <= kBucket) ? 2 : 1) : 0;
+      const int32_t lo0 = rec ? ~node : fl, hi0 = rec ? ~node : fh;
+      const int32_t lo1 = nl, hi1 = nh;
       c_nodes += (uint32_t)__popcll(ballot(have));
       const uint64_t pa = ballot(kind0 == 1), pb = ballot(kind1 == 1);
       const int npush = __popcll(pa) + __popcll(pb);
