@@ -87,8 +87,7 @@ enum alb_counter {
   ALB_CTR_SMALL_SCENE = 16,  /* batched queries whose scene map holds fewer
                                 than k records (rows left at -1 / NaN)      */
   ALB_CTR_REPLAY_MISSES = 17,/* replayed events whose record was not found   */
-  ALB_CTR_NEIGHBOURS = 18,   /* records taken by the fixed-radius gather;
-                                k-NN: candidates entering a candidate buffer */
+  ALB_CTR_NEIGHBOURS = 18,   /* records taken by the fixed-radius gather    */
   ALB_N_COUNTERS = 19
 };
 
@@ -410,7 +409,8 @@ int alb_gather_static(const alb_scene_t* scene, const alb_map_t* map, int32_t ph
  * alb_gather_order: order[q_n] (int32) = query offsets sorted by the Morton
  *   key of their positions on the map's grid; workspace from
  *   alb_gather_workspace_size.
- * alb_knn: the k nearest records of the queries in sort slots [c0, c0 + cn):
+ * alb_knn: the k nearest records of the queries in sort slots [c0, c0 + cn),
+ *   cn <= 2^30 (larger calls fail with ALB_ERR_ARG; alb_knn_batched alike):
  *   row r (= slot - c0, or the query offset when rows_by_query != 0) of
  *   nbr[rows * k] gets the neighbour indices, radius[r] the k-th distance and
  *   knorm[r] the kernel normalisation 1/(pi r^2), or 1/(4/3 pi r^3) for

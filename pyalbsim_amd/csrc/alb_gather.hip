@@ -61,6 +61,9 @@ constexpr int kMaxK = kSeed;      // the seed must hold k records
 constexpr int64_t kChunk = 1 << 22;  // queries per k-NN / contribution round
 constexpr int kWorkStride = 32;      // words between the per-XCD query counters (256 B)
 constexpr int kGrab = 4;             // queries a wavefront takes per counter increment
+// slots of one k-NN launch: the kernel's slot arithmetic is 32-bit, and the
+// per-XCD counters run past the end by at most kGrab per wavefront
+constexpr int64_t kMaxSlots = 1ll << 30;
 static_assert(8 * kWorkStride <= ALB_KNN_WORK_WORDS, "work array too small");
 constexpr int kXcds = 8;          // MI355X XCDs; workgroups are dealt to them round-robin
 
@@ -546,7 +549,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   const int k = a.k;
   // per-wave event counts of one launch (diagnostics only; thousands of waves
   // share a launch's queries, so 32 bits hold them)
-  uint32_t c_q = 0, c_cand = 0, c_nodes = 0, c_ovf = 0, c_steps = 0, c_sorts = 0, c_desc = 0, c_small = 0, c_ins = 0;
+  uint32_t c_q = 0, c_cand = 0, c_nodes = 0, c_ovf = 0, c_steps = 0, c_sorts = 0, c_desc = 0, c_small = 0;
 
   // XCD-aware slot ranges: the grid is a multiple of kXcds blocks and block b
   // runs on XCD b % kXcds, so XCD x walks its own contiguous 1/kXcds of the
@@ -557,19 +560,21 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
   // shares left a launch waiting for its slowest wavefronts (SCATTER
   // queries 12 % faster dealt this way).
   const int xcd = (int)(blockIdx.x % kXcds);
-  const int64_t s_lo = a.q_n * xcd / kXcds, s_hi = a.q_n * (xcd + 1) / kXcds;
+  // slot arithmetic in 32 bits (a launch takes fewer than 2^31 slots; the
+  // host checks), which keeps scalar registers free for the traversal
+  const int32_t s_lo = (int32_t)(a.q_n * xcd / kXcds), s_hi = (int32_t)(a.q_n * (xcd + 1) / kXcds);
   unsigned long long* const wk = a.work + xcd * kWorkStride;
-  auto grab = [&]() -> int64_t {
+  auto grab = [&]() -> int32_t {
     unsigned long long v = 0;
     if (lane == 0) v = atomicAdd(wk, (unsigned long long)kGrab);
-    return s_lo + (int64_t)bcast_u64(v, 0);
+    return s_lo + (int32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
   };
-  int64_t next = 0;
-  for (int64_t chunk = grab(); chunk < s_hi; chunk = next) {
+  int32_t next = 0;
+  for (int32_t chunk = grab(); chunk < s_hi; chunk = next) {
   next = grab();
-  const int64_t chunk_end = chunk + kGrab < s_hi ? chunk + kGrab : s_hi;
-  for (int64_t si = chunk; si < chunk_end; ++si) {
-    const int64_t qrel = a.order[a.c0 + si];
+  const int32_t chunk_end = chunk + kGrab < s_hi ? chunk + kGrab : s_hi;
+  for (int32_t si = chunk; si < chunk_end; ++si) {
+    const int32_t qrel = a.order[a.c0 + si];
     const int64_t qg = a.q_begin + qrel;
     const int64_t qi = a.direct ? qrel : si;  // output row
     const double qx = a.qx[qg], qy = a.qy[qg], qz = a.qz[qg];
@@ -785,7 +790,6 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
         theta_exact = true;
       }
     }
-    c_ins += (uint32_t)buf_n;
 
     // ---- 2. pruned traversal of the stacked subtrees
     while (sp > 0) {
@@ -889,6 +893,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
           const uint64_t mt = ballot(tk[u]);
           if (mt == 0) continue;
           bool take = tk[u];
+          uint64_t mt2 = mt;  // the takers, unless a cut lowers theta
           if (buf_n + __popcll(mt) > kBufK) {
             c_sorts++;
             int kept = 0;
@@ -903,15 +908,14 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
               theta_exact = true;
             }
             take = take && d2v[u] <= theta;
+            mt2 = ballot(take);
           }
-          const uint64_t mt2 = ballot(take);
           if (take) {
             const int slot = buf_n + (int)mask_rank(mt2);
             bd[slot] = d2v[u];
             bi[slot] = ix[u];
           }
           buf_n += __popcll(mt2);
-          c_ins += (uint32_t)__popcll(mt2);
           wave_sync();
         }
       }
@@ -937,7 +941,6 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     if (c_sorts) atomicAdd(&a.ctr[ALB_CTR_KNN_SORTS], (unsigned long long)c_sorts);
     if (c_desc) atomicAdd(&a.ctr[ALB_CTR_KNN_DESCENT], (unsigned long long)c_desc);
     if (c_small) atomicAdd(&a.ctr[ALB_CTR_SMALL_SCENE], (unsigned long long)c_small);
-    if (c_ins) atomicAdd(&a.ctr[ALB_CTR_NEIGHBOURS], (unsigned long long)c_ins);
   }
 }
 
@@ -1369,6 +1372,7 @@ extern "C" int alb_knn(const alb_map_t* map, int32_t photon_type, int32_t k, con
   if (!q || !order || !nbr || !radius || !knorm || !work) return set_error(ALB_ERR_ARG, "alb_knn: null argument");
   if (photon_type < 0 || photon_type >= ALB_N_TYPES) return set_error(ALB_ERR_ARG, "alb_knn: bad photon type");
   if (cn < 0 || c0 < 0) return set_error(ALB_ERR_ARG, "alb_knn: bad slot range");
+  if (cn > kMaxSlots) return set_error(ALB_ERR_ARG, "alb_knn: %lld slots in one call (at most 2^30)", (long long)cn);
   if (cn == 0) return ALB_OK;
   int cus;
   if ((rc = device_cus(&cus)) != ALB_OK) return rc;
@@ -1604,6 +1608,8 @@ extern "C" int alb_knn_batched(const alb_map_t* map, const alb_scene_index_t* in
     return set_error(ALB_ERR_ARG, "alb_knn_batched: scene index or query scenes missing");
   if (photon_type < 0 || photon_type >= ALB_N_TYPES) return set_error(ALB_ERR_ARG, "alb_knn_batched: bad photon type");
   if (cn < 0 || c0 < 0) return set_error(ALB_ERR_ARG, "alb_knn_batched: bad slot range");
+  if (cn > kMaxSlots)
+    return set_error(ALB_ERR_ARG, "alb_knn_batched: %lld slots in one call (at most 2^30)", (long long)cn);
   if (cn == 0) return ALB_OK;
   int cus, rc;
   if ((rc = device_cus(&cus)) != ALB_OK) return rc;
