@@ -5,9 +5,10 @@
 // scipy KDTree build of PhotonMapIndex (photon_map_index.py:17).
 //
 //  1. bounding box of the records (block reduction + ordered-int atomics)
-//  2. quantise each record to a uniform grid (2^32 cells per axis over (x,z)
-//     for the planar surface / floor maps, 2^21 over (x,y,z) for scatter
-//     events) and interleave the cell coordinates into a 64-bit Morton key
+//  2. quantise each record to a uniform grid of cubic cells (2^32 cells
+//     along the largest extent of (x,z) for the planar surface / floor maps,
+//     2^21 along the largest of (x,y,z) for scatter events) and interleave
+//     the cell coordinates into a 64-bit Morton key
 //  3. sort (key, record index) pairs (rocPRIM radix sort via hipCUB)
 //  4. gather the float32 SoA into Morton order
 //  5. build the binary radix tree over the sorted keys (Karras, "Maximizing
@@ -66,18 +67,24 @@ __global__ __launch_bounds__(256) void bbox_kernel(const float* __restrict__ x, 
   }
 }
 
-// meta = {origin x,y,z, inverse cell width x,y,z, dims, bits per axis}
+// meta = {origin x,y,z, inverse cell width x,y,z, dims, bits per axis}.
+// The cells are cubes: one width on every keyed axis, set by the largest
+// extent.  Per-axis widths would stretch the tree's boxes by the ratio of the
+// extents (the scatter map of a 400 m shot spans ~240 m across and 5 m in
+// depth, so its boxes were ~48 times flatter than wide), and a k-NN ball
+// then meets many more of them.
 __global__ void meta_kernel(const uint32_t* bb, double* meta, int dims) {
   const int bits = dims == 2 ? 32 : 21;
   const double cells = ldexp(1.0, bits);
+  double ext = 0.0;
   for (int k = 0; k < 3; ++k) {
     const double lo = (double)ord2f(bb[k]);
     const double hi = (double)ord2f(bb[3 + k]);
-    double ext = hi - lo;
-    if (!(ext > 0.0)) ext = 1.0;
     meta[k] = lo;
-    meta[3 + k] = cells / ext;
+    if (dims == 3 || k != 1) ext = fmax(ext, hi - lo);  // planar maps key (x, z)
   }
+  if (!(ext > 0.0)) ext = 1.0;
+  for (int k = 0; k < 3; ++k) meta[3 + k] = cells / ext;
   meta[6] = (double)dims;
   meta[7] = (double)bits;
 }
