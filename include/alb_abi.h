@@ -272,7 +272,10 @@ typedef struct alb_queries {
  *   int32 leaf_mask      bit 0: left child is the single record split,
  *                        bit 1: right child is the single record split+1
  *   float left_box[6]    min x,y,z, max x,y,z of the left child's records
- *   float right_box[6]   same for the right child */
+ *   float right_box[6]   same for the right child
+ * The boxes are written for the root and for every node covering more than
+ * 64 records; a subtree of at most 64 records is searched as one record range,
+ * and its nodes' boxes are left unwritten. */
 typedef struct alb_map {
   int64_t n;                /* records                                             */
   int32_t dims;             /* 2: key over (x,z) (planar maps), 3: over (x,y,z)    */

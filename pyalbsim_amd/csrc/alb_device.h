@@ -491,6 +491,12 @@ struct __attribute__((aligned(16))) Node {
 };
 static_assert(sizeof(Node) == 64, "tree node must be one 64-byte line");
 
+// The build writes child boxes only into the root and into nodes covering
+// more than kBoxedRecords records: the gathers scan any subtree of at most
+// that many records as one record range, so they never read its node's boxes.
+// first/last/split/leaf_mask are written for every node.
+constexpr int kBoxedRecords = 64;
+
 // squared distance in FP64 from a query to a float32 record, terms summed x,
 // y, z in this order (the order the oracle and the neighbour-set proofs use)
 __device__ __forceinline__ double dist2(double qx, double qy, double qz, float x, float y, float z) {

@@ -54,6 +54,7 @@ constexpr int kBuf = 256;         // per-wave candidate buffer (LDS), power of t
 constexpr int kBufLarge = 512;
 constexpr int kMaxKLarge = kBufLarge - 64;
 constexpr int kBucket = 64;       // nodes with <= kBucket records are scanned
+static_assert(kBucket >= kBoxedRecords, "the k-NN reads the boxes of nodes of more than kBucket records");
 constexpr int kPath = 128;        // descent levels kept for the next query (LDS)
 constexpr int kMaxWindow = 7680;  // LDS-privatised waveform bins (60 KB of FP64: with the row tile, 64 KB per block)
 constexpr int kMaxWindowLarge = 16384;  // batched contributions with one 1024-thread block per CU (132 KB)
@@ -520,8 +521,8 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     double bd[kBufK];
     int32_t sn[kStack];
     int32_t bi[kBufK];
-    int32_t rlo[128];
-    int32_t rhi[128];
+    int32_t rlo[64];  // one range per lane and step
+    int32_t rhi[64];
     uint32_t hist[256];
     int32_t path[kPath];
   };
@@ -794,71 +795,61 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     // ---- 2. pruned traversal of the stacked subtrees
     while (sp > 0) {
       c_steps++;
-      const int p = (sp >= kStack - 128) ? 1 : (sp < 64 ? sp : 64);
-      bool have = (int)lane < p;
+      // one lane per child: lanes 2e and 2e + 1 take the left and the right
+      // child of stack entry e, so a step pops at most 32 entries and each
+      // lane computes one box distance.  Pop policy: at most 32 entries while
+      // the stack is below kStack - 64 (a step then pushes at most 64), else
+      // one (the stack then grows by at most one per tree level).
+      const int p = (sp >= kStack - 64) ? 1 : (sp < 32 ? sp : 32);
+      const int e = (int)(lane >> 1);
+      const int side = (int)(lane & 1);
+      bool have = e < p;
       int32_t node = 0;
       if (have) {
-        node = sn[sp - 1 - (int)lane];
-        have = sd[sp - 1 - (int)lane] <= theta;
+        node = sn[sp - 1 - e];
+        have = sd[sp - 1 - e] <= theta;
       }
       sp -= p;
       wave_sync();
-      // per child (0 = farther, 1 = nearer): 1 = push, 2 = scan.  Branch
-      // free: every lane loads a node (lanes without one load node 0, which
-      // exists whenever the stack is used) and computes both children, and
-      // selects keep what applies.  A node of <= kBucket records yields its
-      // children as ranges, each only when its own box is within theta.
-      const bool rec = have && node < 0;  // a single record
+      // Branch free: every lane loads its entry's node header and its own
+      // child's box (lanes without an entry load node 0, which exists
+      // whenever the stack is used), and selects keep what applies.
+      const bool rec = have && node < 0 && side == 0;  // a single record
       const bool inner = have && node >= 0;
-      const Node nd = nodes[inner ? node : 0];
-      const double dl = box_dist2(qx, qy, qz, nd.lb);
-      const double dr = box_dist2(qx, qy, qz, nd.rb);
-      const bool left_near = dl <= dr;
-      const int32_t fl = left_near ? nd.split + 1 : nd.first, fh = left_near ? nd.last : nd.split;
-      const int32_t nl = left_near ? nd.first : nd.split + 1, nh = left_near ? nd.split : nd.last;
-      const double md0 = left_near ? dr : dl, md1 = left_near ? dl : dr;
-      const bool fleaf = (nd.leaf_mask & (left_near ? 2 : 1)) != 0;
-      const bool nleaf = (nd.leaf_mask & (left_near ? 1 : 2)) != 0;
-      const int32_t id0 = left_near ? nd.split + 1 : nd.split, id1 = left_near ? nd.split : nd.split + 1;
-      const bool far_ok = inner && md0 <= theta && !(fl >= s0 && fh <= s1);
-      const bool near_ok = inner && md1 <= theta && !(nl >= s0 && nh <= s1);
-      const int kind0 = rec ? 2 : (far_ok ? ((fleaf || fh - fl + 1 <= kBucket) ? 2 : 1) : 0);
-      const int kind1 = near_ok ? ((nleaf || nh - nl + 1 <= kBucket) ? 2 : 1) : 0;
-      const int32_t lo0 = rec ? ~node : fl, hi0 = rec ? ~node : fh;
-      const int32_t lo1 = nl, hi1 = nh;
-      c_nodes += (uint32_t)__popcll(ballot(have));
-      const uint64_t pa = ballot(kind0 == 1), pb = ballot(kind1 == 1);
-      const int npush = __popcll(pa) + __popcll(pb);
+      const Node* np = nodes + (inner ? node : 0);
+      const int4 hd = *reinterpret_cast<const int4*>(np);  // first, last, split, leaf_mask
+      const float2* bp = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(np) + 16 + 24 * side);
+      const float2 b01 = bp[0], b23 = bp[1], b45 = bp[2];
+      const float box[6] = {b01.x, b01.y, b23.x, b23.y, b45.x, b45.y};
+      // a subtree of <= kBucket records stacked by the descent (far sides are
+      // stacked whatever their size) has no boxes of its own: one range,
+      // taken by the entry's left lane
+      const bool whole = inner && hd.y - hd.x + 1 <= kBucket;
+      const int32_t lo = side ? hd.z + 1 : hd.x, hi = side ? hd.y : hd.z;
+      const bool leaf = ((hd.w >> side) & 1) != 0;
+      const double d = box_dist2(qx, qy, qz, box);
+      const bool ok = inner && !whole && d <= theta && !(lo >= s0 && hi <= s1);
+      const int kind = (rec || (whole && side == 0)) ? 2 : (ok ? ((leaf || hi - lo + 1 <= kBucket) ? 2 : 1) : 0);
+      const int32_t range_lo = rec ? ~node : (whole ? hd.x : lo), range_hi = rec ? ~node : (whole ? hd.y : hi);
+      c_nodes += (uint32_t)__popcll(ballot(have && side == 0));
+      const uint64_t pm = ballot(kind == 1);
+      const int npush = __popcll(pm);
       if (sp + npush > kStack) {  // excluded by the pop policy; refuse rather than corrupt
         c_ovf++;
         break;
       }
-      {
-        int off = sp + (int)mask_rank(pa) + (int)mask_rank(pb);
-        if (kind0 == 1) {
-          sn[off] = id0;
-          sd[off] = md0;
-          ++off;
-        }
-        if (kind1 == 1) {
-          sn[off] = id1;
-          sd[off] = md1;
-        }
+      if (kind == 1) {
+        const int off = sp + (int)mask_rank(pm);
+        sn[off] = side ? hd.z + 1 : hd.z;  // an internal child's node index
+        sd[off] = d;
       }
       sp += npush;
-      const uint64_t ra = ballot(kind0 == 2), rb = ballot(kind1 == 2);
-      const int nr = __popcll(ra) + __popcll(rb);
-      {
-        int off = (int)mask_rank(ra) + (int)mask_rank(rb);
-        if (kind0 == 2) {
-          rlo[off] = lo0;
-          rhi[off] = hi0;
-          ++off;
-        }
-        if (kind1 == 2) {
-          rlo[off] = lo1;
-          rhi[off] = hi1;
-        }
+      const uint64_t rm = ballot(kind == 2);
+      const int nr = __popcll(rm);
+      if (kind == 2) {
+        const int off = (int)mask_rank(rm);
+        rlo[off] = range_lo;
+        rhi[off] = range_hi;
       }
       wave_sync();
       // scan the ranges four at a time.  The twelve coordinate loads are

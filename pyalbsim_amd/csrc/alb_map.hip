@@ -15,9 +15,13 @@
 //     parallelism in the construction of BVHs, octrees, and k-d trees", HPG
 //     2012): internal node i covers a contiguous record range; equal keys are
 //     split by index
-//  6. exact float32 bounding boxes per node, bottom-up: a child's prefix
-//     length is always longer than its parent's, so one launch per prefix
-//     length (longest first) finds every child box already written.
+//  6. exact float32 bounding boxes of the children of the root and of every
+//     node covering more than kBoxedRecords records, bottom-up: a child's
+//     prefix length is always longer than its parent's, so one launch per
+//     prefix length (longest first) finds every child box already written;
+//     a child of at most kBoxedRecords records takes the box of its records.
+//     The gathers scan such small subtrees as record ranges and never read
+//     their nodes' boxes.
 #include <hipcub/hipcub.hpp>
 
 #include "alb_device.h"
@@ -26,6 +30,7 @@
 namespace alb {
 
 constexpr int kLevels = 128;  // prefix lengths 0..63 (+64..95 for equal keys)
+constexpr int kNoBox = 255;   // level of a node whose boxes are not written
 
 __device__ __forceinline__ uint32_t f2ord(float f) {
   const uint32_t u = __float_as_uint(f);
@@ -181,9 +186,15 @@ __device__ __forceinline__ void build_node(const uint64_t* __restrict__ k, int64
   nd->last = (int32_t)hi;
   nd->split = (int32_t)g;
   nd->leaf_mask = (lo == g ? 1 : 0) | (hi == g + 1 ? 2 : 0);
-  const int lv = dnode < 0 ? 0 : (dnode >= kLevels ? kLevels - 1 : dnode);
-  level[i] = (uint8_t)lv;
-  atomicAdd(&lh[lv], 1u);
+  // only the root and nodes of more than kBoxedRecords records get boxes
+  // (level kNoBox: skipped by the box passes)
+  if (hi - lo + 1 > kBoxedRecords || i == 0) {
+    const int lv = dnode < 0 ? 0 : (dnode >= kLevels ? kLevels - 1 : dnode);
+    level[i] = (uint8_t)lv;
+    atomicAdd(&lh[lv], 1u);
+  } else {
+    level[i] = (uint8_t)kNoBox;
+  }
 }
 
 __global__ void level_scan_kernel(const uint32_t* hist, uint32_t* offset, uint32_t* cursor) {
@@ -203,23 +214,39 @@ __global__ __launch_bounds__(256) void level_scatter_kernel(const uint8_t* __res
   for (int t = threadIdx.x; t < kLevels; t += blockDim.x) cnt[t] = 0u;
   __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lv = i < m ? (int)level[i] : 0;
-  const uint32_t local = i < m ? atomicAdd(&cnt[lv], 1u) : 0u;
+  const int lv = i < m ? (int)level[i] : kNoBox;
+  const bool boxed = lv < kLevels;
+  const uint32_t local = boxed ? atomicAdd(&cnt[lv], 1u) : 0u;
   __syncthreads();
   for (int t = threadIdx.x; t < kLevels; t += blockDim.x)
     if (cnt[t]) base[t] = atomicAdd(&cursor[t], cnt[t]);
   __syncthreads();
-  if (i < m) order[base[lv] + local] = (int32_t)i;
+  if (boxed) order[base[lv] + local] = (int32_t)i;
 }
 
-// box of one child: the record itself for a leaf, else the union of the child
-// node's two child boxes (written by an earlier, longer-prefix level)
+// box of one child covering records [lo, hi]: the record itself for a leaf,
+// the records' own box for a subtree of at most kBoxedRecords records (its
+// node has no boxes), else the union of the child node's two child boxes
+// (written by an earlier, longer-prefix level)
 __device__ __forceinline__ void child_box(const alb_map_t& m, const Node* __restrict__ nodes, int32_t c, bool leaf,
-                                          float b[6]) {
+                                          int32_t lo, int32_t hi, float b[6]) {
   if (leaf) {
     b[0] = b[3] = m.x[c];
     b[1] = b[4] = m.y[c];
     b[2] = b[5] = m.z[c];
+  } else if (hi - lo + 1 <= kBoxedRecords) {
+    b[0] = b[3] = m.x[lo];
+    b[1] = b[4] = m.y[lo];
+    b[2] = b[5] = m.z[lo];
+    for (int32_t r = lo + 1; r <= hi; ++r) {
+      const float x = m.x[r], y = m.y[r], z = m.z[r];
+      b[0] = fminf(b[0], x);
+      b[1] = fminf(b[1], y);
+      b[2] = fminf(b[2], z);
+      b[3] = fmaxf(b[3], x);
+      b[4] = fmaxf(b[4], y);
+      b[5] = fmaxf(b[5], z);
+    }
   } else {
     const Node& cn = nodes[c];
 #pragma unroll
@@ -238,8 +265,8 @@ __global__ __launch_bounds__(256) void box_level_kernel(alb_map_t m, const int32
     const int32_t node = order[t];
     Node& nd = nodes[node];
     float a[6], b[6];
-    child_box(m, nodes, nd.split, nd.leaf_mask & 1, a);
-    child_box(m, nodes, nd.split + 1, nd.leaf_mask & 2, b);
+    child_box(m, nodes, nd.split, nd.leaf_mask & 1, nd.first, nd.split, a);
+    child_box(m, nodes, nd.split + 1, nd.leaf_mask & 2, nd.split + 1, nd.last, b);
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       nd.lb[c] = a[c];
@@ -408,7 +435,9 @@ static int map_build(const alb_records_t* in, int64_t n, int32_t dims, alb_map_t
     hipLaunchKernelGGL(radix_tree_kernel, dim3(g2), dim3(B), 0, s, map->keys, n, (Node*)map->nodes, level, hist);
     hipLaunchKernelGGL(level_scan_kernel, dim3(1), dim3(1), 0, s, hist, offset, cursor);
     hipLaunchKernelGGL(level_scatter_kernel, dim3(g2), dim3(B), 0, s, level, n - 1, cursor, order);
-    const unsigned g3 = g2 < 1024 ? g2 : 1024;
+    // boxed nodes cover more than kBoxedRecords records: fewer than n / 32
+    const unsigned g3b = (unsigned)((n / 32 + B - 1) / B);
+    const unsigned g3 = g3b < 1 ? 1 : (g3b < 1024 ? g3b : 1024);
     for (int lv = kLevels - 1; lv >= 0; --lv)
       hipLaunchKernelGGL(box_level_kernel, dim3(g3), dim3(B), 0, s, m, order, offset, lv);
   }

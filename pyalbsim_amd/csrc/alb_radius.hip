@@ -37,6 +37,7 @@ constexpr int kRWaves = 4;          // wavefronts per block
 constexpr int kRStack = 1024;       // per-wave node stack (LDS)
 constexpr int kRRanges = 128;       // ranges per step: two children per lane
 constexpr int kRBucket = 64;        // subtrees of <= kRBucket records are scanned
+static_assert(kRBucket >= kBoxedRecords, "the radius gather reads the boxes of nodes of more than kRBucket records");
 constexpr int kRMaxWindow = 2048;   // LDS-privatised waveform bins
 constexpr int kRXcds = 8;           // MI355X XCDs; workgroups are dealt round-robin
 

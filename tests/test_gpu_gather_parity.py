@@ -59,12 +59,17 @@ def test_map_build_is_a_permutation_with_containing_boxes(gpu, scene):
         # every record is a leaf of exactly one node
         leaves = np.concatenate([split[(mask & 1) > 0], split[(mask & 2) > 0] + 1])
         assert np.array_equal(np.sort(leaves), np.arange(n))
-        sample = np.random.default_rng(0).choice(n - 1, size=min(n - 1, 400), replace=False)
+        # boxes are written for the root and the nodes of more than 64 records
+        # (smaller subtrees are scanned as record ranges)
+        boxed = np.flatnonzero((last - first + 1 > 64) | (np.arange(n - 1) == 0))
+        assert len(boxed) > 0
+        sample = np.random.default_rng(0).choice(boxed, size=min(len(boxed), 400), replace=False)
         for i in sample:
             lp = xyz[first[i]:split[i] + 1]
             rp = xyz[split[i] + 1:last[i] + 1]
             assert np.array_equal(boxes[i, :3], lp.min(0)) and np.array_equal(boxes[i, 3:6], lp.max(0))
             assert np.array_equal(boxes[i, 6:9], rp.min(0)) and np.array_equal(boxes[i, 9:], rp.max(0))
+        for i in np.random.default_rng(1).choice(n - 1, size=min(n - 1, 400), replace=False):
             if not mask[i] & 1:
                 c = split[i]
                 assert first[c] == first[i] and last[c] == split[i]
