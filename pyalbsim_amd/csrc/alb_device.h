@@ -311,6 +311,27 @@ __device__ __forceinline__ void normalize_f32(float& x, float& y, float& z) {
   z = z / n;
 }
 
+// x / y in float32 for float32 x and y, given ry = 1 / (double)y (IEEE).  The
+// FP64 product is within 2^-52 of the exact quotient, relative: 2^-28 float32
+// ulps.  An exact quotient of two float32 values lies at least 2^-25 ulps
+// from every rounding midpoint (x = y * m for a midpoint m would make an odd
+// 25-bit significand times y's significand fit in 24 bits).  So the product
+// rounds to the IEEE float32 quotient.
+__device__ __forceinline__ float div_f32_recip(float x, double ry) { return (float)((double)x * ry); }
+
+// normalize_f32 with its three float32 divisions taken through one FP64
+// reciprocal (div_f32_recip): the same results
+__device__ __forceinline__ void normalize_f32_recip(float& x, float& y, float& z) {
+  float s = x * x + y * y;
+  s = s + z * z;
+  float n = sqrtf(s);
+  if (n == 0.0f) n = 1.0f;
+  const double rn = 1.0 / (double)n;
+  x = div_f32_recip(x, rn);
+  y = div_f32_recip(y, rn);
+  z = div_f32_recip(z, rn);
+}
+
 __device__ __forceinline__ void normalize_f64_keep_zero(double& x, double& y, double& z) {
   const double n = norm3(x, y, z);
   if (n != 0.0) {
@@ -321,13 +342,16 @@ __device__ __forceinline__ void normalize_f64_keep_zero(double& x, double& y, do
 }
 
 // Cook-Torrance reflection towards the sensor, float32 result
-// (microfacet_brdf.py:96-147 called as sea_surface.py:35-62: wi = -d_p, wo = -d_s)
-__device__ __forceinline__ float brdf_reflect(float px, float py, float pz, double sx, double sy,
-                                              double sz, double alpha, double F0) {
+// (microfacet_brdf.py:96-147 called as sea_surface.py:35-62: wi = -d_p, wo = -d_s),
+// with the outgoing direction o = normalised -s given (the gather stages it
+// once per sensor vertex)
+__device__ __forceinline__ float brdf_reflect_o(float px, float py, float pz, double ox, double oy, double oz,
+                                                double alpha, double F0) {
   float wx = -px, wy = -py, wz = -pz;
-  normalize_f32(wx, wy, wz);
-  double ox = -sx, oy = -sy, oz = -sz;
-  normalize_f64_keep_zero(ox, oy, oz);
+  normalize_f32_recip(wx, wy, wz);
+  const double ndwi = clipd((double)wy, 0.0, 1.0);
+  const double ndwo = clipd(oy, 0.0, 1.0);
+  if (!(ndwi > 0.0 && ndwo > 0.0)) return 0.0f;
   double hx = (double)wx + ox, hy = (double)wy + oy, hz = (double)wz + oz;
   {
     double n = norm3(hx, hy, hz);
@@ -336,19 +360,23 @@ __device__ __forceinline__ float brdf_reflect(float px, float py, float pz, doub
     hy = hy / n;
     hz = hz / n;
   }
-  const double ndwi = clipd((double)wy, 0.0, 1.0);
-  const double ndwo = clipd(oy, 0.0, 1.0);
   double widh = (double)wx * hx;
   widh = widh + (double)wy * hy;
   widh = widh + (double)wz * hz;
   widh = clipd(widh, 0.0, 1.0);
   const double ndh = clipd(hy, 0.0, 1.0);
-  if (!(ndwi > 0.0 && ndwo > 0.0)) return 0.0f;
   const double D = ggx_d(ndh, alpha);
   const double G = smith_g1(ndwi, alpha) * smith_g1(ndwo, alpha);
   const double F = schlick(widh, F0);
   const double den = 4.0 * ndwi * ndwo + ALB_FLT_EPS;
   return (float)((D * G * F) / den);
+}
+
+__device__ __forceinline__ float brdf_reflect(float px, float py, float pz, double sx, double sy,
+                                              double sz, double alpha, double F0) {
+  double ox = -sx, oy = -sy, oz = -sz;
+  normalize_f64_keep_zero(ox, oy, oz);
+  return brdf_reflect_o(px, py, pz, ox, oy, oz, alpha, F0);
 }
 
 // brdf_reflect together with d log(BRDF) / d(alpha^2) (0 where the BRDF is 0):
@@ -395,26 +423,26 @@ __device__ __forceinline__ float brdf_reflect_dlog(float px, float py, float pz,
 
 // Walter BTDF + surface albedo / pi (microfacet_brdf.py:150-226 called as
 // sea_surface.py:64-100 with eta_i = n_water, eta_o = 1)
-__device__ __forceinline__ double btdf_transmit(float px, float py, float pz, double sx, double sy,
-                                                double sz, double alpha, double n_w, double F0,
-                                                double albedo) {
+// Walter BTDF towards the sensor plus the surface albedo / pi
+// (microfacet_brdf.py:150-226 called as sea_surface.py:64-100), with the
+// outgoing direction o = normalised -s given
+__device__ __forceinline__ double btdf_transmit_o(float px, float py, float pz, double ox, double oy, double oz,
+                                                  double alpha, double n_w, double F0, double albedo) {
   float wx = px, wy = py, wz = pz;
-  normalize_f32(wx, wy, wz);
-  double ox = -sx, oy = -sy, oz = -sz;
-  normalize_f64_keep_zero(ox, oy, oz);
+  normalize_f32_recip(wx, wy, wz);
   const double eta_i = n_w, eta_o = 1.0, eta = eta_i / eta_o;
-  double hx = (double)wx + eta * ox, hy = (double)wy + eta * oy, hz = (double)wz + eta * oz;
-  {
-    double n = norm3(hx, hy, hz);
-    if (n == 0.0) n = 1.0;
-    hx = hx / n;
-    hy = hy / n;
-    hz = hz / n;
-  }
   const double ndwi = clipd((double)wy, -1.0, 1.0);
   const double ndwo = clipd(oy, -1.0, 1.0);
   double btdf = 0.0;
-  if (ndwi * ndwo < 0.0) {
+  if (ndwi * ndwo < 0.0) {  // the half vector only matters here
+    double hx = (double)wx + eta * ox, hy = (double)wy + eta * oy, hz = (double)wz + eta * oz;
+    {
+      double n = norm3(hx, hy, hz);
+      if (n == 0.0) n = 1.0;
+      hx = hx / n;
+      hy = hy / n;
+      hz = hz / n;
+    }
     const double ndh = clipd(hy, 0.0, 1.0);
     double widh = (double)wx * hx;
     widh = widh + (double)wy * hy;
@@ -432,6 +460,14 @@ __device__ __forceinline__ double btdf_transmit(float px, float py, float pz, do
     btdf = (1.0 - F) * D * G * factor / (fabs(ndwi) * fabs(ndwo) + ALB_FLT_EPS);
   }
   return btdf + albedo / ALB_PI;
+}
+
+__device__ __forceinline__ double btdf_transmit(float px, float py, float pz, double sx, double sy,
+                                                double sz, double alpha, double n_w, double F0,
+                                                double albedo) {
+  double ox = -sx, oy = -sy, oz = -sz;
+  normalize_f64_keep_zero(ox, oy, oz);
+  return btdf_transmit_o(px, py, pz, ox, oy, oz, alpha, n_w, F0, albedo);
 }
 
 // Phase-function mass of the scattering-angle bin towards the sensor
@@ -577,18 +613,29 @@ __device__ __forceinline__ uint64_t morton_key(double px, double py, double pz, 
   return (spread3(qx) << 2) | (spread3(qy) << 1) | spread3(qz);
 }
 
-// Phase mass of the scattering-angle bin towards the sensor with the per-bin
-// masses tabulated on the host (same values as scatter_mult computes).
-// the same with the outgoing direction o = normalised -s given
-__device__ __forceinline__ int scatter_bin_o(float px, float py, float pz, double ox, double oy, double oz, int n) {
-  normalize_f32(px, py, pz);
+// width of the gather's scattering-angle bins, (pi - eps) / n in float32
+// (physics/scatter/energy.py:41-47)
+__device__ __forceinline__ double scatter_bin_width(int n) {
+  return (double)((3.14159265358979323846f - 1.1920928955078125e-07f) / (float)n);
+}
+
+// Scattering-angle bin towards the sensor, int((acos(c) - eps) / delta), with
+// the outgoing direction o = normalised -s given, delta = scatter_bin_width(n)
+// and rdelta = 1 / delta.  The quotient comes from the reciprocal: it is
+// within 2^-37 of the FP64 quotient (q < 2^15), so the two truncate alike
+// unless an integer lies within 2^-32 of it, and then the quotient is taken.
+__device__ __forceinline__ int scatter_bin_o(float px, float py, float pz, double ox, double oy, double oz, int n,
+                                             double delta, double rdelta) {
+  normalize_f32_recip(px, py, pz);
   double c = (double)px * ox;
   c = c + (double)py * oy;
   c = c + (double)pz * oz;
   c = clipd(c, -1.0, 1.0);
-  const double ang = acos(c);
-  const float delta_f = (3.14159265358979323846f - 1.1920928955078125e-07f) / (float)n;
-  long long i = (long long)((ang - ALB_FLT_EPS) / (double)delta_f);
+  const double x = acos(c) - ALB_FLT_EPS;
+  double q = x * rdelta;
+  const double fr = q - floor(q);
+  if (fr < 0x1p-32 || fr > 1.0 - 0x1p-32) q = x / delta;
+  long long i = (long long)q;
   if (i < 0) i = 0;
   if (i > n - 1) i = n - 1;
   return (int)i;

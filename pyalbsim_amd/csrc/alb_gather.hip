@@ -965,11 +965,13 @@ __global__ __launch_bounds__(256) void knorm_perm_kernel(const double* __restric
 constexpr int kRowTile = 64;
 
 struct RowTile {
-  double s0[kRowTile], s1[kRowTile], s2[kRowTile];  // query direction, or -s normalised (SCATTER)
+  double s0[kRowTile], s1[kRowTile], s2[kRowTile];  // -s normalised (BOTTOM: the query direction s)
   double kn[kRowTile];                              // kernel normalisation
   double mul[kRowTile];                             // BOTTOM multiplier
   const double* pm[kRowTile];                       // SCATTER phase masses of the query's layer
   int32_t pn[kRowTile];
+  double dl[kRowTile], rdl[kRowTile];               // SCATTER angle-bin width and its reciprocal
+  double rm[kRowTile];                              // 1 / float32(sample multiplier)
   float e[kRowTile], t[kRowTile];                   // query energy and time
   int32_t scene[kRowTile];
 };
@@ -978,15 +980,19 @@ template <int TYPE>
 __device__ __forceinline__ void stage_row(RowTile& R, int r, const alb_scene_t& sc, const alb_queries_t& q,
                                           int64_t qg, double knorm) {
   double sx = q.dx[qg], sy = q.dy[qg], sz = q.dz[qg];
-  if constexpr (TYPE == ALB_SCATTER) {
+  if constexpr (TYPE != ALB_BOTTOM_REFLECTION) {  // the outgoing direction -s normalised
     double ox = -sx, oy = -sy, oz = -sz;
     normalize_f64_keep_zero(ox, oy, oz);
     sx = ox;
     sy = oy;
     sz = oz;
+  }
+  if constexpr (TYPE == ALB_SCATTER) {
     int pn;
     R.pm[r] = gather_phase_mass(sc, q.y[qg], &pn);
     R.pn[r] = pn;
+    R.dl[r] = scatter_bin_width(pn);
+    R.rdl[r] = 1.0 / R.dl[r];
   } else if constexpr (TYPE == ALB_BOTTOM_REFLECTION) {
     R.mul[r] = bottom_mult(sc, sx, sy, sz);
   }
@@ -994,6 +1000,7 @@ __device__ __forceinline__ void stage_row(RowTile& R, int r, const alb_scene_t& 
   R.s1[r] = sy;
   R.s2[r] = sz;
   R.kn[r] = knorm;
+  R.rm[r] = 1.0 / (double)(float)sc.mult;
   R.e[r] = q.energy[qg];
   R.t[r] = q.time[qg];
 }
@@ -1029,12 +1036,12 @@ __device__ __forceinline__ double pair_mult(const RowTile& R, int r, const alb_s
   if constexpr (TYPE == ALB_BOTTOM_REFLECTION) {
     return R.mul[r];
   } else if constexpr (TYPE == ALB_SCATTER) {
-    return R.pm[r][scatter_bin_o(p.dx, p.dy, p.dz, R.s0[r], R.s1[r], R.s2[r], R.pn[r])];
+    return R.pm[r][scatter_bin_o(p.dx, p.dy, p.dz, R.s0[r], R.s1[r], R.s2[r], R.pn[r], R.dl[r], R.rdl[r])];
   } else if constexpr (TYPE == ALB_SURFACE_REFLECTION) {
-    return (double)brdf_reflect(p.dx, p.dy, p.dz, R.s0[r], R.s1[r], R.s2[r], sc.alpha_ggx, sc.F0);
+    return (double)brdf_reflect_o(p.dx, p.dy, p.dz, R.s0[r], R.s1[r], R.s2[r], sc.alpha_ggx, sc.F0);
   } else {
-    return btdf_transmit(p.dx, p.dy, p.dz, R.s0[r], R.s1[r], R.s2[r], sc.alpha_ggx, sc.n_water, sc.F0,
-                         sc.surface_albedo);
+    return btdf_transmit_o(p.dx, p.dy, p.dz, R.s0[r], R.s1[r], R.s2[r], sc.alpha_ggx, sc.n_water, sc.F0,
+                           sc.surface_albedo);
   }
 }
 
@@ -1046,7 +1053,6 @@ __global__ __launch_bounds__(256) void contrib_kernel(ContribArgs a) {
   __shared__ RowTile R;
   for (int t = threadIdx.x; t < a.win_len; t += blockDim.x) hist[t] = 0.0;
   const int k = a.sc.k;
-  const float fmult = (float)a.sc.mult;
   const alb_map_t& m = a.map;
   // XCD-aware tile ranges (as in knn_kernel)
   const int64_t tiles = (a.q_n + kRowTile - 1) / kRowTile;
@@ -1077,7 +1083,7 @@ __global__ __launch_bounds__(256) void contrib_kernel(ContribArgs a) {
       const float ee = R.e[r] * p.e;
       const double e = (double)ee * R.kn[r] * mul;
       const float tt = R.t[r] + m.time[i];
-      const int64_t bin = (int64_t)(tt / fmult);
+      const int64_t bin = (int64_t)div_f32_recip(tt, R.rm[r]);  // tt / fmult
       if (bin >= a.win0 && bin < (int64_t)a.win0 + a.win_len) atomicAdd(&hist[bin - a.win0], e);
       else if (bin >= 0 && bin < a.n_bins)
         unsafeAtomicAdd(a.rep ? &a.rep[(int64_t)(blockIdx.x % a.n_rep) * a.n_bins + bin] : &a.wf[bin], e);
@@ -1177,7 +1183,7 @@ __global__ __launch_bounds__(BLOCK) void contrib_batched_kernel(ContribBatchedAr
       const float ee = R.e[r] * p.e;
       const double e = (double)ee * R.kn[r] * mul;
       const float tt = R.t[r] + m.time[i];
-      const int64_t bin = (int64_t)(tt / (float)S.mult);
+      const int64_t bin = (int64_t)div_f32_recip(tt, R.rm[r]);  // tt / float32(mult)
       if (sc == cur && bin >= cw0 && bin < (int64_t)cw0 + a.win_len) {
         atomicAdd(&hist[bin - cw0], e);
       } else {

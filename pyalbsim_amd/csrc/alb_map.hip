@@ -152,8 +152,10 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(const uint64_t* __restr
   __shared__ uint32_t lh[kLevels];
   for (int t = threadIdx.x; t < kLevels; t += blockDim.x) lh[t] = 0u;
   __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n - 1) build_node(k, n, i, nodes, level, lh);
+  // grid-stride: a few thousand blocks, so the level histogram takes few
+  // global atomics (a block per 256 nodes made millions on a few addresses)
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n - 1; i += (int64_t)gridDim.x * blockDim.x)
+    build_node(k, n, i, nodes, level, lh);
   __syncthreads();
   for (int t = threadIdx.x; t < kLevels; t += blockDim.x)
     if (lh[t]) atomicAdd(&hist[t], lh[t]);
@@ -208,20 +210,32 @@ __global__ void level_scan_kernel(const uint32_t* hist, uint32_t* offset, uint32
   offset[kLevels] = acc;
 }
 
+// Boxed nodes into level order.  Block b takes a contiguous chunk of nodes:
+// it counts the chunk's nodes per level in LDS, reserves each level's run with
+// one global atomic, then writes the chunk's node indices into the runs (in
+// any order within a level: boxes do not depend on it).
 __global__ __launch_bounds__(256) void level_scatter_kernel(const uint8_t* __restrict__ level, int64_t m,
                                                             uint32_t* cursor, int32_t* __restrict__ order) {
   __shared__ uint32_t cnt[kLevels], base[kLevels];
   for (int t = threadIdx.x; t < kLevels; t += blockDim.x) cnt[t] = 0u;
   __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lv = i < m ? (int)level[i] : kNoBox;
-  const bool boxed = lv < kLevels;
-  const uint32_t local = boxed ? atomicAdd(&cnt[lv], 1u) : 0u;
+  const int64_t chunk = (m + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < m ? lo + chunk : m;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const int lv = (int)level[i];
+    if (lv < kLevels) atomicAdd(&cnt[lv], 1u);
+  }
   __syncthreads();
-  for (int t = threadIdx.x; t < kLevels; t += blockDim.x)
+  for (int t = threadIdx.x; t < kLevels; t += blockDim.x) {
     if (cnt[t]) base[t] = atomicAdd(&cursor[t], cnt[t]);
+    cnt[t] = 0u;
+  }
   __syncthreads();
-  if (boxed) order[base[lv] + local] = (int32_t)i;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const int lv = (int)level[i];
+    if (lv < kLevels) order[base[lv] + atomicAdd(&cnt[lv], 1u)] = (int32_t)i;
+  }
 }
 
 // box of one child covering records [lo, hi]: the record itself for a leaf,
@@ -432,9 +446,10 @@ static int map_build(const alb_records_t* in, int64_t n, int32_t dims, alb_map_t
   if (in->fwi_x && m.fwi_x) hipLaunchKernelGGL(permute_points_kernel, dim3(grid), dim3(B), 0, s, *in, m);
   if (n > 1) {
     const unsigned g2 = (unsigned)((n - 1 + B - 1) / B);
-    hipLaunchKernelGGL(radix_tree_kernel, dim3(g2), dim3(B), 0, s, map->keys, n, (Node*)map->nodes, level, hist);
+    const unsigned gt = g2 < 2048 ? g2 : 2048;  // grid-stride launches
+    hipLaunchKernelGGL(radix_tree_kernel, dim3(gt), dim3(B), 0, s, map->keys, n, (Node*)map->nodes, level, hist);
     hipLaunchKernelGGL(level_scan_kernel, dim3(1), dim3(1), 0, s, hist, offset, cursor);
-    hipLaunchKernelGGL(level_scatter_kernel, dim3(g2), dim3(B), 0, s, level, n - 1, cursor, order);
+    hipLaunchKernelGGL(level_scatter_kernel, dim3(gt), dim3(B), 0, s, level, n - 1, cursor, order);
     // boxed nodes cover more than kBoxedRecords records: fewer than n / 32
     const unsigned g3b = (unsigned)((n / 32 + B - 1) / B);
     const unsigned g3 = g3b < 1 ? 1 : (g3b < 1024 ? g3b : 1024);
