@@ -918,8 +918,10 @@ __global__ __launch_bounds__(256, (LAYERED || HEAT) ? 2 : 4) void trace_kernel(a
 // pass scene 0 reproduces the single-scene trace.  Lanes whose photon has finished take the next index
 // from a global counter every refill_interval iterations, which keeps the
 // wave's lanes busy when path lengths differ.
+// Three waves per SIMD (168 VGPRs, a few spilled): the 256-scene sweep's
+// forward pass 77 -> 68 ms against two waves; four were slower (more spills).
 template <bool FWD>
-__global__ __launch_bounds__(256) void trace_batched_kernel(const alb_scene_t* __restrict__ scenes, int32_t n_scenes,
+__global__ __launch_bounds__(256, 3) void trace_batched_kernel(const alb_scene_t* __restrict__ scenes, int32_t n_scenes,
                                                             const int64_t* __restrict__ offsets, alb_rng_t rng,
                                                             Sinks sk, unsigned long long* work,
                                                             int32_t refill_interval, unsigned long long* ctr) {
@@ -1008,7 +1010,7 @@ __device__ __forceinline__ void flush_acc(ReplayAcc& a, double* acc) {
 // so one binary search per photon and type places a cursor) and its adjoint
 // weight multiplies the prefix statistics.
 template <bool FWD>
-__global__ __launch_bounds__(256) void replay_batched_kernel(const alb_scene_t* __restrict__ scenes, int32_t n_scenes,
+__global__ __launch_bounds__(256, 3) void replay_batched_kernel(const alb_scene_t* __restrict__ scenes, int32_t n_scenes,
                                                              const int64_t* __restrict__ offsets, alb_rng_t rng,
                                                              ReplayIdx ix, unsigned long long* work,
                                                              int32_t refill_interval, double* acc,
