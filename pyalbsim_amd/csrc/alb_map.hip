@@ -252,14 +252,26 @@ __device__ __forceinline__ void child_box(const alb_map_t& m, const Node* __rest
     b[0] = b[3] = m.x[lo];
     b[1] = b[4] = m.y[lo];
     b[2] = b[5] = m.z[lo];
-    for (int32_t r = lo + 1; r <= hi; ++r) {
-      const float x = m.x[r], y = m.y[r], z = m.z[r];
-      b[0] = fminf(b[0], x);
-      b[1] = fminf(b[1], y);
-      b[2] = fminf(b[2], z);
-      b[3] = fmaxf(b[3], x);
-      b[4] = fmaxf(b[4], y);
-      b[5] = fmaxf(b[5], z);
+    // eight records per round, their loads issued together (a record past hi
+    // repeats record lo, which changes no extreme)
+    for (int32_t r0 = lo + 1; r0 <= hi; r0 += 8) {
+      float x[8], y[8], z[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int32_t r = r0 + u <= hi ? r0 + u : lo;
+        x[u] = m.x[r];
+        y[u] = m.y[r];
+        z[u] = m.z[r];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        b[0] = fminf(b[0], x[u]);
+        b[1] = fminf(b[1], y[u]);
+        b[2] = fminf(b[2], z[u]);
+        b[3] = fmaxf(b[3], x[u]);
+        b[4] = fmaxf(b[4], y[u]);
+        b[5] = fmaxf(b[5], z[u]);
+      }
     }
   } else {
     const Node& cn = nodes[c];
