@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define ALB_ABI_VERSION 10
+#define ALB_ABI_VERSION 11
 
 /* photon types, reference enum order (BOTTOM_REFLECTION = auto() = 1, ...) */
 enum alb_photon_type {
@@ -538,16 +538,21 @@ int alb_replay_index_build(const int64_t* tags, const int32_t* scenes, int64_t n
 
 /* Replay the photons of alb_trace_batched (same scenes, offsets, rng, pass)
  * with the path statistics above, look every record / query up in index[4]
- * and add, per scene s, acc[5 s + ...] += weight * (n, [scatter record], L,
- * m, G) of its prefix.  Nothing is stored per path.  work_counter as in
- * alb_trace_batched; counters[ALB_CTR_REPLAY_MISSES] counts events whose
- * record was not in the index (0 when the sinks did not overflow).
- * Single constant-layer scenes only (layers == NULL): the statistics are
- * those of one homogeneous water column. */
+ * and add, per scene s, acc[6 s + ...] += weight * (n, [scatter record], L,
+ * m, G, d) of its prefix, where d counts the completed free paths ended by a
+ * scatter (their density; n counts the omega0 factors).  Nothing is stored
+ * per path.  stat_layer = -1 counts the whole water column; stat_layer = l
+ * counts only layer l of a layered column of constant layers (L: the water
+ * length in layer l; n: omega0 factors of layer l; d and [scatter record]:
+ * free paths ended in a sub-step of layer l; m and G: whole path), which
+ * gives the gradients of each layer's absorption and scattering (a single
+ * constant layer is layer 0).  work_counter as in alb_trace_batched;
+ * counters[ALB_CTR_REPLAY_MISSES] counts events whose record was not in the
+ * index (0 when the sinks did not overflow). */
 int alb_replay_batched(const alb_scene_t* scenes, int32_t n_scenes, const int64_t* photon_offsets,
                        const alb_rng_t* rng, int32_t pass, const alb_replay_index_t* index,
                        unsigned long long* work_counter, int32_t block, int32_t refill_interval,
-                       double* acc, unsigned long long* counters, void* stream);
+                       double* acc, unsigned long long* counters, int32_t stat_layer, void* stream);
 
 #ifdef __cplusplus
 }

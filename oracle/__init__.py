@@ -57,7 +57,7 @@ def lib():
             L.orc_uniform_at.restype = C.c_double
             L.orc_trace.argtypes = [P, P, C.c_int32, C.c_int64, C.c_int64, P, P, P]
             L.orc_trace.restype = C.c_int64
-            L.orc_trace_stats.argtypes = [P, P, C.c_int32, C.c_int64, C.c_int64, P, P, P, P]
+            L.orc_trace_stats.argtypes = [P, P, C.c_int32, C.c_int64, C.c_int64, P, P, P, P, C.c_int32]
             L.orc_trace_stats.restype = C.c_int64
             L.orc_knn_one.argtypes = [P, P, P, C.c_int64, C.c_double, C.c_double, C.c_double,
                                       C.c_int32, P, P]
@@ -279,19 +279,25 @@ def _alloc_sink(capacity: int, query: bool, points: bool = False):
 
 
 class _StatsSink(C.Structure):
-    _fields_ = [("lam", C.c_void_p), ("gs", C.c_void_p), ("nsc", C.c_void_p), ("nfl", C.c_void_p)]
+    _fields_ = [("lam", C.c_void_p), ("gs", C.c_void_p), ("nsc", C.c_void_p), ("nfl", C.c_void_p),
+                ("dsc", C.c_void_p), ("rdl", C.c_void_p)]
 
 
 def trace(model, key, pass_: int, photon_begin: int, n: int, replay: np.ndarray | None = None,
           stream: int | None = None, per_photon: int = 12, stats: bool = False,
-          phase_score: np.ndarray | None = None, points: bool = False, heat: dict | None = None):
+          phase_score: np.ndarray | None = None, points: bool = False, heat: dict | None = None,
+          stat_layer: int = -1):
     """Trace n photons on the CPU; returns (sinks, counters, n_exhausted).
 
     sinks[t] is a dict of NumPy arrays (records for the forward pass, queries
     for the backward pass) truncated to the emitted count.  stats: also return
     each record's adjoint path statistics ("lam" water length whose optical
     depth was accumulated, "gs" phase-sampling score sum with the per-segment
-    scores phase_score, "nsc" scatters, "nfl" floor reflections before it).
+    scores phase_score, "nsc" scatters whose omega0 was applied, "dsc"
+    completed free paths ended by a scatter, "nfl" floor reflections before
+    it, and "rdl" 1 for a scatter record whose own free path counts).
+    stat_layer >= 0 counts the water length and the scatters of that layer of
+    a layered column only (a single constant layer is layer 0).
     points: forward records also carry the interaction points (fwi_*, sfi_*).
     heat: optional {"bins", "water_extent", "seafloor_extent"}; receives the
     forward scatter-radius counts under "radius" ([bins, bins]).
@@ -322,14 +328,15 @@ def trace(model, key, pass_: int, photon_begin: int, n: int, replay: np.ndarray 
         ctr = np.zeros(abi.N_COUNTERS, dtype=np.uint64)
         if stats:
             st_arrs = [{"lam": np.zeros(cap), "gs": np.zeros(cap), "nsc": np.zeros(cap, dtype=np.int32),
-                        "nfl": np.zeros(cap, dtype=np.int32)} for _ in range(4)]
+                        "nfl": np.zeros(cap, dtype=np.int32), "dsc": np.zeros(cap, dtype=np.int32),
+                        "rdl": np.zeros(cap, dtype=np.int32)} for _ in range(4)]
             ss = (_StatsSink * 4)()
             for t in range(4):
-                for f in ("lam", "gs", "nsc", "nfl"):
+                for f in ("lam", "gs", "nsc", "nfl", "dsc", "rdl"):
                     setattr(ss[t], f, _ptr(st_arrs[t][f]))
             ex = lib().orc_trace_stats(C.byref(hs.s), C.byref(rng), pass_, photon_begin, n,
                                        C.cast(rec_arr, C.c_void_p), C.cast(q_arr, C.c_void_p), _ptr(ctr),
-                                       C.cast(ss, C.c_void_p))
+                                       C.cast(ss, C.c_void_p), int(stat_layer))
         else:
             ex = lib().orc_trace(C.byref(hs.s), C.byref(rng), pass_, photon_begin, n,
                                  C.cast(rec_arr, C.c_void_p), C.cast(q_arr, C.c_void_p), _ptr(ctr))
@@ -345,7 +352,7 @@ def trace(model, key, pass_: int, photon_begin: int, n: int, replay: np.ndarray 
         if points and not query:
             d.update({f: arrs[f][:c].copy() for f in abi.INTERACTION_FIELDS})
         if stats:
-            d.update({f: st_arrs[t][f][:c].copy() for f in ("lam", "gs", "nsc", "nfl")})
+            d.update({f: st_arrs[t][f][:c].copy() for f in ("lam", "gs", "nsc", "nfl", "dsc", "rdl")})
         out.append(d)
     if heat is not None:
         b = hs.heat["bins"]

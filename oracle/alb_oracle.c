@@ -318,7 +318,7 @@ int orc_refract(const double d[3], const double n[3], double eta, double out[3])
  * weighted by omega0 and floor reflections so far. */
 typedef struct orc_path {
   double lam, gs;
-  int32_t nsc, nfl;
+  int32_t nsc, nfl, dsc;  /* dsc: completed free paths ended by a scatter */
 } orc_path;
 
 /* optional per-record statistics, parallel to one sink */
@@ -327,6 +327,8 @@ typedef struct orc_stats_sink {
   double* gs;
   int32_t* nsc;
   int32_t* nfl;
+  int32_t* dsc;
+  int32_t* rdl;  /* 1: a scatter record whose own free path ended in a counted layer */
 } orc_stats_sink;
 
 /* Per-call state of one trace: the optional statistics sinks and the state
@@ -336,17 +338,22 @@ typedef struct orc_stats_sink {
  * from several threads do not share anything mutable. */
 typedef struct orc_ctx {
   const orc_stats_sink* stats;  /* [4] or NULL */
+  int stat_layer;               /* -1: every layer counts; l: only layer l (water length, scatters) */
   orc_path path;
   float fwi[3], sfi[3];
 } orc_ctx;
 
-static void orc_emit_stats(const orc_ctx* cx, int type, unsigned long long i, int64_t capacity) {
+static int orc_stat_in(const orc_ctx* cx, int layer) { return cx->stat_layer < 0 || cx->stat_layer == layer; }
+
+static void orc_emit_stats(const orc_ctx* cx, int type, unsigned long long i, int64_t capacity, int rdl) {
   const orc_stats_sink* st = cx->stats;
   if (!st || !st[type].lam || (int64_t)i >= capacity) return;
   st[type].lam[i] = cx->path.lam;
   st[type].gs[i] = cx->path.gs;
   st[type].nsc[i] = cx->path.nsc;
   st[type].nfl[i] = cx->path.nfl;
+  if (st[type].dsc) st[type].dsc[i] = cx->path.dsc;
+  if (st[type].rdl) st[type].rdl[i] = rdl;
 }
 
 /* stored with every forward record whose sink holds the interaction points */
@@ -535,6 +542,7 @@ static int orc_trace_photon(orc_ctx* cx, const alb_scene_t* sc, const alb_rng_t*
   cx->path.gs = 0.0;
   cx->path.nsc = 0;
   cx->path.nfl = 0;
+  cx->path.dsc = 0;
   const double y_ws = sc->y_ws, y_sf = sc->y_sf;
   const double up[3] = {0.0, 1.0, 0.0}, down[3] = {0.0, -1.0, 0.0};
   const double alpha = (double)sc->alpha_l;
@@ -582,15 +590,15 @@ static int orc_trace_photon(orc_ctx* cx, const alb_scene_t* sc, const alb_rng_t*
       /* attenuation at the pre-step position (the lookup above) */
       tau = (float)((double)tau + (layered ? look[2] : alpha) * len);
       int scatter = tau >= tau_t;
-      if (!scatter) cx->path.lam += len;
+      const int li = layered ? (int)look[0] : 0;  /* the sub-step's layer (pre-step position) */
+      if (orc_stat_in(cx, li)) cx->path.lam += len;
       p[0] = nx[0]; p[1] = nx[1]; p[2] = nx[2];
       if (ctr) ctr[ALB_CTR_WATER_STEPS]++;
       if (scatter) {
-        cx->path.lam += len;
         if (fwd) orc_emit_record(cx, &out[ALB_SCATTER], p, d, E, t_evt, tag);
         else orc_emit_query(&q[ALB_SCATTER], p, d, E, t_evt, tag);
         orc_emit_stats(cx, ALB_SCATTER, (fwd ? *out[ALB_SCATTER].count : *q[ALB_SCATTER].count) - 1,
-                       fwd ? out[ALB_SCATTER].capacity : q[ALB_SCATTER].capacity);
+                       fwd ? out[ALB_SCATTER].capacity : q[ALB_SCATTER].capacity, orc_stat_in(cx, li));
         event_no++;
         if (ctr) ctr[ALB_CTR_SCATTER]++;
         /* sample_scattering_directions_batch (physics/scatter/direction.py:7-51) */
@@ -598,12 +606,14 @@ static int orc_trace_photon(orc_ctx* cx, const alb_scene_t* sc, const alb_rng_t*
         double u2 = orc_uniform(&st);
         double nd[3];
         float w0 = sc->omega0;
+        int li2 = 0;  /* the layer whose omega0 applies (new position) */
         if (layered) {
           /* phase function and albedo of the layer at the new position
            * (core/simulation.py:362-368, water.py:372-403) */
           double lk[4];
           orc_layer_lookup(sc, (p[1] - y_ws) * -1.0, lk);
-          const alb_layer_t* L = &sc->layers[(int)lk[0]];
+          li2 = (int)lk[0];
+          const alb_layer_t* L = &sc->layers[li2];
           orc_scatter_direction_tab(L->phase_theta, L->phase_cdf, L->phase_n, d, u1, u2, nd);
           w0 = (float)lk[3];
           /* phase-sampling score (one constant layer: its table is the scene's) */
@@ -612,7 +622,8 @@ static int orc_trace_photon(orc_ctx* cx, const alb_scene_t* sc, const alb_rng_t*
           orc_scatter_direction(sc, d, u1, u2, nd);
           if (sc->phase_score) cx->path.gs += sc->phase_score[orc_phase_segment(u1, sc)];
         }
-        cx->path.nsc++;
+        if (orc_stat_in(cx, li2)) cx->path.nsc++;
+        if (orc_stat_in(cx, li)) cx->path.dsc++;
         d[0] = nd[0]; d[1] = nd[1]; d[2] = nd[2];
         E = E * w0;
         tau = tau - tau_t;
@@ -633,14 +644,14 @@ static int orc_trace_photon(orc_ctx* cx, const alb_scene_t* sc, const alb_rng_t*
       if (fwd) {
         orc_emit_record(cx, &out[ALB_SURFACE_REFLECTION], P, d, E, t_evt, tag);
         orc_emit_stats(cx, ALB_SURFACE_REFLECTION, *out[ALB_SURFACE_REFLECTION].count - 1,
-                       out[ALB_SURFACE_REFLECTION].capacity);
+                       out[ALB_SURFACE_REFLECTION].capacity, 0);
       } else {
         orc_emit_query(&q[ALB_SURFACE_REFLECTION], P, d, E, t_evt, tag);
         orc_emit_query(&q[ALB_SURFACE_TRANSMISSION_UP], P, d, E, t_evt, tag);
         orc_emit_stats(cx, ALB_SURFACE_REFLECTION, *q[ALB_SURFACE_REFLECTION].count - 1,
-                       q[ALB_SURFACE_REFLECTION].capacity);
+                       q[ALB_SURFACE_REFLECTION].capacity, 0);
         orc_emit_stats(cx, ALB_SURFACE_TRANSMISSION_UP, *q[ALB_SURFACE_TRANSMISSION_UP].count - 1,
-                       q[ALB_SURFACE_TRANSMISSION_UP].capacity);
+                       q[ALB_SURFACE_TRANSMISSION_UP].capacity, 0);
       }
       event_no++;
       if (ctr) ctr[ALB_CTR_ENTER]++;
@@ -666,7 +677,7 @@ static int orc_trace_photon(orc_ctx* cx, const alb_scene_t* sc, const alb_rng_t*
       if (!reflect && fwd) {
         orc_emit_record(cx, &out[ALB_SURFACE_TRANSMISSION_UP], P, d, E, t_evt, tag);
         orc_emit_stats(cx, ALB_SURFACE_TRANSMISSION_UP, *out[ALB_SURFACE_TRANSMISSION_UP].count - 1,
-                       out[ALB_SURFACE_TRANSMISSION_UP].capacity);
+                       out[ALB_SURFACE_TRANSMISSION_UP].capacity, 0);
       }
       event_no++;
       if (ctr) ctr[ALB_CTR_EXIT]++;
@@ -684,13 +695,15 @@ static int orc_trace_photon(orc_ctx* cx, const alb_scene_t* sc, const alb_rng_t*
       if (reflect) {
         /* attenuation at the position after the reflection (core/simulation.py:495-499) */
         double a_post = alpha;
+        int lp = 0;
         if (layered) {
           double lk[4];
           orc_layer_lookup(sc, (p[1] - y_ws) * -1.0, lk);
           a_post = lk[2];
+          lp = (int)lk[0];
         }
         tau = (float)((double)tau + a_post * len);
-        cx->path.lam += len;
+        if (orc_stat_in(cx, lp)) cx->path.lam += len;
       }
     } else { /* ST_FLOOR */
       /* handle_seafloor (core/simulation.py:504-570) */
@@ -707,7 +720,7 @@ static int orc_trace_photon(orc_ctx* cx, const alb_scene_t* sc, const alb_rng_t*
       else orc_emit_query(&q[ALB_BOTTOM_REFLECTION], P, d, E, t_evt, tag);
       orc_emit_stats(cx, ALB_BOTTOM_REFLECTION,
                      (fwd ? *out[ALB_BOTTOM_REFLECTION].count : *q[ALB_BOTTOM_REFLECTION].count) - 1,
-                     fwd ? out[ALB_BOTTOM_REFLECTION].capacity : q[ALB_BOTTOM_REFLECTION].capacity);
+                     fwd ? out[ALB_BOTTOM_REFLECTION].capacity : q[ALB_BOTTOM_REFLECTION].capacity, 0);
       event_no++;
       if (ctr) ctr[ALB_CTR_FLOOR]++;
       /* cosine-weighted hemisphere about +y (physics/reflection/lambertian.py:7-27) */
@@ -724,13 +737,15 @@ static int orc_trace_photon(orc_ctx* cx, const alb_scene_t* sc, const alb_rng_t*
       for (int i = 0; i < 3; ++i) p[i] = P[i] + d[i] * rem;
       /* attenuation at the position after the reflection (core/simulation.py:563-566) */
       double a_post = alpha;
+      int lp = 0;
       if (layered) {
         double lk[4];
         orc_layer_lookup(sc, (p[1] - y_ws) * -1.0, lk);
         a_post = lk[2];
+        lp = (int)lk[0];
       }
       tau = (float)((double)tau + a_post * len);
-      cx->path.lam += len;
+      if (orc_stat_in(cx, lp)) cx->path.lam += len;
       cx->path.nfl++;
       E = E * (float)sc->floor_albedo;
     }
@@ -744,10 +759,11 @@ static int orc_trace_photon(orc_ctx* cx, const alb_scene_t* sc, const alb_rng_t*
  * replay stream ran out. */
 static int64_t orc_trace_ctx(const alb_scene_t* sc, const alb_rng_t* rng, int32_t pass, int64_t photon_begin,
                              int64_t n, const alb_records_t* out, const alb_queries_t* q,
-                             unsigned long long* counters, const orc_stats_sink* stats) {
+                             unsigned long long* counters, const orc_stats_sink* stats, int stat_layer) {
   orc_ctx cx;
   memset(&cx, 0, sizeof cx);
   cx.stats = stats;
+  cx.stat_layer = stat_layer;
   int64_t exhausted = 0;
   for (int64_t i = 0; i < n; ++i) {
     int ex = orc_trace_photon(&cx, sc, rng, pass, (uint64_t)(photon_begin + i), i, out, q, counters);
@@ -762,16 +778,17 @@ static int64_t orc_trace_ctx(const alb_scene_t* sc, const alb_rng_t* rng, int32_
 int64_t orc_trace(const alb_scene_t* sc, const alb_rng_t* rng, int32_t pass,
                   int64_t photon_begin, int64_t n, const alb_records_t* out,
                   const alb_queries_t* q, unsigned long long* counters) {
-  return orc_trace_ctx(sc, rng, pass, photon_begin, n, out, q, counters, NULL);
+  return orc_trace_ctx(sc, rng, pass, photon_begin, n, out, q, counters, NULL, -1);
 }
 
 /* orc_trace with the adjoint path statistics of every record / query
- * written into stats[type] at the record's slot (test of the replay trace). */
+ * written into stats[type] at the record's slot (test of the replay trace);
+ * stat_layer >= 0 counts the water length and scatters of that layer only. */
 int64_t orc_trace_stats(const alb_scene_t* sc, const alb_rng_t* rng, int32_t pass,
                         int64_t photon_begin, int64_t n, const alb_records_t* out,
                         const alb_queries_t* q, unsigned long long* counters,
-                        const orc_stats_sink* stats) {
-  return orc_trace_ctx(sc, rng, pass, photon_begin, n, out, q, counters, stats);
+                        const orc_stats_sink* stats, int32_t stat_layer) {
+  return orc_trace_ctx(sc, rng, pass, photon_begin, n, out, q, counters, stats, stat_layer);
 }
 
 /* --------------------------------------------------------------- k-NN */

@@ -5,7 +5,7 @@ import ctypes as C
 
 import numpy as np
 
-ALB_ABI_VERSION = 10
+ALB_ABI_VERSION = 11
 KNN_WORK_WORDS = 256  # ALB_KNN_WORK_WORDS: scratch words of alb_knn / alb_knn_batched
 GATHER_STATIC_MAX_QUERIES = 1 << 20  # ALB_GATHER_STATIC_MAX_QUERIES: queries per alb_gather_static call
 MAX_K = 448  # largest photon_mapping_k the k-NN kernels take (LDS candidate buffer of 512)

@@ -54,6 +54,7 @@ from .model import (SCATTER_DIVISIONS, henyey_greenstein_backscatter, henyey_gre
 from .sweep import shot_config
 
 PARAMS = ("a", "b", "g", "depth", "albedo", "roughness")
+REPLAY_SUMS = 6  # per-scene sums of alb_replay_batched
 
 
 def phase_g_tables(g: float, h: float = 1e-5):
@@ -149,13 +150,13 @@ def _replay_index(torch, sink, count, offsets, dev):
     return keys, slots, ws
 
 
-def adjoint_sums(res, dloss):
-    """Per-scene sums from a batched forward result kept with adjoint=True.
+def adjoint_weights(res, dloss):
+    """Adjoint weights of a batched forward result kept with adjoint=True.
 
     dloss: flat float64 device tensor laid out like res.waveforms.  Returns
-    (acc [S, 5] = sums of w (n, [scatter], L, m, G) over records and queries,
-    direct [S, 3] = sums of w d log mult for (g, alpha^2, albedo * albedo)).
-    """
+    (weights, direct): weights = (per-type record weights, per-type query
+    weights), device tensors indexed by sink slot; direct [S, 3] = sums of
+    w d log mult for (g, alpha^2, albedo * albedo)."""
     import torch
 
     st = res.state
@@ -178,46 +179,153 @@ def adjoint_sums(res, dloss):
                                              runtime._p(rw["knorm"]), runtime._p(dl), runtime._p(st["wf_off"]),
                                              runtime._p(a_rec[t]), runtime._p(a_q[t]), runtime._p(direct), s),
                        "alb_adjoint_contrib")
-    acc = torch.zeros(5 * n, dtype=torch.float64, device=dev)
+    return (a_rec, a_q), direct.view(n, 3).cpu().numpy()
+
+
+def replay_sums(res, weights, stat_layer: int = -1) -> np.ndarray:
+    """Per-scene sums [S, 6] of w (n, [scatter], L, m, G, d) over records and
+    queries, by path replay (alb_replay_batched): n omega0 factors, [scatter]
+    a scatter record's own free-path density, L water length, m floor
+    reflections, G phase-sampling score, d completed free paths ended by a
+    scatter.  stat_layer = l restricts n, [scatter], L and d to layer l of
+    layered columns (a single constant layer is layer 0); -1 counts all."""
+    import torch
+
+    st = res.state
+    L = runtime.load_library()
+    tab = st["table"]
+    n = tab.n
+    dev = res.waveforms.device
+    s = C.c_void_p(runtime._stream_ptr())
+    a_rec, a_q = weights
+    acc = torch.zeros(REPLAY_SUMS * n, dtype=torch.float64, device=dev)
     ctr = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device=dev)
     keep = []
-    for pass_, sinks, counts, weights, nph in ((abi.PASS_FORWARD, st["records"], st["rcounts"], a_rec, st["nf"]),
-                                               (abi.PASS_BACKWARD, st["queries"], st["qcounts"], a_q, st["nb"])):
+    for pass_, sinks, counts, wts, nph in ((abi.PASS_FORWARD, st["records"], st["rcounts"], a_rec, st["nf"]),
+                                           (abi.PASS_BACKWARD, st["queries"], st["qcounts"], a_q, st["nb"])):
         offsets = torch.as_tensor(np.concatenate([[0], np.cumsum(nph)]).astype(np.int64), device=dev)
         idx = (abi.ReplayIndex * 4)()
         for t in range(4):
-            keys, slots, ws = _replay_index(torch, sinks[t], counts[t], offsets, dev)
-            keep.append((keys, slots, ws))
+            key = ("replay_index", pass_, t)
+            if key not in st:  # the sorted index of a sink serves every replay of this result
+                st[key] = _replay_index(torch, sinks[t], counts[t], offsets, dev)
+            keys, slots, _ = st[key]
             idx[t].keys = runtime._p(keys)
             idx[t].slots = runtime._p(slots)
             idx[t].n = counts[t] if t in st["rows"] else 0
-            idx[t].weight = runtime._p(weights[t])
+            idx[t].weight = runtime._p(wts[t])
         rng = abi.make_rng(st["key"], pass_ | (int(st["base"]) << 1))
         work = torch.zeros(1, dtype=torch.int64, device=dev)
         runtime._check(L.alb_replay_batched(runtime._p(tab.dev), n, runtime._p(offsets), C.byref(rng), pass_,
                                             C.cast(idx, C.c_void_p), runtime._p(work), st["block"], st["refill"],
-                                            runtime._p(acc), runtime._p(ctr), s), "alb_replay_batched")
+                                            runtime._p(acc), runtime._p(ctr), int(stat_layer), s),
+                       "alb_replay_batched")
         keep.append((offsets, work))
     misses = int(ctr[abi.CTR_REPLAY_MISSES].item())
     if misses:
         raise runtime.AlbError(f"path replay did not find {misses} records")
     del keep
-    return acc.view(n, 5).cpu().numpy(), direct.view(n, 3).cpu().numpy()
+    return acc.view(n, REPLAY_SUMS).cpu().numpy()
+
+
+def adjoint_sums(res, dloss):
+    """Per-scene sums from a batched forward result kept with adjoint=True.
+
+    dloss: flat float64 device tensor laid out like res.waveforms.  Returns
+    (acc [S, 6] = replay_sums over the whole column, direct [S, 3] = sums of
+    w d log mult for (g, alpha^2, albedo * albedo)).
+    """
+    weights, direct = adjoint_weights(res, dloss)
+    return replay_sums(res, weights), direct
 
 
 def gradients_from_sums(acc, direct, derivs) -> np.ndarray:
     """theta-gradients [S, 6] (depth column left at 0) from the per-scene sums."""
     out = np.zeros((len(derivs), 6))
     for s, d in enumerate(derivs):
-        c0, cs, c2, c3, c4 = acc[s]
+        c0, cs, c2, c3, c4, c5 = acc[s]
         alpha, a = d.alpha, d.a
-        dl_dalpha = cs / alpha + c0 / (alpha - a) - c2
+        # free-path densities (alpha e^{-alpha L}) and omega0 = (alpha - a) / alpha factors
+        dl_dalpha = (cs + c5) / alpha + c0 * (1.0 / (alpha - a) - 1.0 / alpha) - c2
         out[s, 0] = dl_dalpha * d.dalpha[0] - c0 / (alpha - a)
         out[s, 1] = dl_dalpha * d.dalpha[1]
         out[s, 2] = dl_dalpha * d.dalpha[2] + c4 + direct[s, 0]
         out[s, 4] = (c3 + direct[s, 2]) / d.albedo
         out[s, 5] = direct[s, 1] * 4.0 * d.roughness ** 3  # alpha^2 = r^4
     return out
+
+
+def layer_derivatives(model):
+    """Per layer of a layered column of constant layers: (alpha, a,
+    d alpha / d a, d alpha / d b), alpha the float32 lidar attenuation the
+    kernels use, by central differences of the Churnside formula with the
+    layer's salinity and backscatter fraction."""
+    cfg = model.config
+    H = cfg.scene.flying_height
+    lam = cfg.laser.wavelength
+    div = cfg.laser.divergence_angle
+    out = []
+    for layer, lm in zip(cfg.water.layers, model.layers):
+        a, b = float(layer.absorption_coefficient), float(layer.scattering_coefficient)
+
+        def al(aa, bb):
+            return lidar_attenuation(aa, bb, layer.salinity, lam, H, div, lm.backscatter)
+
+        ha, hb = 1e-6 * max(abs(a), 1e-3), 1e-6 * max(abs(b), 1e-3)
+        out.append((float(np.float32(lm.alpha64)), a, (al(a + ha, b) - al(a - ha, b)) / (2 * ha),
+                    (al(a, b + hb) - al(a, b - hb)) / (2 * hb)))
+    return out
+
+
+def layered_gradients(res, dloss, models):
+    """Loss gradients of scenes whose water column is a stack of constant
+    layers (the reference's TurbidityLayerConfig list, physics/models/
+    water.py:253-416), from a batched forward result kept with adjoint=True.
+
+    Returns one dict per scene: "a" and "b" (per layer, d loss / d
+    absorption and scattering coefficient), "albedo" and "roughness".  One
+    replay per layer restricts the water length and scatter statistics to
+    that layer; omega0 counts at the scatter's new position and free-path
+    densities at the sub-step's layer, as the trace applies them.  A scene
+    with a single constant layer gives layer 0 (same values as
+    gradients_from_sums).  The gather's phase mass is the layer's fixed
+    Fournier-Forand table, so there is no phase parameter."""
+    weights, direct = adjoint_weights(res, dloss)
+    layers = [m.layers if m.layers else None for m in models]
+    n_layers = max(len(x) if x else 1 for x in layers)
+    sums = [replay_sums(res, weights, l) for l in range(n_layers)]
+    out = []
+    for s, m in enumerate(models):
+        if m.layers and any(lm.profiled for lm in m.layers):
+            raise NotImplementedError("layered gradients need constant layers (no depth profiles)")
+        derivs = layer_derivatives(m) if m.layers else [(float(m.alpha_l), float(m.absorption),
+                                                          *_single_layer_dalpha(m))]
+        ga, gb = [], []
+        for l, (alpha, a, da_a, da_b) in enumerate(derivs):
+            c0, cs, c2, _, _, c5 = sums[l][s]
+            dl_dalpha = (cs + c5) / alpha + c0 * (1.0 / (alpha - a) - 1.0 / alpha) - c2
+            ga.append(dl_dalpha * da_a - c0 / (alpha - a))
+            gb.append(dl_dalpha * da_b)
+        albedo = m.floor_albedo
+        rough = m.roughness
+        out.append({"a": np.array(ga), "b": np.array(gb),
+                    "albedo": (sums[0][s, 3] + direct[s, 2]) / albedo,
+                    "roughness": direct[s, 1] * 4.0 * rough ** 3})
+    return out
+
+
+def _single_layer_dalpha(m):
+    layer = m.config.water.layers[0]
+    cfg = m.config
+    a, b = float(layer.absorption_coefficient), float(layer.scattering_coefficient)
+    bp = m.backscatter
+
+    def al(aa, bb):
+        return lidar_attenuation(aa, bb, layer.salinity, cfg.laser.wavelength, cfg.scene.flying_height,
+                                 cfg.laser.divergence_angle, bp)
+
+    ha, hb = 1e-6 * max(abs(a), 1e-3), 1e-6 * max(abs(b), 1e-3)
+    return (al(a + ha, b) - al(a - ha, b)) / (2 * ha), (al(a, b + hb) - al(a, b - hb)) / (2 * hb)
 
 
 class BatchedWaveformFn:
@@ -334,3 +442,82 @@ def batched_waveforms(theta, n_forward, n_backward, key, **spec):
         _FN = BatchedWaveformFn.build()
     spec = dict(spec, n_forward=n_forward, n_backward=n_backward, key=key)
     return _FN.apply(theta, spec)
+
+
+def layered_config(base, row):
+    """base (a SimulationConfig whose water column is a stack of constant
+    layers) with the layers' absorption and scattering coefficients and the
+    floor albedo and surface roughness taken from row = (a_0 .. a_{L-1},
+    b_0 .. b_{L-1}, albedo, roughness)."""
+    import dataclasses
+
+    from .config import SeaFloorConfig, SeaSurfaceConfig, WaterConfig
+
+    layers = base.water.layers
+    nl = len(layers)
+    row = [float(v) for v in row]
+    if len(row) != 2 * nl + 2:
+        raise ValueError(f"expected {2 * nl + 2} parameters for {nl} layers, got {len(row)}")
+    new = tuple(dataclasses.replace(ly, absorption_coefficient=row[i], scattering_coefficient=row[nl + i])
+                for i, ly in enumerate(layers))
+    return dataclasses.replace(base, water=dataclasses.replace(base.water, layers=new),
+                               sea_floor=dataclasses.replace(base.sea_floor, albedo=row[2 * nl]),
+                               sea_surface=dataclasses.replace(base.sea_surface, roughness=row[2 * nl + 1]))
+
+
+class LayeredWaveformFn:
+    """torch.autograd.Function over the batched form for layered columns:
+    params [S, 2 L + 2] (float64; per scene (a_0 .. a_{L-1}, b_0 .. b_{L-1},
+    albedo, roughness) of the base configuration spec["configs"][s], all with
+    L constant layers) -> gated waveforms [S, 4, win_len]; the backward pass
+    is layered_gradients (path replay, one replay per layer)."""
+
+    @staticmethod
+    def build():
+        import torch
+
+        from .model import build_scene
+
+        class _Fn(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, params, spec):
+                from .execution.batched import simulate_batched
+
+                p = params.detach().cpu().numpy().astype(np.float64)
+                models = [build_scene(layered_config(cfg, p[s])) for s, cfg in enumerate(spec["configs"])]
+                wl = spec.get("win_len", 1000)
+                res = simulate_batched(models, spec["n_forward"], spec["n_backward"], spec["key"],
+                                       scene_ids=_scene_ids(spec, len(models)), win_len=wl, strict=False,
+                                       adjoint=True, block=spec.get("block", 256), refill=spec.get("refill", 4))
+                ctx.res, ctx.models, ctx.wl, ctx.device = res, models, wl, params.device
+                return torch.stack([res.gated(s, models[s].gate(wl)) for s in range(len(models))])
+
+            @staticmethod
+            def backward(ctx, grad_out):
+                res, models, wl = ctx.res, ctx.models, ctx.wl
+                dloss = torch.zeros_like(res.waveforms)
+                go = grad_out.to(res.waveforms.device, torch.float64)
+                for s, m in enumerate(models):
+                    g0, gl = m.gate(wl)
+                    o, nb = int(res.offsets[s]), m.n_bins
+                    dloss[o:o + 4 * nb].view(4, nb)[:, g0:g0 + gl] = go[s]
+                g = layered_gradients(res, dloss, models)
+                ctx.res = None
+                rows = [np.concatenate([x["a"], x["b"], [x["albedo"], x["roughness"]]]) for x in g]
+                return torch.as_tensor(np.array(rows), dtype=torch.float64, device=ctx.device), None
+
+        return _Fn
+
+
+_LFN = None
+
+
+def layered_waveforms(params, configs, n_forward, n_backward, key, **spec):
+    """Differentiable gated waveforms [S, 4, win_len] of layered columns:
+    params [S, 2 L + 2] = (a_0 .. a_{L-1}, b_0 .. b_{L-1}, albedo, roughness)
+    applied to the base configurations configs[s]."""
+    global _LFN
+    if _LFN is None:
+        _LFN = LayeredWaveformFn.build()
+    spec = dict(spec, configs=list(configs), n_forward=n_forward, n_backward=n_backward, key=key)
+    return _LFN.apply(params, spec)

@@ -94,9 +94,12 @@ __global__ __launch_bounds__(256) void adjoint_contrib_kernel(AdjointArgs a) {
       ch = 2;
       dl = 1.0;  // the host divides by the albedo
     } else if (a.type == ALB_SCATTER) {
-      const int b = scatter_bin(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, S->phase_n);
-      mul = S->phase_mass[b];
-      if (S->phase_dlog_mass) {
+      // the phase masses of the layer at the query vertex, as the gather takes them
+      int pn;
+      const double* pm = gather_phase_mass(*S, a.q.y[qg], &pn);
+      const int b = scatter_bin(m.dx[i], m.dy[i], m.dz[i], sdx, sdy, sdz, pn);
+      mul = pm[b];
+      if (S->phase_dlog_mass) {  // HG scenes: one table for the column
         ch = 0;
         dl = S->phase_dlog_mass[b];
       }

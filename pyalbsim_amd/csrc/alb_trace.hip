@@ -213,9 +213,12 @@ struct Photon {
   uint32_t event_no;  // events so far (record tag)
   bool alive;
   // path statistics of the replay (adjoint) trace only
+  // (restricted to one layer of a layered column when the replay asks for
+  // it: lam, nsc and dsc then count only that layer's part)
   double lam;         // water path length whose optical depth was accumulated
   double gs;          // sum of phase-sampling scores d log p / dg
   int nsc, nfl;       // scatters (omega0 applied) and floor reflections so far
+  int dsc;            // completed free paths ended by a scatter (their density)
   // heat maps only: first water interaction and latest seafloor interaction
   // (float32, NaN before the interaction; core/simulation.py:85-86, 393, 521)
   float fx, fy, fz, bx, by, bz;
@@ -227,9 +230,10 @@ struct Emit {
   double x, y, z, dx, dy, dz;
   float E, t;
   uint32_t eno;
-  // replay trace: the path statistics of the record's prefix
+  // replay trace: the path statistics of the record's prefix, and rdl = 1 for
+  // a scatter record whose own free-path density counts (its sub-step's layer)
   double lam, gs;
-  int nsc, nfl;
+  int nsc, nfl, dsc, rdl;
   // heat maps: the photon's interaction points at the event
   float fx, fy, fz, bx, by, bz;
 };
@@ -241,6 +245,7 @@ __device__ __forceinline__ void snapshot(Emit& e, const Photon& ph) {
     e.gs = ph.gs;
     e.nsc = ph.nsc;
     e.nfl = ph.nfl;
+    e.dsc = ph.dsc;
   }
   if constexpr (HEAT) {
     e.fx = ph.fx; e.fy = ph.fy; e.fz = ph.fz;
@@ -371,9 +376,12 @@ __device__ __forceinline__ int64_t first_step_through_plane(const alb_scene_t& s
 // surface reflection from the layer of the position after the event
 // (core/simulation.py:495-499, 563-566).  Positions are updated by the same
 // repeated additions, so they match the reference's bit for bit.
+// does layer l count in the replay's statistics (stat_layer < 0: every layer)
+__device__ __forceinline__ bool stat_in(int stat_layer, int l) { return stat_layer < 0 || stat_layer == l; }
+
 template <bool STATS, bool HEAT = false>
 __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, const alb_rng_t& rng, Stream& st,
-                                              Photon& ph, Emit& e, Tally& c) {
+                                              Photon& ph, Emit& e, Tally& c, int stat_layer) {
   const double two_pi = 2.0 * ALB_PI;
   const int64_t S = sc.n_substeps;
   const double y_ws = sc.y_ws, y_sf = sc.y_sf;
@@ -386,7 +394,8 @@ __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, c
     }
     const double y = ph.py;
     const double dep = (y - y_ws) * -1.0;
-    const alb_layer_t& L = Ls[layer_of(Ls, nl, dep)];
+    const int li = layer_of(Ls, nl, dep);
+    const alb_layer_t& L = Ls[li];
     const double zl = dep - L.top;
     const double v = layer_velocity(L, zl);
     const double nx = ph.px + (ph.dx * v) * sc.dt, ny = ph.py + (ph.dy * v) * sc.dt, nz = ph.pz + (ph.dz * v) * sc.dt;
@@ -414,7 +423,9 @@ __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, c
       const float a32 = layer_alpha32(L, zl);
       ph.tau = (float)((double)ph.tau + (double)a32 * slen);
       c.ws++;
-      if constexpr (STATS) ph.lam += slen;
+      if constexpr (STATS) {
+        if (stat_in(stat_layer, li)) ph.lam += slen;
+      }
       const bool scat = ph.tau >= ph.tau_t;
       ph.px = nx;
       ph.py = ny;
@@ -431,8 +442,10 @@ __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, c
       e.eno = ph.event_no++;
       snapshot<STATS, HEAT>(e, ph);
       const double dep2 = (ph.py - y_ws) * -1.0;
-      const alb_layer_t& L2 = Ls[layer_of(Ls, nl, dep2)];
+      const int li2 = layer_of(Ls, nl, dep2);
+      const alb_layer_t& L2 = Ls[li2];
       const double zl2 = dep2 - L2.top;
+      if constexpr (STATS) e.rdl = stat_in(stat_layer, li) ? 1 : 0;  // the free path ended in the sub-step's layer
       const double u = uniform(rng, st);
       double theta;
       if constexpr (STATS) {  // phase-sampling score of the adjoint (one constant layer: the scene's table)
@@ -456,7 +469,10 @@ __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, c
       ph.dx = (double)(float)w0;
       ph.dy = (double)(float)w1;
       ph.dz = (double)(float)w2;
-      if constexpr (STATS) ph.nsc++;
+      if constexpr (STATS) {  // omega0 of the new position's layer; the density of the sub-step's
+        if (stat_in(stat_layer, li2)) ph.nsc++;
+        if (stat_in(stat_layer, li)) ph.dsc++;
+      }
       ph.E = ph.E * layer_omega0(L2, zl2);
       ph.tau = ph.tau - ph.tau_t;
       ph.tau_t = tau_target(uniform(rng, st));
@@ -502,10 +518,11 @@ __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, c
       ph.py = Py + ph.dy * rem;
       ph.pz = Pz + ph.dz * rem;
       const double dep2 = (ph.py - y_ws) * -1.0;
-      const alb_layer_t& L2 = Ls[layer_of(Ls, nl, dep2)];
+      const int li2 = layer_of(Ls, nl, dep2);
+      const alb_layer_t& L2 = Ls[li2];
       ph.tau = (float)((double)ph.tau + (double)layer_alpha32(L2, dep2 - L2.top) * slen);
       if constexpr (STATS) {
-        ph.lam += slen;
+        if (stat_in(stat_layer, li2)) ph.lam += slen;
         ph.nfl++;
       }
       ph.E = ph.E * (float)sc.floor_albedo;
@@ -539,9 +556,12 @@ __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, c
     ph.pz = Pz + ph.dz * rem;
     if (reflect) {
       const double dep2 = (ph.py - y_ws) * -1.0;
-      const alb_layer_t& L2 = Ls[layer_of(Ls, nl, dep2)];
+      const int li2 = layer_of(Ls, nl, dep2);
+      const alb_layer_t& L2 = Ls[li2];
       ph.tau = (float)((double)ph.tau + (double)layer_alpha32(L2, dep2 - L2.top) * slen);
-      if constexpr (STATS) ph.lam += slen;
+      if constexpr (STATS) {
+        if (stat_in(stat_layer, li2)) ph.lam += slen;
+      }
     }
     ph.s++;
     break;
@@ -558,10 +578,12 @@ __device__ __forceinline__ void water_stepped(const alb_scene_t& sc, bool fwd, c
 // phase-sampling score) and snapshot them into e; the physics is unchanged.
 // LAYERED: the scene may be a layered column or have a tilted floor (checked
 // at run time); without it the kernel carries only the flat floor's
-// closed-form water legs.
+// closed-form water legs.  stat_layer >= 0 restricts the water length and
+// scatter statistics to that layer of a layered column (a single constant
+// layer is layer 0).
 template <bool STATS, bool LAYERED = true, bool HEAT = false>
 __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, const alb_rng_t& rng, Stream& st,
-                                           Photon& ph, Emit& e, Tally& c) {
+                                           Photon& ph, Emit& e, Tally& c, int stat_layer = -1) {
   const double two_pi = 2.0 * ALB_PI;
   const int64_t S = sc.n_substeps;
   const double y_ws = sc.y_ws, y_sf = sc.y_sf;
@@ -615,7 +637,7 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
     ph.alive = false;  // below the floor and not rising: the reference finds no event
     return;
   } else if (LAYERED && sc.layers) {
-    water_stepped<STATS, HEAT>(sc, fwd, rng, st, ph, e, c);
+    water_stepped<STATS, HEAT>(sc, fwd, rng, st, ph, e, c, stat_layer);
     return;
   } else {
     // ---------------------------------------------------------- water leg
@@ -673,15 +695,22 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
       e.E = ph.E;
       e.t = (float)(ph.s + j);
       e.eno = ph.event_no++;
-      if constexpr (STATS) ph.lam += a1 * len;
+      const bool in0 = stat_in(stat_layer, 0);  // the closed-form legs: one constant layer
+      if constexpr (STATS) {
+        if (in0) ph.lam += a1 * len;
+      }
       snapshot<STATS, HEAT>(e, ph);
+      if constexpr (STATS) e.rdl = in0 ? 1 : 0;
       const double u = uniform(rng, st);
       double theta;
       if constexpr (STATS) {
         int seg;
         theta = interp_seg(u, sc.phase_cdf, sc.phase_theta, sc.phase_n, &seg);
         if (sc.phase_score) ph.gs += sc.phase_score[seg];
-        ph.nsc++;
+        if (in0) {
+          ph.nsc++;
+          ph.dsc++;
+        }
       } else {
         theta = interp(u, sc.phase_cdf, sc.phase_theta, sc.phase_n);
       }
@@ -708,7 +737,9 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
       const double nx = qx + Dx, ny = qy + Dy, nz = qz + Dz;
       const double svx = nx - qx, svy = ny - qy, svz = nz - qz;
       const double slen = norm3(svx, svy, svz);
-      if constexpr (STATS) ph.lam += (double)jb * len;
+      if constexpr (STATS) {
+        if (stat_in(stat_layer, 0)) ph.lam += (double)jb * len;
+      }
       if (to_floor) {
         // handle_seafloor (core/simulation.py:504-570)
         c.fl++;
@@ -729,7 +760,7 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
         e.eno = ph.event_no++;
         snapshot<STATS, HEAT>(e, ph);
         if constexpr (STATS) {
-          ph.lam += slen;
+          if (stat_in(stat_layer, 0)) ph.lam += slen;
           ph.nfl++;
         }
         const double u1 = uniform(rng, st);
@@ -771,7 +802,7 @@ __device__ __forceinline__ void next_event(const alb_scene_t& sc, bool fwd, cons
           snapshot<STATS, HEAT>(e, ph);
         }
         if constexpr (STATS) {
-          if (reflect) ph.lam += slen;
+          if (reflect && stat_in(stat_layer, 0)) ph.lam += slen;
         }
         if (reflect) {
           ph.dy = -ph.dy;
@@ -988,19 +1019,21 @@ struct ReplayIdx {
 };
 
 // Per-scene sums of weight * (n, [scatter], L, m, G) of the lane's records.
+constexpr int kReplaySums = 6;  // per scene: nsc, [scatter record], lam, nfl, gs, dsc (alb_replay_batched)
+
 struct ReplayAcc {
-  double v[5];
+  double v[kReplaySums];
   int32_t sid;
 };
 
 __device__ __forceinline__ void flush_acc(ReplayAcc& a, double* acc) {
   if (a.sid >= 0) {
 #pragma unroll
-    for (int c = 0; c < 5; ++c)
-      if (a.v[c] != 0.0) unsafeAtomicAdd(&acc[5 * (int64_t)a.sid + c], a.v[c]);
+    for (int c = 0; c < kReplaySums; ++c)
+      if (a.v[c] != 0.0) unsafeAtomicAdd(&acc[kReplaySums * (int64_t)a.sid + c], a.v[c]);
   }
 #pragma unroll
-  for (int c = 0; c < 5; ++c) a.v[c] = 0.0;
+  for (int c = 0; c < kReplaySums; ++c) a.v[c] = 0.0;
 }
 
 // Path replay for the adjoint: the photons of trace_batched_kernel, traced
@@ -1014,7 +1047,7 @@ __global__ __launch_bounds__(256, 3) void replay_batched_kernel(const alb_scene_
                                                              const int64_t* __restrict__ offsets, alb_rng_t rng,
                                                              ReplayIdx ix, unsigned long long* work,
                                                              int32_t refill_interval, double* acc,
-                                                             unsigned long long* ctr) {
+                                                             unsigned long long* ctr, int32_t stat_layer) {
   const uint32_t lane = lane_id();
   const int64_t total = offsets[n_scenes];
   Photon ph;
@@ -1027,7 +1060,7 @@ __global__ __launch_bounds__(256, 3) void replay_batched_kernel(const alb_scene_
   Stream st{0, 0, 0u, false, 0.0};
   alb_rng_t lrng = rng;
   Tally c{};
-  ReplayAcc a{{0.0, 0.0, 0.0, 0.0, 0.0}, -1};
+  ReplayAcc a{{0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, -1};
   unsigned long long misses = 0;
   for (uint32_t it = 0;; ++it) {
     if (it % (uint32_t)refill_interval == 0) {
@@ -1064,14 +1097,15 @@ __global__ __launch_bounds__(256, 3) void replay_batched_kernel(const alb_scene_
             ph.gs = 0.0;
             ph.nsc = 0;
             ph.nfl = 0;
+            ph.dsc = 0;
           }
         }
       }
     }
     if (!ballot(ph.alive || !done)) break;
     if (ph.alive) {
-      Emit e{-1, 0, 0, 0, 0, 0, 0, 0.0f, 0.0f, 0u, 0.0, 0.0, 0, 0};
-      next_event<true, true>(scenes[sid], FWD, lrng, st, ph, e, c);  // tilted floors too
+      Emit e{-1, 0, 0, 0, 0, 0, 0, 0.0f, 0.0f, 0u, 0.0, 0.0, 0, 0, 0, 0};
+      next_event<true, true>(scenes[sid], FWD, lrng, st, ph, e, c, stat_layer);  // tilted floors too
       if (e.et >= 0) {
         const uint64_t key = ((uint64_t)g << 16) | (uint64_t)(e.eno & 0xFFFFu);
 #pragma unroll
@@ -1084,10 +1118,11 @@ __global__ __launch_bounds__(256, 3) void replay_batched_kernel(const alb_scene_
             const double A = ix.t[t].weight[ix.t[t].slots[p]];
             if (A != 0.0) {
               a.v[0] += A * (double)e.nsc;
-              if (t == ALB_SCATTER) a.v[1] += A;
+              if (t == ALB_SCATTER && e.rdl) a.v[1] += A;
               a.v[2] += A * e.lam;
               a.v[3] += A * (double)e.nfl;
               a.v[4] += A * e.gs;
+              a.v[5] += A * (double)e.dsc;
             }
           } else {
             misses++;
@@ -1206,10 +1241,11 @@ extern "C" int alb_trace_batched(const alb_scene_t* scenes, int32_t n_scenes, co
 extern "C" int alb_replay_batched(const alb_scene_t* scenes, int32_t n_scenes, const int64_t* photon_offsets,
                                   const alb_rng_t* rng, int32_t pass, const alb_replay_index_t* index,
                                   unsigned long long* work_counter, int32_t block, int32_t refill_interval,
-                                  double* acc, unsigned long long* counters, void* stream) {
+                                  double* acc, unsigned long long* counters, int32_t stat_layer, void* stream) {
   using namespace alb;
   if (!scenes || !photon_offsets || !rng || !index || !work_counter || !acc)
     return set_error(ALB_ERR_ARG, "alb_replay_batched: null argument");
+  if (stat_layer < -1) return set_error(ALB_ERR_ARG, "alb_replay_batched: stat_layer must be -1 or a layer index");
   if (n_scenes <= 0 || n_scenes > (1 << 20)) return set_error(ALB_ERR_ARG, "alb_replay_batched: bad scene count");
   if (pass != ALB_PASS_FORWARD && pass != ALB_PASS_BACKWARD)
     return set_error(ALB_ERR_ARG, "alb_replay_batched: pass must be 0 (forward) or 1 (backward)");
@@ -1237,9 +1273,9 @@ extern "C" int alb_replay_batched(const alb_scene_t* scenes, int32_t n_scenes, c
   hipStream_t s = (hipStream_t)stream;
   if (pass == ALB_PASS_FORWARD)
     hipLaunchKernelGGL(replay_batched_kernel<true>, dim3(grid), dim3(block), 0, s, scenes, n_scenes, photon_offsets,
-                       *rng, ix, work_counter, refill_interval, acc, counters);
+                       *rng, ix, work_counter, refill_interval, acc, counters, stat_layer);
   else
     hipLaunchKernelGGL(replay_batched_kernel<false>, dim3(grid), dim3(block), 0, s, scenes, n_scenes, photon_offsets,
-                       *rng, ix, work_counter, refill_interval, acc, counters);
+                       *rng, ix, work_counter, refill_interval, acc, counters, stat_layer);
   return check_launch("alb_replay_batched");
 }

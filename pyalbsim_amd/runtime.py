@@ -91,7 +91,7 @@ def load_library(path: str = LIB_PATH):
     L.alb_replay_index_workspace_size.restype = C.c_int
     L.alb_replay_index_build.argtypes = [P, P, C.c_int64, P, P, P, P, C.c_size_t, P]
     L.alb_replay_index_build.restype = C.c_int
-    L.alb_replay_batched.argtypes = [P, C.c_int32, P, P, C.c_int32, P, P, C.c_int32, C.c_int32, P, P, P]
+    L.alb_replay_batched.argtypes = [P, C.c_int32, P, P, C.c_int32, P, P, C.c_int32, C.c_int32, P, P, C.c_int32, P]
     L.alb_replay_batched.restype = C.c_int
     if L.alb_abi_version() != abi.ALB_ABI_VERSION:
         raise AlbError("libalbsim_hip.so ABI version mismatch")
@@ -535,8 +535,8 @@ class DeviceSceneTable:
             th, cd, ms, sc, dm = self.tables[h]
             structs[i] = abi.make_scene(m, _p(th), _p(cd), _p(ms), _p(sc), _p(dm))
             if m.layers:
-                if self.adjoint and (len(m.layers) > 1 or m.layers[0].profiled):
-                    raise NotImplementedError("the adjoint covers single constant-layer scenes only")
+                if self.adjoint and any(lm.profiled for lm in m.layers):
+                    raise NotImplementedError("the adjoint covers columns of constant layers (no depth profiles)")
                 if id(m) not in self.layer_tables:
                     self.layer_tables[id(m)] = (m, _upload_layers(m, device))
                 structs[i].n_layers = len(m.layers)

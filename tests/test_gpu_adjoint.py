@@ -186,7 +186,7 @@ def test_autograd_function_and_depth(gpu):
     assert np.allclose(th1.grad.cpu().numpy()[:, 3], (up - base) / (1e-3 * THETAS[:2, 3]), rtol=1e-9)
 
 
-def _replay_sums(res, weights_by_type, pass_):
+def _replay_sums(res, weights_by_type, pass_, stat_layer=-1):
     """Run the replay trace of one pass with the given per-slot weights."""
     import ctypes as C
 
@@ -208,17 +208,17 @@ def _replay_sums(res, weights_by_type, pass_):
         keep.append((keys, slots, ws, w))
         idx[t].keys, idx[t].slots, idx[t].n, idx[t].weight = runtime._p(keys), runtime._p(slots), counts[t], \
             runtime._p(w)
-    acc = torch.zeros(5 * tab.n, dtype=torch.float64, device="cuda")
+    acc = torch.zeros(adjoint.REPLAY_SUMS * tab.n, dtype=torch.float64, device="cuda")
     ctr = torch.zeros(abi.N_COUNTERS, dtype=torch.int64, device="cuda")
     work = torch.zeros(1, dtype=torch.int64, device="cuda")
     rng = abi.make_rng(st["key"], pass_)
     rc = L.alb_replay_batched(runtime._p(tab.dev), tab.n, runtime._p(offsets), C.byref(rng), pass_,
                               C.cast(idx, C.c_void_p), runtime._p(work), 128, 2, runtime._p(acc), runtime._p(ctr),
-                              C.c_void_p(runtime._stream_ptr()))
+                              int(stat_layer), C.c_void_p(runtime._stream_ptr()))
     assert rc == 0
     torch.cuda.synchronize()
     assert int(ctr[abi.CTR_REPLAY_MISSES].item()) == 0
-    return acc.view(tab.n, 5).cpu().numpy()
+    return acc.view(tab.n, adjoint.REPLAY_SUMS).cpu().numpy()
 
 
 @pytest.mark.parametrize("tilt", [None, (9.0, 120.0)])
@@ -247,7 +247,7 @@ def test_replay_statistics_match_time_stepped_oracle(gpu, pass_, tilt):
         v = sinks[t].view(counts[t])
         gpu_rec.append({"tag": v["tag"].cpu().numpy(), "scene": v["scene"].cpu().numpy(),
                         "y": v["y"].cpu().numpy().astype(np.float64)})
-    expect = np.zeros((2, 5))
+    expect = np.zeros((2, adjoint.REPLAY_SUMS))
     weights = [np.zeros(max(counts[t], 1)) for t in range(4)]
     checked = 0
     for s, m in enumerate(models):
@@ -275,8 +275,8 @@ def test_replay_statistics_match_time_stepped_oracle(gpu, pass_, tilt):
                 w = rng.uniform(0.5, 1.5)
                 weights[t][i] = w
                 c = cpu[t]
-                expect[s] += w * np.array([c["nsc"][j], 1.0 if t == abi.SCATTER else 0.0, c["lam"][j], c["nfl"][j],
-                                           c["gs"][j]])
+                expect[s] += w * np.array([c["nsc"][j], float(c["rdl"][j]) if t == abi.SCATTER else 0.0, c["lam"][j],
+                                           c["nfl"][j], c["gs"][j], c["dsc"][j]])
                 checked += 1
         assert len(bad) <= 0.01 * nph[s] + 2
     assert checked > 1000
