@@ -196,6 +196,10 @@ def main(argv=None):
     ap.add_argument("--inversion-full-iteration", action="store_true",
                     help="also time one inversion iteration over 4096 scenes at 2e4 photons per scene "
                          "(and scale it to 2e5 photons x 200 iterations)")
+    ap.add_argument("--inversion-full-photons", type=float, default=2e4,
+                    help="forward photons per scene of --inversion-full-iteration (2e5: the workload's count)")
+    ap.add_argument("--inversion-full-group", type=int, default=256)
+    ap.add_argument("--inversion-full-iters", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 2 GiB device copy at the end (reference reading for HBM byte counters)")
     args = ap.parse_args(argv)
@@ -345,13 +349,15 @@ def main(argv=None):
     if not args.no_batched:
         if args.sweep_full > 0:
             leg("batched", bench_batched_full, args.sweep_full)
-        leg("batched_small", bench_batched, args.sweep_scenes)
+        if args.sweep_scenes > 0:
+            leg("batched_small", bench_batched, args.sweep_scenes)
     if not args.no_adjoint:
         leg("adjoint", bench_adjoint, args.adjoint_scenes, int(args.adjoint_photons))
     if not args.no_inversion:
         leg("inversion", bench_inversion, args.inversion_scenes, args.inversion_iters, lr=args.inversion_lr)
         if args.inversion_full_iteration:
-            leg("inversion_full_iteration", bench_inversion_full_iteration)
+            leg("inversion_full_iteration", bench_inversion_full_iteration, n_forward=int(args.inversion_full_photons),
+                group=args.inversion_full_group, iterations=args.inversion_full_iters)
     if args.copy_calibration:
         x = torch.empty(1 << 28, dtype=torch.float64, device="cuda")
         y = torch.empty_like(x)
@@ -442,22 +448,24 @@ def bench_inversion(n_scenes: int, iters: int, n_forward: int = 20000, lr: float
             "batched_s_per_iteration": sb, "plain_s_per_iteration": sp, "speedup": sp / sb}
 
 
-def bench_inversion_full_iteration(n_scenes: int = 4096, n_forward: int = 20000, group: int = 256):
-    """One Adam iteration over every inversion scene (batched, in groups of
-    `group` scenes); seconds per iteration at n_forward photons and the
-    estimate for 2e5 photons (the work is linear in photons) and 200
+def bench_inversion_full_iteration(n_scenes: int = 4096, n_forward: int = 20000, group: int = 256,
+                                   iterations: int = 2):
+    """Adam iterations over every inversion scene (batched, in groups of
+    `group` scenes); seconds of the last iteration at n_forward photons and
+    the estimate for 2e5 photons (the work is linear in photons) and 200
     iterations."""
     from pyalbsim_amd.inversion import invert, inversion_scenes, inversion_tilts
 
     th = inversion_scenes(n_scenes, seed=3)
     tilts = inversion_tilts(n_scenes, seed=3)
-    r = invert(th, n_forward, n_forward // 4, iterations=2, lr=1e-2, tilts=tilts, group_scenes=group,
+    r = invert(th, n_forward, n_forward // 4, iterations=iterations, lr=1e-2, tilts=tilts, group_scenes=group,
                log=lambda m: print(m, file=sys.stderr, flush=True))
     s_it = float(r.seconds_per_iteration[-1])
     est = s_it * (2e5 / n_forward)
     print("inversion full iteration done", file=sys.stderr, flush=True)
+    timed = "the second timed" if iterations > 1 else "timed, with the HIP-graph capture of the groups that keep graphs"
     return {"workload": f"{n_scenes} inversion scenes, {n_forward} forward + {n_forward // 4} backward photons per "
-                        f"scene, 2 Adam iterations (the second timed) in groups of {group} scenes",
+                        f"scene, {iterations} Adam iteration(s) ({timed}) in groups of {group} scenes",
             "s_per_iteration": s_it, "estimated_s_per_iteration_at_2e5_photons": est,
             "estimated_s_for_200_iterations_at_2e5_photons": 200 * est,
             "depth_rmse": r.depth_rmse}
@@ -523,7 +531,8 @@ def bench_batched_full(n_scenes: int):
     nb = [x.n_backward for x in sc]
     paths = sum(nf) + sum(nb)
     key = (0xBA7C, 1)
-    block, refill, _ = tune_batched([x.model for x in sc], nf, key)
+    # the tuning pilot: a sixteenth of each scene's photons, at most 5e6 in all
+    block, refill, _ = tune_batched([x.model for x in sc], nf, key, pilot_fraction=min(1 / 16, 5e6 / sum(nf)))
     torch.cuda.synchronize()
     res = run_sweep(sc, key, block=block, refill=refill, strict=False,
                     log=lambda m: print(m, file=sys.stderr, flush=True))
