@@ -432,11 +432,12 @@ int alb_gather_static(const alb_scene_t* scene, const alb_map_t* map, int32_t ph
 int alb_gather_order(const alb_map_t* map, const alb_queries_t* q, int64_t q_begin, int64_t q_n,
                      int32_t* order, void* workspace, size_t workspace_bytes, void* stream);
 #define ALB_KNN_WORK_WORDS 256
-/* Largest k that alb_knn, alb_knn_batched, alb_gather and alb_gather_static
- * take (the reference takes any photon_mapping_k, config/simulation.py:23-29):
- * the candidates live in a per-wavefront LDS buffer of 512 entries, and a
- * buffer cut must free 64 of them.  Larger k fails with ALB_ERR_UNSUPPORTED. */
-#define ALB_MAX_K 448
+/* Largest k that alb_knn, alb_knn_batched and alb_gather take (the
+ * reference takes any photon_mapping_k, config/simulation.py:23-29): the
+ * candidates live in a per-wavefront LDS buffer of 256, 512 or 1024 entries
+ * (k <= 192, <= 448, <= 960), and a buffer cut must free 64 of them.  Larger
+ * k fails with ALB_ERR_UNSUPPORTED; alb_gather_static takes k <= 448. */
+#define ALB_MAX_K 960
 int alb_knn(const alb_map_t* map, int32_t photon_type, int32_t k, const alb_queries_t* q,
             int64_t q_begin, const int32_t* order, int64_t c0, int64_t cn, int32_t rows_by_query,
             int32_t* nbr, double* radius, double* knorm, unsigned long long* counters,

@@ -8,7 +8,7 @@ import numpy as np
 ALB_ABI_VERSION = 11
 KNN_WORK_WORDS = 256  # ALB_KNN_WORK_WORDS: scratch words of alb_knn / alb_knn_batched
 GATHER_STATIC_MAX_QUERIES = 1 << 20  # ALB_GATHER_STATIC_MAX_QUERIES: queries per alb_gather_static call
-MAX_K = 448  # largest photon_mapping_k the k-NN kernels take (LDS candidate buffer of 512)
+MAX_K = 960  # largest photon_mapping_k the k-NN kernels take (LDS candidate buffer of 1024)
 GATHER_STATIC_SCRATCH_BYTES = 64 << 20  # ALB_GATHER_STATIC_SCRATCH_BYTES: its static device array per GPU
 N_TYPES = 4
 N_COUNTERS = 19

@@ -50,9 +50,12 @@ constexpr int kStack = 256;       // per-wave node stack (LDS)
 constexpr int kSeed = 192;        // records of the descent seed (<= kBuf)
 constexpr int kBuf = 256;         // per-wave candidate buffer (LDS), power of two
 // k above kMaxK runs a variant with a buffer of kBufLarge entries (and a seed
-// of kBufLarge - 64); a buffer cut must free 64 slots, so k <= buffer - 64
+// of kBufLarge - 64), k above kMaxKLarge one of kBufHuge entries (two waves
+// per block for its LDS); a buffer cut must free 64 slots, so k <= buffer - 64
 constexpr int kBufLarge = 512;
 constexpr int kMaxKLarge = kBufLarge - 64;
+constexpr int kBufHuge = 1024;
+constexpr int kMaxKHuge = kBufHuge - 64;  // the largest k of the gathers
 constexpr int kBucket = 64;       // nodes with <= kBucket records are scanned
 static_assert(kBucket >= kBoxedRecords, "the k-NN reads the boxes of nodes of more than kBucket records");
 constexpr int kPath = 128;        // descent levels kept for the next query (LDS)
@@ -526,7 +529,8 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     uint32_t hist[256];
     int32_t path[kPath];
   };
-  __shared__ WaveLds lds_s[kWaves];
+  constexpr int kW = BUF > kBufLarge ? 2 : kWaves;  // wavefronts per block (LDS)
+  __shared__ WaveLds lds_s[kW];
 
   const uint32_t lane = lane_id();
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: scalar slot loop
@@ -1005,7 +1009,7 @@ __device__ __forceinline__ void stage_row(RowTile& R, int r, const alb_scene_t& 
   R.t[r] = q.time[qg];
 }
 
-// it / d for it < kRowTile * kMaxKLarge < 2^24 (the float quotient is within one
+// it / d for it < kRowTile * kMaxKHuge < 2^24 (the float quotient is within one
 // of it and corrected)
 __device__ __forceinline__ int small_div(int it, int d, float inv_d) {
   int q = (int)(((float)it + 0.5f) * inv_d);
@@ -1315,9 +1319,13 @@ static void launch_knn(const alb_map_t& map, int k, int64_t blocks, const alb::K
   if (k <= kMaxK) {
     if (off32) hipLaunchKernelGGL((knn_kernel<true, kBuf>), g, b, 0, s, ka);
     else hipLaunchKernelGGL((knn_kernel<false, kBuf>), g, b, 0, s, ka);
-  } else {
+  } else if (k <= kMaxKLarge) {
     if (off32) hipLaunchKernelGGL((knn_kernel<true, kBufLarge>), g, b, 0, s, ka);
     else hipLaunchKernelGGL((knn_kernel<false, kBufLarge>), g, b, 0, s, ka);
+  } else {  // two wavefronts per block; the per-XCD dealing takes any block count
+    const dim3 b2(64 * 2);
+    if (off32) hipLaunchKernelGGL((knn_kernel<true, kBufHuge>), g, b2, 0, s, ka);
+    else hipLaunchKernelGGL((knn_kernel<false, kBufHuge>), g, b2, 0, s, ka);
   }
 }
 
@@ -1327,7 +1335,7 @@ static int check_map(const alb_map_t* map, int k, const char* who) {
   if (map->n <= 0) return set_error(ALB_ERR_ARG, "%s: empty photon map", who);
   if (k <= 0 || k > map->n)
     return set_error(ALB_ERR_ARG, "%s: k = %d exceeds the map size %lld", who, k, (long long)map->n);
-  if (k > kMaxKLarge) return set_error(ALB_ERR_UNSUPPORTED, "%s: k = %d above %d", who, k, kMaxKLarge);
+  if (k > kMaxKHuge) return set_error(ALB_ERR_UNSUPPORTED, "%s: k = %d above %d", who, k, kMaxKHuge);
   if (!map->keys || !map->meta || !map->x || (map->n > 1 && !map->nodes))
     return set_error(ALB_ERR_ARG, "%s: map not built", who);
   return ALB_OK;
@@ -1540,6 +1548,8 @@ extern "C" int alb_gather_static(const alb_scene_t* scene, const alb_map_t* map,
     return set_error(ALB_ERR_ARG, "alb_gather_static: %lld queries exceed the maximum of %d per call",
                      (long long)q_n, ALB_GATHER_STATIC_MAX_QUERIES);
   GatherWs L;
+  if (scene->k > kMaxKLarge)
+    return set_error(ALB_ERR_UNSUPPORTED, "alb_gather_static: k = %d above %d", scene->k, kMaxKLarge);
   rc = gather_layout(q_n, scene->k, q_n < kStaticChunk ? q_n : kStaticChunk, &L);
   if (rc != ALB_OK) return rc;
   if (L.total > kStaticBytes)
@@ -1597,8 +1607,8 @@ extern "C" int alb_knn_batched(const alb_map_t* map, const alb_scene_index_t* in
   if (!map || !index || !q || !order || !nbr || !radius || !knorm || !work)
     return set_error(ALB_ERR_ARG, "alb_knn_batched: null argument");
   if (map->n <= 0) return set_error(ALB_ERR_ARG, "alb_knn_batched: empty photon map");
-  if (k <= 0 || k > kMaxKLarge)
-    return set_error(ALB_ERR_UNSUPPORTED, "alb_knn_batched: k = %d outside [1, %d]", k, kMaxKLarge);
+  if (k <= 0 || k > kMaxKHuge)
+    return set_error(ALB_ERR_UNSUPPORTED, "alb_knn_batched: k = %d outside [1, %d]", k, kMaxKHuge);
   if (!map->keys || !map->meta || !map->x || (map->n > 1 && !map->nodes))
     return set_error(ALB_ERR_ARG, "alb_knn_batched: map not built");
   if (!index->first || !index->last || !index->root || !q->scene)
@@ -1632,8 +1642,8 @@ extern "C" int alb_contrib_batched(const alb_scene_t* scenes, int32_t n_scenes, 
   if (n_scenes < 1) return set_error(ALB_ERR_ARG, "alb_contrib_batched: bad scene count");
   if (!q->scene) return set_error(ALB_ERR_ARG, "alb_contrib_batched: queries carry no scene array");
   if (photon_type < 0 || photon_type >= ALB_N_TYPES) return set_error(ALB_ERR_ARG, "alb_contrib_batched: bad photon type");
-  if (k <= 0 || k > kMaxKLarge)
-    return set_error(ALB_ERR_UNSUPPORTED, "alb_contrib_batched: k = %d outside [1, %d]", k, kMaxKLarge);
+  if (k <= 0 || k > kMaxKHuge)
+    return set_error(ALB_ERR_UNSUPPORTED, "alb_contrib_batched: k = %d outside [1, %d]", k, kMaxKHuge);
   if (win_len < 0 || win_len > kMaxWindowLarge)
     return set_error(ALB_ERR_ARG, "alb_contrib_batched: window length %d outside [0, %d]", win_len, kMaxWindowLarge);
   if (n_replicas < 0 || (n_replicas > 0 && (!replicas || nb_max <= 0)))

@@ -84,8 +84,8 @@ def test_k_limit_matches_the_header_and_is_checked_before_device_work():
 
     for k in (abi.MAX_K + 1, 1000):
         model = build_scene(SimulationConfig(photon_mapping_k=k))
-        with pytest.raises(runtime.AlbError, match=r"outside \[1, 448\]"):
+        with pytest.raises(runtime.AlbError, match=rf"outside \[1, {abi.MAX_K}\]"):
             simulate(model, 1000, 250, (1, 2))
-        with pytest.raises(runtime.AlbError, match=r"outside \[1, 448\]"):
+        with pytest.raises(runtime.AlbError, match=rf"outside \[1, {abi.MAX_K}\]"):
             simulate_batched([model], 1000, 250, (1, 2))
     runtime.check_k([build_scene(SimulationConfig(photon_mapping_k=abi.MAX_K))])

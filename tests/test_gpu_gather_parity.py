@@ -162,9 +162,10 @@ def test_gather_in_many_rounds_equals_one_round(gpu, monkeypatch):
         assert np.allclose(x, y, rtol=1e-10, atol=1e-14 * float(np.abs(y).max()))
 
 
-@pytest.mark.parametrize("k", [193, 256, 448])
+@pytest.mark.parametrize("k", [193, 256, 448, 449, 700, 960])
 def test_large_k_sets_bit_exact_and_waveform_matches(gpu, k):
-    """k above 192 runs the k-NN variant with the 512-entry candidate buffer:
+    """k above 192 runs the k-NN variant with the 512-entry candidate buffer,
+    above 448 the one with 1024 entries (two wavefronts per block):
     neighbour sets and k-th distances equal brute force bit for bit, and the
     waveforms the oracle's re-sum."""
     torch = gpu
