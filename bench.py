@@ -459,11 +459,11 @@ def bench_inversion_full_iteration(n_scenes: int = 4096, n_forward: int = 20000,
     th = inversion_scenes(n_scenes, seed=3)
     tilts = inversion_tilts(n_scenes, seed=3)
     r = invert(th, n_forward, n_forward // 4, iterations=iterations, lr=1e-2, tilts=tilts, group_scenes=group,
-               log=lambda m: print(m, file=sys.stderr, flush=True))
+               graphs=iterations > 1, log=lambda m: print(m, file=sys.stderr, flush=True))
     s_it = float(r.seconds_per_iteration[-1])
     est = s_it * (2e5 / n_forward)
     print("inversion full iteration done", file=sys.stderr, flush=True)
-    timed = "the second timed" if iterations > 1 else "timed, with the HIP-graph capture of the groups that keep graphs"
+    timed = "the second timed" if iterations > 1 else "timed; every pass eager, no HIP graphs"
     return {"workload": f"{n_scenes} inversion scenes, {n_forward} forward + {n_forward // 4} backward photons per "
                         f"scene, {iterations} Adam iteration(s) ({timed}) in groups of {group} scenes",
             "s_per_iteration": s_it, "estimated_s_per_iteration_at_2e5_photons": est,

@@ -217,7 +217,7 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
            key=(0x1A7E, 5), target_key=(0x7A26, 6), form: str = "batched", win_len: int = 1000,
            depth_rel: float = 1e-2, x0=None, log=None, shape_weight: float = 1000.0,
            tilts=None, loss: str = "hybrid", start: str = "timing",
-           group_scenes: int = None) -> InversionResult:
+           group_scenes: int = None, graphs: bool = True) -> InversionResult:
     """Adam on (depth, b, albedo) scaled to [0, 1]; returns the estimates and
     the depth RMSE and loss of every iteration.
 
@@ -233,7 +233,9 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
     iteration and replayed after it; a graph keeps its record and query
     sinks, so with several groups a group whose graphs would leave less than
     half of the device memory free runs its passes eagerly instead (the same
-    results: a replay equals the eager trace bit for bit)."""
+    results: a replay equals the eager trace bit for bit).  graphs=False runs
+    every pass eagerly (a single iteration has nothing to replay, and the
+    graphs' sinks would hold memory the next group needs)."""
     import torch
 
     from .adjoint import batched_waveforms, gated_waveforms
@@ -255,12 +257,23 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
     else:  # plain form: one scene per launch
         groups = [(s, s + 1) for s in range(S)]
     specs = [dict(spec, tilts=None if tl is None else tl[a:b], scene_base=a,
-                  graphs={} if form == "batched" else None) for a, b in groups]
+                  graphs={} if (form == "batched" and graphs) else None) for a, b in groups]
+
+    last = [time.perf_counter()]
+
+    def progress(msg):  # at most one line per 20 s, so long runs show their progress
+        if log is not None and time.perf_counter() - last[0] > 20.0:
+            last[0] = time.perf_counter()
+            log(msg)
 
     with torch.no_grad():  # one pass per group: nothing to replay
-        target = torch.cat([gated_waveforms(theta_true[a:b], n_forward, n_backward, target_key,
-                                            **dict(specs[i], graphs=None))
-                            for i, (a, b) in enumerate(groups)])
+        parts = []
+        for i, (a, b) in enumerate(groups):
+            parts.append(gated_waveforms(theta_true[a:b], n_forward, n_backward, target_key,
+                                         **dict(specs[i], graphs=None)))
+            progress(f"targets: group {i + 1}/{len(groups)} done")
+        target = torch.cat(parts)
+        del parts
     x_init = np.full((S, len(UNKNOWN)), 0.5) if x0 is None else np.array(x0, dtype=np.float64)
     if x0 is None and start == "timing":
         from .adjoint import _models
@@ -300,6 +313,7 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
             pg = scene_loss(batched_waveforms(th_g, n_forward, n_backward, key, **specs[i]), target[a:b])
             pg.sum().backward()
             per[a:b] = pg.detach()
+            progress(f"iteration {it + 1}: group {i + 1}/{len(groups)} done")
             if it == 0 and len(groups) > 1 and specs[i]["graphs"] is not None:
                 del pg
                 free, total = _free_and_total()
