@@ -172,17 +172,13 @@ __device__ __forceinline__ int64_t advance_optical_depth(float& tau, float tau_t
     if (c < 1) c = 1;
     while (c > 1 && !(t0 + (double)(c - 1) * d + I < top)) --c;
     while (t0 + (double)c * d + I < top) ++c;
-    // first sub-step reaching the target: t0 + nt*d >= tau_t, nt >= 1; only
-    // whether it comes within the binade's c sub-steps matters, so the count
-    // is capped at c + 2 (the corrections then stay short however far the
-    // target lies)
+    // first sub-step reaching the target: t0 + nt*d >= tau_t, nt >= 1
     int64_t nt = 1;
     if (t0 + d < tt) {
-      const double est = ceil((tt - t0) * rd);
-      nt = est > (double)(c + 2) ? c + 2 : (int64_t)est;
+      nt = (int64_t)ceil((tt - t0) * rd);
       if (nt < 1) nt = 1;
       while (nt > 1 && t0 + (double)(nt - 1) * d >= tt) --nt;
-      while (nt <= c && t0 + (double)nt * d < tt) ++nt;
+      while (t0 + (double)nt * d < tt) ++nt;
     }
     if (nt <= c && nt <= left) {
       tau = (float)(t0 + (double)nt * d);
