@@ -290,28 +290,36 @@ def layered_gradients(res, dloss, models):
     with a single constant layer gives layer 0 (same values as
     gradients_from_sums).  The gather's phase mass is the layer's fixed
     Fournier-Forand table, so there is no phase parameter."""
+    for m in models:
+        if m.layers and any(lm.profiled for lm in m.layers):
+            raise NotImplementedError("layered gradients need constant layers (no depth profiles)")
     weights, direct = adjoint_weights(res, dloss)
-    layers = [m.layers if m.layers else None for m in models]
-    n_layers = max(len(x) if x else 1 for x in layers)
+    n_layers = max(len(m.layers) if m.layers else 1 for m in models)
     sums = [replay_sums(res, weights, l) for l in range(n_layers)]
     out = []
     for s, m in enumerate(models):
-        if m.layers and any(lm.profiled for lm in m.layers):
-            raise NotImplementedError("layered gradients need constant layers (no depth profiles)")
         derivs = layer_derivatives(m) if m.layers else [(float(m.alpha_l), float(m.absorption),
                                                           *_single_layer_dalpha(m))]
-        ga, gb = [], []
-        for l, (alpha, a, da_a, da_b) in enumerate(derivs):
-            c0, cs, c2, _, _, c5 = sums[l][s]
-            dl_dalpha = (cs + c5) / alpha + c0 * (1.0 / (alpha - a) - 1.0 / alpha) - c2
-            ga.append(dl_dalpha * da_a - c0 / (alpha - a))
-            gb.append(dl_dalpha * da_b)
-        albedo = m.floor_albedo
-        rough = m.roughness
-        out.append({"a": np.array(ga), "b": np.array(gb),
-                    "albedo": (sums[0][s, 3] + direct[s, 2]) / albedo,
-                    "roughness": direct[s, 1] * 4.0 * rough ** 3})
+        out.append(layer_gradients_from_sums([sums[l][s] for l in range(len(derivs))], direct[s], derivs,
+                                             m.floor_albedo, m.roughness))
     return out
+
+
+def layer_gradients_from_sums(layer_sums, direct, derivs, albedo: float, roughness: float) -> dict:
+    """One scene's gradients from its per-layer replay sums (layer_sums[l] =
+    replay_sums(..., stat_layer=l)[scene]), its direct sums and, per layer,
+    (alpha, a, d alpha / d a, d alpha / d b):
+    d L / d alpha_l = (d_l + [scatter]_l) / alpha_l + n_l (1 / (alpha_l - a_l)
+    - 1 / alpha_l) - L_l, d L / d a_l = d L / d alpha_l d alpha_l / d a_l
+    - n_l / (alpha_l - a_l), d L / d b_l = d L / d alpha_l d alpha_l / d b_l;
+    the floor reflections m and the direct sums are whole-path terms."""
+    ga, gb = [], []
+    for (c0, cs, c2, _, _, c5), (alpha, a, da_a, da_b) in zip(layer_sums, derivs):
+        dl_dalpha = (cs + c5) / alpha + c0 * (1.0 / (alpha - a) - 1.0 / alpha) - c2
+        ga.append(dl_dalpha * da_a - c0 / (alpha - a))
+        gb.append(dl_dalpha * da_b)
+    return {"a": np.array(ga), "b": np.array(gb), "albedo": (layer_sums[0][3] + direct[2]) / albedo,
+            "roughness": direct[1] * 4.0 * roughness ** 3}
 
 
 def _single_layer_dalpha(m):
