@@ -16,10 +16,11 @@
 //     2012): internal node i covers a contiguous record range; equal keys are
 //     split by index
 //  6. exact float32 bounding boxes of the children of the root and of every
-//     node covering more than kBoxedRecords records, bottom-up: a child's
-//     prefix length is always longer than its parent's, so one launch per
-//     prefix length (longest first) finds every child box already written;
-//     a child of at most kBoxedRecords records takes the box of its records.
+//     node covering more than kBoxedRecords records: first every child of at
+//     most kBoxedRecords records (or a single record) takes the box of its
+//     records, all in one launch; then the other children, bottom-up: a
+//     child's prefix length is always longer than its parent's, so one launch
+//     per prefix length (longest first) finds every child box already written.
 //     The gathers scan such small subtrees as record ranges and never read
 //     their nodes' boxes.
 #include <hipcub/hipcub.hpp>
@@ -238,65 +239,85 @@ __global__ __launch_bounds__(256) void level_scatter_kernel(const uint8_t* __res
   }
 }
 
-// box of one child covering records [lo, hi]: the record itself for a leaf,
-// the records' own box for a subtree of at most kBoxedRecords records (its
-// node has no boxes), else the union of the child node's two child boxes
-// (written by an earlier, longer-prefix level)
-__device__ __forceinline__ void child_box(const alb_map_t& m, const Node* __restrict__ nodes, int32_t c, bool leaf,
-                                          int32_t lo, int32_t hi, float b[6]) {
-  if (leaf) {
-    b[0] = b[3] = m.x[c];
-    b[1] = b[4] = m.y[c];
-    b[2] = b[5] = m.z[c];
-  } else if (hi - lo + 1 <= kBoxedRecords) {
-    b[0] = b[3] = m.x[lo];
-    b[1] = b[4] = m.y[lo];
-    b[2] = b[5] = m.z[lo];
-    // eight records per round, their loads issued together (a record past hi
-    // repeats record lo, which changes no extreme)
-    for (int32_t r0 = lo + 1; r0 <= hi; r0 += 8) {
-      float x[8], y[8], z[8];
+// box of a terminal child covering records [lo, hi]: the record itself for a
+// leaf, else the records' own box (a subtree of at most kBoxedRecords
+// records, whose node has no boxes)
+__device__ __forceinline__ void terminal_box(const alb_map_t& m, int32_t lo, int32_t hi, float b[6]) {
+  b[0] = b[3] = m.x[lo];
+  b[1] = b[4] = m.y[lo];
+  b[2] = b[5] = m.z[lo];
+  // eight records per round, their loads issued together (a record past hi
+  // repeats record lo, which changes no extreme)
+  for (int32_t r0 = lo + 1; r0 <= hi; r0 += 8) {
+    float x[8], y[8], z[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int32_t r = r0 + u <= hi ? r0 + u : lo;
-        x[u] = m.x[r];
-        y[u] = m.y[r];
-        z[u] = m.z[r];
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        b[0] = fminf(b[0], x[u]);
-        b[1] = fminf(b[1], y[u]);
-        b[2] = fminf(b[2], z[u]);
-        b[3] = fmaxf(b[3], x[u]);
-        b[4] = fmaxf(b[4], y[u]);
-        b[5] = fmaxf(b[5], z[u]);
-      }
+    for (int u = 0; u < 8; ++u) {
+      const int32_t r = r0 + u <= hi ? r0 + u : lo;
+      x[u] = m.x[r];
+      y[u] = m.y[r];
+      z[u] = m.z[r];
     }
-  } else {
-    const Node& cn = nodes[c];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      b[a] = fminf(cn.lb[a], cn.rb[a]);
-      b[3 + a] = fmaxf(cn.lb[3 + a], cn.rb[3 + a]);
+    for (int u = 0; u < 8; ++u) {
+      b[0] = fminf(b[0], x[u]);
+      b[1] = fminf(b[1], y[u]);
+      b[2] = fminf(b[2], z[u]);
+      b[3] = fmaxf(b[3], x[u]);
+      b[4] = fmaxf(b[4], y[u]);
+      b[5] = fmaxf(b[5], z[u]);
     }
   }
 }
 
+// child side (0 left, 1 right) of a boxed node: its record range, and whether
+// it is terminal (a single record or a subtree of at most kBoxedRecords)
+__device__ __forceinline__ bool child_range(const Node& nd, int side, int32_t* lo, int32_t* hi) {
+  *lo = side ? nd.split + 1 : nd.first;
+  *hi = side ? nd.last : nd.split;
+  return ((nd.leaf_mask >> side) & 1) != 0 || *hi - *lo + 1 <= kBoxedRecords;
+}
+
+// Terminal child boxes of every boxed node, in one launch: they depend on no
+// other box, so no level order is needed, and the record scans (the bulk of
+// the box work) run with the whole map's parallelism instead of one level's.
+// Thread t takes side t & 1 of boxed node order[t >> 1].
+__global__ __launch_bounds__(256) void box_terminal_kernel(alb_map_t m, const int32_t* __restrict__ order,
+                                                           const uint32_t* __restrict__ offset) {
+  Node* nodes = (Node*)m.nodes;
+  const uint32_t items = 2u * offset[kLevels];
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < items; t += gridDim.x * blockDim.x) {
+    Node& nd = nodes[order[t >> 1]];
+    const int side = (int)(t & 1u);
+    int32_t lo, hi;
+    if (!child_range(nd, side, &lo, &hi)) continue;
+    float b[6];
+    terminal_box(m, lo, hi, b);
+    float* dst = side ? nd.rb : nd.lb;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) dst[c] = b[c];
+  }
+}
+
+// Boxed children of the boxed nodes of one prefix length: the union of the
+// child node's two boxes, written by an earlier launch (a child's prefix is
+// longer than its parent's, and the launches run longest first).
 __global__ __launch_bounds__(256) void box_level_kernel(alb_map_t m, const int32_t* __restrict__ order,
                                                         const uint32_t* __restrict__ offset, int lv) {
   Node* nodes = (Node*)m.nodes;
   const uint32_t beg = offset[lv], end = offset[lv + 1];
   for (uint32_t t = beg + blockIdx.x * blockDim.x + threadIdx.x; t < end; t += gridDim.x * blockDim.x) {
-    const int32_t node = order[t];
-    Node& nd = nodes[node];
-    float a[6], b[6];
-    child_box(m, nodes, nd.split, nd.leaf_mask & 1, nd.first, nd.split, a);
-    child_box(m, nodes, nd.split + 1, nd.leaf_mask & 2, nd.split + 1, nd.last, b);
+    Node& nd = nodes[order[t]];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      nd.lb[c] = a[c];
-      nd.rb[c] = b[c];
+    for (int side = 0; side < 2; ++side) {
+      int32_t lo, hi;
+      if (child_range(nd, side, &lo, &hi)) continue;
+      const Node& cn = nodes[side ? nd.split + 1 : nd.split];
+      float* dst = side ? nd.rb : nd.lb;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        dst[a] = fminf(cn.lb[a], cn.rb[a]);
+        dst[3 + a] = fmaxf(cn.lb[3 + a], cn.rb[3 + a]);
+      }
     }
   }
 }
@@ -465,6 +486,10 @@ static int map_build(const alb_records_t* in, int64_t n, int32_t dims, alb_map_t
     // boxed nodes cover more than kBoxedRecords records: fewer than n / 32
     const unsigned g3b = (unsigned)((n / 32 + B - 1) / B);
     const unsigned g3 = g3b < 1 ? 1 : (g3b < 1024 ? g3b : 1024);
+    // terminal child boxes: two per boxed node
+    const unsigned g4b = (unsigned)((n / 16 + B - 1) / B);
+    const unsigned g4 = g4b < 1 ? 1 : (g4b < 4096 ? g4b : 4096);
+    hipLaunchKernelGGL(box_terminal_kernel, dim3(g4), dim3(B), 0, s, m, order, offset);
     for (int lv = kLevels - 1; lv >= 0; --lv)
       hipLaunchKernelGGL(box_level_kernel, dim3(g3), dim3(B), 0, s, m, order, offset, lv);
   }
