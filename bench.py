@@ -193,6 +193,17 @@ def main(argv=None):
                     help="sweep scenes run at their full photon counts (grouped batched vs plain); 0: none")
     ap.add_argument("--adjoint-scenes", type=int, default=256)
     ap.add_argument("--adjoint-photons", type=float, default=1e6)
+    ap.add_argument("--adjoint-fd-replicas", type=int, default=0,
+                    help="also compare the adjoint's a, b, g gradients with central CRN differences over this "
+                         "many independent replicas of each of the first --adjoint-fd-scenes sweep scenes")
+    ap.add_argument("--adjoint-fd-scenes", type=int, default=2)
+    ap.add_argument("--adjoint-fd-first", type=int, default=0, help="first sweep scene of that comparison")
+    ap.add_argument("--adjoint-fd-photons", type=float, default=1e5, help="forward photons per replica")
+    ap.add_argument("--adjoint-fd-eps", type=float, default=5e-2, help="relative step of the differences")
+    ap.add_argument("--adjoint-fd-params", default="abg", help="parameters compared, of a, b, g")
+    ap.add_argument("--adjoint-fd-loss", choices=("mse", "linear"), default="mse",
+                    help="mse: sum (W - T)^2; linear: sum 2 (W_ref - T) W with W_ref the scene's mean waveform "
+                         "from separate runs (the MSE linearised at the mean: no variance term)")
     ap.add_argument("--inversion-full-iteration", action="store_true",
                     help="also time one inversion iteration over 4096 scenes at 2e4 photons per scene "
                          "(and scale it to 2e5 photons x 200 iterations)")
@@ -353,6 +364,11 @@ def main(argv=None):
             leg("batched_small", bench_batched, args.sweep_scenes)
     if not args.no_adjoint:
         leg("adjoint", bench_adjoint, args.adjoint_scenes, int(args.adjoint_photons))
+        if args.adjoint_fd_replicas > 0:
+            leg("adjoint_vs_differences", bench_adjoint_fd, args.adjoint_fd_scenes, args.adjoint_fd_replicas,
+                first=args.adjoint_fd_first, loss=args.adjoint_fd_loss, nf=int(args.adjoint_fd_photons),
+                group=max(1, int(128 * 1e5 // args.adjoint_fd_photons)), eps=args.adjoint_fd_eps,
+                params=args.adjoint_fd_params)
     if not args.no_inversion:
         leg("inversion", bench_inversion, args.inversion_scenes, args.inversion_iters, lr=args.inversion_lr)
         if args.inversion_full_iteration:
@@ -622,6 +638,126 @@ def bench_adjoint(n_scenes: int, photons: int = 1000000):
             "depth_difference": "one-sided, common random numbers: one more forward pass",
             "parameters": list(adjoint.PARAMS), "groups": len(groups),
             "forward_paths_per_s": n_scenes * (nf + nb) / tf}
+
+
+def bench_adjoint_fd(n_scenes: int, replicas: int, nf: int = 100000, eps: float = 5e-2, group: int = 128,
+                     first: int = 0, loss: str = "mse", params: str = "abg"):
+    """Adjoint a, b, g gradients against central differences with common
+    random numbers, per sweep scene, over many independent replicas.
+
+    Scene s (LHS seed 1) with its MSE target (parameters perturbed by
+    U[-10 %, 10 %], seed 2; 4 x nf photons in 4 averaged runs) is replicated
+    `replicas` times, each replica its own Philox stream, nf forward and nf / 4
+    backward photons, in batched runs of `group` replicas.  Each replica's
+    adjoint gradient is paired with its own central difference at relative
+    step eps (same streams on both sides).  Reported per parameter: the mean
+    adjoint gradient, the mean difference, their relative gap, and the
+    standard error of the paired difference relative to the mean difference.
+    Both estimators are unbiased for the gradient of the expected loss up to
+    O(eps^2) curvature and O(1/N) self-interaction terms, so the gap measures
+    the adjoint's error within that standard error.
+
+    loss="linear" replaces the MSE by sum 2 (W_ref - T) W, W_ref the scene's
+    mean waveform from four separate runs of 4 x nf photons: the MSE
+    linearised at the mean.  Its expected value is linear in the expected
+    waveform, so its gradient has no variance term.  The MSE's expected
+    value also holds the waveform's variance, whose derivative the CRN
+    difference sees and the path-replay gradient (samples held fixed) does
+    not; where a scene's gated signal is carried by few heavy contributions
+    (a floor return inside the gate at 1e5 photons) that term is large."""
+    import numpy as np
+    import torch
+
+    from pyalbsim_amd import adjoint
+    from pyalbsim_amd.execution.batched import simulate_batched
+    from pyalbsim_amd.sweep import hypercube
+
+    nb, wl, key = nf // 4, 1000, (5, 6)
+    th_all = np.array([[x.params[c] for c in adjoint.PARAMS] for x in hypercube(256, 1)[:first + n_scenes]])
+    pert_all = th_all * (1 + np.random.default_rng(2).uniform(-0.1, 0.1, th_all.shape))
+    pert_all[:, 2] = np.clip(pert_all[:, 2], 0.5, 0.97)
+    cols = tuple((c, n) for c, n in ((0, "a"), (1, "b"), (2, "g")) if n in params)
+
+    def losses(theta, ids, T, adj, R=None):
+        models = [adjoint.model_from_theta(theta)] * len(ids)
+        res = simulate_batched(models, nf, nb, key, scene_ids=ids, win_len=wl, strict=False, adjoint=adj)
+        m = models[0]
+        g0, gl = m.gate(wl)
+        L = np.zeros(len(ids))
+        d = torch.zeros_like(res.waveforms) if adj else None
+        for i in range(len(ids)):
+            w = res.gated(i, m.gate(wl))
+            L[i] = float(((w - T) ** 2).sum()) if R is None else float((R * w).sum())
+            if adj:
+                o = int(res.offsets[i])
+                d[o:o + 4 * m.n_bins].view(4, m.n_bins)[:, g0:g0 + gl] = 2 * (w - T) if R is None else R
+        if not adj:
+            return L, None
+        acc, direct = adjoint.adjoint_sums(res, d)
+        derivs = [adjoint.scene_derivatives(m, theta)] * len(ids)
+        return L, adjoint.gradients_from_sums(acc, direct, derivs)
+
+    out = {"workload": f"first {n_scenes} Latin-hypercube sweep scenes (seed 1), MSE over the {wl}-bin gate against "
+                       f"targets from parameters perturbed by U[-10%,10%] (seed 2); {replicas} independent replicas "
+                       f"per scene ({nf} forward + {nb} backward photons each, batched {group} at a time); central "
+                       f"differences with common random numbers at relative step {eps}; loss {loss}",
+           "scenes": []}
+    t0 = time.perf_counter()
+    for s in range(first, first + n_scenes):
+        th = th_all[s]
+        tm = [adjoint.model_from_theta(pert_all[s])] * 4
+        rt = simulate_batched(tm, 4 * nf, nf, (77, 78), win_len=wl, strict=False)
+        T = sum(rt.gated(r, tm[r].gate(wl)) for r in range(4)) / 4
+        del rt
+        R = None
+        if loss == "linear":
+            mm = [adjoint.model_from_theta(th)] * 4
+            rr = simulate_batched(mm, 4 * nf, nf, (79, 80), win_len=wl, strict=False)
+            R = 2 * (sum(rr.gated(r, mm[r].gate(wl)) for r in range(4)) / 4 - T)
+            del rr
+        G, FD = [], {c: [] for c, _ in cols}
+        g0, gs = 0, group
+        while g0 < replicas:
+            ids = range(1 + s * replicas + g0, 1 + s * replicas + min(replicas, g0 + gs))
+            try:
+                gr = losses(th, ids, T, True, R)[1]
+                fdr = {}
+                for c, _ in cols:
+                    tp, tq = th.copy(), th.copy()
+                    tp[c] *= 1 + eps
+                    tq[c] *= 1 - eps
+                    fdr[c] = (losses(tp, ids, T, False, R)[0] - losses(tq, ids, T, False, R)[0]) / (2 * eps * th[c])
+            except torch.OutOfMemoryError:
+                # a turbid scene's kept adjoint state (neighbour rows) outgrew the
+                # device at this group size: the same replicas again, half as many
+                # per batched run (their streams, and so their values, are unchanged)
+                if gs == 1:
+                    raise
+                torch.cuda.empty_cache()
+                gs //= 2
+                continue
+            G.append(gr)
+            for c, _ in cols:
+                FD[c].append(fdr[c])
+            g0 += len(ids)
+            torch.cuda.synchronize()
+            print(f"adjoint vs differences: scene {s}, replicas {g0} of {replicas} ({gs} per run)",
+                  file=sys.stderr, flush=True)
+        G = np.concatenate(G)
+        row = {"scene": s, "theta": [float(v) for v in th]}
+        for c, name in cols:
+            fd = np.concatenate(FD[c])
+            d = G[:, c] - fd
+            mf = float(fd.mean())
+            row[name] = {"adjoint": float(G[:, c].mean()), "difference": mf,
+                         "relative_gap": float(d.mean() / abs(mf)),
+                         "relative_se": float(d.std(ddof=1) / np.sqrt(len(d)) / abs(mf)),
+                         "adjoint_relative_sd_per_replica": float(G[:, c].std(ddof=1) / abs(G[:, c].mean())),
+                         "difference_relative_sd_per_replica": float(fd.std(ddof=1) / abs(mf))}
+        out["scenes"].append(row)
+        print(json.dumps({"adjoint_vs_differences_scene": row}), file=sys.stderr, flush=True)
+    out["seconds"] = time.perf_counter() - t0
+    return out
 
 
 if __name__ == "__main__":
