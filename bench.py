@@ -31,6 +31,10 @@ The other forms of the metric:
     backward) photons, MSE against targets from perturbed parameters: wall
     time of forward + path-replay adjoint over the forward alone, in
     memory-bounded scene groups;
+  * adjoint_vs_differences (opt-in, --adjoint-fd-replicas R): the adjoint's
+    a, b, g gradients against central differences with common random
+    numbers, paired per replica over R independent replicas of each of
+    --adjoint-fd-scenes sweep scenes (MSE or linearised loss);
   * inversion: Adam over the batched adjoint on --inversion-scenes synthetic
     scenes (unknown depth, b, albedo), --inversion-iters iterations; depth
     RMSE and seconds per iteration, batched vs plain; with
@@ -40,6 +44,9 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--forward N]
        [--backward N] [--no-cpu] [--no-batched] [--no-adjoint] [--no-radius]
        [--no-inversion] [--sweep-full N] [--adjoint-scenes N]
        [--inversion-scenes N] [--inversion-full-iteration]
+       [--adjoint-fd-replicas R --adjoint-fd-scenes N --adjoint-fd-first S
+        --adjoint-fd-photons N --adjoint-fd-eps E --adjoint-fd-loss mse|linear
+        --adjoint-fd-params abg]
 """
 from __future__ import annotations
 
