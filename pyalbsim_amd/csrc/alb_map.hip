@@ -136,12 +136,13 @@ __global__ __launch_bounds__(256) void permute_points_kernel(alb_records_t in, a
   m.sfi_x[i] = in.sfi_x[j]; m.sfi_y[i] = in.sfi_y[j]; m.sfi_z[i] = in.sfi_z[j];
 }
 
-// common-prefix length of keys i and j; equal keys continue with the index bits
-__device__ __forceinline__ int delta(const uint64_t* __restrict__ k, int64_t n, int64_t i, int64_t j) {
+// common-prefix length of keys i and j (ki = k[i]); equal keys continue with
+// the index bits
+__device__ __forceinline__ int delta(const uint64_t* __restrict__ k, int64_t n, int64_t i, uint64_t ki, int64_t j) {
   if (j < 0 || j >= n) return -1;
-  const uint64_t a = k[i], b = k[j];
-  if (a == b) return 64 + __clz((uint32_t)i ^ (uint32_t)j);
-  return __clzll(a ^ b);
+  const uint64_t b = k[j];
+  if (ki == b) return 64 + __clz((uint32_t)i ^ (uint32_t)j);
+  return __clzll(ki ^ b);
 }
 
 __device__ __forceinline__ void build_node(const uint64_t* __restrict__ k, int64_t n, int64_t i,
@@ -164,23 +165,26 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(const uint64_t* __restr
 
 __device__ __forceinline__ void build_node(const uint64_t* __restrict__ k, int64_t n, int64_t i,
                                            Node* __restrict__ nodes, uint8_t* __restrict__ level, uint32_t* lh) {
-  const int d = (delta(k, n, i, i + 1) - delta(k, n, i, i - 1)) >= 0 ? 1 : -1;
-  const int dmin = delta(k, n, i, i - d);
+  const uint64_t ki = k[i];
+  const int d = (delta(k, n, i, ki, i + 1) - delta(k, n, i, ki, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta(k, n, i, ki, i - d);
   int64_t lmax = 2;
-  while (delta(k, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  while (delta(k, n, i, ki, i + lmax * d) > dmin) lmax <<= 1;
   int64_t l = 0;
   for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
-    if (delta(k, n, i, i + (l + t) * d) > dmin) l += t;
+    if (delta(k, n, i, ki, i + (l + t) * d) > dmin) l += t;
   const int64_t j = i + l * d;
-  const int dnode = delta(k, n, i, j);
+  const int dnode = delta(k, n, i, ki, j);
+  // split: binary search with steps ceil(l / 2^e), e = 1, 2, ... (a shift, not
+  // a 64-bit division)
   int64_t s = 0;
-  int64_t div = 2;
-  int64_t t = (l + div - 1) / div;
+  int e = 1;
+  int64_t t = (l + 1) >> 1;
   while (true) {
-    if (delta(k, n, i, i + (s + t) * d) > dnode) s += t;
+    if (delta(k, n, i, ki, i + (s + t) * d) > dnode) s += t;
     if (t <= 1) break;
-    div <<= 1;
-    t = (l + div - 1) / div;
+    ++e;
+    t = (l + ((int64_t)1 << e) - 1) >> e;
   }
   const int64_t g = i + s * d + (d < 0 ? -1 : 0);
   const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
