@@ -186,6 +186,30 @@ def test_autograd_function_and_depth(gpu):
     assert np.allclose(th1.grad.cpu().numpy()[:, 3], (up - base) / (1e-3 * THETAS[:2, 3]), rtol=1e-9)
 
 
+def test_independent_run_loss_recipe(gpu):
+    """Gradient of the squared error of the mean waveform: the loss is
+    evaluated on an independent run (another key) and replayed through the
+    first, wf + (wf2 - wf).detach().  Its value is the independent run's loss,
+    and since path replay is linear in dL/dW its gradient equals the first
+    run's gradient for the fixed weights 2 (wf2 - T)."""
+    torch = gpu
+    th = torch.tensor(THETAS[:2], dtype=torch.float64, requires_grad=True)
+    wf = adjoint.batched_waveforms(th, NF, NB, (21, 22), win_len=WL, depth_grad="none")
+    wf2 = adjoint.batched_waveforms(th.detach(), NF, NB, (23, 24), win_len=WL, depth_grad="none")
+    assert not torch.equal(wf.detach(), wf2)  # independent photons
+    T = 0.5 * wf2.detach() + 0.5 * wf.detach().mean()
+    L = ((wf + (wf2 - wf).detach() - T) ** 2).sum()
+    lv = float(L.detach())
+    assert abs(lv - float(((wf2 - T) ** 2).sum())) <= 1e-10 * abs(lv)
+    L.backward()
+    th3 = torch.tensor(THETAS[:2], dtype=torch.float64, requires_grad=True)
+    wf3 = adjoint.batched_waveforms(th3, NF, NB, (21, 22), win_len=WL, depth_grad="none")
+    (wf3 * (2 * (wf2 - T)).detach()).sum().backward()
+    g, g3 = th.grad.cpu().numpy(), th3.grad.cpu().numpy()
+    assert np.all(np.isfinite(g)) and np.any(g != 0)
+    assert np.allclose(g, g3, rtol=1e-9, atol=0)
+
+
 def _replay_sums(res, weights_by_type, pass_, stat_layer=-1):
     """Run the replay trace of one pass with the given per-slot weights."""
     import ctypes as C
