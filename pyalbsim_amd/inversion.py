@@ -217,7 +217,7 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
            key=(0x1A7E, 5), target_key=(0x7A26, 6), form: str = "batched", win_len: int = 1000,
            depth_rel: float = 1e-2, x0=None, log=None, shape_weight: float = 1000.0,
            tilts=None, loss: str = "hybrid", start: str = "timing",
-           group_scenes: int = None, graphs: bool = True) -> InversionResult:
+           group_scenes: int = None, graphs: bool = True, independent_key=None) -> InversionResult:
     """Adam on (depth, b, albedo) scaled to [0, 1]; returns the estimates and
     the depth RMSE and loss of every iteration.
 
@@ -235,7 +235,13 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
     half of the device memory free runs its passes eagerly instead (the same
     results: a replay equals the eager trace bit for bit).  graphs=False runs
     every pass eagerly (a single iteration has nothing to replay, and the
-    graphs' sinks would hold memory the next group needs)."""
+    graphs' sinks would hold memory the next group needs).
+
+    independent_key: evaluate the loss on a second forward run with these
+    Philox keys (same scene streams, other photons) and replay it through the
+    first, w + (w2 - w).detach(): the gradient of the loss of the mean
+    waveform, without the variance term that path replay of a loss that is
+    not linear in w leaves out.  One more forward pass per iteration."""
     import torch
 
     from .adjoint import batched_waveforms, gated_waveforms
@@ -310,7 +316,12 @@ def invert(theta_true: np.ndarray, n_forward: int, n_backward: int, iterations: 
         for i, (a, b) in enumerate(groups):  # forward and backward of one group at a time
             th_g = base[a:b].clone()
             th_g[:, idx] = lo_t + x[a:b] * span_t
-            pg = scene_loss(batched_waveforms(th_g, n_forward, n_backward, key, **specs[i]), target[a:b])
+            w = batched_waveforms(th_g, n_forward, n_backward, key, **specs[i])
+            if independent_key is not None:
+                with torch.no_grad():
+                    w2 = gated_waveforms(th_g, n_forward, n_backward, independent_key, **dict(specs[i], graphs=None))
+                w = w + (w2 - w).detach()
+            pg = scene_loss(w, target[a:b])
             pg.sum().backward()
             per[a:b] = pg.detach()
             progress(f"iteration {it + 1}: group {i + 1}/{len(groups)} done")

@@ -40,6 +40,17 @@ def test_depth_moves_toward_truth_on_clear_scenes(gpu, loss):
     assert res.depth_rmse[-1] < res.depth_rmse[0]
 
 
+def test_independent_run_loss_moves_depth_toward_truth(gpu):
+    """independent_key: the loss is evaluated on a second forward run (other
+    photons) and replayed through the first; the clear scenes' depths still
+    move toward the truth."""
+    th = _clear_scenes()
+    res = invert(th, 40000, 10000, iterations=10, lr=0.02, x0=_x0(th, 4.0), independent_key=(0x1A7E, 9))
+    err = np.abs(res.theta[:, COLS["depth"]] - th[:, COLS["depth"]])
+    assert np.all(np.isfinite(res.loss)) and np.all(np.isfinite(res.theta))
+    assert err.mean() < 0.6 * 4.0, err
+
+
 def test_plain_form_runs(gpu):
     th = _clear_scenes()[:2]
     res = invert(th, 20000, 5000, iterations=2, lr=0.02, x0=_x0(th, 4.0), form="plain")
