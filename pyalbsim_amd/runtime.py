@@ -352,7 +352,7 @@ def build_map(records: Sink, n: int, dims: int, stream=None) -> DeviceMap:
     return m
 
 
-CHUNK = 1 << 24  # queries per k-NN / contribution round at k = 100 (2^22: SCATTER k-NN 3 % slower; rows take 4 k B per query)
+CHUNK = 1 << 24  # queries per k-NN / contribution round at k = 100 (2^22: SCATTER k-NN 3 % slower; 2^25: the same step time; rows take 4 k B per query)
 
 
 def gather_chunk(q_n: int, k: int) -> int:
