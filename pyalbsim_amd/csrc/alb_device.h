@@ -588,6 +588,16 @@ __device__ __forceinline__ uint64_t spread3(uint64_t v) {  // 21 bits -> every t
   return v;
 }
 
+// float32 <-> unsigned keys that order like the floats (bounding boxes by
+// integer atomicMin / atomicMax)
+__device__ __forceinline__ uint32_t f2ord(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
 __device__ __forceinline__ uint64_t quant(double p, double lo, double inv, double maxc) {
   double c = floor((p - lo) * inv);
   c = c < 0.0 ? 0.0 : (c > maxc ? maxc : c);

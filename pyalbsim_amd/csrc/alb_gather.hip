@@ -27,8 +27,9 @@
 //      FP64 in one fixed order, box distances are, by monotone rounding,
 //      never larger than those of the records inside, and every cut keeps all
 //      entries at or below its bound.
-//  Queries are first sorted by the Morton key of their position on the map's
-//  grid, so neighbouring wavefronts walk the same nodes and buckets (L2 hits).
+//  Queries are first sorted by the Morton key of their position on a grid over
+//  the queries' own box, so neighbouring wavefronts walk the same nodes and
+//  buckets (L2 hits), also where the queries lie far outside the records.
 //  contrib_kernel — one thread per (query, neighbour): energy multiplier,
 //   kernel normalisation and time bin; FP64 sums go to an LDS-privatised
 //   window of the waveform (ds_add_f64), bins outside it to global atomics.
@@ -1218,6 +1219,67 @@ __global__ __launch_bounds__(256) void replica_reduce_kernel(const double* __res
   }
 }
 
+// Grid of the query sort: the map's, or the queries' own bounding box (cubic
+// cells, the map's dims and bits per axis) where the queries spread far wider
+// than the records.  On the map's grid, queries outside the
+// records' box are clamped onto its boundary cells: the surface-reflection
+// map's records lie in the laser spot while its queries spread tens of
+// metres around it, so nearly all of them shared a few keys and reached the
+// k-NN in trace order.  The order changes no result: each query's neighbour
+// set is exact in any order.
+__global__ void query_bbox_init_kernel(uint32_t* bb) {
+  if (threadIdx.x < 3) bb[threadIdx.x] = 0xFFFFFFFFu;
+  else if (threadIdx.x < 6) bb[threadIdx.x] = 0u;
+}
+
+__global__ __launch_bounds__(256) void query_bbox_kernel(alb_queries_t q, int64_t q_begin, int64_t q_n, uint32_t* bb) {
+  uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < q_n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t a = f2ord((float)q.x[q_begin + i]), b = f2ord((float)q.y[q_begin + i]),
+                   c = f2ord((float)q.z[q_begin + i]);
+    mn[0] = min(mn[0], a); mx[0] = max(mx[0], a);
+    mn[1] = min(mn[1], b); mx[1] = max(mx[1], b);
+    mn[2] = min(mn[2], c); mx[2] = max(mx[2], c);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      mn[k] = min(mn[k], (uint32_t)__shfl_xor((int)mn[k], off));
+      mx[k] = max(mx[k], (uint32_t)__shfl_xor((int)mx[k], off));
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      atomicMin(&bb[k], mn[k]);
+      atomicMax(&bb[3 + k], mx[k]);
+    }
+  }
+}
+
+// qmeta: the map meta's layout (origin, inverse cell width, dims, bits) over
+// the queries' box; a query rounded outside it is clamped by quant, which only
+// moves it to a boundary cell of the sort
+__global__ void query_meta_kernel(const uint32_t* bb, const double* __restrict__ map_meta, double* qmeta) {
+  const int dims = (int)map_meta[6];
+  const double cells = ldexp(1.0, (int)map_meta[7]);
+  double ext = 0.0;
+  for (int k = 0; k < 3; ++k)
+    if (dims == 3 || k != 1) ext = fmax(ext, (double)ord2f(bb[3 + k]) - (double)ord2f(bb[k]));
+  if (!(ext > 0.0)) ext = 1.0;
+  // the map's grid unless the queries spread more than four times as wide as
+  // the records (a sort over all maps on the queries' grid was slower on the
+  // sweep scenes, whose queries and records mostly share their extent)
+  const double map_ext = cells / map_meta[3];
+  const bool own = ext > 4.0 * map_ext;
+  for (int k = 0; k < 8; ++k) qmeta[k] = map_meta[k];
+  if (own) {
+    for (int k = 0; k < 3; ++k) qmeta[k] = (double)ord2f(bb[k]);
+    for (int k = 0; k < 3; ++k) qmeta[3 + k] = cells / ext;
+  }
+}
+
 __global__ __launch_bounds__(256) void query_key_kernel(alb_queries_t q, int64_t q_begin, int64_t q_n,
                                                         const double* __restrict__ meta, uint64_t* __restrict__ keys,
                                                         int32_t* __restrict__ vals) {
@@ -1258,7 +1320,7 @@ __global__ __launch_bounds__(256) void query_key_batched_kernel(alb_queries_t q,
 static size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct GatherWs {
-  size_t keys_a, keys_b, vals_a, order, temp, temp_bytes, nbr, rad, knorm, work, total;
+  size_t keys_a, keys_b, vals_a, order, temp, temp_bytes, nbr, rad, knorm, work, qgrid, total;
   int64_t chunk, knorm_rows;
 };
 
@@ -1284,6 +1346,7 @@ static int gather_layout(int64_t q_n, int k, int64_t chunk, GatherWs* L) {
   L->knorm_rows = q_n > chunk ? q_n : chunk;
   L->knorm = off; off = al256(off + (size_t)(L->knorm_rows > 0 ? L->knorm_rows : 1) * sizeof(double));
   L->work = off; off = al256(off + ALB_KNN_WORK_WORDS * sizeof(unsigned long long));
+  L->qgrid = off; off = al256(off + 8 * sizeof(double) + 8 * sizeof(uint32_t));  // query-sort grid
   L->total = off;
   return ALB_OK;
 }
@@ -1358,8 +1421,13 @@ extern "C" int alb_gather_order(const alb_map_t* map, const alb_queries_t* q, in
   uint64_t* keys_a = (uint64_t*)(ws + L.keys_a);
   uint64_t* keys_b = (uint64_t*)(ws + L.keys_b);
   int32_t* vals_a = (int32_t*)(ws + L.vals_a);
-  hipLaunchKernelGGL(query_key_kernel, dim3((unsigned)((q_n + 255) / 256)), dim3(256), 0, s, *q, q_begin, q_n,
-                     map->meta, keys_a, vals_a);
+  double* qmeta = (double*)(ws + L.qgrid);
+  uint32_t* qbb = (uint32_t*)(qmeta + 8);
+  const unsigned gq = (unsigned)((q_n + 255) / 256);
+  hipLaunchKernelGGL(query_bbox_init_kernel, dim3(1), dim3(64), 0, s, qbb);
+  hipLaunchKernelGGL(query_bbox_kernel, dim3(gq < 2048 ? gq : 2048), dim3(256), 0, s, *q, q_begin, q_n, qbb);
+  hipLaunchKernelGGL(query_meta_kernel, dim3(1), dim3(1), 0, s, qbb, map->meta, qmeta);
+  hipLaunchKernelGGL(query_key_kernel, dim3(gq), dim3(256), 0, s, *q, q_begin, q_n, qmeta, keys_a, vals_a);
   size_t tb = L.temp_bytes;
   hipError_t e = hipcub::DeviceRadixSort::SortPairs((void*)(ws + L.temp), tb, keys_a, keys_b, vals_a, order,
                                                     (int)q_n, 0, map->dims == 2 ? 64 : 63, s);
@@ -1586,11 +1654,17 @@ extern "C" int alb_gather_order_batched(const alb_map_t* map, const alb_scene_in
   const int key_bits = map->dims == 2 ? 64 : 63;
   const int sb = index->scene_bits;
   const unsigned g = (unsigned)((q_n + 255) / 256);
+  // the sort's grid, as in alb_gather_order
+  double* qmeta = (double*)(ws + L.qgrid);
+  uint32_t* qbb = (uint32_t*)(qmeta + 8);
+  hipLaunchKernelGGL(query_bbox_init_kernel, dim3(1), dim3(64), 0, s, qbb);
+  hipLaunchKernelGGL(query_bbox_kernel, dim3(g < 2048 ? g : 2048), dim3(256), 0, s, *q, (int64_t)0, q_n, qbb);
+  hipLaunchKernelGGL(query_meta_kernel, dim3(1), dim3(1), 0, s, qbb, map->meta, qmeta);
   if (sb > 0)
-    hipLaunchKernelGGL(query_key_batched_kernel, dim3(g), dim3(256), 0, s, *q, q_n, map->meta, key_bits, sb, keys_a,
+    hipLaunchKernelGGL(query_key_batched_kernel, dim3(g), dim3(256), 0, s, *q, q_n, qmeta, key_bits, sb, keys_a,
                        vals_a);
   else
-    hipLaunchKernelGGL(query_key_kernel, dim3(g), dim3(256), 0, s, *q, (int64_t)0, q_n, map->meta, keys_a, vals_a);
+    hipLaunchKernelGGL(query_key_kernel, dim3(g), dim3(256), 0, s, *q, (int64_t)0, q_n, qmeta, keys_a, vals_a);
   size_t tb = L.temp_bytes;
   hipError_t e = hipcub::DeviceRadixSort::SortPairs((void*)(ws + L.temp), tb, keys_a, keys_b, vals_a, order,
                                                     (int)q_n, 0, key_bits, s);

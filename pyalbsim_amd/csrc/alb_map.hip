@@ -33,13 +33,6 @@ namespace alb {
 constexpr int kLevels = 128;  // prefix lengths 0..63 (+64..95 for equal keys)
 constexpr int kNoBox = 255;   // level of a node whose boxes are not written
 
-__device__ __forceinline__ uint32_t f2ord(float f) {
-  const uint32_t u = __float_as_uint(f);
-  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float ord2f(uint32_t u) {
-  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
-}
 
 __global__ void bbox_init_kernel(uint32_t* bb, uint32_t* hist) {
   if (threadIdx.x < 3) bb[threadIdx.x] = 0xFFFFFFFFu;
