@@ -27,8 +27,9 @@
 //      FP64 in one fixed order, box distances are, by monotone rounding,
 //      never larger than those of the records inside, and every cut keeps all
 //      entries at or below its bound.
-//  Queries are first sorted by the Morton key of their position on a grid over
-//  the queries' own box, so neighbouring wavefronts walk the same nodes and
+//  Queries are first sorted by the Morton key of their position on the map's
+//  grid (on a grid over their own box where they spread more than four times
+//  as wide as the records), so neighbouring wavefronts walk the same nodes and
 //  buckets (L2 hits), also where the queries lie far outside the records.
 //  contrib_kernel — one thread per (query, neighbour): energy multiplier,
 //   kernel normalisation and time bin; FP64 sums go to an LDS-privatised
